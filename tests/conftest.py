@@ -1,0 +1,45 @@
+import gzip
+import os
+import shutil
+import sys
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+PKG = os.path.join(ROOT, "xcalibre_tw.jl_amd")
+for p in (PKG, ROOT):
+    if p not in sys.path:
+        sys.path.insert(0, p)
+
+GOLDEN = os.path.join(ROOT, "tests", "golden")
+
+
+def pytest_configure(config):
+    config.addinivalue_line("markers", "gpu: needs an MI355X (gfx950) device")
+    config.addinivalue_line("markers", "slow: full-size case (minutes on CPU)")
+
+
+@pytest.fixture(scope="session")
+def quad40_path(tmp_path_factory):
+    """The reference's own sample mesh unv_sample_meshes/quad40.unv (fixture data)."""
+    d = tmp_path_factory.mktemp("unv")
+    out = d / "quad40.unv"
+    with gzip.open(os.path.join(GOLDEN, "quad40.unv.gz"), "rb") as f, open(out, "wb") as g:
+        shutil.copyfileobj(f, g)
+    return str(out)
+
+
+@pytest.fixture(scope="session")
+def quad40(quad40_path):
+    from fvmhip import build_mesh
+
+    return build_mesh(quad40_path, scale=0.001)
+
+
+def has_gpu():
+    try:
+        from fvmhip import device_count
+
+        return device_count() > 0
+    except Exception:
+        return False

@@ -1,0 +1,164 @@
+"""GPU parity: libfvmhip (HIP, gfx950) against the oracle on the same inputs.
+
+The FP64 path is required to be bit-identical to the oracle (same operation
+order, same canonical reductions, no FMA); every comparison below is exact
+unless it says otherwise.
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import fvmhip as F
+from fvmhip import _lib as L
+from oracle import ref
+from tests.cases import bfs_model, cavity_model, config, flatplate_model, rel_l2, same_bits
+
+pytestmark = pytest.mark.gpu
+
+
+def _sims(model, cfg, pref=None, algorithm=L.ALG_SIMPLE, conv_break=1e-7):
+    case = F.CaseDesc(model, cfg, pref, algorithm, conv_break)
+    return F.Simulation(model, cfg, pref, algorithm, conv_break), ref.RefSimulation(model, case)
+
+
+PHASE_ARRAYS = ["ux_A", "ux_b", "uy_A", "uy_b", "ux_dinv", "Ux", "Uy", "rD", "rDf", "Hv_x", "Hv_y", "divHv", "p_A",
+                "p_b", "p_dinv", "p", "gradp_x", "gradp_y", "pf", "Uf_x", "Uf_y", "mdotf"]
+
+
+def test_cavity_first_iterations_bitwise(quad40):
+    """Every intermediate array of the first two SIMPLE iterations of C1."""
+    model = cavity_model(quad40)
+    g, r = _sims(model, config(2), pref=0.0)
+    for it in range(2):
+        assert g.run(1) == 1 and r.run(1) == 1
+        for name in PHASE_ARRAYS:
+            a, b = g.get(name), r.get(name)
+            assert same_bits(a, b), f"iteration {it + 1}: {name} differs (rel {rel_l2(a, b):.3e})"
+        for fld in ("Ux", "Uy", "p"):
+            assert g.stats(fld)["niter"] == r.stats(fld)["niter"], fld
+    Rg, Rr = g.residuals(), r.residuals()
+    for a, b in zip(Rg, Rr):
+        assert same_bits(a, b)
+    assert np.isnan(Rg[1][0])  # R_uy[1] = 0/0 (b == 0 at the first iteration)
+
+
+@pytest.mark.parametrize("usolver,psolver,upre,ppre", [
+    (F.BicgstabSolver, F.CgSolver, F.DILU, F.DILU),
+    (F.GmresSolver, F.GmresSolver, F.DILU, F.DILU),
+    (F.BicgstabSolver, F.BicgstabSolver, F.ILU0, F.ILU0),
+    (F.GmresSolver, F.CgSolver, F.Jacobi, F.Jacobi),
+    (F.BicgstabSolver, F.GmresSolver, F.NormDiagonal, F.DILU),
+])
+def test_cavity_solver_matrix_bitwise(quad40, usolver, psolver, upre, ppre):
+    model = cavity_model(quad40)
+    g, r = _sims(model, config(12, U_solver=usolver, p_solver=psolver, U_pre=upre, p_pre=ppre), pref=0.0)
+    assert g.run(12) == r.run(12)
+    for a, b in zip(g.residuals(), r.residuals()):
+        assert same_bits(a, b)
+    for name in ("Ux", "Uy", "p", "mdotf"):
+        assert same_bits(g.get(name), r.get(name)), name
+
+
+def test_cavity_c1_full_run(quad40):
+    """C1 as specified: 1000 iterations (break at 1e-7); histories and fields."""
+    model = cavity_model(quad40)
+    g, r = _sims(model, config(1000), pref=0.0)
+    ng, nr = g.run(1000), r.run(1000)
+    assert ng == nr
+    Rg, Rr = g.residuals(), r.residuals()
+    for a, b in zip(Rg, Rr):
+        assert same_bits(a, b)
+    for name in ("Ux", "Uy", "p"):
+        a, b = g.get(name), r.get(name)
+        assert rel_l2(a, b) <= 1e-10
+        assert same_bits(a, b), name
+
+
+def test_upwind_midpoint_euler_piso(quad40):
+    """Upwind divergence, Midpoint gradients, Euler time term, PISO loop."""
+    model = cavity_model(quad40)
+    cfg = config(4, schemes=dict(time=F.Euler, divergence=F.Upwind, gradient=F.Midpoint), dt=0.05)
+    g, r = _sims(model, cfg, pref=0.0, algorithm=L.ALG_PISO)
+    assert g.run(4) == r.run(4) == 4
+    for name in ("Ux", "Uy", "p", "gradp_x", "divHv", "ux_A", "ux_b"):
+        assert same_bits(g.get(name), r.get(name)), name
+
+
+def test_bfs_small_bitwise():
+    """A small BFS (graded mesh, Dirichlet outlet pressure, Neumann outlet U)."""
+    mesh = F.backward_facing_step(nx=120, ny=30)
+    model = bfs_model(mesh, nu=1e-3)
+    g, r = _sims(model, config(20, schemes=dict(divergence=F.Upwind)), conv_break=1e-6)
+    assert g.run(20) == r.run(20)
+    for a, b in zip(g.residuals(), r.residuals()):
+        assert same_bits(a, b)
+    for name in ("Ux", "Uy", "p"):
+        assert same_bits(g.get(name), r.get(name)), name
+
+
+def test_komega_flatplate_bitwise():
+    mesh = F.flat_plate(nx=60, ny=30, first=1e-3, ratio=1.1)
+    model = flatplate_model(mesh)
+    cfg = config(5, schemes=dict(divergence=F.Upwind), turbulent=True, p_solver=F.BicgstabSolver)
+    g, r = _sims(model, cfg)
+    assert g.run(5) == r.run(5)
+    for name in ("Ux", "Uy", "p", "k", "omega", "nut", "nutf", "Pk"):
+        a, b = g.get(name), r.get(name)
+        assert same_bits(a, b), f"{name} rel {rel_l2(a, b):.3e}"
+
+
+# ---------------------------------------------------------------------------
+# Operator level, at full size
+# ---------------------------------------------------------------------------
+@pytest.fixture(scope="module")
+def bfs_full():
+    return F.backward_facing_step()  # 2000 x 500, 1M cells
+
+
+def _eqn(mesh):
+    dm = mesh.device()
+    h = C.c_void_p()
+    L.check(L.lib().fvh_eqn_create(dm.h, C.byref(h)))
+    nz, b = C.c_void_p(), C.c_void_p()
+    L.check(L.lib().fvh_eqn_arrays(h, C.byref(nz), C.byref(b)))
+    return dm, h, nz.value, b.value
+
+
+def _to_dev(x):
+    import torch
+
+    return torch.from_numpy(np.ascontiguousarray(x, np.float64)).cuda()
+
+
+def _hip_copy(dst, src, nbytes):
+    hip = C.CDLL("libamdhip64.so")
+    assert hip.hipMemcpy(C.c_void_p(dst), C.c_void_p(src), C.c_size_t(nbytes), 3) == 0  # device to device
+
+
+def test_spmv_and_dilu_full_size_bitwise(bfs_full):
+    import torch
+
+    mesh = bfs_full
+    dm, h, nzp, bp = _eqn(mesh)
+    try:
+        rp, ci = dm.pattern()
+        rng = np.random.default_rng(7)
+        nnz = ci.size
+        rows = np.repeat(np.arange(mesh.ncells), np.diff(rp))
+        vals = np.where(rows == ci, 4.5, 0.0) - rng.uniform(0.1, 1.0, nnz) * (rows != ci)
+        vt = _to_dev(vals)
+        torch.cuda.synchronize()
+        _hip_copy(nzp, vt.data_ptr(), 8 * nnz)
+        x = rng.standard_normal(mesh.ncells)
+        xt = _to_dev(x)
+        yt = torch.zeros(mesh.ncells, dtype=torch.float64, device="cuda")
+        torch.cuda.synchronize()
+        L.check(L.lib().fvh_spmv(h, C.c_void_p(xt.data_ptr()), C.c_void_p(yt.data_ptr())))
+        assert same_bits(yt.cpu().numpy(), ref.spmv_csr(mesh, vals, x))
+        L.check(L.lib().fvh_precond_set(h, L.PRECOND_DILU, None))
+        L.check(L.lib().fvh_precond_update(h))
+        L.check(L.lib().fvh_precond_apply(h, C.c_void_p(yt.data_ptr()), C.c_void_p(xt.data_ptr())))
+        assert same_bits(yt.cpu().numpy(), ref.precond_apply_csr(mesh, vals, L.PRECOND_DILU, x))
+    finally:
+        L.lib().fvh_eqn_destroy(h)

@@ -1,0 +1,149 @@
+// Device-side internals of libfvmhip.so: buffers, the device mesh view that
+// kernels receive, the per-equation state, Krylov scalars and the canonical
+// reduction order.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+#include <map>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "fvh_internal.h"
+
+#define HIP_TRY(expr)                                                                               \
+    do {                                                                                            \
+        hipError_t e_ = (expr);                                                                     \
+        if (e_ != hipSuccess)                                                                       \
+            return fvh::set_error(FVH_ERR_HIP, "%s:%d %s: %s", __FILE__, __LINE__, #expr,          \
+                                  hipGetErrorString(e_));                                           \
+    } while (0)
+
+#define FVH_TRY(expr)                    \
+    do {                                 \
+        int rc_ = (expr);                \
+        if (rc_ != FVH_OK) return rc_;   \
+    } while (0)
+
+namespace fvh {
+
+// Canonical summation order shared with the oracle: chunks of kChunk
+// elements, each summed by kBlock lanes (lane t takes t, t+256, ... in
+// order), then a pairwise tree (t, t+s) for s = 128 ... 1; the chunk
+// partials are reduced the same way by one block.
+constexpr int kBlock = 256;
+constexpr int kPer = 8;
+constexpr int kChunk = kBlock * kPer;
+
+inline int nchunks(int64_t n) { return (int)((n + kChunk - 1) / kChunk); }
+
+hipStream_t stream();
+int ensure_device();
+
+template <class T>
+struct DBuf {
+    T *p = nullptr;
+    size_t n = 0;
+    DBuf() = default;
+    DBuf(const DBuf &) = delete;
+    DBuf &operator=(const DBuf &) = delete;
+    ~DBuf() { release(); }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    int alloc(size_t count) {
+        release();
+        if (count == 0) count = 1;
+        hipError_t e = hipMalloc((void **)&p, count * sizeof(T));
+        if (e != hipSuccess) {
+            p = nullptr;
+            return set_error(FVH_ERR_HIP, "hipMalloc(%zu bytes): %s", count * sizeof(T), hipGetErrorString(e));
+        }
+        n = count;
+        return FVH_OK;
+    }
+    int zero() {
+        HIP_TRY(hipMemsetAsync(p, 0, n * sizeof(T), stream()));
+        return FVH_OK;
+    }
+    int upload(const std::vector<T> &v) {
+        FVH_TRY(alloc(v.size()));
+        if (!v.empty()) HIP_TRY(hipMemcpy(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+        return FVH_OK;
+    }
+    int download(std::vector<T> &v, size_t count) const {
+        v.resize(count);
+        if (count) HIP_TRY(hipMemcpy(v.data(), p, count * sizeof(T), hipMemcpyDeviceToHost));
+        return FVH_OK;
+    }
+};
+
+// What kernels see of the mesh (plain pointers, int32 indices).
+struct DevMesh {
+    int n, nf, nbf, nnz, ncf;
+    const int *rowptr, *col, *diag, *tslot;         // CSR pattern of A
+    const int *cf_ptr, *cf_face, *cf_nbr, *cf_slot;  // cell -> internal faces (faces_range order)
+    const double *cf_sign, *cf_w, *cf_omw;           // nsign, divergence weights of that side
+    const int *own1, *own2;                          // face owners (boundary: own1 == own2)
+    const double *fw, *farea, *fdelta;               // face weight, area, delta
+    const double *Sfx, *Sfy, *Sfz;                   // area * normal
+    const double *nx, *ny, *nz;                      // unit normals
+    const double *vol;                               // cell volumes
+    const double *ccx, *ccy, *ccz, *fcx, *fcy, *fcz;  // centres
+    const int *cb_ptr, *cb_face;                     // cell -> boundary faces, ascending face id
+    const int *bf_face, *bf_cell, *bf_patch;         // boundary faces in boundary (patch) order
+};
+
+// Per-cell boundary-condition entries of one BC tuple, in the order
+// apply_boundary_conditions! visits them (tuple order, then patch order).
+struct BcPlan {
+    DBuf<int> ptr, face, bc;
+};
+
+// Sweep schedule of the lower (forward) and upper (backward) triangles.
+struct Schedule {
+    int nlev = 0;
+    DBuf<int> fptr, frows, bptr, brows;
+    std::vector<int> h_fptr, h_bptr;
+    int band_nx = 0;  // >0: A is a banded grid (deps only r-1, r-nx)
+};
+
+}  // namespace fvh
+
+struct fvh_mesh {
+    fvh::DevMesh d{};
+    int64_t n = 0, nf = 0, nbf = 0, nnz = 0;
+    int nboundaries = 0;
+    // owned device storage
+    fvh::DBuf<int> rowptr, col, diag, tslot, cf_ptr, cf_face, cf_nbr, cf_slot, own1, own2, cb_ptr, cb_face,
+        bf_face, bf_cell, bf_patch;
+    fvh::DBuf<double> cf_sign, cf_w, cf_omw, fw, farea, fdelta, Sfx, Sfy, Sfz, nx, ny, nz, vol, ccx, ccy, ccz, fcx,
+        fcy, fcz;
+    // host copies used by set-up code
+    std::vector<int> h_rowptr, h_col, h_diag, h_bf_face, h_bf_cell, h_bf_patch;
+    std::vector<int> h_bstart;  // boundary b occupies bf[h_bstart[b], h_bstart[b+1])
+    std::vector<double> h_fdelta, h_vol, h_nx, h_ny, h_nz;
+    fvh::Schedule sched;
+    std::map<std::vector<int>, std::unique_ptr<fvh::BcPlan>> bc_plans;
+    // ILU(0) update lists: for each lower entry (i,k), pairs (slot(i,j), slot(k,j)), j > k
+    bool ilu_ready = false;
+    fvh::DBuf<int> ilu_uptr, ilu_pairs;
+};
+
+namespace fvh {
+
+// Krylov scalars, resident on the device so that an iteration never waits
+// for the host.
+struct KScal {
+    double rho, alpha, omega, beta, rnorm, eps, gamma, pnorm2, rnorm0, ts, tt, sigma;
+    double hbis, zeta;
+    int done, iter, status, k;
+};
+
+int bc_plan(fvh_mesh *m, const fvh_bc *bcs, int nbc, BcPlan **out);
+
+}  // namespace fvh

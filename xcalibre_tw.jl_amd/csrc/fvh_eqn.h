@@ -1,0 +1,87 @@
+// The equation handle: A values on the mesh CSR pattern, b, R, Fx, the
+// preconditioner attached to it and the Krylov workspace.
+#pragma once
+
+#include "k_calc.h"
+
+namespace fvh {
+
+constexpr int kMaxTerms = 4;
+constexpr int kMaxSrc = 2;
+
+struct KTerm {
+    int op, scheme;
+    double sign;
+    const double *flux;  // face array (Laplacian, Divergence) or cell array (Si); null = 1
+    const double *phi;   // cell values of the unknown
+};
+
+enum { ASM_FULL = 0, ASM_BC_ONLY = 1, ASM_RELAX_ONLY = 2 };
+
+struct AsmArgs {
+    int mode;
+    int nterms;
+    KTerm t[kMaxTerms];
+    int nsrc;
+    double ssign[kMaxSrc];
+    const double *sfield[kMaxSrc];
+    const double *prev;  // Time{Euler} previous values (null = 0)
+    double rdt;
+    const int *bptr, *bface, *bbc;  // fused boundary conditions (null = none)
+    BcVals bc;
+    int relax;
+    double alpha;
+    const double *rfield;
+    int setref;
+    double pref;
+    int refcell;
+};
+
+// Krylov workspace, allocated on first use and kept across solves.
+struct KrylovWork {
+    int64_t n = 0;
+    int cap_basis = 0;
+    DBuf<double> v[10];          // FP64 vectors
+    DBuf<double> basis;          // GMRES Krylov basis (cap_basis+1) x n
+    DBuf<double> part;           // reduction partials
+    DBuf<double> small;          // GMRES R, c, s, z, y
+    DBuf<KScal> ks;
+    KScal *h_ks = nullptr;       // pinned mirror
+    // mixed precision (FP32 inner)
+    DBuf<float> f[10];
+    DBuf<float> valf;            // FP32 copy of A
+    DBuf<unsigned short> valh;   // FP16 copy of the off-diagonals
+    ~KrylovWork() {
+        if (h_ks) (void)hipHostFree(h_ks);
+    }
+};
+
+}  // namespace fvh
+
+struct fvh_eqn {
+    fvh_mesh *m = nullptr;
+    fvh::DBuf<double> nz, b, R, Fx;
+    // preconditioner
+    int pkind = FVH_PRECOND_NONE;
+    fvh_eqn *pmat = nullptr;    // equation whose A the preconditioner reads
+    fvh_eqn *pshare = nullptr;  // use that equation's preconditioner object instead
+    fvh::DBuf<double> D, Dinv, lu, pstore;
+    fvh::KrylovWork kw;
+    fvh::DBuf<double> red;  // residual reductions
+};
+
+namespace fvh {
+
+int launch_assemble(fvh_eqn *e, const AsmArgs &a);
+int make_terms(const fvh_term *terms, int nterms, AsmArgs &a);
+int check_bc_terms(const AsmArgs &a, const fvh_bc *bcs, int nbc);
+
+// solve-side entry points used by the drivers
+int precond_update(fvh_eqn *e);
+int precond_apply(fvh_eqn *e, double *y, const double *x, const KScal *ks, double sigma);
+int launch_spmv(fvh_eqn *e, const double *x, double *y, const KScal *ks);
+int solve(fvh_eqn *e, const fvh_solver_cfg &cfg, double *x, fvh_stats *st);
+int residual(fvh_eqn *e, const double *phi, double *res_host);
+int residual_async(fvh_eqn *e, const double *phi, double *res_dev);
+
+}  // namespace fvh

@@ -1,0 +1,150 @@
+// DILU / ILU(0) recurrences, level-scheduled in the reference's row order.
+//
+// The DILU diagonal recurrence (preconditioners_1_DILU.jl:56-105) and both
+// sweeps (:107-162) are evaluated row by row with each row's terms in the
+// reference's order, so the preconditioner and its rounding are unchanged.
+// Rows of one level have no dependency on each other; one work-group walks
+// the levels with a barrier between them (rows of a level in parallel).
+#include "k_sweep.h"
+
+namespace fvh {
+
+constexpr int kSweepBlock = 1024;
+
+__global__ void __launch_bounds__(kSweepBlock)
+    k_dilu_update_levels(int nlev, const int *__restrict__ lptr, const int *__restrict__ lrows,
+                         const int *__restrict__ rowptr, const int *__restrict__ col, const int *__restrict__ diag,
+                         const int *__restrict__ tslot, const double *__restrict__ val, double *D, double *Dinv) {
+    for (int l = 0; l < nlev; ++l) {
+        for (int k = lptr[l] + threadIdx.x; k < lptr[l + 1]; k += kSweepBlock) {
+            const int j = lrows[k];
+            double d = val[diag[j]];
+            for (int p = rowptr[j]; p < diag[j]; ++p) {
+                const int i = col[p];
+                d = d - val[p] * val[tslot[p]] / D[i];
+            }
+            D[j] = d;
+            Dinv[j] = 1.0 / d;
+        }
+        __syncthreads();
+    }
+}
+
+__global__ void __launch_bounds__(kSweepBlock)
+    k_dilu_apply_levels(int nlev, const int *__restrict__ fptr, const int *__restrict__ frows,
+                        const int *__restrict__ bptr, const int *__restrict__ brows, const int *__restrict__ rowptr,
+                        const int *__restrict__ col, const int *__restrict__ diag, const double *__restrict__ val,
+                        const double *__restrict__ Dinv, const double *__restrict__ x, double *y, const KScal *ks) {
+    if (ks && ks->done) return;
+    // forward: y_i = x_i - sum_{j<i} (A_ij y_j) Dinv_j, j ascending
+    for (int l = 0; l < nlev; ++l) {
+        for (int k = fptr[l] + threadIdx.x; k < fptr[l + 1]; k += kSweepBlock) {
+            const int i = frows[k];
+            double s = x[i];
+            for (int p = rowptr[i]; p < diag[i]; ++p) {
+                const int j = col[p];
+                s = s - val[p] * y[j] * Dinv[j];
+            }
+            y[i] = s;
+        }
+        __syncthreads();
+    }
+    // backward: z_i = y_i Dinv_i - sum_{j>i} (A_ij z_j) Dinv_i, j ascending
+    for (int l = 0; l < nlev; ++l) {
+        for (int k = bptr[l] + threadIdx.x; k < bptr[l + 1]; k += kSweepBlock) {
+            const int i = brows[k];
+            const double di = Dinv[i];
+            double s = y[i] * di;
+            for (int p = diag[i] + 1; p < rowptr[i + 1]; ++p) s = s - val[p] * y[col[p]] * di;
+            y[i] = s;
+        }
+        __syncthreads();
+    }
+}
+
+__global__ void k_copy(int n, const double *__restrict__ a, double *__restrict__ b) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) b[i] = a[i];
+}
+
+// ILU(0), Saad Alg. 10.4 (IKJ) row by row on A's pattern
+__global__ void __launch_bounds__(kSweepBlock)
+    k_ilu0_factor_levels(int nlev, const int *__restrict__ lptr, const int *__restrict__ lrows,
+                         const int *__restrict__ rowptr, const int *__restrict__ col, const int *__restrict__ diag,
+                         const int *__restrict__ uptr, const int *__restrict__ pairs, double *lu) {
+    for (int l = 0; l < nlev; ++l) {
+        for (int k = lptr[l] + threadIdx.x; k < lptr[l + 1]; k += kSweepBlock) {
+            const int i = lrows[k];
+            for (int pk = rowptr[i]; pk < diag[i]; ++pk) {
+                const int kk = col[pk];
+                double lik = lu[pk] / lu[diag[kk]];
+                lu[pk] = lik;
+                for (int q = uptr[pk]; q < uptr[pk + 1]; q += 2) lu[pairs[q]] = lu[pairs[q]] - lik * lu[pairs[q + 1]];
+            }
+        }
+        __syncthreads();
+    }
+}
+
+__global__ void __launch_bounds__(kSweepBlock)
+    k_ilu0_apply_levels(int nlev, const int *__restrict__ fptr, const int *__restrict__ frows,
+                        const int *__restrict__ bptr, const int *__restrict__ brows, const int *__restrict__ rowptr,
+                        const int *__restrict__ col, const int *__restrict__ diag, const double *__restrict__ lu,
+                        const double *__restrict__ x, double *y, const KScal *ks) {
+    if (ks && ks->done) return;
+    for (int l = 0; l < nlev; ++l) {
+        for (int k = fptr[l] + threadIdx.x; k < fptr[l + 1]; k += kSweepBlock) {
+            const int i = frows[k];
+            double s = x[i];
+            for (int p = rowptr[i]; p < diag[i]; ++p) s = s - lu[p] * y[col[p]];
+            y[i] = s;
+        }
+        __syncthreads();
+    }
+    for (int l = 0; l < nlev; ++l) {
+        for (int k = bptr[l] + threadIdx.x; k < bptr[l + 1]; k += kSweepBlock) {
+            const int i = brows[k];
+            double s = y[i];
+            for (int p = diag[i] + 1; p < rowptr[i + 1]; ++p) s = s - lu[p] * y[col[p]];
+            y[i] = s / lu[diag[i]];
+        }
+        __syncthreads();
+    }
+}
+
+int dilu_update(fvh_mesh *m, const double *val, double *D, double *Dinv) {
+    const Schedule &s = m->sched;
+    hipLaunchKernelGGL(k_dilu_update_levels, dim3(1), dim3(kSweepBlock), 0, stream(), s.nlev, s.fptr.p, s.frows.p,
+                       m->d.rowptr, m->d.col, m->d.diag, m->d.tslot, val, D, Dinv);
+    HIP_TRY(hipGetLastError());
+    return FVH_OK;
+}
+
+int dilu_apply(fvh_mesh *m, const double *val, const double *Dinv, const double *x, double *y, const KScal *ks) {
+    const Schedule &s = m->sched;
+    hipLaunchKernelGGL(k_dilu_apply_levels, dim3(1), dim3(kSweepBlock), 0, stream(), s.nlev, s.fptr.p, s.frows.p, s.bptr.p,
+                       s.brows.p, m->d.rowptr, m->d.col, m->d.diag, val, Dinv, x, y, ks);
+    HIP_TRY(hipGetLastError());
+    return FVH_OK;
+}
+
+int ilu0_factor(fvh_mesh *m, const double *val, double *lu) {
+    const Schedule &s = m->sched;
+    const int nnz = (int)m->nnz;
+    hipLaunchKernelGGL(k_copy, dim3((nnz + 255) / 256), dim3(256), 0, stream(), nnz, val, lu);
+    HIP_TRY(hipGetLastError());
+    hipLaunchKernelGGL(k_ilu0_factor_levels, dim3(1), dim3(kSweepBlock), 0, stream(), s.nlev, s.fptr.p, s.frows.p,
+                       m->d.rowptr, m->d.col, m->d.diag, m->ilu_uptr.p, m->ilu_pairs.p, lu);
+    HIP_TRY(hipGetLastError());
+    return FVH_OK;
+}
+
+int ilu0_apply(fvh_mesh *m, const double *lu, const double *x, double *y, const KScal *ks) {
+    const Schedule &s = m->sched;
+    hipLaunchKernelGGL(k_ilu0_apply_levels, dim3(1), dim3(kSweepBlock), 0, stream(), s.nlev, s.fptr.p, s.frows.p, s.bptr.p,
+                       s.brows.p, m->d.rowptr, m->d.col, m->d.diag, lu, x, y, ks);
+    HIP_TRY(hipGetLastError());
+    return FVH_OK;
+}
+
+}  // namespace fvh

@@ -1,0 +1,393 @@
+// Library core: error channel, device/stream management and the device mesh
+// (CSR pattern, face -> CSR slot maps, boundary tables, sweep schedules).
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+
+#include "fvh_device.h"
+
+namespace fvh {
+
+static thread_local std::string g_last_error;
+static hipStream_t g_stream = nullptr;
+static int g_device_ok = -1;
+
+int set_error(int code, const char *fmt, ...) {
+    char buf[1024];
+    va_list ap;
+    va_start(ap, fmt);
+    std::vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_last_error = buf;
+    return code;
+}
+
+// The library works on the current HIP device with one non-blocking
+// default-priority stream.
+int ensure_device() {
+    if (g_device_ok == 1) return FVH_OK;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n == 0) {
+        g_device_ok = 0;
+        return set_error(FVH_ERR_NODEV, "no HIP device available");
+    }
+    hipDeviceProp_t prop;
+    HIP_TRY(hipGetDeviceProperties(&prop, 0));
+    if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+        return set_error(FVH_ERR_NODEV, "device is %s; this library is built for gfx950", prop.gcnArchName);
+    HIP_TRY(hipStreamCreateWithFlags(&g_stream, hipStreamNonBlocking));
+    g_device_ok = 1;
+    return FVH_OK;
+}
+
+hipStream_t stream() { return g_stream; }
+
+// Per-cell boundary-condition lists for one BC tuple, cached on the mesh.
+int bc_plan(fvh_mesh *m, const fvh_bc *bcs, int nbc, BcPlan **out) {
+    std::vector<int> key(nbc);
+    for (int i = 0; i < nbc; ++i) {
+        if (bcs[i].patch < 0 || bcs[i].patch >= m->nboundaries)
+            return set_error(FVH_ERR_ARG, "boundary condition %d names patch %d of %d", i, bcs[i].patch, m->nboundaries);
+        key[i] = bcs[i].patch;
+    }
+    auto it = m->bc_plans.find(key);
+    if (it != m->bc_plans.end()) {
+        *out = it->second.get();
+        return FVH_OK;
+    }
+    const int n = (int)m->n;
+    std::vector<std::vector<std::pair<int, int>>> per(n);
+    for (int k = 0; k < nbc; ++k) {
+        int b = bcs[k].patch;
+        for (int i = m->h_bstart[b]; i < m->h_bstart[b + 1]; ++i) per[m->h_bf_cell[i]].push_back({m->h_bf_face[i], k});
+    }
+    std::vector<int> ptr(n + 1, 0), face, bc;
+    for (int c = 0; c < n; ++c) {
+        for (auto &e : per[c]) {
+            face.push_back(e.first);
+            bc.push_back(e.second);
+        }
+        ptr[c + 1] = (int)face.size();
+    }
+    auto plan = std::make_unique<BcPlan>();
+    FVH_TRY(plan->ptr.upload(ptr));
+    FVH_TRY(plan->face.upload(face));
+    FVH_TRY(plan->bc.upload(bc));
+    *out = plan.get();
+    m->bc_plans.emplace(key, std::move(plan));
+    return FVH_OK;
+}
+
+}  // namespace fvh
+
+using namespace fvh;
+
+extern "C" const char *fvh_version(void) { return "fvmhip 0.1.0 (gfx950)"; }
+
+extern "C" const char *fvh_status_string(int s) {
+    switch (s) {
+        case FVH_OK: return "ok";
+        case FVH_ERR_ARG: return "bad argument";
+        case FVH_ERR_HIP: return "HIP runtime error";
+        case FVH_ERR_NODEV: return "no usable gfx950 device";
+        case FVH_ERR_MESH: return "inconsistent mesh";
+        case FVH_ERR_UNSUPPORTED: return "unsupported term/boundary combination";
+        case FVH_ERR_IO: return "I/O error";
+        case FVH_ERR_SIZE: return "size beyond maximum";
+        default: return "unknown status";
+    }
+}
+
+extern "C" int fvh_last_error(char *buf, int64_t len) {
+    if (!buf || len <= 0) return FVH_ERR_ARG;
+    std::snprintf(buf, (size_t)len, "%s", g_last_error.c_str());
+    return FVH_OK;
+}
+
+extern "C" int fvh_device_count(int32_t *n) {
+    if (!n) return set_error(FVH_ERR_ARG, "null");
+    int c = 0;
+    if (hipGetDeviceCount(&c) != hipSuccess) c = 0;
+    *n = c;
+    return FVH_OK;
+}
+
+namespace {
+
+struct V3 {
+    double x, y, z;
+};
+inline double vnorm(V3 a) { return std::sqrt((a.x * a.x + a.y * a.y) + a.z * a.z); }
+
+int build_schedule(fvh_mesh *m) {
+    const int n = (int)m->n;
+    const auto &rp = m->h_rowptr, &col = m->h_col, &dg = m->h_diag;
+    std::vector<int> lf(n, 0), lb(n, 0);
+    int nlev = 0;
+    for (int i = 0; i < n; ++i) {
+        int l = 0;
+        for (int p = rp[i]; p < dg[i]; ++p) l = std::max(l, lf[col[p]] + 1);
+        lf[i] = l;
+        nlev = std::max(nlev, l + 1);
+    }
+    int nlevb = 0;
+    for (int i = n - 1; i >= 0; --i) {
+        int l = 0;
+        for (int p = dg[i] + 1; p < rp[i + 1]; ++p) l = std::max(l, lb[col[p]] + 1);
+        lb[i] = l;
+        nlevb = std::max(nlevb, l + 1);
+    }
+    auto group = [n](const std::vector<int> &lev, int nl, std::vector<int> &ptr, std::vector<int> &rows, bool desc) {
+        ptr.assign(nl + 1, 0);
+        for (int i = 0; i < n; ++i) ptr[lev[i] + 1]++;
+        for (int l = 0; l < nl; ++l) ptr[l + 1] += ptr[l];
+        rows.assign(n, 0);
+        std::vector<int> pos(ptr.begin(), ptr.end() - 1);
+        if (!desc)
+            for (int i = 0; i < n; ++i) rows[pos[lev[i]]++] = i;
+        else
+            for (int i = n - 1; i >= 0; --i) rows[pos[lev[i]]++] = i;
+    };
+    std::vector<int> frows, brows;
+    group(lf, nlev, m->sched.h_fptr, frows, false);
+    group(lb, nlevb, m->sched.h_bptr, brows, true);
+    m->sched.nlev = nlev;
+    FVH_TRY(m->sched.fptr.upload(m->sched.h_fptr));
+    FVH_TRY(m->sched.frows.upload(frows));
+    FVH_TRY(m->sched.bptr.upload(m->sched.h_bptr));
+    FVH_TRY(m->sched.brows.upload(brows));
+    // banded grid: every lower neighbour of row r is r-1 (r % nx != 0) or r-nx
+    int nx = 0;
+    bool band = n > 1;
+    for (int i = 0; i < n && band; ++i)
+        for (int p = rp[i]; p < dg[i]; ++p) {
+            int d = i - col[p];
+            if (d == 1) continue;
+            if (nx == 0) nx = d;
+            if (d != nx) { band = false; break; }
+        }
+    if (band && nx > 1 && n % nx == 0) {
+        for (int i = 0; i < n && band; ++i)
+            for (int p = rp[i]; p < dg[i]; ++p)
+                if (i - col[p] == 1 && i % nx == 0) { band = false; break; }
+        m->sched.band_nx = band ? nx : 0;
+    } else {
+        m->sched.band_nx = 0;
+    }
+    return FVH_OK;
+}
+
+}  // namespace
+
+extern "C" int fvh_mesh_create(const fvh_mesh_desc *d, fvh_mesh **out) {
+    FVH_GUARD_BEGIN
+    if (!d || !out) return set_error(FVH_ERR_ARG, "fvh_mesh_create: null argument");
+    FVH_TRY(ensure_device());
+    if (d->ncells <= 0 || d->nfaces <= 0 || d->ncells >= (1LL << 30) || d->nfaces >= (1LL << 30))
+        return set_error(FVH_ERR_SIZE, "mesh sizes out of range");
+    const int64_t b = d->index_base;
+    const int n = (int)d->ncells, nf = (int)d->nfaces, ncf = (int)d->ncellfaces;
+    auto m = std::make_unique<fvh_mesh>();
+    m->n = n;
+    m->nf = nf;
+    m->nboundaries = (int)d->nboundaries;
+    // cell -> faces
+    std::vector<int> cf_ptr(n + 1, 0), cf_face(ncf), cf_nbr(ncf);
+    std::vector<double> cf_sign(ncf);
+    for (int c = 0; c < n; ++c) {
+        int64_t s = d->cell_faces_range[2 * c] - b, e = d->cell_faces_range[2 * c + 1] - b + 1;
+        if (s != cf_ptr[c] || e < s || e > ncf) return set_error(FVH_ERR_MESH, "faces_range of cell %d is not contiguous", c);
+        cf_ptr[c + 1] = (int)e;
+    }
+    if (cf_ptr[n] != ncf) return set_error(FVH_ERR_MESH, "faces_range does not cover cell_faces");
+    for (int i = 0; i < ncf; ++i) {
+        cf_face[i] = (int)(d->cell_faces[i] - b);
+        cf_nbr[i] = (int)(d->cell_neighbours[i] - b);
+        cf_sign[i] = (double)d->cell_nsign[i];
+        if (cf_face[i] < 0 || cf_face[i] >= nf || cf_nbr[i] < 0 || cf_nbr[i] >= n)
+            return set_error(FVH_ERR_MESH, "cell_faces entry %d out of range", i);
+    }
+    std::vector<int> own1(nf), own2(nf);
+    for (int f = 0; f < nf; ++f) {
+        own1[f] = (int)(d->face_owners[2 * f] - b);
+        own2[f] = (int)(d->face_owners[2 * f + 1] - b);
+        if (own1[f] < 0 || own1[f] >= n || own2[f] < 0 || own2[f] >= n) return set_error(FVH_ERR_MESH, "face %d owners out of range", f);
+    }
+    // boundaries
+    std::vector<int> bf_face, bf_cell, bf_patch, bstart(d->nboundaries + 1, 0);
+    for (int64_t g = 0; g < d->nboundaries; ++g) {
+        int64_t s = d->boundary_range[2 * g] - b, e = d->boundary_range[2 * g + 1] - b;
+        for (int64_t i = s; i <= e; ++i) {
+            bf_face.push_back((int)(d->boundary_faces[i] - b));
+            bf_cell.push_back((int)(d->boundary_cells[i] - b));
+            bf_patch.push_back((int)g);
+        }
+        bstart[g + 1] = (int)bf_face.size();
+    }
+    const int nbf = (int)bf_face.size();
+    m->nbf = nbf;
+    // the reference loops `for f in 1:nbfaces` over boundary faces and
+    // `nbfaces+1:end` over internal ones; the mesh must be laid out so
+    for (int f = 0; f < nf; ++f)
+        if ((f < nbf) != (own1[f] == own2[f]))
+            return set_error(FVH_ERR_MESH, "face %d: boundary faces must be the first %d faces", f, nbf);
+    for (int i = 0; i < nbf; ++i)
+        if (bf_face[i] < 0 || bf_face[i] >= nbf || own1[bf_face[i]] != bf_cell[i])
+            return set_error(FVH_ERR_MESH, "boundary entry %d inconsistent", i);
+    // CSR pattern (transpose of the reference CSC; the pattern is symmetric)
+    std::vector<int> rowptr(n + 1, 0), col, diag(n);
+    col.reserve(n + ncf);
+    std::vector<int> tmp;
+    for (int c = 0; c < n; ++c) {
+        tmp.assign(1, c);
+        for (int i = cf_ptr[c]; i < cf_ptr[c + 1]; ++i) tmp.push_back(cf_nbr[i]);
+        std::sort(tmp.begin(), tmp.end());
+        tmp.erase(std::unique(tmp.begin(), tmp.end()), tmp.end());
+        for (int j : tmp) {
+            if (j == c) diag[c] = (int)col.size();
+            col.push_back(j);
+        }
+        rowptr[c + 1] = (int)col.size();
+    }
+    const int nnz = (int)col.size();
+    m->nnz = nnz;
+    auto find = [&](int r, int c) -> int {
+        auto bb = col.begin() + rowptr[r], ee = col.begin() + rowptr[r + 1];
+        auto it = std::lower_bound(bb, ee, c);
+        return (it != ee && *it == c) ? (int)(it - col.begin()) : -1;
+    };
+    std::vector<int> tslot(nnz);
+    for (int r = 0; r < n; ++r)
+        for (int p = rowptr[r]; p < rowptr[r + 1]; ++p) {
+            tslot[p] = find(col[p], r);
+            if (tslot[p] < 0) return set_error(FVH_ERR_MESH, "connectivity is not symmetric at (%d,%d)", r, col[p]);
+        }
+    std::vector<int> cf_slot(ncf);
+    std::vector<char> used(nnz, 0);
+    std::vector<double> cf_w(ncf), cf_omw(ncf);
+    for (int c = 0; c < n; ++c)
+        for (int i = cf_ptr[c]; i < cf_ptr[c + 1]; ++i) {
+            int s = find(c, cf_nbr[i]);
+            if (used[s]) return set_error(FVH_ERR_MESH, "cells %d and %d share more than one face", c, cf_nbr[i]);
+            used[s] = 1;
+            cf_slot[i] = s;
+            // Divergence{Linear} weight of this side (Discretise_1_schemes.jl:64)
+            int f = cf_face[i], nb = cf_nbr[i];
+            V3 xf{d->face_centre[3 * f], d->face_centre[3 * f + 1], d->face_centre[3 * f + 2]};
+            V3 xC{d->cell_centre[3 * c], d->cell_centre[3 * c + 1], d->cell_centre[3 * c + 2]};
+            V3 xN{d->cell_centre[3 * nb], d->cell_centre[3 * nb + 1], d->cell_centre[3 * nb + 2]};
+            double w = vnorm({xf.x - xC.x, xf.y - xC.y, xf.z - xC.z}) / vnorm({xN.x - xC.x, xN.y - xC.y, xN.z - xC.z});
+            cf_w[i] = w;
+            cf_omw[i] = 1.0 - w;
+        }
+    // cell -> boundary faces in ascending face id
+    std::vector<int> cb_ptr(n + 1, 0), cb_face;
+    for (int f = 0; f < nbf; ++f) cb_ptr[own1[f] + 1]++;
+    for (int c = 0; c < n; ++c) cb_ptr[c + 1] += cb_ptr[c];
+    cb_face.resize(nbf);
+    {
+        std::vector<int> pos(cb_ptr.begin(), cb_ptr.end() - 1);
+        for (int f = 0; f < nbf; ++f) cb_face[pos[own1[f]]++] = f;
+    }
+    // face geometry
+    std::vector<double> fw(nf), farea(nf), fdelta(nf), Sfx(nf), Sfy(nf), Sfz(nf), nxv(nf), nyv(nf), nzv(nf), fcx(nf), fcy(nf), fcz(nf);
+    for (int f = 0; f < nf; ++f) {
+        fw[f] = d->face_weight[f];
+        farea[f] = d->face_area[f];
+        fdelta[f] = d->face_delta[f];
+        nxv[f] = d->face_normal[3 * f];
+        nyv[f] = d->face_normal[3 * f + 1];
+        nzv[f] = d->face_normal[3 * f + 2];
+        Sfx[f] = farea[f] * nxv[f];
+        Sfy[f] = farea[f] * nyv[f];
+        Sfz[f] = farea[f] * nzv[f];
+        fcx[f] = d->face_centre[3 * f];
+        fcy[f] = d->face_centre[3 * f + 1];
+        fcz[f] = d->face_centre[3 * f + 2];
+    }
+    std::vector<double> vol(n), ccx(n), ccy(n), ccz(n);
+    for (int c = 0; c < n; ++c) {
+        vol[c] = d->cell_volume[c];
+        ccx[c] = d->cell_centre[3 * c];
+        ccy[c] = d->cell_centre[3 * c + 1];
+        ccz[c] = d->cell_centre[3 * c + 2];
+    }
+    FVH_TRY(m->rowptr.upload(rowptr));
+    FVH_TRY(m->col.upload(col));
+    FVH_TRY(m->diag.upload(diag));
+    FVH_TRY(m->tslot.upload(tslot));
+    FVH_TRY(m->cf_ptr.upload(cf_ptr));
+    FVH_TRY(m->cf_face.upload(cf_face));
+    FVH_TRY(m->cf_nbr.upload(cf_nbr));
+    FVH_TRY(m->cf_slot.upload(cf_slot));
+    FVH_TRY(m->cf_sign.upload(cf_sign));
+    FVH_TRY(m->cf_w.upload(cf_w));
+    FVH_TRY(m->cf_omw.upload(cf_omw));
+    FVH_TRY(m->own1.upload(own1));
+    FVH_TRY(m->own2.upload(own2));
+    FVH_TRY(m->cb_ptr.upload(cb_ptr));
+    FVH_TRY(m->cb_face.upload(cb_face));
+    FVH_TRY(m->bf_face.upload(bf_face));
+    FVH_TRY(m->bf_cell.upload(bf_cell));
+    FVH_TRY(m->bf_patch.upload(bf_patch));
+    FVH_TRY(m->fw.upload(fw));
+    FVH_TRY(m->farea.upload(farea));
+    FVH_TRY(m->fdelta.upload(fdelta));
+    FVH_TRY(m->Sfx.upload(Sfx));
+    FVH_TRY(m->Sfy.upload(Sfy));
+    FVH_TRY(m->Sfz.upload(Sfz));
+    FVH_TRY(m->nx.upload(nxv));
+    FVH_TRY(m->ny.upload(nyv));
+    FVH_TRY(m->nz.upload(nzv));
+    FVH_TRY(m->vol.upload(vol));
+    FVH_TRY(m->ccx.upload(ccx));
+    FVH_TRY(m->ccy.upload(ccy));
+    FVH_TRY(m->ccz.upload(ccz));
+    FVH_TRY(m->fcx.upload(fcx));
+    FVH_TRY(m->fcy.upload(fcy));
+    FVH_TRY(m->fcz.upload(fcz));
+    m->h_rowptr = rowptr;
+    m->h_col = col;
+    m->h_diag = diag;
+    m->h_bf_face = bf_face;
+    m->h_bf_cell = bf_cell;
+    m->h_bf_patch = bf_patch;
+    m->h_bstart = bstart;
+    m->h_fdelta = fdelta;
+    m->h_vol = vol;
+    m->h_nx = nxv;
+    m->h_ny = nyv;
+    m->h_nz = nzv;
+    FVH_TRY(build_schedule(m.get()));
+    DevMesh &v = m->d;
+    v.n = n; v.nf = nf; v.nbf = nbf; v.nnz = nnz; v.ncf = ncf;
+    v.rowptr = m->rowptr.p; v.col = m->col.p; v.diag = m->diag.p; v.tslot = m->tslot.p;
+    v.cf_ptr = m->cf_ptr.p; v.cf_face = m->cf_face.p; v.cf_nbr = m->cf_nbr.p; v.cf_slot = m->cf_slot.p;
+    v.cf_sign = m->cf_sign.p; v.cf_w = m->cf_w.p; v.cf_omw = m->cf_omw.p;
+    v.own1 = m->own1.p; v.own2 = m->own2.p; v.fw = m->fw.p; v.farea = m->farea.p; v.fdelta = m->fdelta.p;
+    v.Sfx = m->Sfx.p; v.Sfy = m->Sfy.p; v.Sfz = m->Sfz.p; v.nx = m->nx.p; v.ny = m->ny.p; v.nz = m->nz.p;
+    v.vol = m->vol.p; v.ccx = m->ccx.p; v.ccy = m->ccy.p; v.ccz = m->ccz.p; v.fcx = m->fcx.p; v.fcy = m->fcy.p; v.fcz = m->fcz.p;
+    v.cb_ptr = m->cb_ptr.p; v.cb_face = m->cb_face.p;
+    v.bf_face = m->bf_face.p; v.bf_cell = m->bf_cell.p; v.bf_patch = m->bf_patch.p;
+    *out = m.release();
+    return FVH_OK;
+    FVH_GUARD_END
+}
+
+extern "C" void fvh_mesh_destroy(fvh_mesh *m) { delete m; }
+
+extern "C" int fvh_mesh_pattern(const fvh_mesh *m, int64_t *nnz, int32_t *rowptr, int32_t *colidx) {
+    if (!m) return set_error(FVH_ERR_ARG, "null mesh");
+    if (nnz) *nnz = m->nnz;
+    if (rowptr) std::copy(m->h_rowptr.begin(), m->h_rowptr.end(), rowptr);
+    if (colidx) std::copy(m->h_col.begin(), m->h_col.end(), colidx);
+    return FVH_OK;
+}
+
+extern "C" int fvh_mesh_sweep_info(const fvh_mesh *m, int64_t *nlevels, int64_t *band_nx) {
+    if (!m) return set_error(FVH_ERR_ARG, "null mesh");
+    if (nlevels) *nlevels = m->sched.nlev;
+    if (band_nx) *band_nx = m->sched.band_nx;
+    return FVH_OK;
+}

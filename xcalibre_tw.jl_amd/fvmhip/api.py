@@ -1,0 +1,351 @@
+"""Host-side mirror of the reference's user API, driving libfvmhip.
+
+Names, argument meanings and defaults follow XCALibre_TW.jl:
+  RANS{Laminar}/RANS{KOmega}(mesh, viscosity) ... src/RANSModels/RANSModels_0_types.jl:22,
+                                                  src/RANSModels/RANSModels_kOmega.jl:9
+  @assign! model U (Dirichlet(:inlet, v), ...) .. src/Discretise/Discretise_0_types.jl:80-169
+  set_schemes(time, divergence, laplacian, gradient)  Discretise_0_types.jl:171-181
+  set_solver(field; solver, preconditioner, convergence, relax, itmax=100,
+             atol=sqrt(eps), rtol=1e-3) ............. src/Solve/Solve_1_api.jl:5-26
+  set_runtime(iterations, write_interval, time_step) src/Solve/Solve_1_api.jl:28-30
+  Configuration(schemes, solvers, runtime) ......... src/Simulate/Simulate_0_types.jl:3-7
+  initialise!(field, value) ........................ src/Fields/Fields_0_types.jl:205-221
+  simple!(model, config; pref) -> (R_ux, R_uy, R_p)  src/Solvers/Solvers_1_SIMPLE.jl:3
+  piso!(model, config; pref) ....................... src/Solvers/Solvers_2_PISO.jl:3
+Julia's `!` and `:symbol` become plain names and strings.  Errors the
+reference throws as strings are raised as ValueError.
+"""
+import ctypes as C
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _lib as L
+
+# ---- schemes -------------------------------------------------------------
+Steady, Euler, Linear, Upwind, Orthogonal, Midpoint = L.STEADY, L.EULER, L.LINEAR, L.UPWIND, L.ORTHOGONAL, L.MIDPOINT
+
+
+def set_schemes(time=Steady, divergence=Linear, laplacian=Linear, gradient=Orthogonal):
+    return dict(time=time, divergence=divergence, laplacian=laplacian, gradient=gradient)
+
+
+# ---- solvers and preconditioners ----------------------------------------
+BicgstabSolver, CgSolver, GmresSolver = L.BICGSTAB, L.CG, L.GMRES
+
+
+class PreconditionerType:
+    kind = None
+
+
+class DILU(PreconditionerType):
+    kind = L.PRECOND_DILU
+
+
+class ILU0(PreconditionerType):
+    kind = L.PRECOND_ILU0
+
+
+class Jacobi(PreconditionerType):
+    kind = L.PRECOND_JACOBI
+
+
+class NormDiagonal(PreconditionerType):
+    kind = L.PRECOND_NORMDIAG
+
+
+class LDL(PreconditionerType):
+    """Exact sparse LDL' (LDLFactorizations.jl): a direct solve, not on the device path."""
+    kind = None
+
+
+def set_solver(field, *, solver, preconditioner, convergence, relax, itmax=100, atol=None, rtol=1e-3, precision=L.FP64):
+    if atol is None:
+        atol = math.sqrt(np.finfo(np.float64).eps)
+    if isinstance(preconditioner, type):
+        preconditioner = preconditioner()
+    if preconditioner.kind is None:
+        raise ValueError(f"{type(preconditioner).__name__} preconditioner is not supported by the device path")
+    return dict(solver=solver, preconditioner=preconditioner, convergence=float(convergence), relax=float(relax),
+                itmax=int(itmax), atol=float(atol), rtol=float(rtol), precision=int(precision))
+
+
+def set_runtime(*, iterations, write_interval, time_step):
+    return dict(iterations=int(iterations), dt=float(time_step), write_interval=int(write_interval))
+
+
+@dataclass
+class Configuration:
+    schemes: dict
+    solvers: dict
+    runtime: dict
+
+
+# ---- fields and boundary conditions --------------------------------------
+@dataclass
+class ConstantScalar:
+    values: float
+
+
+WALL_DEFAULTS = dict(kappa=0.41, beta1=0.075, cmu=0.09, B=5.2, E=9.8)  # Discretise_0_types.jl:61,69,77
+
+
+@dataclass
+class BoundaryCondition:
+    ID: str
+    value: object = 0.0
+    kind: int = field(default=L.BC_DIRICHLET, repr=False)
+
+
+def Dirichlet(ID, value):
+    if isinstance(value, (list, tuple, np.ndarray)) and len(value) != 3:
+        raise ValueError("Only vectors with three components can be used")
+    return BoundaryCondition(str(ID).lstrip(":"), value, L.BC_DIRICHLET)
+
+
+def Neumann(ID, value=0.0):
+    return BoundaryCondition(str(ID).lstrip(":"), value, L.BC_NEUMANN)
+
+
+def KWallFunction(ID, value=None):
+    return BoundaryCondition(str(ID).lstrip(":"), dict(value or WALL_DEFAULTS), L.BC_KWALL)
+
+
+def OmegaWallFunction(ID, value=None):
+    return BoundaryCondition(str(ID).lstrip(":"), dict(value or WALL_DEFAULTS), L.BC_OMEGAWALL)
+
+
+def NutWallFunction(ID, value=None):
+    return BoundaryCondition(str(ID).lstrip(":"), dict(value or WALL_DEFAULTS), L.BC_NUTWALL)
+
+
+class ScalarField:
+    def __init__(self, mesh):
+        self.mesh = mesh
+        self.values = np.zeros(mesh.ncells)
+        self.BCs = ()
+
+
+class VectorField:
+    def __init__(self, mesh):
+        self.mesh = mesh
+        self.x, self.y, self.z = ScalarField(mesh), ScalarField(mesh), ScalarField(mesh)
+        self.BCs = ()
+
+
+def initialise(f, value):
+    """initialise!(field, value): scalar or 3-vector."""
+    if isinstance(f, VectorField):
+        if len(value) != 3:
+            raise ValueError("Vectors should have 3 components")
+        f.x.values[:] = value[0]
+        f.y.values[:] = value[1]
+        f.z.values[:] = value[2]
+    else:
+        f.values[:] = value
+
+
+class Laminar:
+    pass
+
+
+class KOmega:
+    pass
+
+
+class RANS:
+    """RANS{Laminar}(mesh=..., viscosity=...) / RANS{KOmega}(...)."""
+
+    def __init__(self, model, *, mesh, viscosity):
+        self.model = model
+        self.mesh = mesh
+        self.U = VectorField(mesh)
+        self.p = ScalarField(mesh)
+        self.nu = viscosity
+        if model is KOmega:
+            self.turbulence = dict(k=ScalarField(mesh), omega=ScalarField(mesh), nut=ScalarField(mesh))
+        else:
+            self.turbulence = None
+
+
+def isturbulent(model):
+    return model.model is KOmega
+
+
+def assign(model, name, *bcs):
+    """@assign! model U (...) / @assign! model turbulence k (...)."""
+    if name in ("U", "p"):
+        f = getattr(model, name)
+    else:
+        f = model.turbulence[name]
+    for bc in bcs:
+        model.mesh.boundary_index(bc.ID)  # throws for unknown patches, as boundary_index does
+    f.BCs = tuple(bcs)
+    return model
+
+
+# ---- translation to the C case description --------------------------------
+def _bcs(mesh, bcs, vector):
+    arr = (L.BC * max(1, len(bcs)))()
+    for i, bc in enumerate(bcs):
+        arr[i].kind = bc.kind
+        arr[i].patch = mesh.boundary_index(bc.ID)
+        if isinstance(bc.value, dict):
+            for j, key in enumerate(("kappa", "beta1", "cmu", "B", "E")):
+                arr[i].wf[j] = float(bc.value[key])
+        elif isinstance(bc.value, (list, tuple, np.ndarray)):
+            for j in range(3):
+                arr[i].value[j] = float(bc.value[j])
+        else:
+            for j in range(3):
+                arr[i].value[j] = float(bc.value)
+    return arr
+
+
+def _schemes(s):
+    out = L.Schemes()
+    out.time, out.divergence, out.laplacian, out.gradient = s["time"], s["divergence"], s["laplacian"], s["gradient"]
+    return out
+
+
+def _solver(v):
+    out = L.SolverCfg()
+    out.solver = v["solver"]
+    out.precond = v["preconditioner"].kind
+    out.itmax = v["itmax"]
+    out.precision = v.get("precision", L.FP64)
+    out.convergence = v["convergence"]
+    out.relax = v["relax"]
+    out.atol = v["atol"]
+    out.rtol = v["rtol"]
+    return out
+
+
+class CaseDesc:
+    """fvh_case for (model, config; pref) plus the buffers it points into."""
+
+    def __init__(self, model, config, pref=None, algorithm=L.ALG_SIMPLE, conv_break=1e-7):
+        mesh = model.mesh
+        c = L.Case()
+        c.model = L.KOMEGA if isturbulent(model) else L.LAMINAR
+        c.algorithm = algorithm
+        c.iterations = config.runtime["iterations"]
+        c.has_pref = 0 if pref is None else 1
+        c.pref = 0.0 if pref is None else float(pref)
+        nu = model.nu.values if isinstance(model.nu, ConstantScalar) else float(model.nu)
+        c.nu = float(nu)
+        c.dt = config.runtime["dt"]
+        c.conv_break = conv_break
+        self._keep = []
+
+        def bcset(bcs):
+            arr = _bcs(mesh, bcs, True)
+            self._keep.append(arr)
+            s = L.BCSet()
+            s.nbc = len(bcs)
+            s.bcs = C.cast(arr, C.POINTER(L.BC))
+            return s
+
+        c.U = bcset(model.U.BCs)
+        c.p = bcset(model.p.BCs)
+        sc, sv = config.schemes, config.solvers
+        c.sU, c.sp = _schemes(sc["U"]), _schemes(sc["p"])
+        c.vU, c.vp = _solver(sv["U"]), _solver(sv["p"])
+        if c.model == L.KOMEGA:
+            t = model.turbulence
+            c.k, c.omega, c.nut = bcset(t["k"].BCs), bcset(t["omega"].BCs), bcset(t["nut"].BCs)
+            c.sk, c.somega = _schemes(sc["k"]), _schemes(sc["omega"])
+            c.vk, c.vomega = _solver(sv["k"]), _solver(sv["omega"])
+        else:
+            c.k = c.omega = c.nut = bcset(())
+        self.c = c
+
+
+def _fields(model):
+    n = model.mesh.ncells
+    U, p = model.U, model.p
+    out = dict(Ux=U.x.values, Uy=U.y.values, Uz=U.z.values, p=p.values)
+    if isturbulent(model):
+        t = model.turbulence
+        out.update(k=t["k"].values, omega=t["omega"].values, nut=t["nut"].values)
+    for k, v in out.items():
+        out[k] = np.ascontiguousarray(v, np.float64)
+        assert out[k].size == n
+    return out
+
+
+class Simulation:
+    """The loop set-up of simple!/piso! on the device, steppable.
+
+    `run(n)` executes up to n loop iterations (stopping at the reference's
+    convergence break); `get(name)` copies a state array to the host."""
+
+    def __init__(self, model, config, pref=None, algorithm=L.ALG_SIMPLE, conv_break=1e-7):
+        self.model, self.config = model, config
+        self.case = CaseDesc(model, config, pref, algorithm, conv_break)
+        self.dm = model.mesh.device()
+        f = _fields(model)
+        self._f = f
+        h = C.c_void_p()
+        g = lambda k: L.dptr(f[k]) if k in f else None  # noqa: E731
+        L.check(L.lib().fvh_sim_create(self.dm.h, C.byref(self.case.c), g("Ux"), g("Uy"), g("Uz"), g("p"), g("k"),
+                                       g("omega"), g("nut"), C.byref(h)), "fvh_sim_create")
+        self.h = h
+        self.iterations = self.case.c.iterations
+
+    def run(self, n):
+        done = C.c_int32()
+        L.check(L.lib().fvh_sim_iterate(self.h, int(n), C.byref(done)), "fvh_sim_iterate")
+        return done.value
+
+    def residuals(self):
+        R = [np.ones(self.iterations) for _ in range(3)]
+        L.check(L.lib().fvh_sim_residuals(self.h, *(L.dptr(r) for r in R)))
+        return tuple(R)
+
+    def get(self, name):
+        ln = C.c_int64()
+        L.check(L.lib().fvh_sim_get(self.h, name.encode(), None, C.byref(ln)), f"fvh_sim_get({name})")
+        out = np.zeros(ln.value)
+        L.check(L.lib().fvh_sim_get(self.h, name.encode(), L.dptr(out), C.byref(ln)))
+        return out
+
+    def stats(self, field_name):
+        s = L.Stats()
+        L.check(L.lib().fvh_sim_stats(self.h, field_name.encode(), C.byref(s)))
+        return dict(niter=s.niter, status=L.KRYLOV_STATUS.get(s.status, s.status), rnorm=s.rnorm, rnorm0=s.rnorm0,
+                    refinements=s.refinements)
+
+    def write_back(self):
+        m = self.model
+        m.U.x.values[:] = self.get("Ux")
+        m.U.y.values[:] = self.get("Uy")
+        m.p.values[:] = self.get("p")
+        if isturbulent(m):
+            for k in ("k", "omega", "nut"):
+                m.turbulence[k].values[:] = self.get(k)
+
+    def __del__(self):
+        try:
+            if self.h:
+                L.lib().fvh_sim_destroy(self.h)
+        except Exception:
+            pass
+
+
+def simple(model, config, pref=None, conv_break=1e-7):
+    """simple!(model, config; pref): returns (R_ux, R_uy, R_p); fields are
+    updated in place on the model, as in the reference."""
+    sim = Simulation(model, config, pref, L.ALG_SIMPLE, conv_break)
+    sim.run(sim.iterations)
+    sim.write_back()
+    return sim.residuals()
+
+
+def piso(model, config, pref=None):
+    """piso!(model, config; pref)."""
+    sim = Simulation(model, config, pref, L.ALG_PISO)
+    sim.run(sim.iterations)
+    sim.write_back()
+    return sim.residuals()
