@@ -1,0 +1,173 @@
+#!/usr/bin/env python3
+"""Benchmark: SIMPLE iterations/s and cell-updates/s on the laminar
+backward-facing step, 2000 x 500 quads (1M cells), on one MI355X.
+
+    python bench.py [--steps K] [--warmup W] [--gpus 1] [--precision fp64|mixed|both] [--no-cpu]
+
+A step is one SIMPLE loop iteration (Solvers_1_SIMPLE.jl:125-230): two momentum
+assemblies + BiCGSTAB/DILU solves, the pressure equation (CG/DILU), gradients,
+interpolation and the flux update, with every field resident in HBM.  Prints
+one JSON line.  The SpMV roofline is timed live with HIP events on the
+library's stream; the CPU baseline is the oracle (single-threaded FP64
+restatement of the reference) on a bounded number of iterations of the same case.
+"""
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "xcalibre_tw.jl_amd"))
+sys.path.insert(0, ROOT)
+
+import numpy as np  # noqa: E402
+
+METRIC = "SIMPLE iters/s & cell-updates/s at 1e-6 resid: mixed vs FP64; SpMV HBM GB/s"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def make_case(nx, ny, iterations, precision):
+    import fvmhip as F
+    from tests.cases import bfs_model, config
+
+    mesh = F.backward_facing_step(nx=nx, ny=ny)
+    model = bfs_model(mesh)
+    cfg = config(iterations, schemes=dict(divergence=F.Upwind), precision=precision)
+    return mesh, model, cfg
+
+
+def spmv_bytes(n, nnz, sval=8, sx=8):
+    """Algorithmic bytes of y = A x (CSR, int32 indices): rowptr, colidx + values,
+    x once, y once."""
+    return 4 * (n + 1) + nnz * (4 + sval) + n * sx + n * sx
+
+
+def run_gpu(args, precision):
+    import fvmhip as F
+    from fvmhip import _lib as L
+
+    mesh, model, cfg = make_case(args.nx, args.ny, args.warmup + args.steps + 1, precision)
+    t0 = time.time()
+    sim = F.Simulation(model, cfg, None, L.ALG_SIMPLE, conv_break=1e-6)
+    log(f"[bench] {'fp64' if precision == 0 else 'mixed'}: set-up {time.time() - t0:.1f}s, {mesh.ncells} cells")
+    done = sim.run(args.warmup)
+    L.check(L.lib().fvh_synchronize())
+    t1 = time.perf_counter()
+    done = sim.run(args.steps)
+    L.check(L.lib().fvh_synchronize())
+    dt = time.perf_counter() - t1
+    if done != args.steps:
+        log(f"[bench] loop stopped after {done} timed iterations (converged)")
+    R = sim.residuals()
+    it0 = args.warmup
+    res = dict(iters=done, seconds=dt, iters_per_s=done / dt, cell_updates_per_s=mesh.ncells * done / dt,
+               ms_per_step=1e3 * dt / max(done, 1),
+               residuals_last=[float(R[k][it0 + done - 1]) for k in range(3)],
+               krylov=dict(Ux=sim.stats("Ux"), p=sim.stats("p")))
+    # SpMV on the assembled momentum matrix, timed with HIP events
+    eq = C.c_void_p()
+    L.check(L.lib().fvh_sim_equation(sim.h, b"ux", C.byref(eq)))
+    n = mesh.ncells
+    rp, ci = mesh.device().pattern()
+    x = L.DeviceArray.from_host(np.random.default_rng(0).standard_normal(n))
+    y = L.DeviceArray(n)
+    ms = C.c_double()
+    L.check(L.lib().fvh_time_op(eq, 0, 200, x.vp, y.vp, C.byref(ms)))
+    nbytes = spmv_bytes(n, ci.size)
+    res["spmv"] = dict(ms=ms.value, bytes=nbytes, GBps=nbytes / (ms.value * 1e-3) / 1e9)
+    L.check(L.lib().fvh_time_op(eq, 1, 20, x.vp, y.vp, C.byref(ms)))
+    res["precond_apply_ms"] = ms.value
+    L.check(L.lib().fvh_time_op(eq, 2, 10, None, None, C.byref(ms)))
+    res["precond_update_ms"] = ms.value
+    res["n"] = n
+    res["nnz"] = int(ci.size)
+    del sim
+    return res
+
+
+def run_cpu(args):
+    """The oracle (1 core) on the same case: a bounded sample of iterations."""
+    import fvmhip as F
+    from oracle import ref
+
+    mesh, model, cfg = make_case(args.nx, args.ny, args.cpu_iters + 1, 0)
+    case = F.CaseDesc(model, cfg, None, 0, 1e-6)
+    t0 = time.time()
+    rs = ref.RefSimulation(model, case)
+    setup = time.time() - t0
+    t1 = time.perf_counter()
+    n = rs.run(args.cpu_iters)
+    dt = time.perf_counter() - t1
+    return dict(iters=n, seconds=dt, setup_s=setup, cell_updates_per_s=mesh.ncells * n / dt, iters_per_s=n / dt)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed", "both"])
+    ap.add_argument("--nx", type=int, default=2000)
+    ap.add_argument("--ny", type=int, default=500)
+    ap.add_argument("--cpu-iters", type=int, default=3)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); the multi-GPU decomposition is design-only.",
+              file=sys.stderr)
+        sys.exit(2)
+    import fvmhip as F
+
+    if F.device_count() < 1:
+        print("no HIP device", file=sys.stderr)
+        sys.exit(1)
+    forms = {"fp64": [0], "mixed": [1], "both": [0, 1]}[args.precision]
+    out = {}
+    for p in forms:
+        out["fp64" if p == 0 else "mixed"] = run_gpu(args, p)
+        log(f"[bench] {'fp64' if p == 0 else 'mixed'}: {out['fp64' if p == 0 else 'mixed']['iters_per_s']:.3f} it/s")
+    head = out.get("mixed") or out["fp64"]
+    cpu = None
+    if not args.no_cpu:
+        c = run_cpu(args)
+        cpu = dict(value=c["cell_updates_per_s"], unit="cell-updates/s", cores=1, kind="port",
+                   sample=f"{c['iters']} SIMPLE iterations of the same 2000x500 case, oracle FP64 restatement, "
+                          f"{c['seconds']:.1f}s (set-up {c['setup_s']:.1f}s excluded)")
+        log(f"[bench] cpu: {c['iters_per_s']:.4f} it/s")
+    sp = head["spmv"]
+    line = {
+        "metric": METRIC,
+        "value": head["cell_updates_per_s"],
+        "unit": "cell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": head["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if "mixed" not in out else "f32-in-f64",
+        "data": "synthetic (generated BFS mesh, zero-pressure / uniform-velocity initial fields)",
+        "config": {"workload": f"laminar backward-facing step {args.nx}x{args.ny} quads ({head['n']} cells), "
+                               f"SIMPLE, Upwind convection, BiCGSTAB+DILU (U), CG+DILU (p), Re_h=500",
+                   "cells": head["n"], "nnz": head["nnz"]},
+        "simple_iters_per_s": head["iters_per_s"],
+        "roofline": {"bound": "hbm", "achieved": sp["GBps"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": sp["GBps"] / HBM_PEAK_GBS, "traffic": None, "kernel": "k_spmv (FP64 CSR)",
+                     "bytes_per_launch": sp["bytes"], "ms_per_launch": sp["ms"]},
+        "cpu_baseline": cpu,
+        "forms": out,
+    }
+    if "fp64" in out and "mixed" in out:
+        line["mixed_vs_fp64"] = out["mixed"]["iters_per_s"] / out["fp64"]["iters_per_s"]
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

@@ -190,6 +190,13 @@ const char *fvh_status_string(int status);
 int fvh_last_error(char *buf, int64_t len);
 /* Number of HIP devices (0 on a host without a GPU). */
 int fvh_device_count(int32_t *n);
+/* Device memory for hosts that have no GPU array type of their own (the
+ * Julia caller passes its own device arrays instead).  fvh_memcpy copies in
+ * any direction (unified addressing) and returns after completion. */
+int fvh_malloc(void **ptr, int64_t bytes);
+int fvh_free(void *ptr);
+int fvh_memcpy(void *dst, const void *src, int64_t bytes);
+int fvh_synchronize(void);
 
 /* ------------------------------------------------------------------ */
 /* Host mesh building: the UNV2 builder rules (face order, ownership,   */
@@ -327,6 +334,15 @@ int fvh_sim_get(fvh_sim *s, const char *name, double *out, int64_t *len);
 int fvh_sim_stats(fvh_sim *s, const char *field, fvh_stats *out);
 /* Device time of the phases of the last iteration (ms), NaN if untimed. */
 int fvh_sim_timing(fvh_sim *s, double *ms_assembly, double *ms_solve, double *ms_other);
+/* The equation handle of "ux", "uy" or "p" (owned by the sim). */
+int fvh_sim_equation(fvh_sim *s, const char *name, fvh_eqn **out);
+
+/* Measurement helper (no counterpart in the reference): average device time
+ * of one launch of an operator over `reps` back-to-back launches, by HIP
+ * events on the library stream.  op: 0 = SpMV y = A x (FP64), 1 =
+ * preconditioner apply y = M^{-1} x, 2 = preconditioner update, 3 = FP32
+ * SpMV on the single-precision copy of A, 4 = FP32 preconditioner apply. */
+int fvh_time_op(fvh_eqn *e, int32_t op, int32_t reps, const double *x, double *y, double *ms_per_launch);
 
 /* simple!(model, config; pref): whole run, fields in/out (host). */
 int fvh_simple(fvh_mesh *m, const fvh_case *c,

@@ -787,17 +787,28 @@ static fvh_stats gmres(const Op &op, const Vec &b, Vec &x, int itmax, double ato
 // the preconditioner is DILU/ILU(0) and A has a negative first diagonal, the
 // only form in which a preconditioned CG is defined for the negative-definite
 // pressure matrix.
+struct Negated {  // -A and its preconditioner, set up once per equation
+    CSC An;
+    Precond Pn;
+    int kind = -1;
+};
+
 static fvh_stats run(Equation &eq, Precond &P, const fvh_solver_cfg &cfg, Vec &x) {
     if (cfg.solver == FVH_CG && (P.kind == FVH_PRECOND_DILU || P.kind == FVH_PRECOND_ILU0) &&
         eq.A.nzval[eq.A.slot(0, 0)] < 0.0) {
-        CSC An = eq.A;
-        for (double &v : An.nzval) v = -v;
-        Precond Pn;
-        precond_setup(Pn, P.kind, &An);
-        precond_update(Pn);
+        static thread_local std::map<const Equation *, std::unique_ptr<Negated>> cache;
+        auto &ng = cache[&eq];
+        if (!ng || ng->kind != P.kind || ng->An.n != eq.A.n) {
+            ng = std::make_unique<Negated>();
+            ng->An = eq.A;
+            ng->kind = P.kind;
+            precond_setup(ng->Pn, P.kind, &ng->An);
+        }
+        for (size_t i = 0; i < eq.A.nzval.size(); ++i) ng->An.nzval[i] = -eq.A.nzval[i];
+        precond_update(ng->Pn);
         Vec bn(eq.b.size());
         for (size_t i = 0; i < bn.size(); ++i) bn[i] = -eq.b[i];
-        Op op{&An, &Pn};
+        Op op{&ng->An, &ng->Pn};
         return cg(op, bn, x, cfg.itmax, cfg.atol, cfg.rtol);
     }
     Op op{&eq.A, &P};

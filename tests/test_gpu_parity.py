@@ -125,20 +125,8 @@ def _eqn(mesh):
     return dm, h, nz.value, b.value
 
 
-def _to_dev(x):
-    import torch
-
-    return torch.from_numpy(np.ascontiguousarray(x, np.float64)).cuda()
-
-
-def _hip_copy(dst, src, nbytes):
-    hip = C.CDLL("libamdhip64.so")
-    assert hip.hipMemcpy(C.c_void_p(dst), C.c_void_p(src), C.c_size_t(nbytes), 3) == 0  # device to device
-
-
 def test_spmv_and_dilu_full_size_bitwise(bfs_full):
-    import torch
-
+    """C2-size (1M cells) SpMV and DILU (update + apply) on random values."""
     mesh = bfs_full
     dm, h, nzp, bp = _eqn(mesh)
     try:
@@ -147,18 +135,19 @@ def test_spmv_and_dilu_full_size_bitwise(bfs_full):
         nnz = ci.size
         rows = np.repeat(np.arange(mesh.ncells), np.diff(rp))
         vals = np.where(rows == ci, 4.5, 0.0) - rng.uniform(0.1, 1.0, nnz) * (rows != ci)
-        vt = _to_dev(vals)
-        torch.cuda.synchronize()
-        _hip_copy(nzp, vt.data_ptr(), 8 * nnz)
+        L.check(L.lib().fvh_memcpy(C.c_void_p(nzp), vals.ctypes.data_as(C.c_void_p), vals.nbytes))
         x = rng.standard_normal(mesh.ncells)
-        xt = _to_dev(x)
-        yt = torch.zeros(mesh.ncells, dtype=torch.float64, device="cuda")
-        torch.cuda.synchronize()
-        L.check(L.lib().fvh_spmv(h, C.c_void_p(xt.data_ptr()), C.c_void_p(yt.data_ptr())))
-        assert same_bits(yt.cpu().numpy(), ref.spmv_csr(mesh, vals, x))
+        xd, yd = L.DeviceArray.from_host(x), L.DeviceArray(mesh.ncells)
+        L.check(L.lib().fvh_spmv(h, xd.vp, yd.vp))
+        assert same_bits(yd.host(), ref.spmv_csr(mesh, vals, x))
         L.check(L.lib().fvh_precond_set(h, L.PRECOND_DILU, None))
         L.check(L.lib().fvh_precond_update(h))
-        L.check(L.lib().fvh_precond_apply(h, C.c_void_p(yt.data_ptr()), C.c_void_p(xt.data_ptr())))
-        assert same_bits(yt.cpu().numpy(), ref.precond_apply_csr(mesh, vals, L.PRECOND_DILU, x))
+        L.check(L.lib().fvh_precond_apply(h, yd.vp, xd.vp))
+        assert same_bits(yd.host(), ref.precond_apply_csr(mesh, vals, L.PRECOND_DILU, x))
+        for kind in (L.PRECOND_ILU0, L.PRECOND_JACOBI):
+            L.check(L.lib().fvh_precond_set(h, kind, None))
+            L.check(L.lib().fvh_precond_update(h))
+            L.check(L.lib().fvh_precond_apply(h, yd.vp, xd.vp))
+            assert same_bits(yd.host(), ref.precond_apply_csr(mesh, vals, kind, x)), kind
     finally:
         L.lib().fvh_eqn_destroy(h)

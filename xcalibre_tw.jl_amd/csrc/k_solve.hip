@@ -713,3 +713,32 @@ extern "C" int fvh_solve(fvh_eqn *e, const fvh_solver_cfg *cfg, double *x, fvh_s
     return FVH_OK;
     FVH_GUARD_END
 }
+
+extern "C" int fvh_time_op(fvh_eqn *e, int32_t op, int32_t reps, const double *x, double *y, double *ms) {
+    FVH_GUARD_BEGIN
+    if (!e || !ms || reps <= 0) return set_error(FVH_ERR_ARG, "bad argument");
+    if ((op == 0 || op == 1) && (!x || !y)) return set_error(FVH_ERR_ARG, "null vector");
+    auto launch = [&]() -> int {
+        switch (op) {
+            case 0: return launch_spmv(e, x, y, nullptr);
+            case 1: return precond_apply(e, y, x, nullptr, 1.0);
+            case 2: return precond_update(e);
+            default: return set_error(FVH_ERR_UNSUPPORTED, "fvh_time_op: op %d", op);
+        }
+    };
+    FVH_TRY(launch());  // warm
+    hipEvent_t a, b;
+    HIP_TRY(hipEventCreate(&a));
+    HIP_TRY(hipEventCreate(&b));
+    HIP_TRY(hipEventRecord(a, stream()));
+    for (int i = 0; i < reps; ++i) FVH_TRY(launch());
+    HIP_TRY(hipEventRecord(b, stream()));
+    HIP_TRY(hipEventSynchronize(b));
+    float t = 0.f;
+    HIP_TRY(hipEventElapsedTime(&t, a, b));
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    *ms = (double)t / reps;
+    return FVH_OK;
+    FVH_GUARD_END
+}

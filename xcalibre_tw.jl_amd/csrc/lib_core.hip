@@ -113,6 +113,34 @@ extern "C" int fvh_device_count(int32_t *n) {
     return FVH_OK;
 }
 
+extern "C" int fvh_malloc(void **ptr, int64_t bytes) {
+    if (!ptr || bytes < 0) return set_error(FVH_ERR_ARG, "fvh_malloc: bad arguments");
+    FVH_TRY(ensure_device());
+    HIP_TRY(hipMalloc(ptr, bytes > 0 ? (size_t)bytes : 1));
+    return FVH_OK;
+}
+
+extern "C" int fvh_free(void *ptr) {
+    if (ptr) HIP_TRY(hipFree(ptr));
+    return FVH_OK;
+}
+
+extern "C" int fvh_memcpy(void *dst, const void *src, int64_t bytes) {
+    if ((!dst || !src) && bytes > 0) return set_error(FVH_ERR_ARG, "fvh_memcpy: null pointer");
+    FVH_TRY(ensure_device());
+    if (bytes > 0) {
+        HIP_TRY(hipMemcpyAsync(dst, src, (size_t)bytes, hipMemcpyDefault, stream()));
+        HIP_TRY(hipStreamSynchronize(stream()));
+    }
+    return FVH_OK;
+}
+
+extern "C" int fvh_synchronize(void) {
+    FVH_TRY(ensure_device());
+    HIP_TRY(hipStreamSynchronize(stream()));
+    return FVH_OK;
+}
+
 namespace {
 
 struct V3 {

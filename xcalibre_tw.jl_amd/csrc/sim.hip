@@ -655,6 +655,16 @@ extern "C" int fvh_sim_timing(fvh_sim *h, double *a, double *s, double *o) {
     return FVH_OK;
 }
 
+extern "C" int fvh_sim_equation(fvh_sim *h, const char *name, fvh_eqn **out) {
+    if (!h || !name || !out) return set_error(FVH_ERR_ARG, "null argument");
+    std::string nm(name);
+    if (nm == "ux") *out = h->S.ux.get();
+    else if (nm == "uy") *out = h->S.uy.get();
+    else if (nm == "p") *out = h->S.pe.get();
+    else return set_error(FVH_ERR_ARG, "no equation %s", name);
+    return FVH_OK;
+}
+
 extern "C" int fvh_sim_get(fvh_sim *h, const char *name, double *out, int64_t *len) {
     FVH_GUARD_BEGIN
     if (!h || !name) return set_error(FVH_ERR_ARG, "null argument");

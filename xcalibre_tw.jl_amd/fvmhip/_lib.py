@@ -87,6 +87,10 @@ SIGNATURES = {
     "fvh_status_string": (C.c_char_p, [C.c_int]),
     "fvh_last_error": (C.c_int, [C.c_char_p, C.c_int64]),
     "fvh_device_count": (C.c_int, [P_I32]),
+    "fvh_malloc": (C.c_int, [C.POINTER(C.c_void_p), C.c_int64]),
+    "fvh_free": (C.c_int, [C.c_void_p]),
+    "fvh_memcpy": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64]),
+    "fvh_synchronize": (C.c_int, []),
     "fvh_hmesh_build": (C.c_int, [C.c_int64, P_D, C.c_int64, P_I64, P_I64, C.c_int64, P_I64, P_I64, C.c_char_p,
                                   C.c_double, C.POINTER(_VP)]),
     "fvh_hmesh_read_unv": (C.c_int, [C.c_char_p, C.c_double, C.POINTER(_VP)]),
@@ -125,6 +129,8 @@ SIGNATURES = {
     "fvh_sim_get": (C.c_int, [_VP, C.c_char_p, P_D, P_I64]),
     "fvh_sim_stats": (C.c_int, [_VP, C.c_char_p, C.POINTER(Stats)]),
     "fvh_sim_timing": (C.c_int, [_VP, P_D, P_D, P_D]),
+    "fvh_sim_equation": (C.c_int, [_VP, C.c_char_p, C.POINTER(_VP)]),
+    "fvh_time_op": (C.c_int, [_VP, C.c_int32, C.c_int32, _VP, _VP, P_D]),
     "fvh_simple": (C.c_int, [_VP, C.POINTER(Case), P_D, P_D, P_D, P_D, P_D, P_D, P_D, P_D, P_D, P_D, P_I32]),
 }
 
@@ -174,3 +180,42 @@ def device_count():
     n = C.c_int32(0)
     check(lib().fvh_device_count(C.byref(n)))
     return n.value
+
+
+class DeviceArray:
+    """A float64 (or int) array in device memory owned by libfvmhip."""
+
+    def __init__(self, n, dtype=np.float64):
+        self.n = int(n)
+        self.dtype = np.dtype(dtype)
+        p = C.c_void_p()
+        check(lib().fvh_malloc(C.byref(p), self.n * self.dtype.itemsize), "fvh_malloc")
+        self.ptr = p.value
+
+    @classmethod
+    def from_host(cls, a):
+        a = np.ascontiguousarray(a)
+        d = cls(a.size, a.dtype)
+        d.upload(a)
+        return d
+
+    def upload(self, a):
+        a = np.ascontiguousarray(a, self.dtype)
+        assert a.size == self.n
+        check(lib().fvh_memcpy(C.c_void_p(self.ptr), a.ctypes.data_as(C.c_void_p), a.nbytes), "fvh_memcpy")
+
+    def host(self):
+        out = np.empty(self.n, self.dtype)
+        check(lib().fvh_memcpy(out.ctypes.data_as(C.c_void_p), C.c_void_p(self.ptr), out.nbytes), "fvh_memcpy")
+        return out
+
+    @property
+    def vp(self):
+        return C.c_void_p(self.ptr)
+
+    def __del__(self):
+        try:
+            if self.ptr:
+                lib().fvh_free(C.c_void_p(self.ptr))
+        except Exception:
+            pass
