@@ -151,3 +151,28 @@ def test_spmv_and_dilu_full_size_bitwise(bfs_full):
             assert same_bits(yd.host(), ref.precond_apply_csr(mesh, vals, kind, x)), kind
     finally:
         L.lib().fvh_eqn_destroy(h)
+
+
+@pytest.mark.parametrize("sweep", ["levels", "band"])
+def test_sweep_paths_bitwise(sweep, monkeypatch):
+    """Both DILU schedules (level sets / banded anti-diagonals) against the oracle."""
+    if sweep == "levels":
+        monkeypatch.setenv("FVH_SWEEP", "levels")
+    else:
+        monkeypatch.delenv("FVH_SWEEP", raising=False)
+    mesh = F.backward_facing_step(nx=150, ny=70)  # 2 waves of grid rows, uneven
+    nlev, band = mesh.device().sweep_info()
+    assert nlev == 150 + 70 - 1
+    assert (band == 150) == (sweep == "band")
+    model = bfs_model(mesh, nu=1e-3)
+    g, r = _sims(model, config(6, schemes=dict(divergence=F.Upwind)), conv_break=1e-6)
+    assert g.run(6) == r.run(6)
+    for name in ("ux_dinv", "p_dinv", "Ux", "Uy", "p"):
+        assert same_bits(g.get(name), r.get(name)), name
+    for a, b in zip(g.residuals(), r.residuals()):
+        assert same_bits(a, b)
+
+
+def test_quad40_is_banded(quad40):
+    nlev, band = quad40.device().sweep_info()
+    assert band == 40 and nlev == 79

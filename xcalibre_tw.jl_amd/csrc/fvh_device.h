@@ -104,12 +104,27 @@ struct BcPlan {
     DBuf<int> ptr, face, bc;
 };
 
+// Anti-diagonal geometry of a banded grid, by value in kernel arguments.
+struct BandGeom {
+    int nx, ny, nT;
+    const int *offj;              // packed position of (T - t, t) is offj[T] + t
+    const unsigned char *flags;   // packed: which of S, W, E, N neighbours exist
+};
+
+// Band coefficients of one matrix in packed (anti-diagonal) order.
+struct BandVals {
+    double *ad, *ls, *lw, *ue, *un, *ts, *tw;
+};
+
 // Sweep schedule of the lower (forward) and upper (backward) triangles.
 struct Schedule {
     int nlev = 0;
     DBuf<int> fptr, frows, bptr, brows;
     std::vector<int> h_fptr, h_bptr;
     int band_nx = 0;  // >0: A is a banded grid (deps only r-1, r-nx)
+    BandGeom g{};
+    DBuf<int> b_offj, b_iperm, b_perm;
+    DBuf<unsigned char> b_flags;
 };
 
 }  // namespace fvh

@@ -58,6 +58,13 @@ struct KrylovWork {
 
 }  // namespace fvh
 
+namespace fvh {
+struct BandStore {
+    DBuf<double> ad, ls, lw, ue, un, ts, tw, dinv, xp;
+    BandVals v{};
+};
+}  // namespace fvh
+
 struct fvh_eqn {
     fvh_mesh *m = nullptr;
     fvh::DBuf<double> nz, b, R, Fx;
@@ -66,6 +73,7 @@ struct fvh_eqn {
     fvh_eqn *pmat = nullptr;    // equation whose A the preconditioner reads
     fvh_eqn *pshare = nullptr;  // use that equation's preconditioner object instead
     fvh::DBuf<double> D, Dinv, lu, pstore;
+    fvh::BandStore bd;  // DILU on banded grids (packed order)
     fvh::KrylovWork kw;
     fvh::DBuf<double> red;  // residual reductions
 };
@@ -83,5 +91,6 @@ int launch_spmv(fvh_eqn *e, const double *x, double *y, const KScal *ks);
 int solve(fvh_eqn *e, const fvh_solver_cfg &cfg, double *x, fvh_stats *st);
 int residual(fvh_eqn *e, const double *phi, double *res_host);
 int residual_async(fvh_eqn *e, const double *phi, double *res_dev);
+int precond_dinv_natural(fvh_eqn *e, const double **out);
 
 }  // namespace fvh

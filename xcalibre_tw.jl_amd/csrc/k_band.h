@@ -1,0 +1,16 @@
+// Banded-grid DILU recurrences (anti-diagonal schedule, one thread per grid row).
+#pragma once
+
+#include "fvh_device.h"
+
+struct fvh_eqn;
+
+namespace fvh {
+
+int band_setup(fvh_mesh *m);
+int band_alloc(fvh_eqn *e);
+int band_dilu_update(fvh_eqn *e, const double *val);
+int band_dilu_apply(fvh_eqn *e, const double *x, double *y, const KScal *ks);
+int band_dinv_natural(fvh_eqn *e, double *out);
+
+}  // namespace fvh

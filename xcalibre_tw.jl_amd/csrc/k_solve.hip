@@ -16,6 +16,7 @@
 #include <limits>
 
 #include "fvh_eqn.h"
+#include "k_band.h"
 #include "k_sweep.h"
 #include "k_util.h"
 
@@ -188,7 +189,9 @@ int precond_update(fvh_eqn *eq) {
                                e->pstore.p);
             HIP_TRY(hipGetLastError());
             return FVH_OK;
-        case FVH_PRECOND_DILU: return dilu_update(m, src->nz.p, e->D.p, e->Dinv.p);
+        case FVH_PRECOND_DILU:
+            if (m->sched.band_nx > 0) return band_dilu_update(e, src->nz.p);
+            return dilu_update(m, src->nz.p, e->D.p, e->Dinv.p);
         case FVH_PRECOND_ILU0: return ilu0_factor(m, src->nz.p, e->lu.p);
         default: return set_error(FVH_ERR_UNSUPPORTED, "preconditioner %d", e->pkind);
     }
@@ -209,7 +212,9 @@ int precond_apply(fvh_eqn *eq, double *y, const double *x, const KScal *ks, doub
             const double *s = e->pstore.p;
             return map(n, ks, [=] __device__(int i) { y[i] = s[i] * x[i]; });
         }
-        case FVH_PRECOND_DILU: return dilu_apply(m, src->nz.p, e->Dinv.p, x, y, ks);
+        case FVH_PRECOND_DILU:
+            if (m->sched.band_nx > 0) return band_dilu_apply(e, x, y, ks);
+            return dilu_apply(m, src->nz.p, e->Dinv.p, x, y, ks);
         case FVH_PRECOND_ILU0: return ilu0_apply(m, e->lu.p, x, y, ks);
         default: return set_error(FVH_ERR_UNSUPPORTED, "preconditioner %d", e->pkind);
     }
@@ -698,9 +703,19 @@ extern "C" int fvh_precond_apply(fvh_eqn *e, double *y, const double *x) {
     return FVH_OK;
 }
 
+int fvh::precond_dinv_natural(fvh_eqn *e, const double **out) {
+    if (e->pkind != FVH_PRECOND_DILU) return set_error(FVH_ERR_ARG, "not a DILU preconditioner");
+    if (e->m->sched.band_nx > 0) FVH_TRY(band_dinv_natural(e, e->Dinv.p));
+    *out = e->Dinv.p;
+    return FVH_OK;
+}
+
 extern "C" int fvh_precond_dinv(fvh_eqn *e, double *dinv) {
-    if (!e || !dinv || e->pkind != FVH_PRECOND_DILU) return set_error(FVH_ERR_ARG, "not a DILU preconditioner");
-    HIP_TRY(hipMemcpy(dinv, e->Dinv.p, sizeof(double) * e->m->n, hipMemcpyDeviceToHost));
+    if (!e || !dinv) return set_error(FVH_ERR_ARG, "null argument");
+    const double *src = nullptr;
+    FVH_TRY(precond_dinv_natural(e, &src));
+    HIP_TRY(hipStreamSynchronize(stream()));
+    HIP_TRY(hipMemcpy(dinv, src, sizeof(double) * e->m->n, hipMemcpyDeviceToHost));
     return FVH_OK;
 }
 

@@ -4,8 +4,10 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 
 #include "fvh_device.h"
+#include "k_band.h"
 
 namespace fvh {
 
@@ -195,6 +197,8 @@ int build_schedule(fvh_mesh *m) {
             if (nx == 0) nx = d;
             if (d != nx) { band = false; break; }
         }
+    const char *force = std::getenv("FVH_SWEEP");
+    if (force && std::strcmp(force, "levels") == 0) band = false;
     if (band && nx > 1 && n % nx == 0) {
         for (int i = 0; i < n && band; ++i)
             for (int p = rp[i]; p < dg[i]; ++p)
@@ -388,6 +392,7 @@ extern "C" int fvh_mesh_create(const fvh_mesh_desc *d, fvh_mesh **out) {
     m->h_ny = nyv;
     m->h_nz = nzv;
     FVH_TRY(build_schedule(m.get()));
+    FVH_TRY(band_setup(m.get()));
     DevMesh &v = m->d;
     v.n = n; v.nf = nf; v.nbf = nbf; v.nnz = nnz; v.ncf = ncf;
     v.rowptr = m->rowptr.p; v.col = m->col.p; v.diag = m->diag.p; v.tslot = m->tslot.p;

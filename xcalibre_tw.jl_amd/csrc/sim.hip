@@ -688,10 +688,10 @@ extern "C" int fvh_sim_get(fvh_sim *h, const char *name, double *out, int64_t *l
         src = it->second.first;
         cnt = it->second.second;
     } else if (nm == "ux_dinv" && S.ux->pkind == FVH_PRECOND_DILU) {
-        src = S.ux->Dinv.p;
+        FVH_TRY(precond_dinv_natural(S.ux.get(), &src));
         cnt = n;
     } else if (nm == "p_dinv" && S.pe->pkind == FVH_PRECOND_DILU) {
-        src = S.pe->Dinv.p;
+        FVH_TRY(precond_dinv_natural(S.pe.get(), &src));
         cnt = n;
     } else {
         return set_error(FVH_ERR_ARG, "unknown array %s", name);
