@@ -29,6 +29,14 @@ constexpr int kPD = 8;  // prefetch distance (steps)
 
 enum : unsigned char { F_S = 1, F_W = 2, F_E = 4, F_N = 8 };
 
+// Work-group barrier that orders LDS only: global loads issued for later
+// steps stay in flight across it (a full __syncthreads waits for them).
+__device__ inline void lds_barrier() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 __device__ inline double dpp_shr1(double v) {  // lane l <- lane l-1 (lane 0 <- 0)
     int lo = __double2loint(v), hi = __double2hiint(v);
     lo = __builtin_amdgcn_update_dpp(0, lo, 0x138, 0xF, 0xF, false);
@@ -76,29 +84,89 @@ __global__ void k_band_scatter(int n, const int *__restrict__ iperm, const doubl
     if (p < n) x[iperm[p]] = xp[p];
 }
 
+// Anti-diagonal walk: offj(T) = (first packed position of diagonal T) - jmin(T),
+// advanced in scalar registers (no per-step loads of the offset table).
+struct Walk {
+    int T, offj, jmin, jmax;
+    __device__ void init(const BandGeom &g, int T0) {
+        T = T0;
+        offj = (T0 >= 0 && T0 < g.nT) ? g.offj[T0] : 0;
+        jmin = max(0, T0 - g.nx + 1);
+        jmax = min(g.ny - 1, T0);
+    }
+    __device__ void next(const BandGeom &g) {  // T -> T+1
+        int cnt = jmax - jmin + 1;
+        offj += cnt;
+        T += 1;
+        int jmin1 = max(0, T - g.nx + 1);
+        offj -= jmin1 - jmin;
+        jmin = jmin1;
+        jmax = min(g.ny - 1, T);
+    }
+    __device__ void prev(const BandGeom &g) {  // T -> T-1
+        T -= 1;
+        int jmin1 = max(0, T - g.nx + 1), jmax1 = min(g.ny - 1, T);
+        int cnt = jmax1 - jmin1 + 1;
+        offj = offj + jmin - jmin1 - cnt;
+        jmin = jmin1;
+        jmax = jmax1;
+    }
+    // packed position of thread t on this diagonal, clamped into [0, n) so the
+    // load is always in bounds (the value of an inactive thread is unused)
+    __device__ int pos(int t, int n) const {
+        int q = offj + t;
+        return q < 0 ? 0 : (q >= n ? n - 1 : q);
+    }
+    __device__ bool active(int t) const { return T >= 0 && t >= jmin && t <= jmax; }
+};
+
 // ---------------------------------------------------------------------------
 // Diagonal recurrence (update_dilu_diagonal!)
 // ---------------------------------------------------------------------------
+struct PreU {
+    double ad, ls, ts, lw, tw;
+    unsigned f;
+};
+
 __global__ void __launch_bounds__(1024) k_band_dilu_update(BandGeom g, BandVals bv, double *__restrict__ dinv) {
     __shared__ double sh_d[16][2];
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    const int n = g.nx * g.ny;
     const bool row = t < g.ny;
+    PreU buf[kPD];
+    Walk cw, pw;  // current and prefetch walks
+    cw.init(g, 0);
+    pw.init(g, 0);
+    auto load = [&](const Walk &w) -> PreU {
+        const int p = w.pos(t, n);
+        return PreU{bv.ad[p], bv.ls[p], bv.ts[p], bv.lw[p], bv.tw[p], g.flags[p]};
+    };
+#pragma unroll
+    for (int k = 0; k < kPD; ++k) {
+        buf[k] = load(pw);
+        pw.next(g);
+    }
     double dprev = 0.0;  // D of (i-1, j)
-    for (int T = 0; T < g.nT; ++T) {
-        double ds = dpp_shr1(dprev);
-        if (lane == 0 && wave > 0) ds = sh_d[wave - 1][(T + 1) & 1];
-        const int i = T - t;
-        if (row && i >= 0 && i < g.nx) {
-            const int p = g.offj[T] + t;
-            const unsigned char f = g.flags[p];
-            double d = bv.ad[p];
-            if (f & F_S) d = d - bv.ls[p] * bv.ts[p] / ds;
-            if (f & F_W) d = d - bv.lw[p] * bv.tw[p] / dprev;
-            dinv[p] = 1.0 / d;
-            dprev = d;
+    for (int T0 = 0; T0 < g.nT; T0 += kPD) {
+#pragma unroll
+        for (int k = 0; k < kPD; ++k) {
+            const PreU c = buf[k];
+            double ds = dpp_shr1(dprev);
+            const double dl = sh_d[wave > 0 ? wave - 1 : 0][(cw.T + 1) & 1];
+            ds = (lane == 0 && wave > 0) ? dl : ds;
+            double d1 = c.ad - c.ls * c.ts / ds;
+            double d = (c.f & F_S) ? d1 : c.ad;
+            double d2 = d - c.lw * c.tw / dprev;
+            d = (c.f & F_W) ? d2 : d;
+            const bool act = row && cw.T < g.nT && cw.active(t);
+            if (act) dinv[cw.offj + t] = 1.0 / d;
+            dprev = act ? d : dprev;
+            if (lane == 63) sh_d[wave][cw.T & 1] = dprev;
+            lds_barrier();
+            buf[k] = load(pw);
+            pw.next(g);
+            cw.next(g);
         }
-        if (lane == 63) sh_d[wave][T & 1] = dprev;
-        __syncthreads();
     }
 }
 
@@ -107,102 +175,100 @@ __global__ void __launch_bounds__(1024) k_band_dilu_update(BandGeom g, BandVals 
 // ---------------------------------------------------------------------------
 struct Pre {
     double b, d, c1, c2;
-    unsigned char f;
+    unsigned f;
 };
 
 __global__ void __launch_bounds__(1024) k_band_dilu_apply(BandGeom g, BandVals bv, const double *__restrict__ dinv,
-                                                          const double *bp, double *yp,
-                                                          const KScal *ks) {
+                                                          const double *bp, double *yp, const KScal *ks) {
     if (ks && ks->done) return;
     __shared__ double sh_x[16][2], sh_d[16][2];
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    const int n = g.nx * g.ny;
     const bool row = t < g.ny;
     const int nwaves = (blockDim.x + 63) >> 6;
-    // ---- forward ----
+    const int wl = wave > 0 ? wave - 1 : 0, wr = wave + 1 < nwaves ? wave + 1 : wave;
+    // ---- forward: y_r = (b_r - (A_rs y_s) Dinv_s) - (A_rw y_w) Dinv_w ----
     {
         Pre buf[kPD];
-        auto load = [&](int T) -> Pre {
-            Pre q{0.0, 0.0, 0.0, 0.0, 0};
-            const int i = T - t;
-            if (row && T < g.nT && i >= 0 && i < g.nx) {
-                const int p = g.offj[T] + t;
-                q.f = g.flags[p];
-                q.b = bp[p];
-                q.d = dinv[p];
-                q.c1 = bv.ls[p];
-                q.c2 = bv.lw[p];
-            }
-            return q;
+        Walk cw, pw;
+        cw.init(g, 0);
+        pw.init(g, 0);
+        auto load = [&](const Walk &w) -> Pre {
+            const int p = w.pos(t, n);
+            return Pre{bp[p], dinv[p], bv.ls[p], bv.lw[p], g.flags[p]};
         };
 #pragma unroll
-        for (int k = 0; k < kPD; ++k) buf[k] = load(k);
+        for (int k = 0; k < kPD; ++k) {
+            buf[k] = load(pw);
+            pw.next(g);
+        }
         double xprev = 0.0, dprev = 0.0;
         for (int T0 = 0; T0 < g.nT; T0 += kPD) {
 #pragma unroll
             for (int k = 0; k < kPD; ++k) {
-                const int T = T0 + k;
-                const Pre cur = buf[k];
-                buf[k] = load(T + kPD);
+                const Pre c = buf[k];
                 double xs = dpp_shr1(xprev), ds = dpp_shr1(dprev);
-                if (lane == 0 && wave > 0) {
-                    xs = sh_x[wave - 1][(T + 1) & 1];
-                    ds = sh_d[wave - 1][(T + 1) & 1];
-                }
-                const int i = T - t;
-                if (row && T < g.nT && i >= 0 && i < g.nx) {
-                    double x = cur.b;
-                    if (cur.f & F_S) x = x - cur.c1 * xs * ds;
-                    if (cur.f & F_W) x = x - cur.c2 * xprev * dprev;
-                    yp[g.offj[T] + t] = x;
-                    xprev = x;
-                    dprev = cur.d;
-                }
+                const int par = (cw.T + 1) & 1;
+                const double xl = sh_x[wl][par], dl = sh_d[wl][par];
+                const bool fromlds = lane == 0 && wave > 0;
+                xs = fromlds ? xl : xs;
+                ds = fromlds ? dl : ds;
+                double x1 = c.b - c.c1 * xs * ds;
+                double x = (c.f & F_S) ? x1 : c.b;
+                double x2 = x - c.c2 * xprev * dprev;
+                x = (c.f & F_W) ? x2 : x;
+                const bool act = row && cw.T < g.nT && cw.active(t);
+                if (act) yp[cw.offj + t] = x;
+                xprev = act ? x : xprev;
+                dprev = act ? c.d : dprev;
                 if (lane == 63) {
-                    sh_x[wave][T & 1] = xprev;
-                    sh_d[wave][T & 1] = dprev;
+                    sh_x[wave][cw.T & 1] = xprev;
+                    sh_d[wave][cw.T & 1] = dprev;
                 }
-                __syncthreads();
+                lds_barrier();
+                buf[k] = load(pw);
+                pw.next(g);
+                cw.next(g);
             }
         }
     }
-    // ---- backward (anti-diagonals in reverse) ----
+    __syncthreads();  // all forward stores complete before the backward loads
+    // ---- backward: z_r = (y_r Dinv_r - (A_re z_e) Dinv_r) - (A_rn z_n) Dinv_r ----
     {
         Pre buf[kPD];
-        auto load = [&](int T) -> Pre {
-            Pre q{0.0, 0.0, 0.0, 0.0, 0};
-            const int i = T - t;
-            if (row && T >= 0 && i >= 0 && i < g.nx) {
-                const int p = g.offj[T] + t;
-                q.f = g.flags[p];
-                q.b = yp[p];
-                q.d = dinv[p];
-                q.c1 = bv.ue[p];
-                q.c2 = bv.un[p];
-            }
-            return q;
+        Walk cw, pw;
+        cw.init(g, g.nT - 1);
+        pw.init(g, g.nT - 1);
+        auto load = [&](const Walk &w) -> Pre {
+            const int p = w.pos(t, n);
+            return Pre{yp[p], dinv[p], bv.ue[p], bv.un[p], g.flags[p]};
         };
-        const int Tlast = g.nT - 1;
 #pragma unroll
-        for (int k = 0; k < kPD; ++k) buf[k] = load(Tlast - k);
+        for (int k = 0; k < kPD; ++k) {
+            buf[k] = load(pw);
+            pw.prev(g);
+        }
         double zprev = 0.0;
-        for (int T0 = Tlast; T0 >= 0; T0 -= kPD) {
+        for (int T0 = g.nT - 1; T0 >= 0; T0 -= kPD) {
 #pragma unroll
             for (int k = 0; k < kPD; ++k) {
-                const int T = T0 - k;
-                const Pre cur = buf[k];
-                buf[k] = load(T - kPD);
+                const Pre c = buf[k];
                 double zn = dpp_shl1(zprev);
-                if (lane == 63 && wave + 1 < nwaves) zn = sh_x[wave + 1][(T + 1) & 1];
-                const int i = T - t;
-                if (row && T >= 0 && i >= 0 && i < g.nx) {
-                    double z = cur.b * cur.d;
-                    if (cur.f & F_E) z = z - cur.c1 * zprev * cur.d;
-                    if (cur.f & F_N) z = z - cur.c2 * zn * cur.d;
-                    yp[g.offj[T] + t] = z;
-                    zprev = z;
-                }
-                if (lane == 0) sh_x[wave][T & 1] = zprev;
-                __syncthreads();
+                const double zl = sh_x[wr][(cw.T + 1) & 1];
+                zn = (lane == 63 && wave + 1 < nwaves) ? zl : zn;
+                double z = c.b * c.d;
+                double z1 = z - c.c1 * zprev * c.d;
+                z = (c.f & F_E) ? z1 : z;
+                double z2 = z - c.c2 * zn * c.d;
+                z = (c.f & F_N) ? z2 : z;
+                const bool act = row && cw.active(t);
+                if (act) yp[cw.offj + t] = z;
+                zprev = act ? z : zprev;
+                if (lane == 0) sh_x[wave][cw.T & 1] = zprev;
+                lds_barrier();
+                buf[k] = load(pw);
+                pw.prev(g);
+                cw.prev(g);
             }
         }
     }
