@@ -29,6 +29,25 @@ def quad40_path(tmp_path_factory):
     return str(out)
 
 
+def _unzip(tmp_path_factory, name):
+    d = tmp_path_factory.mktemp("unv")
+    out = d / name
+    with gzip.open(os.path.join(GOLDEN, name + ".gz"), "rb") as f, open(out, "wb") as g:
+        shutil.copyfileobj(f, g)
+    return str(out)
+
+
+@pytest.fixture(scope="session")
+def bfs10_path(tmp_path_factory):
+    """unv_sample_meshes/backwardFacingStep_10mm.unv (mesh of CASE_UNV_BFS.jl)."""
+    return _unzip(tmp_path_factory, "backwardFacingStep_10mm.unv")
+
+
+@pytest.fixture(scope="session")
+def trig40_path(tmp_path_factory):
+    return _unzip(tmp_path_factory, "trig40.unv")
+
+
 @pytest.fixture(scope="session")
 def quad40(quad40_path):
     from fvmhip import build_mesh

@@ -23,6 +23,8 @@
 #include "fvh_eqn.h"
 #include "k_band.h"
 
+#include <cstdlib>
+
 namespace fvh {
 
 constexpr int kPD = 8;  // prefetch distance (steps)
@@ -178,6 +180,7 @@ struct Pre {
     unsigned f;
 };
 
+template <int kPD>
 __global__ void __launch_bounds__(1024) k_band_dilu_apply(BandGeom g, BandVals bv, const double *__restrict__ dinv,
                                                           const double *bp, double *yp, const KScal *ks) {
     if (ks && ks->done) return;
@@ -366,8 +369,20 @@ int band_dilu_apply(fvh_eqn *e, const double *x, double *y, const KScal *ks) {
     double *xp = e->bd.xp.p;
     hipLaunchKernelGGL(k_band_gather, grid1d(n), dim3(256), 0, stream(), n, s.b_iperm.p, x, xp, ks);
     HIP_TRY(hipGetLastError());
-    hipLaunchKernelGGL(k_band_dilu_apply, dim3(1), dim3(threads_for(s.g.ny)), 0, stream(), s.g, e->bd.v, e->bd.dinv.p, xp,
-                       xp, ks);
+    static int pd = -1;
+    if (pd < 0) {
+        const char *env = std::getenv("FVH_BAND_PD");
+        pd = env ? std::atoi(env) : 8;
+    }
+    const dim3 blk(threads_for(s.g.ny));
+    if (pd == 16)
+        hipLaunchKernelGGL((k_band_dilu_apply<16>), dim3(1), blk, 0, stream(), s.g, e->bd.v, e->bd.dinv.p, xp, xp, ks);
+    else if (pd == 24)
+        hipLaunchKernelGGL((k_band_dilu_apply<24>), dim3(1), blk, 0, stream(), s.g, e->bd.v, e->bd.dinv.p, xp, xp, ks);
+    else if (pd == 4)
+        hipLaunchKernelGGL((k_band_dilu_apply<4>), dim3(1), blk, 0, stream(), s.g, e->bd.v, e->bd.dinv.p, xp, xp, ks);
+    else
+        hipLaunchKernelGGL((k_band_dilu_apply<8>), dim3(1), blk, 0, stream(), s.g, e->bd.v, e->bd.dinv.p, xp, xp, ks);
     HIP_TRY(hipGetLastError());
     hipLaunchKernelGGL(k_band_scatter, grid1d(n), dim3(256), 0, stream(), n, s.b_iperm.p, xp, y, ks);
     HIP_TRY(hipGetLastError());
