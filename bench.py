@@ -2,14 +2,17 @@
 """Benchmark: SIMPLE iterations/s and cell-updates/s on the laminar
 backward-facing step, 2000 x 500 quads (1M cells), on one MI355X.
 
-    python bench.py [--steps K] [--warmup W] [--gpus 1] [--precision fp64|mixed|both] [--no-cpu]
+    python bench.py [--steps K] [--warmup W] [--gpus 1] [--precision both|fp64|mixed] [--no-cpu]
 
 A step is one SIMPLE loop iteration (Solvers_1_SIMPLE.jl:125-230): two momentum
 assemblies + BiCGSTAB/DILU solves, the pressure equation (CG/DILU), gradients,
-interpolation and the flux update, with every field resident in HBM.  Prints
-one JSON line.  The SpMV roofline is timed live with HIP events on the
-library's stream; the CPU baseline is the oracle (single-threaded FP64
-restatement of the reference) on a bounded number of iterations of the same case.
+interpolation and the flux update, with every field resident in HBM.  Both
+forms run by default: plain FP64 (the headline `value`) and mixed precision
+(FP32 Krylov inner solves in FP64 defect correction).  Prints one JSON line.
+The operators (DILU apply, the step's dominant kernel, and SpMV) are timed
+live with HIP events on the library's stream; the CPU baseline is the oracle
+(single-threaded FP64 restatement of the reference) on a bounded number of
+iterations of the same case.
 """
 import argparse
 import ctypes as C
@@ -25,7 +28,6 @@ sys.path.insert(0, ROOT)
 import numpy as np  # noqa: E402
 
 METRIC = "SIMPLE iters/s & cell-updates/s at 1e-6 resid: mixed vs FP64; SpMV HBM GB/s"
-HBM_PEAK_GBS = 8000.0
 
 
 def log(*a):
@@ -42,20 +44,16 @@ def make_case(nx, ny, iterations, precision):
     return mesh, model, cfg
 
 
-def spmv_bytes(n, nnz, sval=8, sx=8):
-    """Algorithmic bytes of y = A x (CSR, int32 indices): rowptr, colidx + values,
-    x once, y once."""
-    return 4 * (n + 1) + nnz * (4 + sval) + n * sx + n * sx
-
-
 def run_gpu(args, precision):
     import fvmhip as F
     from fvmhip import _lib as L
+    from fvmhip import roofline as RL
 
+    name = "fp64" if precision == L.FP64 else "mixed"
     mesh, model, cfg = make_case(args.nx, args.ny, args.warmup + args.steps + 1, precision)
     t0 = time.time()
     sim = F.Simulation(model, cfg, None, L.ALG_SIMPLE, conv_break=1e-6)
-    log(f"[bench] {'fp64' if precision == 0 else 'mixed'}: set-up {time.time() - t0:.1f}s, {mesh.ncells} cells")
+    log(f"[bench] {name}: set-up {time.time() - t0:.1f}s, {mesh.ncells} cells")
     done = sim.run(args.warmup)
     L.check(L.lib().fvh_synchronize())
     t1 = time.perf_counter()
@@ -69,24 +67,38 @@ def run_gpu(args, precision):
     res = dict(iters=done, seconds=dt, iters_per_s=done / dt, cell_updates_per_s=mesh.ncells * done / dt,
                ms_per_step=1e3 * dt / max(done, 1),
                residuals_last=[float(R[k][it0 + done - 1]) for k in range(3)],
-               krylov=dict(Ux=sim.stats("Ux"), p=sim.stats("p")))
-    # SpMV on the assembled momentum matrix, timed with HIP events
+               krylov=dict(Ux=sim.stats("Ux"), Uy=sim.stats("Uy"), p=sim.stats("p")))
+    # operators on the assembled momentum matrix, timed with HIP events on the library stream
     eq = C.c_void_p()
     L.check(L.lib().fvh_sim_equation(sim.h, b"ux", C.byref(eq)))
     n = mesh.ncells
     rp, ci = mesh.device().pattern()
+    nnz = int(ci.size)
     x = L.DeviceArray.from_host(np.random.default_rng(0).standard_normal(n))
     y = L.DeviceArray(n)
     ms = C.c_double()
-    L.check(L.lib().fvh_time_op(eq, 0, 200, x.vp, y.vp, C.byref(ms)))
-    nbytes = spmv_bytes(n, ci.size)
-    res["spmv"] = dict(ms=ms.value, bytes=nbytes, GBps=nbytes / (ms.value * 1e-3) / 1e9)
-    L.check(L.lib().fvh_time_op(eq, 1, 20, x.vp, y.vp, C.byref(ms)))
-    res["precond_apply_ms"] = ms.value
-    L.check(L.lib().fvh_time_op(eq, 2, 10, None, None, C.byref(ms)))
-    res["precond_update_ms"] = ms.value
+
+    def timed(op, reps, xv=x.vp, yv=y.vp):
+        L.check(L.lib().fvh_time_op(eq, op, reps, xv, yv, C.byref(ms)))
+        return ms.value
+
+    ops = {}
+    t = timed(0, 200)
+    ops["spmv_fp64"] = dict(ms=t, bytes=RL.spmv(n, nnz), GBps=RL.gbps(RL.spmv(n, nnz), t))
+    t = timed(1, 20)
+    ops["dilu_apply_fp64"] = dict(ms=t, bytes=RL.dilu_apply(n, nnz), GBps=RL.gbps(RL.dilu_apply(n, nnz), t))
+    t = timed(2, 10, None, None)
+    ops["dilu_update_fp64"] = dict(ms=t, bytes=RL.dilu_update(n, nnz), GBps=RL.gbps(RL.dilu_update(n, nnz), t))
+    if precision != L.FP64:
+        b = RL.spmv(n, nnz, 4, 4, 4)
+        t = timed(3, 200)
+        ops["spmv_fp32"] = dict(ms=t, bytes=b, GBps=RL.gbps(b, t))
+        b = RL.dilu_apply(n, nnz, 4, 4, 4, 4, 4)
+        t = timed(4, 20)
+        ops["dilu_apply_fp32"] = dict(ms=t, bytes=b, GBps=RL.gbps(b, t))
+    res["ops"] = ops
     res["n"] = n
-    res["nnz"] = int(ci.size)
+    res["nnz"] = nnz
     del sim
     return res
 
@@ -112,7 +124,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed", "both"])
+    ap.add_argument("--precision", default="both", choices=["fp64", "mixed", "both"])
     ap.add_argument("--nx", type=int, default=2000)
     ap.add_argument("--ny", type=int, default=500)
     ap.add_argument("--cpu-iters", type=int, default=3)
@@ -123,24 +135,32 @@ def main():
               file=sys.stderr)
         sys.exit(2)
     import fvmhip as F
+    from fvmhip import _lib as L
+    from fvmhip import roofline as RL
 
     if F.device_count() < 1:
         print("no HIP device", file=sys.stderr)
         sys.exit(1)
-    forms = {"fp64": [0], "mixed": [1], "both": [0, 1]}[args.precision]
+    forms = {"fp64": [L.FP64], "mixed": [L.MIXED], "both": [L.FP64, L.MIXED]}[args.precision]
     out = {}
     for p in forms:
-        out["fp64" if p == 0 else "mixed"] = run_gpu(args, p)
-        log(f"[bench] {'fp64' if p == 0 else 'mixed'}: {out['fp64' if p == 0 else 'mixed']['iters_per_s']:.3f} it/s")
-    head = out.get("mixed") or out["fp64"]
+        k = "fp64" if p == L.FP64 else "mixed"
+        out[k] = run_gpu(args, p)
+        log(f"[bench] {k}: {out[k]['iters_per_s']:.3f} it/s, {out[k]['ms_per_step']:.1f} ms/iteration")
+    # headline: the FP64 form (the reference's arithmetic) when it ran
+    hk = "fp64" if "fp64" in out else "mixed"
+    head = out[hk]
     cpu = None
     if not args.no_cpu:
         c = run_cpu(args)
         cpu = dict(value=c["cell_updates_per_s"], unit="cell-updates/s", cores=1, kind="port",
-                   sample=f"{c['iters']} SIMPLE iterations of the same 2000x500 case, oracle FP64 restatement, "
-                          f"{c['seconds']:.1f}s (set-up {c['setup_s']:.1f}s excluded)")
+                   sample=f"{c['iters']} SIMPLE iterations of the same {args.nx}x{args.ny} case, oracle FP64 "
+                          f"restatement, {c['seconds']:.1f}s (set-up {c['setup_s']:.1f}s excluded)")
         log(f"[bench] cpu: {c['iters_per_s']:.4f} it/s")
-    sp = head["spmv"]
+    # dominant kernel of the step: the DILU apply (banded sweep), called by every Krylov iteration
+    dk = "dilu_apply_fp64" if hk == "fp64" else "dilu_apply_fp32"
+    d = head["ops"][dk]
+    sp = head["ops"]["spmv_fp64"]
     line = {
         "metric": METRIC,
         "value": head["cell_updates_per_s"],
@@ -152,15 +172,18 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if "mixed" not in out else "f32-in-f64",
+        "dtype": "f64" if hk == "fp64" else "f32-in-f64",
         "data": "synthetic (generated BFS mesh, zero-pressure / uniform-velocity initial fields)",
         "config": {"workload": f"laminar backward-facing step {args.nx}x{args.ny} quads ({head['n']} cells), "
                                f"SIMPLE, Upwind convection, BiCGSTAB+DILU (U), CG+DILU (p), Re_h=500",
-                   "cells": head["n"], "nnz": head["nnz"]},
+                   "cells": head["n"], "nnz": head["nnz"], "form": hk},
         "simple_iters_per_s": head["iters_per_s"],
-        "roofline": {"bound": "hbm", "achieved": sp["GBps"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": sp["GBps"] / HBM_PEAK_GBS, "traffic": None, "kernel": "k_spmv (FP64 CSR)",
-                     "bytes_per_launch": sp["bytes"], "ms_per_launch": sp["ms"]},
+        "roofline": {"bound": "hbm", "achieved": d["GBps"], "peak": RL.HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": d["GBps"] / RL.HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "DILU apply: k_band_gather + k_band_dilu_apply + k_band_scatter",
+                     "bytes_per_launch": d["bytes"], "ms_per_launch": d["ms"]},
+        "spmv": {"kernel": "k_spmv (FP64 CSR)", "GBps": sp["GBps"], "frac": sp["GBps"] / RL.HBM_PEAK_GBS,
+                 "bytes_per_launch": sp["bytes"], "ms_per_launch": sp["ms"]},
         "cpu_baseline": cpu,
         "forms": out,
     }

@@ -66,7 +66,7 @@ enum { FVH_PRECOND_NONE = 0, FVH_PRECOND_DILU = 1, FVH_PRECOND_ILU0 = 2,
 /* Arithmetic of the Krylov inner solve */
 enum { FVH_FP64 = 0,          /* plain FP64, bit-identical to the oracle             */
        FVH_MIXED = 1,         /* FP32 SpMV + preconditioner inside FP64 refinement   */
-       FVH_MIXED_FP16 = 2 };  /* as MIXED, off-diagonals stored as FP16              */
+       FVH_MIXED_FP16 = 2 };  /* reserved (FP16 off-diagonals): FVH_ERR_UNSUPPORTED  */
 
 /* Turbulence models and pressure-velocity algorithms */
 enum { FVH_LAMINAR = 0, FVH_KOMEGA = 1 };
@@ -308,6 +308,14 @@ int fvh_precond_dinv(fvh_eqn *e, double *dinv_host);
  * overwritten)  [src/Solve/Solve_1_api.jl:32-46; Krylov.jl 0.9.5 solve!] */
 int fvh_solve(fvh_eqn *e, const fvh_solver_cfg *cfg, double *x, fvh_stats *stats);
 
+/* FP32 operators of the mixed-precision inner solves (no counterpart in the
+ * reference, which runs in FP64 only): y = A x and y = M^{-1} x on FP32
+ * copies of A and of the preconditioner's data, taken from the current FP64
+ * values (DILU reciprocals computed in FP64 by fvh_precond_update, then
+ * rounded).  x, y: device arrays of ncells floats. */
+int fvh_spmv_f32(fvh_eqn *e, float *y, const float *x);
+int fvh_precond_apply_f32(fvh_eqn *e, float *y, const float *x);
+
 /* ------------------------------------------------------------------ */
 /* Pressure-velocity loops                                             */
 /* ------------------------------------------------------------------ */
@@ -327,8 +335,11 @@ int fvh_sim_iterate(fvh_sim *s, int32_t n, int32_t *done);
 /* Residual histories R_ux, R_uy, R_p (host, length case.iterations,
  * entries not reached keep 1.0 as in Solvers_1_SIMPLE.jl:110-112). */
 int fvh_sim_residuals(fvh_sim *s, double *R_ux, double *R_uy, double *R_p);
-/* Host copy of a named state array ("Ux","Uy","p","k","omega","nut",
- * "mdotf","gradp_x",... see DESIGN); *len receives the length. */
+/* Host copy of a named state array; *len receives the length.  Names:
+ * cell fields Ux Uy Uz p k omega nut nueff rD divHv Hv_x Hv_y gradp_x
+ * gradp_y gradp_z gradU_xx gradU_yx gradU_xy gradU_yy Pk nueffk nueffw;
+ * face fields mdotf rDf pf Uf_x Uf_y Uf_z nutf; systems ux_b uy_b p_b and
+ * ux_A uy_A p_A (CSR order); DILU reciprocals ux_dinv p_dinv. */
 int fvh_sim_get(fvh_sim *s, const char *name, double *out, int64_t *len);
 /* Krylov statistics of the last solve of field "Ux","Uy","p","k","omega". */
 int fvh_sim_stats(fvh_sim *s, const char *field, fvh_stats *out);
@@ -341,7 +352,8 @@ int fvh_sim_equation(fvh_sim *s, const char *name, fvh_eqn **out);
  * of one launch of an operator over `reps` back-to-back launches, by HIP
  * events on the library stream.  op: 0 = SpMV y = A x (FP64), 1 =
  * preconditioner apply y = M^{-1} x, 2 = preconditioner update, 3 = FP32
- * SpMV on the single-precision copy of A, 4 = FP32 preconditioner apply. */
+ * SpMV on the single-precision copy of A, 4 = FP32 preconditioner apply
+ * (ops 3/4 round x to FP32 first, all ones when x is NULL; y is unused). */
 int fvh_time_op(fvh_eqn *e, int32_t op, int32_t reps, const double *x, double *y, double *ms_per_launch);
 
 /* simple!(model, config; pref): whole run, fields in/out (host). */

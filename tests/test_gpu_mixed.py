@@ -1,0 +1,209 @@
+"""Mixed precision: FP32 Krylov inner solves inside FP64 defect correction.
+
+The FP32 operators are checked bit for bit where their arithmetic is fixed
+(SpMV against a float32 restatement, the banded sweep against the
+level-scheduled one), against the FP64 operators within FP32 rounding, and
+the whole path through its result: solves reach FP64 accuracy past the FP32
+limit, and SIMPLE runs converge to the FP64 oracle's fields.
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import fvmhip as F
+from fvmhip import _lib as L
+from oracle import ref
+from tests.cases import bfs_model, cavity_model, config, rel_l2
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN_C1 = "tests/golden/c1_oracle.npz"
+
+
+class Eqn:
+    """A bare equation on a mesh with values set from the host."""
+
+    def __init__(self, mesh):
+        self.mesh = mesh
+        self.dm = mesh.device()
+        self.h = C.c_void_p()
+        L.check(L.lib().fvh_eqn_create(self.dm.h, C.byref(self.h)))
+        nz, b = C.c_void_p(), C.c_void_p()
+        L.check(L.lib().fvh_eqn_arrays(self.h, C.byref(nz), C.byref(b)))
+        self.nz, self.b = nz.value, b.value
+        self.rp, self.ci = self.dm.pattern()
+
+    def set_values(self, vals, rhs=None):
+        vals = np.ascontiguousarray(vals, np.float64)
+        L.check(L.lib().fvh_memcpy(C.c_void_p(self.nz), vals.ctypes.data_as(C.c_void_p), vals.nbytes))
+        if rhs is not None:
+            rhs = np.ascontiguousarray(rhs, np.float64)
+            L.check(L.lib().fvh_memcpy(C.c_void_p(self.b), rhs.ctypes.data_as(C.c_void_p), rhs.nbytes))
+        self.vals = vals
+
+    def dense(self):
+        n = self.mesh.ncells
+        A = np.zeros((n, n))
+        rows = np.repeat(np.arange(n), np.diff(self.rp))
+        A[rows, self.ci] = self.vals
+        return A
+
+    def close(self):
+        L.lib().fvh_eqn_destroy(self.h)
+
+
+def random_values(e, rng, symmetric=False, negate=False):
+    """Diagonally dominant values on the mesh pattern."""
+    n = e.mesh.ncells
+    rows = np.repeat(np.arange(n), np.diff(e.rp))
+    off = rows != e.ci
+    v = np.zeros(e.ci.size)
+    if symmetric:
+        lo, hi = np.minimum(rows, e.ci), np.maximum(rows, e.ci)
+        w = np.random.default_rng(int(rng.integers(1 << 30))).uniform(0.1, 1.0, n * 8)
+        v[off] = -w[(lo * 7 + hi) % w.size][off]
+    else:
+        v[off] = -rng.uniform(0.1, 1.0, int(off.sum()))
+    rowabs = np.bincount(rows, weights=np.abs(v), minlength=n)
+    v[~off] = rowabs + 0.05
+    return -v if negate else v
+
+
+def spmv_f32(e, vals, x):
+    """float32 restatement of the FP32 SpMV: each row summed from 0 in
+    ascending column order, single-precision products and sums, no FMA."""
+    n = e.mesh.ncells
+    v32, x32 = vals.astype(np.float32), x.astype(np.float32)
+    deg = np.diff(e.rp)
+    s = np.zeros(n, np.float32)
+    for k in range(deg.max()):
+        has = deg > k
+        idx = e.rp[:-1][has] + k
+        s[has] = s[has] + v32[idx] * x32[e.ci[idx]]
+    return s
+
+
+def test_fp32_operators(monkeypatch):
+    """FP32 SpMV bit for bit against its restatement; FP32 DILU apply on the
+    banded and the level-set schedules bit for bit equal to each other and
+    within FP32 rounding of the FP64 apply."""
+    rng = np.random.default_rng(11)
+    out = {}
+    for sweep in ("band", "levels"):
+        if sweep == "levels":
+            monkeypatch.setenv("FVH_SWEEP", "levels")
+        else:
+            monkeypatch.delenv("FVH_SWEEP", raising=False)
+        mesh = F.backward_facing_step(nx=150, ny=70)
+        assert (mesh.device().sweep_info()[1] == 150) == (sweep == "band")
+        e = Eqn(mesh)
+        try:
+            vals = random_values(e, np.random.default_rng(5))
+            e.set_values(vals)
+            x = rng.standard_normal(mesh.ncells) if sweep == "band" else out["x"]
+            out["x"] = x
+            x32 = L.DeviceArray.from_host(x.astype(np.float32))
+            y32 = L.DeviceArray(mesh.ncells, np.float32)
+            L.check(L.lib().fvh_spmv_f32(e.h, y32.vp, x32.vp))
+            assert np.array_equal(y32.host(), spmv_f32(e, vals, x))
+            L.check(L.lib().fvh_precond_set(e.h, L.PRECOND_DILU, None))
+            L.check(L.lib().fvh_precond_update(e.h))
+            L.check(L.lib().fvh_precond_apply_f32(e.h, y32.vp, x32.vp))
+            out[sweep] = y32.host()
+            xd, yd = L.DeviceArray.from_host(x), L.DeviceArray(mesh.ncells)
+            L.check(L.lib().fvh_precond_apply(e.h, yd.vp, xd.vp))
+            y64 = yd.host()
+            assert rel_l2(out[sweep].astype(np.float64), y64) < 1e-5
+        finally:
+            e.close()
+    assert np.array_equal(out["band"], out["levels"])
+
+
+@pytest.mark.parametrize("solver,pre,sym,neg", [
+    (F.BicgstabSolver, F.DILU, False, False),
+    (F.CgSolver, F.DILU, True, True),  # CG on -A (negative definite, DILU)
+    (F.GmresSolver, F.ILU0, False, False),
+    (F.BicgstabSolver, F.Jacobi, False, True),
+])
+def test_refinement_reaches_fp64_accuracy(quad40, solver, pre, sym, neg):
+    """rtol 1e-12 is out of FP32's reach: the FP64 defect correction has to
+    repeat the inner solve, and the result matches a dense FP64 solve."""
+    e = Eqn(quad40)
+    try:
+        rng = np.random.default_rng(3)
+        vals = random_values(e, rng, symmetric=sym, negate=neg)
+        rhs = rng.standard_normal(quad40.ncells)
+        e.set_values(vals, rhs)
+        exact = np.linalg.solve(e.dense(), rhs)
+        L.check(L.lib().fvh_precond_set(e.h, pre.kind, None))
+        L.check(L.lib().fvh_precond_update(e.h))
+        results = {}
+        for prec in (L.FP64, L.MIXED):
+            cfg = L.SolverCfg(solver=solver, precond=pre.kind, itmax=400, precision=prec, convergence=1e-7,
+                              relax=1.0, atol=0.0, rtol=1e-12)
+            x = L.DeviceArray.from_host(np.zeros(quad40.ncells))
+            st = L.Stats()
+            L.check(L.lib().fvh_solve(e.h, C.byref(cfg), x.vp, C.byref(st)))
+            results[prec] = (x.host(), st)
+        x64, _ = results[L.FP64]
+        xm, stm = results[L.MIXED]
+        assert rel_l2(x64, exact) < 1e-10
+        assert rel_l2(xm, exact) < 1e-10, f"mixed rel err {rel_l2(xm, exact):.2e}, {stm.refinements} refinements"
+        assert stm.refinements >= 2
+    finally:
+        e.close()
+
+
+def test_single_correction_at_default_tolerance(quad40):
+    """At the solver defaults (rtol 1e-3) one FP32 solve is enough."""
+    e = Eqn(quad40)
+    try:
+        rng = np.random.default_rng(9)
+        vals = random_values(e, rng)
+        rhs = rng.standard_normal(quad40.ncells)
+        e.set_values(vals, rhs)
+        L.check(L.lib().fvh_precond_set(e.h, L.PRECOND_DILU, None))
+        L.check(L.lib().fvh_precond_update(e.h))
+        cfg = L.SolverCfg(solver=L.BICGSTAB, precond=L.PRECOND_DILU, itmax=100, precision=L.MIXED, convergence=1e-7,
+                          relax=1.0, atol=np.sqrt(np.finfo(float).eps), rtol=1e-3)
+        x = L.DeviceArray.from_host(np.zeros(quad40.ncells))
+        st = L.Stats()
+        L.check(L.lib().fvh_solve(e.h, C.byref(cfg), x.vp, C.byref(st)))
+        assert st.refinements == 1 and st.status == 0  # solved
+        r = rhs - e.dense() @ x.host()
+        assert np.linalg.norm(r) < 1e-2 * np.linalg.norm(rhs)
+    finally:
+        e.close()
+
+
+def test_cavity_c1_mixed_fields(quad40):
+    """C1 with U and p solved in mixed precision, run to the reference's
+    convergence break: fields within 1e-6 relative L2 of the FP64 oracle."""
+    model = cavity_model(quad40)
+    cfg = config(1000, precision=L.MIXED)
+    sim = F.Simulation(model, cfg, 0.0)
+    n = sim.run(1000)
+    gold = np.load(GOLDEN_C1)
+    errs = {f: rel_l2(sim.get(f), gold[f]) for f in ("Ux", "Uy", "p")}
+    print(f"C1 mixed: {n} iterations (FP64 oracle {int(gold['iterations'])}); rel L2 {errs}")
+    assert n < 1000  # converged to the break
+    for f, err in errs.items():
+        assert err <= 1e-6, f"{f}: {err:.3e}"
+
+
+def test_bfs_mixed_converges_to_oracle():
+    """A small backward-facing step run to 1e-6 residuals in both forms."""
+    mesh = F.backward_facing_step(nx=120, ny=40)
+    model = bfs_model(mesh, nu=1e-3)
+    iters = 3000
+    cfg = config(iters, schemes=dict(divergence=F.Upwind), precision=L.MIXED)
+    cfg64 = config(iters, schemes=dict(divergence=F.Upwind))
+    g = F.Simulation(model, cfg, None, L.ALG_SIMPLE, 1e-6)
+    r = ref.RefSimulation(model, F.CaseDesc(model, cfg64, None, L.ALG_SIMPLE, 1e-6))
+    ng, nr = g.run(iters), r.run(iters)
+    assert ng < iters and nr < iters
+    errs = {f: rel_l2(g.get(f), r.get(f)) for f in ("Ux", "Uy", "p")}
+    print(f"BFS 120x40 mixed: {ng} iterations vs FP64 oracle {nr}; rel L2 {errs}")
+    for f, err in errs.items():
+        assert err <= 1e-5, f"{f}: {err:.3e}"

@@ -107,13 +107,27 @@ struct BcPlan {
 // Anti-diagonal geometry of a banded grid, by value in kernel arguments.
 struct BandGeom {
     int nx, ny, nT;
-    const int *offj;              // packed position of (T - t, t) is offj[T] + t
-    const unsigned char *flags;   // packed: which of S, W, E, N neighbours exist
+    const int *offj;  // packed position of (T - t, t) is offj[T] + t
 };
 
-// Band coefficients of one matrix in packed (anti-diagonal) order.
-struct BandVals {
-    double *ad, *ls, *lw, *ue, *un, *ts, *tw;
+// Per-row records in packed (anti-diagonal) order.
+template <typename T>
+struct alignas(4 * sizeof(T)) Rec4T {
+    T a, b, c, d;
+};
+using Rec4 = Rec4T<double>;
+using Rec4f = Rec4T<float>;
+struct alignas(16) Rec6 {
+    double a, b, c, d, e, f;
+};
+// u: (A_rr, A_rs, A_sr, A_rw, A_wr, -) for the diagonal recurrence;
+// f: (b_r, Dinv_r, A_rs, A_rw) for the forward sweep;
+// b: (y_r, Dinv_r, A_re, A_rn) for the backward sweep;
+// f32 / b32: FP32 copies of f / b for the mixed-precision inner solves.
+struct BandRecs {
+    Rec6 *u;
+    Rec4 *f, *b;
+    Rec4f *f32, *b32;
 };
 
 // Sweep schedule of the lower (forward) and upper (backward) triangles.
@@ -123,8 +137,7 @@ struct Schedule {
     std::vector<int> h_fptr, h_bptr;
     int band_nx = 0;  // >0: A is a banded grid (deps only r-1, r-nx)
     BandGeom g{};
-    DBuf<int> b_offj, b_iperm, b_perm;
-    DBuf<unsigned char> b_flags;
+    DBuf<int> b_offj, b_iperm;
 };
 
 }  // namespace fvh

@@ -47,10 +47,12 @@ struct KrylovWork {
     DBuf<double> small;          // GMRES R, c, s, z, y
     DBuf<KScal> ks;
     KScal *h_ks = nullptr;       // pinned mirror
-    // mixed precision (FP32 inner)
-    DBuf<float> f[10];
+    // mixed precision (FP32 inner solves)
+    int64_t n32 = 0;
+    int cap_basis32 = 0;
+    DBuf<float> f[10];           // FP32 vectors
+    DBuf<float> basis32;         // GMRES basis of the inner solves
     DBuf<float> valf;            // FP32 copy of A
-    DBuf<unsigned short> valh;   // FP16 copy of the off-diagonals
     ~KrylovWork() {
         if (h_ks) (void)hipHostFree(h_ks);
     }
@@ -60,8 +62,12 @@ struct KrylovWork {
 
 namespace fvh {
 struct BandStore {
-    DBuf<double> ad, ls, lw, ue, un, ts, tw, dinv, xp;
-    BandVals v{};
+    DBuf<Rec6> u;
+    DBuf<Rec4> f, b;
+    DBuf<double> zp;
+    DBuf<Rec4f> f32, b32;  // FP32 copies for the mixed-precision inner solves
+    DBuf<float> zp32;
+    BandRecs r{};
 };
 }  // namespace fvh
 
@@ -73,6 +79,7 @@ struct fvh_eqn {
     fvh_eqn *pmat = nullptr;    // equation whose A the preconditioner reads
     fvh_eqn *pshare = nullptr;  // use that equation's preconditioner object instead
     fvh::DBuf<double> D, Dinv, lu, pstore;
+    fvh::DBuf<float> p32a, p32b;  // FP32 copies: Dinv / Jacobi scale, and A or LU values
     fvh::BandStore bd;  // DILU on banded grids (packed order)
     fvh::KrylovWork kw;
     fvh::DBuf<double> red;  // residual reductions

@@ -11,6 +11,9 @@ int band_setup(fvh_mesh *m);
 int band_alloc(fvh_eqn *e);
 int band_dilu_update(fvh_eqn *e, const double *val);
 int band_dilu_apply(fvh_eqn *e, const double *x, double *y, const KScal *ks);
+// FP32 records from the FP64 ones (after each update), and the FP32 sweeps
+int band_to32(fvh_eqn *e);
+int band_dilu_apply32(fvh_eqn *e, const float *x, float *y, const KScal *ks);
 int band_dinv_natural(fvh_eqn *e, double *out);
 
 }  // namespace fvh

@@ -25,33 +25,98 @@ namespace fvh {
 // ---------------------------------------------------------------------------
 // SpMV  y = sigma * A x  (optionally with the partial sums of w.y)
 // ---------------------------------------------------------------------------
-template <bool DOT>
-__global__ void __launch_bounds__(kBlock) k_spmv(int n, const int *__restrict__ rowptr, const int *__restrict__ col,
-                                                 const double *__restrict__ val, const double *__restrict__ x,
-                                                 double *__restrict__ y, double sigma, const KScal *ks,
-                                                 const double *__restrict__ w, double *part, int nch) {
+// Row i of A x, summed from the first entry in ascending column order.
+template <typename V>
+__device__ inline V row_ax(const int *__restrict__ rowptr, const int *__restrict__ col, const V *__restrict__ val,
+                           const V *__restrict__ x, int i) {
+    V s = 0;
+    const int p1 = rowptr[i + 1];
+    for (int p = rowptr[i]; p < p1; ++p) s = s + val[p] * x[col[p]];
+    return s;
+}
+
+// One thread per row.
+template <typename V>
+__global__ void __launch_bounds__(256) k_spmv(int n, const int *__restrict__ rowptr, const int *__restrict__ col,
+                                              const V *__restrict__ val, const V *__restrict__ x, V *__restrict__ y,
+                                              V sigma, const KScal *ks) {
     if (ks && ks->done) return;
-    double acc[1] = {0.0};
-    const int base = blockIdx.x * kChunk + threadIdx.x;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) y[i] = sigma * row_ax(rowptr, col, val, x, i);
+}
+
+// Canonical partial sums of one 2048-row chunk from per-row terms staged in
+// LDS (p[v][j] = term of row chunk_base + j, 0 for rows past n): lane t < 256
+// adds rows t + 256 k in ascending k, then the same tree as chunk_reduce.
+// All kWide threads of the block call it.
+constexpr int kWide = 1024;
+template <int NV>
+__device__ inline void chunk_reduce_staged(double (&p)[NV][kChunk], double *__restrict__ part, int nch) {
+    const int t = threadIdx.x;
+    __syncthreads();
+    double acc[NV];
 #pragma unroll
-    for (int k = 0; k < kPer; ++k) {
-        int i = base + k * kBlock;
-        if (i < n) {
-            double s = 0.0;
-            for (int p = rowptr[i]; p < rowptr[i + 1]; ++p) s = s + val[p] * x[col[p]];
-            s = sigma * s;
-            y[i] = s;
-            if (DOT) acc[0] = acc[0] + w[i] * s;
+    for (int v = 0; v < NV; ++v) acc[v] = 0.0;
+    if (t < kBlock) {
+#pragma unroll
+        for (int k = 0; k < kPer; ++k)
+#pragma unroll
+            for (int v = 0; v < NV; ++v) acc[v] = acc[v] + p[v][t + k * kBlock];
+    }
+    __syncthreads();
+    if (t < kBlock) {
+#pragma unroll
+        for (int v = 0; v < NV; ++v) p[v][t] = acc[v];
+    }
+    __syncthreads();
+    if (t < 128) {
+#pragma unroll
+        for (int v = 0; v < NV; ++v) p[v][t] = p[v][t] + p[v][t + 128];
+    }
+    __syncthreads();
+    if (t < 64) {
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+            double a = p[v][t] + p[v][t + 64];
+            a = a + __shfl_down(a, 32);
+            a = a + __shfl_down(a, 16);
+            a = a + __shfl_down(a, 8);
+            a = a + __shfl_down(a, 4);
+            a = a + __shfl_down(a, 2);
+            a = a + __shfl_down(a, 1);
+            if (t == 0) part[v * nch + blockIdx.x] = a;
         }
     }
-    if (DOT) chunk_reduce<1>(acc, part, nch);
+}
+
+// y = sigma A x and the partial sums of w.y: one block of kWide threads per
+// chunk, two rows per thread.
+template <typename V>
+__global__ void __launch_bounds__(kWide) k_spmv_dot(int n, const int *__restrict__ rowptr, const int *__restrict__ col,
+                                                    const V *__restrict__ val, const V *__restrict__ x,
+                                                    V *__restrict__ y, V sigma, const KScal *ks,
+                                                    const V *__restrict__ w, double *part, int nch) {
+    if (ks && ks->done) return;
+    __shared__ double p[1][kChunk];
+    const int c0 = blockIdx.x * kChunk;
+#pragma unroll
+    for (int h = 0; h < kChunk / kWide; ++h) {
+        const int j = threadIdx.x + h * kWide, i = c0 + j;
+        double pr = 0.0;
+        if (i < n) {
+            const V s = sigma * row_ax(rowptr, col, val, x, i);
+            y[i] = s;
+            pr = (double)w[i] * (double)s;
+        }
+        p[0][j] = pr;
+    }
+    chunk_reduce_staged<1>(p, part, nch);
 }
 
 int launch_spmv(fvh_eqn *e, const double *x, double *y, const KScal *ks) {
     const fvh_mesh *m = e->m;
-    int nch = nchunks(m->n);
-    hipLaunchKernelGGL((k_spmv<false>), dim3(nch), dim3(kBlock), 0, stream(), (int)m->n, m->d.rowptr, m->d.col, e->nz.p, x, y,
-                       1.0, ks, (const double *)nullptr, (double *)nullptr, nch);
+    hipLaunchKernelGGL((k_spmv<double>), grid1d(m->n), dim3(256), 0, stream(), (int)m->n, m->d.rowptr, m->d.col, e->nz.p,
+                       x, y, 1.0, ks);
     HIP_TRY(hipGetLastError());
     return FVH_OK;
 }
@@ -60,8 +125,8 @@ static int launch_spmv_dot(fvh_eqn *e, const double *x, double *y, double sigma,
                            double *part) {
     const fvh_mesh *m = e->m;
     int nch = nchunks(m->n);
-    hipLaunchKernelGGL((k_spmv<true>), dim3(nch), dim3(kBlock), 0, stream(), (int)m->n, m->d.rowptr, m->d.col, e->nz.p, x, y,
-                       sigma, ks, w, part, nch);
+    hipLaunchKernelGGL((k_spmv_dot<double>), dim3(nch), dim3(kWide), 0, stream(), (int)m->n, m->d.rowptr, m->d.col,
+                       e->nz.p, x, y, sigma, ks, w, part, nch);
     HIP_TRY(hipGetLastError());
     return FVH_OK;
 }
@@ -69,24 +134,25 @@ static int launch_spmv_dot(fvh_eqn *e, const double *x, double *y, double sigma,
 // ---------------------------------------------------------------------------
 // Residual: sqrt(mean((Ax-b)^2)) / norm(b)
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(kBlock) k_residual_part(int n, const int *__restrict__ rowptr,
-                                                          const int *__restrict__ col, const double *__restrict__ val,
-                                                          const double *__restrict__ x, const double *__restrict__ b,
-                                                          double *part, int nch) {
-    double acc[2] = {0.0, 0.0};
-    const int base = blockIdx.x * kChunk + threadIdx.x;
+__global__ void __launch_bounds__(kWide) k_residual_part(int n, const int *__restrict__ rowptr,
+                                                         const int *__restrict__ col, const double *__restrict__ val,
+                                                         const double *__restrict__ x, const double *__restrict__ b,
+                                                         double *part, int nch) {
+    __shared__ double p[2][kChunk];
+    const int c0 = blockIdx.x * kChunk;
 #pragma unroll
-    for (int k = 0; k < kPer; ++k) {
-        int i = base + k * kBlock;
+    for (int h = 0; h < kChunk / kWide; ++h) {
+        const int j = threadIdx.x + h * kWide, i = c0 + j;
+        double d2 = 0.0, b2 = 0.0;
         if (i < n) {
-            double s = 0.0;
-            for (int p = rowptr[i]; p < rowptr[i + 1]; ++p) s = s + val[p] * x[col[p]];
-            double d = fabs(s - b[i]);
-            acc[0] = acc[0] + d * d;
-            acc[1] = acc[1] + b[i] * b[i];
+            const double d = fabs(row_ax(rowptr, col, val, x, i) - b[i]);
+            d2 = d * d;
+            b2 = b[i] * b[i];
         }
+        p[0][j] = d2;
+        p[1][j] = b2;
     }
-    chunk_reduce<2>(acc, part, nch);
+    chunk_reduce_staged<2>(p, part, nch);
 }
 
 __global__ void __launch_bounds__(kBlock) k_residual_final(const double *part, int nch, int n, double *out) {
@@ -99,8 +165,8 @@ int residual_async(fvh_eqn *e, const double *phi, double *res_dev) {
     const fvh_mesh *m = e->m;
     const int nch = nchunks(m->n);
     if (e->red.n < (size_t)(2 * nch)) FVH_TRY(e->red.alloc(2 * nch));
-    hipLaunchKernelGGL(k_residual_part, dim3(nch), dim3(kBlock), 0, stream(), (int)m->n, m->d.rowptr, m->d.col, e->nz.p, phi,
-                       e->b.p, e->red.p, nch);
+    hipLaunchKernelGGL(k_residual_part, dim3(nch), dim3(kWide), 0, stream(), (int)m->n, m->d.rowptr, m->d.col, e->nz.p,
+                       phi, e->b.p, e->red.p, nch);
     HIP_TRY(hipGetLastError());
     hipLaunchKernelGGL(k_residual_final, dim3(1), dim3(kBlock), 0, stream(), e->red.p, nch, (int)m->n, res_dev);
     HIP_TRY(hipGetLastError());
@@ -221,6 +287,87 @@ int precond_apply(fvh_eqn *eq, double *y, const double *x, const KScal *ks, doub
 }
 
 // ---------------------------------------------------------------------------
+// FP32 operators of the mixed-precision inner solves
+// ---------------------------------------------------------------------------
+static int launch_spmv32(fvh_eqn *e, const float *x, float *y, float sigma, const KScal *ks, const float *w,
+                         double *part) {
+    const fvh_mesh *m = e->m;
+    if (!w) {
+        hipLaunchKernelGGL((k_spmv<float>), grid1d(m->n), dim3(256), 0, stream(), (int)m->n, m->d.rowptr, m->d.col,
+                           e->kw.valf.p, x, y, sigma, ks);
+    } else {
+        const int nch = nchunks(m->n);
+        hipLaunchKernelGGL((k_spmv_dot<float>), dim3(nch), dim3(kWide), 0, stream(), (int)m->n, m->d.rowptr, m->d.col,
+                           e->kw.valf.p, x, y, sigma, ks, w, part, nch);
+    }
+    HIP_TRY(hipGetLastError());
+    return FVH_OK;
+}
+
+// FP32 copies read by the inner solves: A (for the SpMV) and the
+// preconditioner's data, rounded once from the FP64 values after each update.
+// The DILU reciprocals are computed in FP64 by the update itself.
+static int prepare32(fvh_eqn *e) {
+    const fvh_mesh *m = e->m;
+    const int64_t n = m->n, nnz = m->nnz;
+    KrylovWork &w = e->kw;
+    if (w.valf.n != (size_t)nnz) FVH_TRY(w.valf.alloc(nnz));
+    {
+        const double *a = e->nz.p;
+        float *af = w.valf.p;
+        FVH_TRY(map(nnz, nullptr, [=] __device__(int i) { af[i] = (float)a[i]; }));
+    }
+    fvh_eqn *P = e->pshare ? e->pshare : e;
+    fvh_eqn *src = P->pmat ? P->pmat : P;
+    switch (P->pkind) {
+        case FVH_PRECOND_NONE: return FVH_OK;
+        case FVH_PRECOND_JACOBI:
+        case FVH_PRECOND_NORMDIAG: {
+            if (P->p32a.n != (size_t)n) FVH_TRY(P->p32a.alloc(n));
+            const double *s = P->pstore.p;
+            float *sf = P->p32a.p;
+            return map(n, nullptr, [=] __device__(int i) { sf[i] = (float)s[i]; });
+        }
+        case FVH_PRECOND_DILU: {
+            if (m->sched.band_nx > 0) return band_to32(P);
+            if (P->p32a.n != (size_t)n) FVH_TRY(P->p32a.alloc(n));
+            if (P->p32b.n != (size_t)nnz) FVH_TRY(P->p32b.alloc(nnz));
+            const double *di = P->Dinv.p, *a = src->nz.p;
+            float *df = P->p32a.p, *af = P->p32b.p;
+            FVH_TRY(map(n, nullptr, [=] __device__(int i) { df[i] = (float)di[i]; }));
+            return map(nnz, nullptr, [=] __device__(int i) { af[i] = (float)a[i]; });
+        }
+        case FVH_PRECOND_ILU0: {
+            if (P->p32b.n != (size_t)nnz) FVH_TRY(P->p32b.alloc(nnz));
+            const double *lu = P->lu.p;
+            float *lf = P->p32b.p;
+            return map(nnz, nullptr, [=] __device__(int i) { lf[i] = (float)lu[i]; });
+        }
+        default: return set_error(FVH_ERR_UNSUPPORTED, "preconditioner %d", P->pkind);
+    }
+}
+
+// y = M^{-1} x in FP32 (same recurrences, FP32 copies of their data)
+static int precond_apply32(fvh_eqn *eq, float *y, const float *x, const KScal *ks) {
+    fvh_eqn *e = eq->pshare ? eq->pshare : eq;
+    fvh_mesh *m = e->m;
+    const int n = (int)m->n;
+    switch (e->pkind) {
+        case FVH_PRECOND_NONE: return map(n, ks, [=] __device__(int i) { y[i] = x[i]; });
+        case FVH_PRECOND_JACOBI:
+        case FVH_PRECOND_NORMDIAG: {
+            const float *s = e->p32a.p;
+            return map(n, ks, [=] __device__(int i) { y[i] = s[i] * x[i]; });
+        }
+        case FVH_PRECOND_DILU:
+            if (m->sched.band_nx > 0) return band_dilu_apply32(e, x, y, ks);
+            return dilu_apply32(m, e->p32b.p, e->p32a.p, x, y, ks);
+        case FVH_PRECOND_ILU0: return ilu0_apply32(m, e->p32b.p, x, y, ks);
+        default: return set_error(FVH_ERR_UNSUPPORTED, "preconditioner %d", e->pkind);
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Krylov workspace
 // ---------------------------------------------------------------------------
 static int ensure_work(fvh_eqn *e, int basis) {
@@ -233,11 +380,28 @@ static int ensure_work(fvh_eqn *e, int basis) {
         if (!w.h_ks) HIP_TRY(hipHostMalloc((void **)&w.h_ks, sizeof(KScal), hipHostMallocDefault));
         w.n = n;
         w.cap_basis = 0;
+        w.n32 = 0;
     }
     if (basis > w.cap_basis) {
         FVH_TRY(w.basis.alloc((size_t)(basis + 1) * n));
         FVH_TRY(w.small.alloc((size_t)basis * (basis + 1) / 2 + 4 * (size_t)(basis + 2)));
         w.cap_basis = basis;
+    }
+    return FVH_OK;
+}
+
+static int ensure_work32(fvh_eqn *e, int basis) {
+    FVH_TRY(ensure_work(e, basis));
+    KrylovWork &w = e->kw;
+    const int64_t n = e->m->n;
+    if (w.n32 != n) {
+        for (auto &v : w.f) FVH_TRY(v.alloc(n));
+        w.n32 = n;
+        w.cap_basis32 = 0;
+    }
+    if (basis > w.cap_basis32) {
+        FVH_TRY(w.basis32.alloc((size_t)(basis + 1) * n));
+        w.cap_basis32 = basis;
     }
     return FVH_OK;
 }
@@ -253,19 +417,66 @@ __device__ inline double dsign(double x) { return x > 0.0 ? 1.0 : (x < 0.0 ? -1.
 // Iterations are enqueued in batches between host checks of ks->done.
 static int next_batch(int done_iters) { return done_iters < 8 ? 2 : (done_iters < 32 ? 4 : 8); }
 
+// A linear system as the Krylov recurrences see it.  T is the storage type
+// of the vectors: double for the reference path (start value in x, A and M
+// in FP64), float for the inner solves of mixed precision (start value 0,
+// A and M in FP32).  Scalars and every dot product stay FP64; for T = double
+// each expression below is the FP64 one term for term.
+template <typename T>
+struct Sys {
+    fvh_eqn *e;
+    int n;
+    const T *b;
+    bool warm;
+    T *v[8];
+    T *basis;
+};
+
+template <typename T>
+__device__ inline double D(T x) {
+    return (double)x;
+}
+
+static int sys_spmv(const Sys<double> &S, const double *x, double *y, const KScal *ks) {
+    return launch_spmv(S.e, x, y, ks);
+}
+static int sys_spmv(const Sys<float> &S, const float *x, float *y, const KScal *ks) {
+    return launch_spmv32(S.e, x, y, 1.0f, ks, nullptr, nullptr);
+}
+static int sys_spmv_dot(const Sys<double> &S, const double *x, double *y, double sigma, const KScal *ks,
+                        const double *w, double *part) {
+    return launch_spmv_dot(S.e, x, y, sigma, ks, w, part);
+}
+static int sys_spmv_dot(const Sys<float> &S, const float *x, float *y, double sigma, const KScal *ks, const float *w,
+                        double *part) {
+    return launch_spmv32(S.e, x, y, (float)sigma, ks, w, part);
+}
+static int sys_precond(const Sys<double> &S, double *y, const double *x, const KScal *ks) {
+    return precond_apply(S.e, y, x, ks, 1.0);
+}
+static int sys_precond(const Sys<float> &S, float *y, const float *x, const KScal *ks) {
+    return precond_apply32(S.e, y, x, ks);
+}
+
+static void fill_stats(fvh_eqn *e, fvh_stats *st) {
+    st->niter = e->kw.h_ks->iter;
+    st->status = e->kw.h_ks->status;
+    st->rnorm = e->kw.h_ks->rnorm;
+    st->rnorm0 = e->kw.h_ks->rnorm0;
+}
+
 // ---------------------------------------------------------------------------
 // BiCGSTAB (Krylov.jl bicgstab!, left preconditioning, c = b)
 // ---------------------------------------------------------------------------
-static int bicgstab(fvh_eqn *e, const fvh_solver_cfg &cfg, double *xu, fvh_stats *st) {
-    FVH_TRY(ensure_work(e, 0));
-    const int n = (int)e->m->n;
+template <typename T>
+static int bicgstab(const Sys<T> &S, const fvh_solver_cfg &cfg, T *xu, fvh_stats *st) {
+    fvh_eqn *e = S.e;
+    const int n = S.n;
     KrylovWork &w = e->kw;
-    double *xk = w.v[0].p, *dx = w.v[1].p, *r = w.v[2].p, *p = w.v[3].p, *v = w.v[4].p, *s = w.v[5].p, *q = w.v[6].p,
-           *t = w.v[7].p;
-    const double *b = e->b.p;
+    T *xk = S.v[0], *dx = S.v[1], *r = S.v[2], *p = S.v[3], *v = S.v[4], *s = S.v[5], *q = S.v[6], *t = S.v[7];
+    const T *b = S.b;
     double *part = w.part.p;
     KScal *ks = w.ks.p;
-    const int nch = nchunks(n);
     const double atol = cfg.atol, rtol = cfg.rtol;
     const int itmax = cfg.itmax;
     FVH_TRY(single(ks, false, [=] __device__(KScal * k) {
@@ -276,19 +487,28 @@ static int bicgstab(fvh_eqn *e, const fvh_solver_cfg &cfg, double *xu, fvh_stats
         k->omega = 1.0;
         k->rho = 1.0;
     }));
-    FVH_TRY(map(n, nullptr, [=] __device__(int i) {
-        dx[i] = xu[i];
-        xk[i] = 0.0;
-        s[i] = 0.0;
-        v[i] = 0.0;
-    }));
-    FVH_TRY(launch_spmv(e, dx, q, nullptr));
-    FVH_TRY(map(n, nullptr, [=] __device__(int i) { q[i] = b[i] - q[i]; }));
-    FVH_TRY(precond_apply(e, r, q, nullptr, 1.0));
+    if (S.warm) {
+        FVH_TRY(map(n, nullptr, [=] __device__(int i) {
+            dx[i] = xu[i];
+            xk[i] = 0;
+            s[i] = 0;
+            v[i] = 0;
+        }));
+        FVH_TRY(sys_spmv(S, dx, q, nullptr));
+        FVH_TRY(map(n, nullptr, [=] __device__(int i) { q[i] = b[i] - q[i]; }));
+    } else {
+        FVH_TRY(map(n, nullptr, [=] __device__(int i) {
+            xk[i] = 0;
+            s[i] = 0;
+            v[i] = 0;
+            q[i] = b[i];
+        }));
+    }
+    FVH_TRY(sys_precond(S, r, q, nullptr));
     FVH_TRY(mapred<2>(n, nullptr, part, [=] __device__(int i, double(&a)[2]) {
         p[i] = r[i];
-        a[0] = a[0] + r[i] * r[i];
-        a[1] = a[1] + b[i] * r[i];
+        a[0] = a[0] + D(r[i]) * D(r[i]);
+        a[1] = a[1] + D(b[i]) * D(r[i]);
     }));
     FVH_TRY(finalize<2>(n, part, ks, false, [=] __device__(KScal * k, double(&rr)[2]) {
         double rn = sqrt(rr[0]);
@@ -306,9 +526,9 @@ static int bicgstab(fvh_eqn *e, const fvh_solver_cfg &cfg, double *xu, fvh_stats
     while (!w.h_ks->done && enq < itmax) {
         int nb = std::min(next_batch(enq), itmax - enq);
         for (int it = 0; it < nb; ++it) {
-            FVH_TRY(launch_spmv(e, p, q, ks));
-            FVH_TRY(precond_apply(e, v, q, ks, 1.0));
-            FVH_TRY(mapred<1>(n, ks, part, [=] __device__(int i, double(&a)[1]) { a[0] = a[0] + b[i] * v[i]; }));
+            FVH_TRY(sys_spmv(S, p, q, ks));
+            FVH_TRY(sys_precond(S, v, q, ks));
+            FVH_TRY(mapred<1>(n, ks, part, [=] __device__(int i, double(&a)[1]) { a[0] = a[0] + D(b[i]) * D(v[i]); }));
             FVH_TRY(finalize<1>(n, part, ks, true, [=] __device__(KScal * k, double(&bv)[1]) {
                 k->iter += 1;
                 k->rho = k->beta;
@@ -316,23 +536,23 @@ static int bicgstab(fvh_eqn *e, const fvh_solver_cfg &cfg, double *xu, fvh_stats
             }));
             FVH_TRY(map(n, ks, [=] __device__(int i) {
                 double al = ks->alpha;
-                s[i] = r[i] + (-al) * v[i];
-                xk[i] = xk[i] + al * p[i];
+                s[i] = D(r[i]) + (-al) * D(v[i]);
+                xk[i] = D(xk[i]) + al * D(p[i]);
             }));
-            FVH_TRY(launch_spmv(e, s, q, ks));
-            FVH_TRY(precond_apply(e, t, q, ks, 1.0));
+            FVH_TRY(sys_spmv(S, s, q, ks));
+            FVH_TRY(sys_precond(S, t, q, ks));
             FVH_TRY(mapred<2>(n, ks, part, [=] __device__(int i, double(&a)[2]) {
-                a[0] = a[0] + t[i] * s[i];
-                a[1] = a[1] + t[i] * t[i];
+                a[0] = a[0] + D(t[i]) * D(s[i]);
+                a[1] = a[1] + D(t[i]) * D(t[i]);
             }));
             FVH_TRY(finalize<2>(n, part, ks, true, [=] __device__(KScal * k, double(&tt)[2]) { k->omega = tt[0] / tt[1]; }));
             FVH_TRY(mapred<2>(n, ks, part, [=] __device__(int i, double(&a)[2]) {
                 double om = ks->omega;
-                xk[i] = xk[i] + om * s[i];
-                double ri = s[i] + (-om) * t[i];
+                xk[i] = D(xk[i]) + om * D(s[i]);
+                const T ri = D(s[i]) + (-om) * D(t[i]);
                 r[i] = ri;
-                a[0] = a[0] + b[i] * ri;
-                a[1] = a[1] + ri * ri;
+                a[0] = a[0] + D(b[i]) * D(ri);
+                a[1] = a[1] + D(ri) * D(ri);
             }));
             FVH_TRY(finalize<2>(n, part, ks, true, [=] __device__(KScal * k, double(&rr)[2]) {
                 double next_rho = rr[0];
@@ -351,31 +571,30 @@ static int bicgstab(fvh_eqn *e, const fvh_solver_cfg &cfg, double *xu, fvh_stats
             }));
             FVH_TRY(map(n, ks, [=] __device__(int i) {
                 double om = ks->omega, be = ks->ts;
-                double pi = p[i] + (-om) * v[i];
-                p[i] = r[i] + be * pi;
+                double pi = D(p[i]) + (-om) * D(v[i]);
+                p[i] = D(r[i]) + be * pi;
             }));
         }
         enq += nb;
         FVH_TRY(read_ks(e));
     }
-    FVH_TRY(map(n, nullptr, [=] __device__(int i) { xu[i] = xk[i] + dx[i]; }));
+    if (S.warm) FVH_TRY(map(n, nullptr, [=] __device__(int i) { xu[i] = xk[i] + dx[i]; }));
+    else FVH_TRY(map(n, nullptr, [=] __device__(int i) { xu[i] = xk[i]; }));
     FVH_TRY(read_ks(e));
-    st->niter = w.h_ks->iter;
-    st->status = w.h_ks->status;
-    st->rnorm = w.h_ks->rnorm;
-    st->rnorm0 = w.h_ks->rnorm0;
+    fill_stats(e, st);
     return FVH_OK;
 }
 
 // ---------------------------------------------------------------------------
 // CG (Krylov.jl cg!) on sigma*A x = sigma*b
 // ---------------------------------------------------------------------------
-static int cg(fvh_eqn *e, const fvh_solver_cfg &cfg, double *xu, fvh_stats *st, double sigma) {
-    FVH_TRY(ensure_work(e, 0));
-    const int n = (int)e->m->n;
+template <typename T>
+static int cg(const Sys<T> &S, const fvh_solver_cfg &cfg, T *xu, fvh_stats *st, double sigma) {
+    fvh_eqn *e = S.e;
+    const int n = S.n;
     KrylovWork &w = e->kw;
-    double *xk = w.v[0].p, *dx = w.v[1].p, *r = w.v[2].p, *z = w.v[3].p, *p = w.v[4].p, *Ap = w.v[5].p;
-    const double *b = e->b.p;
+    T *xk = S.v[0], *dx = S.v[1], *r = S.v[2], *z = S.v[3], *p = S.v[4], *Ap = S.v[5];
+    const T *b = S.b;
     double *part = w.part.p;
     KScal *ks = w.ks.p;
     const double atol = cfg.atol, rtol = cfg.rtol;
@@ -387,18 +606,25 @@ static int cg(fvh_eqn *e, const fvh_solver_cfg &cfg, double *xu, fvh_stats *st, 
         k->status = FVH_KRYLOV_ITMAX;
         k->sigma = sigma;
     }));
-    FVH_TRY(map(n, nullptr, [=] __device__(int i) {
-        dx[i] = xu[i];
-        xk[i] = 0.0;
-    }));
-    FVH_TRY(launch_spmv(e, dx, Ap, nullptr));
-    FVH_TRY(map(n, nullptr, [=] __device__(int i) { r[i] = sigma * b[i] - sigma * Ap[i]; }));
-    FVH_TRY(precond_apply(e, z, r, nullptr, 1.0));
+    if (S.warm) {
+        FVH_TRY(map(n, nullptr, [=] __device__(int i) {
+            dx[i] = xu[i];
+            xk[i] = 0;
+        }));
+        FVH_TRY(sys_spmv(S, dx, Ap, nullptr));
+        FVH_TRY(map(n, nullptr, [=] __device__(int i) { r[i] = sigma * D(b[i]) - sigma * D(Ap[i]); }));
+    } else {
+        FVH_TRY(map(n, nullptr, [=] __device__(int i) {
+            xk[i] = 0;
+            r[i] = sigma * D(b[i]);
+        }));
+    }
+    FVH_TRY(sys_precond(S, z, r, nullptr));
     FVH_TRY(mapred<1>(n, nullptr, part, [=] __device__(int i, double(&a)[1]) {
-        double zi = sigma * z[i];
+        const T zi = sigma * D(z[i]);
         z[i] = zi;
         p[i] = zi;
-        a[0] = a[0] + r[i] * zi;
+        a[0] = a[0] + D(r[i]) * D(zi);
     }));
     FVH_TRY(finalize<1>(n, part, ks, false, [=] __device__(KScal * k, double(&g)[1]) {
         double gamma = g[0];
@@ -417,7 +643,7 @@ static int cg(fvh_eqn *e, const fvh_solver_cfg &cfg, double *xu, fvh_stats *st, 
     while (!w.h_ks->done && enq < itmax) {
         int nb = std::min(next_batch(enq), itmax - enq);
         for (int it = 0; it < nb; ++it) {
-            FVH_TRY(launch_spmv_dot(e, p, Ap, sigma, ks, p, part));
+            FVH_TRY(sys_spmv_dot(S, p, Ap, sigma, ks, p, part));
             FVH_TRY(finalize<1>(n, part, ks, true, [=] __device__(KScal * k, double(&pa)[1]) {
                 double pAp = pa[0];
                 if (pAp <= epsT * k->pnorm2 && fabs(pAp) <= epsT * k->pnorm2) {
@@ -429,14 +655,14 @@ static int cg(fvh_eqn *e, const fvh_solver_cfg &cfg, double *xu, fvh_stats *st, 
             }));
             FVH_TRY(map(n, ks, [=] __device__(int i) {
                 double al = ks->alpha;
-                xk[i] = xk[i] + al * p[i];
-                r[i] = r[i] + (-al) * Ap[i];
+                xk[i] = D(xk[i]) + al * D(p[i]);
+                r[i] = D(r[i]) + (-al) * D(Ap[i]);
             }));
-            FVH_TRY(precond_apply(e, z, r, ks, 1.0));
+            FVH_TRY(sys_precond(S, z, r, ks));
             FVH_TRY(mapred<1>(n, ks, part, [=] __device__(int i, double(&a)[1]) {
-                double zi = sigma * z[i];
+                const T zi = sigma * D(z[i]);
                 z[i] = zi;
-                a[0] = a[0] + r[i] * zi;
+                a[0] = a[0] + D(r[i]) * D(zi);
             }));
             FVH_TRY(finalize<1>(n, part, ks, true, [=] __device__(KScal * k, double(&g)[1]) {
                 double gn = g[0];
@@ -456,17 +682,15 @@ static int cg(fvh_eqn *e, const fvh_solver_cfg &cfg, double *xu, fvh_stats *st, 
                     k->status = solved ? FVH_KRYLOV_SOLVED : FVH_KRYLOV_ITMAX;
                 }
             }));
-            FVH_TRY(map(n, ks, [=] __device__(int i) { p[i] = z[i] + ks->beta * p[i]; }));
+            FVH_TRY(map(n, ks, [=] __device__(int i) { p[i] = D(z[i]) + ks->beta * D(p[i]); }));
         }
         enq += nb;
         FVH_TRY(read_ks(e));
     }
-    FVH_TRY(map(n, nullptr, [=] __device__(int i) { xu[i] = xk[i] + dx[i]; }));
+    if (S.warm) FVH_TRY(map(n, nullptr, [=] __device__(int i) { xu[i] = xk[i] + dx[i]; }));
+    else FVH_TRY(map(n, nullptr, [=] __device__(int i) { xu[i] = xk[i]; }));
     FVH_TRY(read_ks(e));
-    st->niter = w.h_ks->iter;
-    st->status = w.h_ks->status;
-    st->rnorm = w.h_ks->rnorm;
-    st->rnorm0 = w.h_ks->rnorm0;
+    fill_stats(e, st);
     return FVH_OK;
 }
 
@@ -495,13 +719,14 @@ __device__ inline void sym_givens(double a, double b, double &c, double &s, doub
     }
 }
 
-static int gmres(fvh_eqn *e, const fvh_solver_cfg &cfg, double *xu, fvh_stats *st) {
+template <typename T>
+static int gmres(const Sys<T> &S, const fvh_solver_cfg &cfg, T *xu, fvh_stats *st) {
+    fvh_eqn *e = S.e;
     const int itmax = cfg.itmax;
-    FVH_TRY(ensure_work(e, itmax > 0 ? itmax : 1));
-    const int n = (int)e->m->n;
+    const int n = S.n;
     KrylovWork &w = e->kw;
-    double *xk = w.v[0].p, *dx = w.v[1].p, *wv = w.v[2].p, *r0 = w.v[3].p, *q = w.v[4].p;
-    double *V = w.basis.p;
+    T *xk = S.v[0], *dx = S.v[1], *wv = S.v[2], *r0 = S.v[3], *q = S.v[4];
+    T *V = S.basis;
     const size_t ld = (size_t)n;
     const int cap = w.cap_basis;
     double *R = w.small.p;                         // packed upper triangle
@@ -509,7 +734,7 @@ static int gmres(fvh_eqn *e, const fvh_solver_cfg &cfg, double *xu, fvh_stats *s
     double *ss = cc + (cap + 2);                   // sines
     double *zz = ss + (cap + 2);                   // rhs of the least-squares problem
     double *yy = zz + (cap + 2);
-    const double *b = e->b.p;
+    const T *b = S.b;
     double *part = w.part.p;
     KScal *ks = w.ks.p;
     const double atol = cfg.atol, rtol = cfg.rtol;
@@ -520,14 +745,21 @@ static int gmres(fvh_eqn *e, const fvh_solver_cfg &cfg, double *xu, fvh_stats *s
         k->k = 0;
         k->status = FVH_KRYLOV_ITMAX;
     }));
-    FVH_TRY(map(n, nullptr, [=] __device__(int i) {
-        dx[i] = xu[i];
-        xk[i] = 0.0;
-    }));
-    FVH_TRY(launch_spmv(e, dx, wv, nullptr));
-    FVH_TRY(map(n, nullptr, [=] __device__(int i) { wv[i] = b[i] - wv[i]; }));
-    FVH_TRY(precond_apply(e, r0, wv, nullptr, 1.0));
-    FVH_TRY(mapred<1>(n, nullptr, part, [=] __device__(int i, double(&a)[1]) { a[0] = a[0] + r0[i] * r0[i]; }));
+    if (S.warm) {
+        FVH_TRY(map(n, nullptr, [=] __device__(int i) {
+            dx[i] = xu[i];
+            xk[i] = 0;
+        }));
+        FVH_TRY(sys_spmv(S, dx, wv, nullptr));
+        FVH_TRY(map(n, nullptr, [=] __device__(int i) { wv[i] = b[i] - wv[i]; }));
+    } else {
+        FVH_TRY(map(n, nullptr, [=] __device__(int i) {
+            xk[i] = 0;
+            wv[i] = b[i];
+        }));
+    }
+    FVH_TRY(sys_precond(S, r0, wv, nullptr));
+    FVH_TRY(mapred<1>(n, nullptr, part, [=] __device__(int i, double(&a)[1]) { a[0] = a[0] + D(r0[i]) * D(r0[i]); }));
     FVH_TRY(finalize<1>(n, part, ks, false, [=] __device__(KScal * k, double(&bb)[1]) {
         double beta = sqrt(bb[0]);
         k->rnorm = beta;
@@ -538,7 +770,7 @@ static int gmres(fvh_eqn *e, const fvh_solver_cfg &cfg, double *xu, fvh_stats *s
         if (beta <= k->eps) { k->done = 1; k->status = FVH_KRYLOV_SOLVED; return; }
         if (itmax <= 0) { k->done = 1; k->status = FVH_KRYLOV_ITMAX; }
     }));
-    FVH_TRY(map(n, ks, [=] __device__(int i) { V[i] = r0[i] / ks->rnorm0; }));
+    FVH_TRY(map(n, ks, [=] __device__(int i) { V[i] = D(r0[i]) / ks->rnorm0; }));
     FVH_TRY(read_ks(e));
     int enq = 0;
     while (!w.h_ks->done && enq < itmax) {
@@ -546,20 +778,20 @@ static int gmres(fvh_eqn *e, const fvh_solver_cfg &cfg, double *xu, fvh_stats *s
         for (int it = 0; it < nb; ++it) {
             const int k = enq + it + 1;  // 1-based inner iteration
             const int nr = (k - 1) * k / 2;
-            double *Vk1 = V + (size_t)(k - 1) * ld;
-            FVH_TRY(launch_spmv(e, Vk1, wv, ks));
-            FVH_TRY(precond_apply(e, q, wv, ks, 1.0));
+            T *Vk1 = V + (size_t)(k - 1) * ld;
+            FVH_TRY(sys_spmv(S, Vk1, wv, ks));
+            FVH_TRY(sys_precond(S, q, wv, ks));
             // modified Gram-Schmidt: R[nr+i] = V_i . q ; q -= R[nr+i] V_i
-            FVH_TRY(mapred<1>(n, ks, part, [=] __device__(int j, double(&a)[1]) { a[0] = a[0] + V[j] * q[j]; }));
+            FVH_TRY(mapred<1>(n, ks, part, [=] __device__(int j, double(&a)[1]) { a[0] = a[0] + D(V[j]) * D(q[j]); }));
             for (int i = 0; i < k; ++i) {
                 double *Ri = R + nr + i;
                 FVH_TRY(finalize<1>(n, part, ks, true, [=] __device__(KScal *, double(&d)[1]) { *Ri = d[0]; }));
-                const double *Vi = V + (size_t)i * ld;
-                const double *Vn = (i + 1 < k) ? V + (size_t)(i + 1) * ld : nullptr;
+                const T *Vi = V + (size_t)i * ld;
+                const T *Vn = (i + 1 < k) ? V + (size_t)(i + 1) * ld : nullptr;
                 FVH_TRY(mapred<1>(n, ks, part, [=] __device__(int j, double(&a)[1]) {
-                    double qj = q[j] + (-(*Ri)) * Vi[j];
+                    const T qj = D(q[j]) + (-(*Ri)) * D(Vi[j]);
                     q[j] = qj;
-                    a[0] = a[0] + (Vn ? Vn[j] : qj) * qj;
+                    a[0] = a[0] + (Vn ? D(Vn[j]) : D(qj)) * D(qj);
                 }));
             }
             FVH_TRY(finalize<1>(n, part, ks, true, [=] __device__(KScal * kk, double(&h)[1]) {
@@ -590,8 +822,8 @@ static int gmres(fvh_eqn *e, const fvh_solver_cfg &cfg, double *xu, fvh_stats *s
                     zz[k] = zeta;
                 }
             }));
-            double *Vk = V + (size_t)k * ld;
-            FVH_TRY(map(n, ks, [=] __device__(int i) { Vk[i] = q[i] / ks->hbis; }));
+            T *Vk = V + (size_t)k * ld;
+            FVH_TRY(map(n, ks, [=] __device__(int i) { Vk[i] = D(q[i]) / ks->hbis; }));
         }
         enq += nb;
         FVH_TRY(read_ks(e));
@@ -612,17 +844,111 @@ static int gmres(fvh_eqn *e, const fvh_solver_cfg &cfg, double *xu, fvh_stats *s
         }
         kk->iter = k;
     }));
+    const bool warm = S.warm;
     FVH_TRY(map(n, nullptr, [=] __device__(int j) {
         const int k = ks->k;
-        double xj = xk[j];
-        for (int i = 0; i < k; ++i) xj = xj + yy[i] * V[(size_t)i * ld + j];
-        xu[j] = xj + dx[j];
+        double xj = D(xk[j]);
+        for (int i = 0; i < k; ++i) xj = xj + yy[i] * D(V[(size_t)i * ld + j]);
+        xu[j] = warm ? xj + D(dx[j]) : xj;
     }));
     FVH_TRY(read_ks(e));
-    st->niter = w.h_ks->iter;
-    st->status = w.h_ks->status;
-    st->rnorm = w.h_ks->rnorm;
-    st->rnorm0 = w.h_ks->rnorm0;
+    fill_stats(e, st);
+    return FVH_OK;
+}
+
+template <typename T>
+static int krylov(const Sys<T> &S, const fvh_solver_cfg &cfg, T *x, fvh_stats *st, double sigma) {
+    if (cfg.solver == FVH_BICGSTAB) return bicgstab(S, cfg, x, st);
+    if (cfg.solver == FVH_GMRES) return gmres(S, cfg, x, st);
+    if (cfg.solver == FVH_CG) return cg(S, cfg, x, st, sigma);
+    return set_error(FVH_ERR_UNSUPPORTED, "solver %d", cfg.solver);
+}
+
+// CG runs on -A x = -b when A is negative definite and M follows its sign
+// (DILU / ILU(0)); the oracle does the same.
+static int cg_sigma(fvh_eqn *e, double *sigma) {
+    *sigma = 1.0;
+    const fvh_eqn *P = e->pshare ? e->pshare : e;
+    if (P->pkind == FVH_PRECOND_DILU || P->pkind == FVH_PRECOND_ILU0) {
+        double a00 = 0.0;
+        HIP_TRY(hipMemcpyAsync(&a00, e->nz.p + e->m->h_diag[0], sizeof(double), hipMemcpyDeviceToHost, stream()));
+        HIP_TRY(hipStreamSynchronize(stream()));
+        if (a00 < 0.0) *sigma = -1.0;
+    }
+    return FVH_OK;
+}
+
+// FP64 defect r = b - A x and its 2-norm (canonical order) on the host
+static int defect64(fvh_eqn *e, const double *x, double *r, double *norm) {
+    const int n = (int)e->m->n;
+    KrylovWork &w = e->kw;
+    const double *b = e->b.p;
+    FVH_TRY(launch_spmv(e, x, r, nullptr));
+    FVH_TRY(mapred<1>(n, nullptr, w.part.p, [=] __device__(int i, double(&a)[1]) {
+        const double ri = b[i] - r[i];
+        r[i] = ri;
+        a[0] = a[0] + ri * ri;
+    }));
+    KScal *ks = w.ks.p;
+    FVH_TRY(finalize<1>(n, w.part.p, ks, false, [=] __device__(KScal * k, double(&s)[1]) { k->tt = sqrt(s[0]); }));
+    FVH_TRY(read_ks(e));
+    *norm = w.h_ks->tt;
+    return FVH_OK;
+}
+
+// Mixed precision: FP64 defect correction around FP32 Krylov solves.
+//   r = b - A x (FP64);  A d = r solved in FP32 from d = 0 with the same
+//   method, preconditioner and stopping rule;  x += d (FP64).
+// The correction is repeated (at most kMaxRefine times, within the itmax
+// budget) while the FP64 defect is above atol + rtol*|r0|, the last
+// correction at least halved it, and the inner solve did not simply run out
+// of iterations: when it stopped as converged, only if the FP64 defect fell
+// short of its claimed reduction by more than 100x (the sign that FP32
+// rounding, not the stopping rule, ended it); when it stopped on a
+// breakdown (GMRES' Hessenberg test, BiCGSTAB's alpha), always.
+constexpr int kMaxRefine = 4;
+
+static int solve_mixed(fvh_eqn *e, const fvh_solver_cfg &cfg, double *x, fvh_stats *st) {
+    const int n = (int)e->m->n;
+    const int basis = cfg.solver == FVH_GMRES ? std::max(cfg.itmax, 1) : 0;
+    FVH_TRY(ensure_work32(e, basis));
+    FVH_TRY(prepare32(e));
+    KrylovWork &w = e->kw;
+    double sigma = 1.0;
+    if (cfg.solver == FVH_CG) FVH_TRY(cg_sigma(e, &sigma));
+    double *r = w.v[8].p;
+    float *r32 = w.f[8].p, *d32 = w.f[9].p;
+    Sys<float> S{e, n, r32, false, {w.f[0].p, w.f[1].p, w.f[2].p, w.f[3].p, w.f[4].p, w.f[5].p, w.f[6].p, w.f[7].p},
+                 w.basis32.p};
+    double rn0 = 0.0, rn = 0.0;
+    FVH_TRY(defect64(e, x, r, &rn0));
+    rn = rn0;
+    fvh_solver_cfg in = cfg;
+    int used = 0;
+    for (int ref = 0; ref < kMaxRefine && used < cfg.itmax; ++ref) {
+        if (rn == 0.0) break;
+        FVH_TRY(map(n, nullptr, [=] __device__(int i) { r32[i] = (float)r[i]; }));
+        in.itmax = cfg.itmax - used;
+        fvh_stats is{};
+        FVH_TRY(krylov(S, in, d32, &is, sigma));
+        FVH_TRY(map(n, nullptr, [=] __device__(int i) { x[i] = x[i] + (double)d32[i]; }));
+        const double rn_prev = rn;
+        FVH_TRY(defect64(e, x, r, &rn));
+        used += is.niter;
+        if (ref == 0) st->rnorm0 = is.rnorm0;
+        st->rnorm = is.rnorm;
+        st->status = is.status;
+        st->refinements = ref + 1;
+        const bool reached = rn <= cfg.atol + cfg.rtol * rn0;
+        const bool progress = rn <= 0.5 * rn_prev;
+        bool again;
+        if (is.status == FVH_KRYLOV_SOLVED)
+            again = is.rnorm0 > 0.0 && rn > 100.0 * (is.rnorm / is.rnorm0) * rn_prev;
+        else
+            again = is.status == FVH_KRYLOV_BREAKDOWN;
+        if (reached || !progress || !again) break;
+    }
+    st->niter = used;
     return FVH_OK;
 }
 
@@ -630,22 +956,18 @@ int solve(fvh_eqn *e, const fvh_solver_cfg &cfg, double *x, fvh_stats *st) {
     fvh_stats local{};
     if (!st) st = &local;
     std::memset(st, 0, sizeof *st);
-    if (cfg.solver == FVH_BICGSTAB) return bicgstab(e, cfg, x, st);
-    if (cfg.solver == FVH_GMRES) return gmres(e, cfg, x, st);
-    if (cfg.solver == FVH_CG) {
-        // CG runs on -A x = -b when A is negative definite and M follows its
-        // sign (DILU / ILU(0)); the oracle does the same.
-        double sigma = 1.0;
-        const fvh_eqn *P = e->pshare ? e->pshare : e;
-        if (P->pkind == FVH_PRECOND_DILU || P->pkind == FVH_PRECOND_ILU0) {
-            double a00 = 0.0;
-            HIP_TRY(hipMemcpyAsync(&a00, e->nz.p + e->m->h_diag[0], sizeof(double), hipMemcpyDeviceToHost, stream()));
-            HIP_TRY(hipStreamSynchronize(stream()));
-            if (a00 < 0.0) sigma = -1.0;
-        }
-        return cg(e, cfg, x, st, sigma);
-    }
-    return set_error(FVH_ERR_UNSUPPORTED, "solver %d", cfg.solver);
+    if (cfg.solver != FVH_BICGSTAB && cfg.solver != FVH_CG && cfg.solver != FVH_GMRES)
+        return set_error(FVH_ERR_UNSUPPORTED, "solver %d", cfg.solver);
+    if (cfg.precision == FVH_MIXED) return solve_mixed(e, cfg, x, st);
+    if (cfg.precision != FVH_FP64) return set_error(FVH_ERR_UNSUPPORTED, "precision %d", cfg.precision);
+    const int basis = cfg.solver == FVH_GMRES ? std::max(cfg.itmax, 1) : 0;
+    FVH_TRY(ensure_work(e, basis));
+    KrylovWork &w = e->kw;
+    Sys<double> S{e, (int)e->m->n, e->b.p, true,
+                  {w.v[0].p, w.v[1].p, w.v[2].p, w.v[3].p, w.v[4].p, w.v[5].p, w.v[6].p, w.v[7].p}, w.basis.p};
+    double sigma = 1.0;
+    if (cfg.solver == FVH_CG) FVH_TRY(cg_sigma(e, &sigma));
+    return krylov(S, cfg, x, st, sigma);
 }
 
 }  // namespace fvh
@@ -722,8 +1044,29 @@ extern "C" int fvh_precond_dinv(fvh_eqn *e, double *dinv) {
 extern "C" int fvh_solve(fvh_eqn *e, const fvh_solver_cfg *cfg, double *x, fvh_stats *st) {
     FVH_GUARD_BEGIN
     if (!e || !cfg || !x) return set_error(FVH_ERR_ARG, "null argument");
-    if (cfg->precision != FVH_FP64) return set_error(FVH_ERR_UNSUPPORTED, "fvh_solve: use FVH_FP64 (mixed precision runs inside the loop drivers)");
     FVH_TRY(solve(e, *cfg, x, st));
+    HIP_TRY(hipStreamSynchronize(stream()));
+    return FVH_OK;
+    FVH_GUARD_END
+}
+
+extern "C" int fvh_spmv_f32(fvh_eqn *e, float *y, const float *x) {
+    FVH_GUARD_BEGIN
+    if (!e || !x || !y) return set_error(FVH_ERR_ARG, "null argument");
+    FVH_TRY(ensure_work32(e, 0));
+    FVH_TRY(prepare32(e));
+    FVH_TRY(launch_spmv32(e, x, y, 1.0f, nullptr, nullptr, nullptr));
+    HIP_TRY(hipStreamSynchronize(stream()));
+    return FVH_OK;
+    FVH_GUARD_END
+}
+
+extern "C" int fvh_precond_apply_f32(fvh_eqn *e, float *y, const float *x) {
+    FVH_GUARD_BEGIN
+    if (!e || !x || !y) return set_error(FVH_ERR_ARG, "null argument");
+    FVH_TRY(ensure_work32(e, 0));
+    FVH_TRY(prepare32(e));
+    FVH_TRY(precond_apply32(e, y, x, nullptr));
     HIP_TRY(hipStreamSynchronize(stream()));
     return FVH_OK;
     FVH_GUARD_END
@@ -733,12 +1076,24 @@ extern "C" int fvh_time_op(fvh_eqn *e, int32_t op, int32_t reps, const double *x
     FVH_GUARD_BEGIN
     if (!e || !ms || reps <= 0) return set_error(FVH_ERR_ARG, "bad argument");
     if ((op == 0 || op == 1) && (!x || !y)) return set_error(FVH_ERR_ARG, "null vector");
+    if (op < 0 || op > 4) return set_error(FVH_ERR_UNSUPPORTED, "fvh_time_op: op %d", op);
+    float *x32 = nullptr, *y32 = nullptr;
+    if (op >= 3) {  // FP32 operators on the inner-solve copies, vectors from x rounded
+        FVH_TRY(ensure_work32(e, 0));
+        FVH_TRY(prepare32(e));
+        x32 = e->kw.f[8].p;
+        y32 = e->kw.f[9].p;
+        const int n = (int)e->m->n;
+        if (x) FVH_TRY(map(n, nullptr, [=] __device__(int i) { x32[i] = (float)x[i]; }));
+        else FVH_TRY(map(n, nullptr, [=] __device__(int i) { x32[i] = 1.0f; }));
+    }
     auto launch = [&]() -> int {
         switch (op) {
             case 0: return launch_spmv(e, x, y, nullptr);
             case 1: return precond_apply(e, y, x, nullptr, 1.0);
             case 2: return precond_update(e);
-            default: return set_error(FVH_ERR_UNSUPPORTED, "fvh_time_op: op %d", op);
+            case 3: return launch_spmv32(e, x32, y32, 1.0f, nullptr, nullptr, nullptr);
+            default: return precond_apply32(e, y32, x32, nullptr);
         }
     };
     FVH_TRY(launch());  // warm

@@ -12,5 +12,8 @@ int dilu_apply(fvh_mesh *m, const double *val, const double *Dinv, const double 
 // ILU(0) factor on A's pattern and its two sweeps
 int ilu0_factor(fvh_mesh *m, const double *val, double *lu);
 int ilu0_apply(fvh_mesh *m, const double *lu, const double *x, double *y, const KScal *ks);
+// the same sweeps in FP32 (FP32 copies of A / Dinv / LU)
+int dilu_apply32(fvh_mesh *m, const float *val, const float *Dinv, const float *x, float *y, const KScal *ks);
+int ilu0_apply32(fvh_mesh *m, const float *lu, const float *x, float *y, const KScal *ks);
 
 }  // namespace fvh

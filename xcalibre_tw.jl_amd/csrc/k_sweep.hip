@@ -30,17 +30,18 @@ __global__ void __launch_bounds__(kSweepBlock)
     }
 }
 
+template <typename T>
 __global__ void __launch_bounds__(kSweepBlock)
     k_dilu_apply_levels(int nlev, const int *__restrict__ fptr, const int *__restrict__ frows,
                         const int *__restrict__ bptr, const int *__restrict__ brows, const int *__restrict__ rowptr,
-                        const int *__restrict__ col, const int *__restrict__ diag, const double *__restrict__ val,
-                        const double *__restrict__ Dinv, const double *__restrict__ x, double *y, const KScal *ks) {
+                        const int *__restrict__ col, const int *__restrict__ diag, const T *__restrict__ val,
+                        const T *__restrict__ Dinv, const T *__restrict__ x, T *y, const KScal *ks) {
     if (ks && ks->done) return;
     // forward: y_i = x_i - sum_{j<i} (A_ij y_j) Dinv_j, j ascending
     for (int l = 0; l < nlev; ++l) {
         for (int k = fptr[l] + threadIdx.x; k < fptr[l + 1]; k += kSweepBlock) {
             const int i = frows[k];
-            double s = x[i];
+            T s = x[i];
             for (int p = rowptr[i]; p < diag[i]; ++p) {
                 const int j = col[p];
                 s = s - val[p] * y[j] * Dinv[j];
@@ -53,8 +54,8 @@ __global__ void __launch_bounds__(kSweepBlock)
     for (int l = 0; l < nlev; ++l) {
         for (int k = bptr[l] + threadIdx.x; k < bptr[l + 1]; k += kSweepBlock) {
             const int i = brows[k];
-            const double di = Dinv[i];
-            double s = y[i] * di;
+            const T di = Dinv[i];
+            T s = y[i] * di;
             for (int p = diag[i] + 1; p < rowptr[i + 1]; ++p) s = s - val[p] * y[col[p]] * di;
             y[i] = s;
         }
@@ -86,16 +87,17 @@ __global__ void __launch_bounds__(kSweepBlock)
     }
 }
 
+template <typename T>
 __global__ void __launch_bounds__(kSweepBlock)
     k_ilu0_apply_levels(int nlev, const int *__restrict__ fptr, const int *__restrict__ frows,
                         const int *__restrict__ bptr, const int *__restrict__ brows, const int *__restrict__ rowptr,
-                        const int *__restrict__ col, const int *__restrict__ diag, const double *__restrict__ lu,
-                        const double *__restrict__ x, double *y, const KScal *ks) {
+                        const int *__restrict__ col, const int *__restrict__ diag, const T *__restrict__ lu,
+                        const T *__restrict__ x, T *y, const KScal *ks) {
     if (ks && ks->done) return;
     for (int l = 0; l < nlev; ++l) {
         for (int k = fptr[l] + threadIdx.x; k < fptr[l + 1]; k += kSweepBlock) {
             const int i = frows[k];
-            double s = x[i];
+            T s = x[i];
             for (int p = rowptr[i]; p < diag[i]; ++p) s = s - lu[p] * y[col[p]];
             y[i] = s;
         }
@@ -104,7 +106,7 @@ __global__ void __launch_bounds__(kSweepBlock)
     for (int l = 0; l < nlev; ++l) {
         for (int k = bptr[l] + threadIdx.x; k < bptr[l + 1]; k += kSweepBlock) {
             const int i = brows[k];
-            double s = y[i];
+            T s = y[i];
             for (int p = diag[i] + 1; p < rowptr[i + 1]; ++p) s = s - lu[p] * y[col[p]];
             y[i] = s / lu[diag[i]];
         }
@@ -120,9 +122,10 @@ int dilu_update(fvh_mesh *m, const double *val, double *D, double *Dinv) {
     return FVH_OK;
 }
 
-int dilu_apply(fvh_mesh *m, const double *val, const double *Dinv, const double *x, double *y, const KScal *ks) {
+template <typename T>
+static int dilu_apply_t(fvh_mesh *m, const T *val, const T *Dinv, const T *x, T *y, const KScal *ks) {
     const Schedule &s = m->sched;
-    hipLaunchKernelGGL(k_dilu_apply_levels, dim3(1), dim3(kSweepBlock), 0, stream(), s.nlev, s.fptr.p, s.frows.p, s.bptr.p,
+    hipLaunchKernelGGL((k_dilu_apply_levels<T>), dim3(1), dim3(kSweepBlock), 0, stream(), s.nlev, s.fptr.p, s.frows.p, s.bptr.p,
                        s.brows.p, m->d.rowptr, m->d.col, m->d.diag, val, Dinv, x, y, ks);
     HIP_TRY(hipGetLastError());
     return FVH_OK;
@@ -139,12 +142,26 @@ int ilu0_factor(fvh_mesh *m, const double *val, double *lu) {
     return FVH_OK;
 }
 
-int ilu0_apply(fvh_mesh *m, const double *lu, const double *x, double *y, const KScal *ks) {
+template <typename T>
+static int ilu0_apply_t(fvh_mesh *m, const T *lu, const T *x, T *y, const KScal *ks) {
     const Schedule &s = m->sched;
-    hipLaunchKernelGGL(k_ilu0_apply_levels, dim3(1), dim3(kSweepBlock), 0, stream(), s.nlev, s.fptr.p, s.frows.p, s.bptr.p,
+    hipLaunchKernelGGL((k_ilu0_apply_levels<T>), dim3(1), dim3(kSweepBlock), 0, stream(), s.nlev, s.fptr.p, s.frows.p, s.bptr.p,
                        s.brows.p, m->d.rowptr, m->d.col, m->d.diag, lu, x, y, ks);
     HIP_TRY(hipGetLastError());
     return FVH_OK;
+}
+
+int dilu_apply(fvh_mesh *m, const double *val, const double *Dinv, const double *x, double *y, const KScal *ks) {
+    return dilu_apply_t<double>(m, val, Dinv, x, y, ks);
+}
+int dilu_apply32(fvh_mesh *m, const float *val, const float *Dinv, const float *x, float *y, const KScal *ks) {
+    return dilu_apply_t<float>(m, val, Dinv, x, y, ks);
+}
+int ilu0_apply(fvh_mesh *m, const double *lu, const double *x, double *y, const KScal *ks) {
+    return ilu0_apply_t<double>(m, lu, x, y, ks);
+}
+int ilu0_apply32(fvh_mesh *m, const float *lu, const float *x, float *y, const KScal *ks) {
+    return ilu0_apply_t<float>(m, lu, x, y, ks);
 }
 
 }  // namespace fvh

@@ -122,6 +122,8 @@ SIGNATURES = {
     "fvh_precond_apply": (C.c_int, [_VP, _VP, _VP]),
     "fvh_precond_dinv": (C.c_int, [_VP, P_D]),
     "fvh_solve": (C.c_int, [_VP, C.POINTER(SolverCfg), _VP, C.POINTER(Stats)]),
+    "fvh_spmv_f32": (C.c_int, [_VP, _VP, _VP]),
+    "fvh_precond_apply_f32": (C.c_int, [_VP, _VP, _VP]),
     "fvh_sim_create": (C.c_int, [_VP, C.POINTER(Case), P_D, P_D, P_D, P_D, P_D, P_D, P_D, C.POINTER(_VP)]),
     "fvh_sim_destroy": (None, [_VP]),
     "fvh_sim_iterate": (C.c_int, [_VP, C.c_int32, P_I32]),
