@@ -239,6 +239,21 @@ int fvh_mesh_pattern(const fvh_mesh *m, int64_t *nnz, int32_t *rowptr, int32_t *
  * width nx (0 when the general level-scheduled sweep is used). */
 int fvh_mesh_sweep_info(const fvh_mesh *m, int64_t *nlevels, int64_t *band_nx);
 
+/* Independent cases (batched mode; no counterpart in the reference, which
+ * runs one case per process).  Declares the mesh as the block-diagonal union
+ * of ncases cases, case k owning cells [cell_offsets[k], cell_offsets[k+1])
+ * (0-based, no face joining two cases).  Every solve on the mesh then keeps
+ * one Krylov state per case and reduces per case, so each case computes
+ * exactly what it computes alone; the DILU sweeps run one work-group per
+ * case when every case is a full i-fastest rectangle. */
+int fvh_mesh_set_cases(fvh_mesh *m, int32_t ncases, const int64_t *cell_offsets);
+/* Number of cases (0 = not split) and whether their sweeps are banded. */
+int fvh_mesh_case_info(const fvh_mesh *m, int32_t *ncases, int32_t *banded);
+/* Autotuned launch shape of the banded DILU sweeps: grid rows per lane
+ * (1 or 2, timed on the mesh's records at its first DILU update; 0 = the
+ * level-scheduled sweeps are used).  No counterpart in the reference. */
+int fvh_mesh_tuning(const fvh_mesh *m, int32_t *rows_per_lane);
+
 /* ------------------------------------------------------------------ */
 /* Calculate  (device pointers)                                        */
 /* ------------------------------------------------------------------ */
@@ -327,6 +342,17 @@ int fvh_sim_create(fvh_mesh *m, const fvh_case *c,
                    const double *Ux, const double *Uy, const double *Uz,
                    const double *p, const double *k, const double *omega,
                    const double *nut, fvh_sim **out);
+/* simple!/piso! over the independent cases of a split mesh (laminar): one
+ * viscosity per case, the setReference! cell of each case (0-based within
+ * the case, -1 = none; the value is c->pref).  Fields are host arrays over
+ * all cells.  Each iteration advances every case; fvh_sim_iterate stops when
+ * all have reached the convergence break. */
+int fvh_sim_create_cases(fvh_mesh *m, const fvh_case *c, const double *nu_case,
+                         const int64_t *ref_cells, const double *Ux, const double *Uy,
+                         const double *Uz, const double *p, fvh_sim **out);
+/* Residual histories and Krylov statistics of case k. */
+int fvh_sim_case_residuals(fvh_sim *s, int32_t k, double *R_ux, double *R_uy, double *R_p);
+int fvh_sim_case_stats(fvh_sim *s, int32_t k, const char *field, fvh_stats *out);
 void fvh_sim_destroy(fvh_sim *s);
 /* Runs up to `n` more iterations of the loop body
  * [Solvers_1_SIMPLE.jl:125-230 / Solvers_2_PISO.jl:125-237]; stops early

@@ -48,6 +48,38 @@ def bfs_model(mesh, U=0.5, nu=1e-5):
     return model
 
 
+def channel_model(mesh, nu=0.01, U=1.0):
+    """Plane channel: uniform inlet, zero-gradient outlet with p = 0, no-slip walls
+    (patches inlet/outlet/bottom/top as the generated channel)."""
+    model = F.RANS(F.Laminar, mesh=mesh, viscosity=F.ConstantScalar(nu))
+    z = [0.0, 0.0, 0.0]
+    F.assign(model, "U", F.Dirichlet("inlet", [U, 0.0, 0.0]), F.Neumann("outlet", 0.0), F.Dirichlet("bottom", z),
+             F.Dirichlet("top", z))
+    F.assign(model, "p", F.Neumann("inlet", 0.0), F.Dirichlet("outlet", 0.0), F.Neumann("bottom", 0.0),
+             F.Neumann("top", 0.0))
+    F.initialise(model.U, [U, 0.0, 0.0])
+    F.initialise(model.p, 0.0)
+    return model
+
+
+def batch_cases(ncases=64, seed=5, lo=32, hi=512):
+    """The batched workload: side lengths exp(U(ln lo, ln hi)) rounded, Re =
+    exp(U(ln 100, ln 1000)) on a unit square with unit lid / inlet velocity
+    (nu = 1/Re); even cases are cavities (pressure reference 0 at their first
+    cell), odd cases channels.  Returns [(model, pref)]."""
+    rng = np.random.default_rng(seed)
+    sides = np.rint(np.exp(rng.uniform(np.log(lo), np.log(hi), ncases))).astype(int)
+    res = np.exp(rng.uniform(np.log(100.0), np.log(1000.0), ncases))
+    out = []
+    for k in range(ncases):
+        n, nu = int(sides[k]), 1.0 / float(res[k])
+        if k % 2 == 0:
+            out.append((cavity_model(F.cavity(n), nu=nu), 0.0))
+        else:
+            out.append((channel_model(F.channel(n, n), nu=nu), None))
+    return out
+
+
 def flatplate_model(mesh, U=10.0, nu=1.5e-5, intensity=0.01, visc_ratio=10.0):
     """k-omega flat plate (CASE_UNV_flatplate_highRe.jl:16-51): k = 1.5 (I U)^2,
     omega = k / (visc_ratio nu)."""

@@ -86,22 +86,23 @@ def spmv_f32(e, vals, x):
 
 def test_fp32_operators(monkeypatch):
     """FP32 SpMV bit for bit against its restatement; FP32 DILU apply on the
-    banded and the level-set schedules bit for bit equal to each other and
-    within FP32 rounding of the FP64 apply."""
+    banded (one and two grid rows per lane) and the level-set schedules bit
+    for bit equal to each other and within FP32 rounding of the FP64 apply."""
     rng = np.random.default_rng(11)
     out = {}
-    for sweep in ("band", "levels"):
+    for sweep in ("band1", "band2", "levels"):
         if sweep == "levels":
             monkeypatch.setenv("FVH_SWEEP", "levels")
         else:
             monkeypatch.delenv("FVH_SWEEP", raising=False)
-        mesh = F.backward_facing_step(nx=150, ny=70)
-        assert (mesh.device().sweep_info()[1] == 150) == (sweep == "band")
+            monkeypatch.setenv("FVH_BAND_ROWS", sweep[-1])
+        mesh = F.backward_facing_step(nx=150, ny=300)
+        assert (mesh.device().sweep_info()[1] == 150) == (sweep != "levels")
         e = Eqn(mesh)
         try:
             vals = random_values(e, np.random.default_rng(5))
             e.set_values(vals)
-            x = rng.standard_normal(mesh.ncells) if sweep == "band" else out["x"]
+            x = rng.standard_normal(mesh.ncells) if "x" not in out else out["x"]
             out["x"] = x
             x32 = L.DeviceArray.from_host(x.astype(np.float32))
             y32 = L.DeviceArray(mesh.ncells, np.float32)
@@ -117,7 +118,8 @@ def test_fp32_operators(monkeypatch):
             assert rel_l2(out[sweep].astype(np.float64), y64) < 1e-5
         finally:
             e.close()
-    assert np.array_equal(out["band"], out["levels"])
+    assert np.array_equal(out["band1"], out["levels"])
+    assert np.array_equal(out["band2"], out["levels"])
 
 
 @pytest.mark.parametrize("solver,pre,sym,neg", [

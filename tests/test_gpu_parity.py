@@ -153,20 +153,24 @@ def test_spmv_and_dilu_full_size_bitwise(bfs_full):
         L.lib().fvh_eqn_destroy(h)
 
 
-@pytest.mark.parametrize("sweep", ["levels", "band"])
-def test_sweep_paths_bitwise(sweep, monkeypatch):
-    """Both DILU schedules (level sets / banded anti-diagonals) against the oracle."""
+@pytest.mark.parametrize("sweep,rows", [("levels", None), ("band", "1"), ("band", "2")])
+def test_sweep_paths_bitwise(sweep, rows, monkeypatch):
+    """Both DILU schedules (level sets / banded anti-diagonals, one or two grid
+    rows per lane) against the oracle.  60 x 300: 5 waves of grid rows at one
+    row per lane, 3 at two; every wave boundary carries the exchange."""
     if sweep == "levels":
         monkeypatch.setenv("FVH_SWEEP", "levels")
     else:
         monkeypatch.delenv("FVH_SWEEP", raising=False)
-    mesh = F.backward_facing_step(nx=150, ny=70)  # 2 waves of grid rows, uneven
+    if rows:
+        monkeypatch.setenv("FVH_BAND_ROWS", rows)
+    mesh = F.backward_facing_step(nx=60, ny=300)
     nlev, band = mesh.device().sweep_info()
-    assert nlev == 150 + 70 - 1
-    assert (band == 150) == (sweep == "band")
+    assert nlev == 60 + 300 - 1
+    assert (band == 60) == (sweep == "band")
     model = bfs_model(mesh, nu=1e-3)
-    g, r = _sims(model, config(6, schemes=dict(divergence=F.Upwind)), conv_break=1e-6)
-    assert g.run(6) == r.run(6)
+    g, r = _sims(model, config(4, schemes=dict(divergence=F.Upwind)), conv_break=1e-6)
+    assert g.run(4) == r.run(4)
     for name in ("ux_dinv", "p_dinv", "Ux", "Uy", "p"):
         assert same_bits(g.get(name), r.get(name)), name
     for a, b in zip(g.residuals(), r.residuals()):

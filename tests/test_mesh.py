@@ -126,3 +126,33 @@ def test_builder_rejects_inconsistent_input():
     ev = np.array([1, 2, 2, 3, 3, 4, 4, 4, 1, 2, 3, 4])  # degenerate edge 4-4
     with pytest.raises(F.FVHError):
         F.from_elements(pts, (eptr, ev), [("all", [1, 2, 3, 4])])
+
+
+def test_union_keeps_every_case_intact():
+    """Batched mode: the union of independent meshes keeps each case's cells,
+    geometry and closed surfaces; patches merge by (family, name) with
+    boundary faces first."""
+    ms = [F.cavity(8), F.channel(6, 4), F.cavity(5)]
+    u = F.union(ms, ["a", "b", "a"])
+    assert u.ncells == sum(m.ncells for m in ms)
+    assert list(u.case_offsets) == [0, 64, 88, 113]
+    assert u.boundary_names == ["a:inlet", "a:outlet", "a:bottom", "a:top", "b:inlet", "b:outlet", "b:bottom", "b:top"]
+    assert closed_surface_defect(u) < 1e-15
+    nb = u.nbfaces
+    assert nb == sum(m.nbfaces for m in ms)
+    assert np.all(u.face_owners[:nb, 0] == u.face_owners[:nb, 1])
+    assert np.all(u.face_owners[nb:, 0] < u.face_owners[nb:, 1])
+    for k, m in enumerate(ms):
+        a, b = u.case_offsets[k], u.case_offsets[k + 1]
+        assert np.array_equal(u.cell_volume[a:b], m.cell_volume)
+        assert np.array_equal(u.cell_centre[a:b], m.cell_centre)
+        # each cell's faces: same count, ascending ids, neighbours in the case
+        rng = u.cell_faces_range[a:b]
+        assert np.array_equal(np.diff(rng, axis=1), np.diff(m.cell_faces_range, axis=1))
+        for c in range(0, b - a, 7):
+            s, e = rng[c]
+            assert np.all(np.diff(u.cell_faces[s:e + 1]) > 0)
+            nbrs = u.cell_neighbours[s:e + 1]
+            assert np.all((nbrs >= a) & (nbrs < b))
+    top_a = u.boundary("a:top")[1]
+    assert top_a.size == 8 + 5 and np.all(top_a[:8] < 64) and np.all(top_a[8:] >= 88)

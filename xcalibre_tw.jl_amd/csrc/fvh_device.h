@@ -130,12 +130,37 @@ struct BandRecs {
     Rec4f *f32, *b32;
 };
 
+// A mesh that is the block-diagonal union of independent cases (cells of
+// case k are [off[k], off[k+1]), no face joins two cases).  Reductions run
+// per case over the case's own 2048-row chunks, so each case's sums are the
+// ones it would have alone; block b of a reduction launch covers chunk
+// (blk_case[b], (blk_row0[b] - off[case]) / 2048).
+struct CaseLayout {
+    int K = 0;
+    int nblk = 0;                        // reduction blocks (chunks) over all cases
+    std::vector<int> h_off, h_cb;        // cell offsets [K+1], first block of each case [K+1]
+    DBuf<int> off, cb, cell_case, blk_case, blk_row0;
+    // per-case banded sweeps (all cases full rectangles, i-fastest)
+    bool band = false;
+    int band_R = 0, band_maxny = 0;  // rows per lane (0: not tuned yet), tallest case
+    DBuf<BandGeom> geoms;                // offj of case k points into offj_all
+    DBuf<int> offj_all, b_iperm;         // union numbering
+    DBuf<int> nxs;
+};
+
+// Device view of a CaseLayout, by value in kernel arguments.
+struct CaseView {
+    int K, nblk;
+    const int *off, *cb, *cell_case, *blk_case, *blk_row0;
+};
+
 // Sweep schedule of the lower (forward) and upper (backward) triangles.
 struct Schedule {
     int nlev = 0;
     DBuf<int> fptr, frows, bptr, brows;
     std::vector<int> h_fptr, h_bptr;
     int band_nx = 0;  // >0: A is a banded grid (deps only r-1, r-nx)
+    int band_R = 0;   // grid rows per lane of the sweeps (0: not tuned yet)
     BandGeom g{};
     DBuf<int> b_offj, b_iperm;
 };
@@ -160,6 +185,12 @@ struct fvh_mesh {
     // ILU(0) update lists: for each lower entry (i,k), pairs (slot(i,j), slot(k,j)), j > k
     bool ilu_ready = false;
     fvh::DBuf<int> ilu_uptr, ilu_pairs;
+    // independent cases (batched mode); null for a single case
+    std::unique_ptr<fvh::CaseLayout> cases;
+    fvh::CaseView case_view() const {
+        const fvh::CaseLayout &c = *cases;
+        return fvh::CaseView{c.K, c.nblk, c.off.p, c.cb.p, c.cell_case.p, c.blk_case.p, c.blk_row0.p};
+    }
 };
 
 namespace fvh {

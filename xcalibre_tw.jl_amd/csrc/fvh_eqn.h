@@ -35,6 +35,7 @@ struct AsmArgs {
     int setref;
     double pref;
     int refcell;
+    const unsigned char *refflag;  // independent cases: cells that take the reference (null: refcell)
 };
 
 // Krylov workspace, allocated on first use and kept across solves.
@@ -53,7 +54,14 @@ struct KrylovWork {
     DBuf<float> f[10];           // FP32 vectors
     DBuf<float> basis32;         // GMRES basis of the inner solves
     DBuf<float> valf;            // FP32 copy of A
+    std::vector<fvh_stats> case_stats;  // last solve, per independent case
+    std::map<std::string, hipGraphExec_t> graphs;  // captured iteration batches (independent cases)
+    void drop_graphs() {
+        for (auto &g : graphs) (void)hipGraphExecDestroy(g.second);
+        graphs.clear();
+    }
     ~KrylovWork() {
+        drop_graphs();
         if (h_ks) (void)hipHostFree(h_ks);
     }
 };
@@ -99,5 +107,17 @@ int solve(fvh_eqn *e, const fvh_solver_cfg &cfg, double *x, fvh_stats *st);
 int residual(fvh_eqn *e, const double *phi, double *res_host);
 int residual_async(fvh_eqn *e, const double *phi, double *res_dev);
 int precond_dinv_natural(fvh_eqn *e, const double **out);
+
+// shared by the single-case and the per-case (batched) solvers
+int ensure_work(fvh_eqn *e, int basis);
+int ensure_work32(fvh_eqn *e, int basis);
+int prepare32(fvh_eqn *e);
+int precond_apply32(fvh_eqn *eq, float *y, const float *x, const KScal *ks);
+int launch_spmv32(fvh_eqn *e, const float *x, float *y, float sigma, const KScal *ks, const float *w, double *part);
+int cg_sigma(fvh_eqn *e, double *sigma);
+// independent cases: every case solved with its own Krylov state; st gets
+// the largest iteration count and the worst status, e->kw.case_stats each case
+int solve_cases(fvh_eqn *e, const fvh_solver_cfg &cfg, double *x, fvh_stats *st);
+int residual_cases_async(fvh_eqn *e, const double *phi, double *out, int stride, int comp);
 
 }  // namespace fvh

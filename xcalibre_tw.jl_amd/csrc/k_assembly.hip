@@ -113,7 +113,7 @@ __global__ void k_assemble(DevMesh m, AsmArgs a, double *__restrict__ nz, double
         diag /= a.alpha;
         b += (1.0 - a.alpha) * diag * a.rfield[c];
     }
-    if (a.setref && c == a.refcell) {
+    if (a.setref && (a.refflag ? a.refflag[c] != 0 : c == a.refcell)) {
         b += diag * a.pref;
         diag += diag;
     }

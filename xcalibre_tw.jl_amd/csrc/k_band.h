@@ -8,6 +8,12 @@ struct fvh_eqn;
 namespace fvh {
 
 int band_setup(fvh_mesh *m);
+// the DILU of this mesh runs on the banded (anti-diagonal) kernels
+inline bool band_path(const fvh_mesh *m) {
+    return m->cases ? m->cases->band : m->sched.band_nx > 0;
+}
+// per-case geometry of a batch of full-rectangle cases
+int band_cases_setup(fvh_mesh *m);
 int band_alloc(fvh_eqn *e);
 int band_dilu_update(fvh_eqn *e, const double *val);
 int band_dilu_apply(fvh_eqn *e, const double *x, double *y, const KScal *ks);
@@ -15,5 +21,7 @@ int band_dilu_apply(fvh_eqn *e, const double *x, double *y, const KScal *ks);
 int band_to32(fvh_eqn *e);
 int band_dilu_apply32(fvh_eqn *e, const float *x, float *y, const KScal *ks);
 int band_dinv_natural(fvh_eqn *e, double *out);
+// grid rows per lane the sweeps use (tuned at the first update)
+int band_rows_per_lane(const fvh_mesh *m);
 
 }  // namespace fvh

@@ -20,6 +20,9 @@
 //   y_r   = (b_r - (A_rs*y_s)*Dinv_s) - (A_rw*y_w)*Dinv_w
 //   z_r   = (y_r*Dinv_r - (A_re*z_e)*Dinv_r) - (A_rn*z_n)*Dinv_r
 // so the result is bit-identical to the level-scheduled kernels.
+#include <cstdlib>
+#include <cstring>
+
 #include "fvh_eqn.h"
 #include "k_band.h"
 
@@ -102,13 +105,23 @@ __device__ inline void st_val(T *base, unsigned idx, T v) {
 // ---------------------------------------------------------------------------
 // Packing (all CUs): natural CSR values -> anti-diagonal records
 // ---------------------------------------------------------------------------
-__global__ void k_band_pack(int n, int nx, const int *__restrict__ iperm, const int *__restrict__ rowptr,
+// Row r of a grid with nx columns whose cells are [o, o + nk) (o = 0, nk = n
+// for a single mesh; one case of a batch otherwise).
+__global__ void k_band_pack(int n, int nx1, const int *__restrict__ cell_case, const int *__restrict__ off,
+                            const int *__restrict__ nxs, const int *__restrict__ iperm, const int *__restrict__ rowptr,
                             const int *__restrict__ diag, const double *__restrict__ val, BandRecs br) {
     int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= n) return;
-    const int r = iperm[p];
-    const int i = r % nx;
-    const bool S = r >= nx, W = i > 0, E = i < nx - 1, N = r + nx < n;
+    int o = 0, nk = n, nx = nx1;
+    if (cell_case) {
+        const int k = cell_case[p];  // packed positions of a case cover its own cell range
+        o = off[k];
+        nk = off[k + 1] - o;
+        nx = nxs[k];
+    }
+    const int r = iperm[p], rl = r - o;
+    const int i = rl % nx;
+    const bool S = rl >= nx, W = i > 0, E = i < nx - 1, N = rl + nx < nk;
     const int d = diag[r];
     Rec6 u;
     u.a = val[d];
@@ -184,109 +197,172 @@ struct Walk {
     // packed position of thread t, clamped into [0, n): every load is in
     // bounds; the value read by a thread off the diagonal is never used
     __device__ unsigned pos(int t, int n) const { return min((unsigned)(offj + t), (unsigned)(n - 1)); }
+    // first of the R adjacent records of rows t .. t+R-1, clamped so that all
+    // R are in bounds (R <= n)
+    template <int R>
+    __device__ unsigned pos_block(int t, int n) const {
+        return min((unsigned)(offj + t), (unsigned)(n - R));
+    }
 };
 
 // ---------------------------------------------------------------------------
 // Diagonal recurrence (update_dilu_diagonal!)
+//
+// Lane t owns the R consecutive grid rows j = R t + q.  At step T row j
+// handles cell (T - j, j); its south value (T - j, j - 1) is row j-1's result
+// of step T-1: the lane's own register for q > 0, lane t-1's last row (DPP /
+// LDS) for q = 0.  Its west value is its own result of step T-1.
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(1024) k_band_dilu_update(BandGeom g, BandRecs br) {
+template <int R>
+__device__ inline void band_update_body(const BandGeom &g, const Rec6 *__restrict__ u, Rec4 *__restrict__ fr,
+                                        Rec4 *__restrict__ br_) {
     __shared__ double sh_d[16][2];
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
     const int wl = wave > 0 ? wave - 1 : 0;
     const int n = g.nx * g.ny;
-    const bool row = t < g.ny, S = t > 0;
-    Rec6 buf[kPD];
+    const int j0 = t * R;
+    Rec6 buf[kPD][R];
     Walk cw, pw;
     cw.init(g, 0);
     pw.init(g, 0);
 #pragma unroll
     for (int k = 0; k < kPD; ++k) {
-        buf[k] = ldrec(br.u, pw.pos(t, n));
+        {
+            const unsigned pb = pw.pos_block<R>(j0, n);
+#pragma unroll
+            for (int q = 0; q < R; ++q) buf[k][q] = ldrec(u, pb + q);
+        }
         __builtin_amdgcn_sched_barrier(0);  // issue in consumption order
         pw.next(g);
     }
-    double dprev = 0.0;  // D of (i-1, j)
+    double dprev[R];  // D of (i-1, j) per row
+#pragma unroll
+    for (int q = 0; q < R; ++q) dprev[q] = 0.0;
     for (int T0 = 0; T0 < g.nT; T0 += kPD) {
 #pragma unroll
         for (int k = 0; k < kPD; ++k) {
-            const Rec6 c = buf[k];  // ad, ls, ts, lw, tw
-            const double ds = dpp_shr1(dprev, sh_d[wl][(cw.T + 1) & 1]);
-            const int i = cw.T - t;
-            double d = c.a;
-            const double d1 = d - c.b * c.c / ds;
-            d = S ? d1 : d;
-            const double d2 = d - c.d * c.e / dprev;
-            d = (i > 0) ? d2 : d;
-            if (row && (unsigned)i < (unsigned)g.nx) {
-                const double r = 1.0 / d;
-                const unsigned p = cw.offj + t;
-                br.f[p].b = r;
-                br.b[p].b = r;
+            const double ds0 = dpp_shr1(dprev[R - 1], sh_d[wl][(cw.T + 1) & 1]);
+            double dn[R];
+#pragma unroll
+            for (int q = 0; q < R; ++q) {
+                const Rec6 c = buf[k][q];  // ad, ls, ts, lw, tw
+                const int j = j0 + q, i = cw.T - j;
+                const double ds = q == 0 ? ds0 : dprev[q - 1];
+                double d = c.a;
+                const double d1 = d - c.b * c.c / ds;
+                d = (j > 0) ? d1 : d;
+                const double d2 = d - c.d * c.e / dprev[q];
+                d = (i > 0) ? d2 : d;
+                if (j < g.ny && (unsigned)i < (unsigned)g.nx) {
+                    const double r = 1.0 / d;
+                    const unsigned p = cw.offj + j;
+                    fr[p].b = r;
+                    br_[p].b = r;
+                }
+                dn[q] = d;
             }
-            dprev = d;
-            if (lane == 63) sh_d[wave][cw.T & 1] = dprev;
+#pragma unroll
+            for (int q = 0; q < R; ++q) dprev[q] = dn[q];
+            if (lane == 63) sh_d[wave][cw.T & 1] = dprev[R - 1];
             lds_barrier();
-            // the slot is consumed above: keep its refill below it
+            // the slots are consumed above: keep their refill below them
             __builtin_amdgcn_sched_barrier(0);
-            buf[k] = ldrec(br.u, pw.pos(t, n));
+            {
+                const unsigned pb = pw.pos_block<R>(j0, n);
+#pragma unroll
+                for (int q = 0; q < R; ++q) buf[k][q] = ldrec(u, pb + q);
+            }
             pw.next(g);
             cw.next(g);
         }
     }
 }
 
+template <int R>
+__global__ void __launch_bounds__(1024 / R) k_band_dilu_update(BandGeom g, BandRecs br) {
+    band_update_body<R>(g, br.u, br.f, br.b);
+}
+
+// one work-group per case of a batch
+template <int R>
+__global__ void __launch_bounds__(1024 / R) k_band_dilu_update_cases(const BandGeom *__restrict__ gs,
+                                                                     const int *__restrict__ off, BandRecs br) {
+    const int k = blockIdx.x, o = off[k];
+    band_update_body<R>(gs[k], br.u + o, br.f + o, br.b + o);
+}
+
 // ---------------------------------------------------------------------------
 // ldiv!: forward then backward sweep over the packed records, in FP64 (the
-// reference's arithmetic) or FP32 (the inner solves of mixed precision)
+// reference's arithmetic) or FP32 (the inner solves of mixed precision); the
+// row ownership is that of the diagonal recurrence.
 // ---------------------------------------------------------------------------
-template <typename T>
-__global__ void __launch_bounds__(1024) k_band_dilu_apply(BandGeom g, const Rec4T<T> *__restrict__ f, Rec4T<T> *bk,
-                                                          T *__restrict__ zp, const KScal *ks) {
-    if (ks && ks->done) return;
+template <typename T, int R>
+__device__ inline void band_apply_body(const BandGeom &g, const Rec4T<T> *__restrict__ f, Rec4T<T> *bk,
+                                       T *__restrict__ zp) {
     // [wave][step parity][value | Dinv] of the wave's edge lane
     __shared__ T sh[16][2][2];
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
     const int n = g.nx * g.ny;
-    const bool row = t < g.ny, S = t > 0, N = t < g.ny - 1;
+    const int j0 = t * R;
     const int nwaves = (blockDim.x + 63) >> 6;
     const int wl = wave > 0 ? wave - 1 : 0, wr = wave + 1 < nwaves ? wave + 1 : wave;
     // ---- forward: y_r = (b_r - (A_rs y_s) Dinv_s) - (A_rw y_w) Dinv_w ----
     {
-        Rec4T<T> buf[kPD];  // b, dinv, A_rs, A_rw
+        Rec4T<T> buf[kPD][R];  // b, dinv, A_rs, A_rw
         Walk cw, pw;
         cw.init(g, 0);
         pw.init(g, 0);
 #pragma unroll
         for (int k = 0; k < kPD; ++k) {
-            buf[k] = ldrec(f, pw.pos(t, n));
+            {
+                const unsigned pb = pw.pos_block<R>(j0, n);
+#pragma unroll
+                for (int q = 0; q < R; ++q) buf[k][q] = ldrec(f, pb + q);
+            }
             __builtin_amdgcn_sched_barrier(0);  // issue in consumption order
             pw.next(g);
         }
-        T xprev = 0, dprev = 0;
+        T xprev[R], dprev[R];
+#pragma unroll
+        for (int q = 0; q < R; ++q) xprev[q] = dprev[q] = 0;
         for (int T0 = 0; T0 < g.nT; T0 += kPD) {
 #pragma unroll
             for (int k = 0; k < kPD; ++k) {
-                const Rec4T<T> c = buf[k];
                 const int par = (cw.T + 1) & 1;
-                const T xs = dpp_shr1(xprev, sh[wl][par][0]);
-                const T ds = dpp_shr1(dprev, sh[wl][par][1]);
-                const int i = cw.T - t;
-                T x = c.a;
-                const T x1 = x - c.c * xs * ds;
-                x = S ? x1 : x;
-                const T x2 = x - c.d * xprev * dprev;
-                x = (i > 0) ? x2 : x;
-                if (row && (unsigned)i < (unsigned)g.nx) st_a(bk, cw.offj + t, x);
-                xprev = x;
-                dprev = keep(c.b);
+                const T xs0 = dpp_shr1(xprev[R - 1], sh[wl][par][0]);
+                const T ds0 = dpp_shr1(dprev[R - 1], sh[wl][par][1]);
+                T xn[R];
+#pragma unroll
+                for (int q = 0; q < R; ++q) {
+                    const Rec4T<T> c = buf[k][q];
+                    const int j = j0 + q, i = cw.T - j;
+                    const T xs = q == 0 ? xs0 : xprev[q - 1];
+                    const T ds = q == 0 ? ds0 : dprev[q - 1];
+                    T x = c.a;
+                    const T x1 = x - c.c * xs * ds;
+                    x = (j > 0) ? x1 : x;
+                    const T x2 = x - c.d * xprev[q] * dprev[q];
+                    x = (i > 0) ? x2 : x;
+                    if (j < g.ny && (unsigned)i < (unsigned)g.nx) st_a(bk, cw.offj + j, x);
+                    xn[q] = x;
+                }
+#pragma unroll
+                for (int q = 0; q < R; ++q) {
+                    xprev[q] = xn[q];
+                    dprev[q] = keep(buf[k][q].b);
+                }
                 if (lane == 63) {
-                    sh[wave][cw.T & 1][0] = xprev;
-                    sh[wave][cw.T & 1][1] = dprev;
+                    sh[wave][cw.T & 1][0] = xprev[R - 1];
+                    sh[wave][cw.T & 1][1] = dprev[R - 1];
                 }
                 lds_barrier();
-                // the slot is consumed above: keep its refill below it
+                // the slots are consumed above: keep their refill below them
                 __builtin_amdgcn_sched_barrier(0);
-                buf[k] = ldrec(f, pw.pos(t, n));
+                {
+                    const unsigned pb = pw.pos_block<R>(j0, n);
+#pragma unroll
+                    for (int q = 0; q < R; ++q) buf[k][q] = ldrec(f, pb + q);
+                }
                 pw.next(g);
                 cw.next(g);
             }
@@ -295,39 +371,76 @@ __global__ void __launch_bounds__(1024) k_band_dilu_apply(BandGeom g, const Rec4
     __syncthreads();  // forward stores complete before the backward loads
     // ---- backward: z_r = (y_r Dinv_r - (A_re z_e) Dinv_r) - (A_rn z_n) Dinv_r ----
     {
-        Rec4T<T> buf[kPD];  // y, dinv, A_re, A_rn
+        Rec4T<T> buf[kPD][R];  // y, dinv, A_re, A_rn
         Walk cw, pw;
         cw.init(g, g.nT - 1);
         pw.init(g, g.nT - 1);
 #pragma unroll
         for (int k = 0; k < kPD; ++k) {
-            buf[k] = ldrec((const Rec4T<T> *)bk, pw.pos(t, n));
+            {
+                const unsigned pb = pw.pos_block<R>(j0, n);
+#pragma unroll
+                for (int q = 0; q < R; ++q) buf[k][q] = ldrec((const Rec4T<T> *)bk, pb + q);
+            }
             __builtin_amdgcn_sched_barrier(0);  // issue in consumption order
             pw.prev(g);
         }
-        T zprev = 0;
+        T zprev[R];  // z of (i+1, j) per row
+#pragma unroll
+        for (int q = 0; q < R; ++q) zprev[q] = 0;
         for (int T0 = g.nT - 1; T0 >= 0; T0 -= kPD) {
 #pragma unroll
             for (int k = 0; k < kPD; ++k) {
-                const Rec4T<T> c = buf[k];
-                const T zn = dpp_shl1(zprev, sh[wr][(cw.T + 1) & 1][0]);
-                const int i = cw.T - t;
-                T z = c.a * c.b;
-                const T z1 = z - c.c * zprev * c.b;
-                z = (i < g.nx - 1) ? z1 : z;
-                const T z2 = z - c.d * zn * c.b;
-                z = N ? z2 : z;
-                if (row && cw.T >= 0 && (unsigned)i < (unsigned)g.nx) st_val(zp, cw.offj + t, z);
-                zprev = z;
-                if (lane == 0) sh[wave][cw.T & 1][0] = zprev;
+                // north of the lane's last row: lane t+1's first row, step T+1
+                const T znl = dpp_shl1(zprev[0], sh[wr][(cw.T + 1) & 1][0]);
+                T zq[R];
+#pragma unroll
+                for (int q = 0; q < R; ++q) {
+                    const Rec4T<T> c = buf[k][q];
+                    const int j = j0 + q, i = cw.T - j;
+                    const T zn = q == R - 1 ? znl : zprev[q + 1];
+                    T z = c.a * c.b;
+                    const T z1 = z - c.c * zprev[q] * c.b;
+                    z = (i < g.nx - 1) ? z1 : z;
+                    const T z2 = z - c.d * zn * c.b;
+                    z = (j < g.ny - 1) ? z2 : z;
+                    if (j < g.ny && cw.T >= 0 && (unsigned)i < (unsigned)g.nx) st_val(zp, cw.offj + j, z);
+                    zq[q] = z;
+                }
+#pragma unroll
+                for (int q = 0; q < R; ++q) zprev[q] = zq[q];
+                if (lane == 0) sh[wave][cw.T & 1][0] = zprev[0];
                 lds_barrier();
                 __builtin_amdgcn_sched_barrier(0);
-                buf[k] = ldrec((const Rec4T<T> *)bk, pw.pos(t, n));
+                {
+                    const unsigned pb = pw.pos_block<R>(j0, n);
+#pragma unroll
+                    for (int q = 0; q < R; ++q) buf[k][q] = ldrec((const Rec4T<T> *)bk, pb + q);
+                }
                 pw.prev(g);
                 cw.prev(g);
             }
         }
     }
+}
+
+template <typename T, int R>
+__global__ void __launch_bounds__(1024 / R) k_band_dilu_apply(BandGeom g, const Rec4T<T> *__restrict__ f,
+                                                              Rec4T<T> *bk, T *__restrict__ zp, const KScal *ks) {
+    if (ks && ks->done) return;
+    band_apply_body<T, R>(g, f, bk, zp);
+}
+
+// one work-group per case of a batch; a case whose solve has finished skips
+template <typename T, int R>
+__global__ void __launch_bounds__(1024 / R) k_band_dilu_apply_cases(const BandGeom *__restrict__ gs,
+                                                                    const int *__restrict__ off,
+                                                                    const Rec4T<T> *__restrict__ f, Rec4T<T> *bk,
+                                                                    T *__restrict__ zp, const KScal *ks) {
+    const int k = blockIdx.x;
+    if (ks && ks[k].done) return;
+    const int o = off[k];
+    band_apply_body<T, R>(gs[k], f + o, bk + o, zp + o);
 }
 
 // ---------------------------------------------------------------------------
@@ -336,45 +449,102 @@ __global__ void __launch_bounds__(1024) k_band_dilu_apply(BandGeom g, const Rec4
 // Anti-diagonal geometry of a banded mesh: offsets and packed row order.
 // The fast path needs the full rectangle pattern (every row r = i + j*nx has
 // exactly the neighbours r-nx, r-1, r+1, r+nx that exist on the grid), so the
-// neighbour flags follow from (i, j).
-int band_setup(fvh_mesh *m) {
-    Schedule &s = m->sched;
-    const int nx = s.band_nx;
-    if (nx <= 0) return FVH_OK;
-    const int n = (int)m->n, ny = n / nx;
-    const auto &rp = m->h_rowptr, &col = m->h_col;
-    bool full = ny <= 1024 && ny >= 1;
-    for (int r = 0; r < n && full; ++r) {
+// neighbour flags follow from (i, j).  Rows [c0, c0 + n) of the CSR pattern.
+static bool full_rect(const std::vector<int> &rp, const std::vector<int> &col, int c0, int n, int nx) {
+    if (nx <= 0 || n % nx != 0) return false;
+    const int ny = n / nx;
+    if (ny < 1 || ny > 1024) return false;
+    int want[5];
+    for (int r = 0; r < n; ++r) {
         const int i = r % nx, j = r / nx;
-        std::vector<int> want;
-        if (j > 0) want.push_back(r - nx);
-        if (i > 0) want.push_back(r - 1);
-        want.push_back(r);
-        if (i < nx - 1) want.push_back(r + 1);
-        if (j < ny - 1) want.push_back(r + nx);
-        if ((int)want.size() != rp[r + 1] - rp[r]) { full = false; break; }
-        for (size_t q = 0; q < want.size(); ++q)
-            if (col[rp[r] + q] != want[q]) { full = false; break; }
+        int w = 0;
+        if (j > 0) want[w++] = r - nx;
+        if (i > 0) want[w++] = r - 1;
+        want[w++] = r;
+        if (i < nx - 1) want[w++] = r + 1;
+        if (j < ny - 1) want[w++] = r + nx;
+        const int a = rp[c0 + r], e = rp[c0 + r + 1];
+        if (w != e - a) return false;
+        for (int q = 0; q < w; ++q)
+            if (col[a + q] - c0 != want[q]) return false;
     }
-    // records are addressed by 32-bit byte offsets (48-byte records at most)
-    if (!full || (int64_t)n * (int64_t)sizeof(Rec6) >= ((int64_t)1 << 32)) {
-        s.band_nx = 0;
-        return FVH_OK;
-    }
+    return true;
+}
+
+// offj[T] = first packed position of diagonal T minus its first row jmin;
+// iperm[p] = row (local) at packed position p.
+static void anti_diagonals(int nx, int ny, std::vector<int> &offj, std::vector<int> &iperm) {
     const int nT = nx + ny - 1;
-    std::vector<int> offj(nT), iperm(n);
+    offj.assign(nT, 0);
+    iperm.assign((size_t)nx * ny, 0);
     int p = 0;
     for (int T = 0; T < nT; ++T) {
         int jmin = std::max(0, T - nx + 1), jmax = std::min(ny - 1, T);
         offj[T] = p - jmin;
         for (int j = jmin; j <= jmax; ++j, ++p) iperm[p] = (T - j) + j * nx;
     }
+}
+
+int band_setup(fvh_mesh *m) {
+    Schedule &s = m->sched;
+    const int nx = s.band_nx;
+    if (nx <= 0) return FVH_OK;
+    const int n = (int)m->n, ny = n / nx;
+    // records are addressed by 32-bit byte offsets (48-byte records at most)
+    if (!full_rect(m->h_rowptr, m->h_col, 0, n, nx) || (int64_t)n * (int64_t)sizeof(Rec6) >= ((int64_t)1 << 32)) {
+        s.band_nx = 0;
+        return FVH_OK;
+    }
+    std::vector<int> offj, iperm;
+    anti_diagonals(nx, ny, offj, iperm);
     FVH_TRY(s.b_offj.upload(offj));
     FVH_TRY(s.b_iperm.upload(iperm));
     s.g.nx = nx;
     s.g.ny = ny;
-    s.g.nT = nT;
+    s.g.nT = nx + ny - 1;
     s.g.offj = s.b_offj.p;
+    return FVH_OK;
+}
+
+
+// Per-case geometry of a batch whose every case is a full rectangle; leaves
+// cases->band false otherwise (the level-scheduled sweeps then cover the
+// union, one case after another in the same levels).
+int band_cases_setup(fvh_mesh *m) {
+    CaseLayout &L = *m->cases;
+    L.band = false;
+    const char *force = std::getenv("FVH_SWEEP");
+    if (force && std::strcmp(force, "levels") == 0) return FVH_OK;
+    if ((int64_t)m->n * (int64_t)sizeof(Rec6) >= ((int64_t)1 << 32)) return FVH_OK;
+    const auto &rp = m->h_rowptr, &col = m->h_col;
+    std::vector<BandGeom> geoms(L.K);
+    std::vector<int> nxs(L.K), offj_all, iperm_all(m->n), toff(L.K);
+    int maxny = 1;
+    for (int k = 0; k < L.K; ++k) {
+        const int c0 = L.h_off[k], nk = L.h_off[k + 1] - c0;
+        int nx = 0;
+        for (int r = 0; r < nk && nx == 0; ++r)
+            for (int q = rp[c0 + r]; q < m->h_diag[c0 + r]; ++q)
+                if (r - (col[q] - c0) != 1) { nx = r - (col[q] - c0); break; }
+        if (nx == 0) nx = nk;  // a single grid row
+        if (!full_rect(rp, col, c0, nk, nx)) return FVH_OK;
+        std::vector<int> offj, iperm;
+        anti_diagonals(nx, nk / nx, offj, iperm);
+        toff[k] = (int)offj_all.size();
+        offj_all.insert(offj_all.end(), offj.begin(), offj.end());
+        for (int q = 0; q < nk; ++q) iperm_all[c0 + q] = c0 + iperm[q];
+        geoms[k] = BandGeom{nx, nk / nx, nx + nk / nx - 1, nullptr};
+        nxs[k] = nx;
+        maxny = std::max(maxny, nk / nx);
+    }
+    FVH_TRY(L.offj_all.upload(offj_all));
+    for (int k = 0; k < L.K; ++k) geoms[k].offj = L.offj_all.p + toff[k];
+    FVH_TRY(L.geoms.upload(geoms));
+    FVH_TRY(L.b_iperm.upload(iperm_all));
+    FVH_TRY(L.nxs.upload(nxs));
+    L.band_R = 0;
+    L.band_maxny = maxny;
+    L.band = true;
     return FVH_OK;
 }
 
@@ -391,20 +561,138 @@ int band_alloc(fvh_eqn *e) {
     return FVH_OK;
 }
 
-static int threads_for(int ny) { return ((ny + 63) / 64) * 64; }
+static int threads_for(int ny, int R) { return (((ny + R - 1) / R + 63) / 64) * 64; }
+
+// Grid rows per lane of the sweeps: 1 or 2, chosen per mesh by timing both
+// on the mesh's own records at its first update (FVH_BAND_ROWS=1|2 forces
+// one).  Two rows per lane let four waves, one per SIMD, cover a tall grid
+// and share the per-step overhead (addressing, exchange, barrier); one row
+// per lane keeps the step shorter when the grid is low.
+static int forced_rows() {
+    const char *env = std::getenv("FVH_BAND_ROWS");
+    if (env && (env[0] == '1' || env[0] == '2') && env[1] == 0) return env[0] - '0';
+    return 0;
+}
+
+struct BandLaunch {
+    bool cases;
+    int R, threads, blocks;
+};
+
+static BandLaunch band_launch(const fvh_mesh *m, int R) {
+    if (m->cases && m->cases->band)
+        return BandLaunch{true, R, threads_for(m->cases->band_maxny, R), m->cases->K};
+    return BandLaunch{false, R, threads_for(m->sched.g.ny, R), 1};
+}
+
+static int band_rows(const fvh_mesh *m) {
+    const int f = forced_rows();
+    if (f) return f;
+    const int t = (m->cases && m->cases->band) ? m->cases->band_R : m->sched.band_R;
+    if (t) return t;
+    const int ny = (m->cases && m->cases->band) ? m->cases->band_maxny : m->sched.g.ny;
+    return ny > 256 ? 2 : 1;  // before tuning
+}
+
+template <typename T>
+static int launch_sweep(const fvh_mesh *m, int R, const Rec4T<T> *f, Rec4T<T> *bk, T *zp, const KScal *ks) {
+    const BandLaunch b = band_launch(m, R);
+    if (b.cases) {
+        const CaseLayout &L = *m->cases;
+        if (R == 2)
+            hipLaunchKernelGGL((k_band_dilu_apply_cases<T, 2>), dim3(b.blocks), dim3(b.threads), 0, stream(), L.geoms.p,
+                               L.off.p, f, bk, zp, ks);
+        else
+            hipLaunchKernelGGL((k_band_dilu_apply_cases<T, 1>), dim3(b.blocks), dim3(b.threads), 0, stream(), L.geoms.p,
+                               L.off.p, f, bk, zp, ks);
+    } else {
+        const BandGeom &g = m->sched.g;
+        if (R == 2)
+            hipLaunchKernelGGL((k_band_dilu_apply<T, 2>), dim3(1), dim3(b.threads), 0, stream(), g, f, bk, zp, ks);
+        else
+            hipLaunchKernelGGL((k_band_dilu_apply<T, 1>), dim3(1), dim3(b.threads), 0, stream(), g, f, bk, zp, ks);
+    }
+    HIP_TRY(hipGetLastError());
+    return FVH_OK;
+}
+
+static int launch_update(const fvh_mesh *m, int R, BandRecs r) {
+    const BandLaunch b = band_launch(m, R);
+    if (b.cases) {
+        const CaseLayout &L = *m->cases;
+        if (R == 2)
+            hipLaunchKernelGGL(k_band_dilu_update_cases<2>, dim3(b.blocks), dim3(b.threads), 0, stream(), L.geoms.p,
+                               L.off.p, r);
+        else
+            hipLaunchKernelGGL(k_band_dilu_update_cases<1>, dim3(b.blocks), dim3(b.threads), 0, stream(), L.geoms.p,
+                               L.off.p, r);
+    } else {
+        if (R == 2)
+            hipLaunchKernelGGL(k_band_dilu_update<2>, dim3(1), dim3(b.threads), 0, stream(), m->sched.g, r);
+        else
+            hipLaunchKernelGGL(k_band_dilu_update<1>, dim3(1), dim3(b.threads), 0, stream(), m->sched.g, r);
+    }
+    HIP_TRY(hipGetLastError());
+    return FVH_OK;
+}
+
+// Times the FP64 sweep at 1 and 2 rows per lane on the freshly packed
+// records (the launches only rewrite the sweep's own scratch fields) and
+// keeps the faster.
+static int tune_rows(fvh_eqn *e) {
+    fvh_mesh *m = e->m;
+    hipEvent_t a, b;
+    HIP_TRY(hipEventCreate(&a));
+    HIP_TRY(hipEventCreate(&b));
+    float best = 0.f;
+    int bestR = 1;
+    for (int R = 1; R <= 2; ++R) {
+        FVH_TRY(launch_sweep<double>(m, R, e->bd.f.p, e->bd.b.p, e->bd.zp.p, nullptr));  // warm
+        HIP_TRY(hipEventRecord(a, stream()));
+        for (int rep = 0; rep < 3; ++rep)
+            FVH_TRY(launch_sweep<double>(m, R, e->bd.f.p, e->bd.b.p, e->bd.zp.p, nullptr));
+        HIP_TRY(hipEventRecord(b, stream()));
+        HIP_TRY(hipEventSynchronize(b));
+        float t = 0.f;
+        HIP_TRY(hipEventElapsedTime(&t, a, b));
+        if (R == 1 || t < best) {
+            best = t;
+            bestR = R;
+        }
+    }
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    if (m->cases && m->cases->band) m->cases->band_R = bestR;
+    else m->sched.band_R = bestR;
+    return FVH_OK;
+}
+
+static const int *band_iperm(const fvh_mesh *m) {
+    return (m->cases && m->cases->band) ? m->cases->b_iperm.p : m->sched.b_iperm.p;
+}
 
 int band_dilu_update(fvh_eqn *e, const double *val) {
     fvh_mesh *m = e->m;
     const Schedule &s = m->sched;
     FVH_TRY(band_alloc(e));
     const int n = (int)m->n;
-    hipLaunchKernelGGL(k_band_pack, grid1d(n), dim3(256), 0, stream(), n, s.g.nx, s.b_iperm.p, m->d.rowptr, m->d.diag,
-                       val, e->bd.r);
+    if (m->cases && m->cases->band) {
+        const CaseLayout &L = *m->cases;
+        hipLaunchKernelGGL(k_band_pack, grid1d(n), dim3(256), 0, stream(), n, 0, L.cell_case.p, L.off.p, L.nxs.p,
+                           L.b_iperm.p, m->d.rowptr, m->d.diag, val, e->bd.r);
+    } else {
+        hipLaunchKernelGGL(k_band_pack, grid1d(n), dim3(256), 0, stream(), n, s.g.nx, (const int *)nullptr,
+                           (const int *)nullptr, (const int *)nullptr, s.b_iperm.p, m->d.rowptr, m->d.diag, val,
+                           e->bd.r);
+    }
     HIP_TRY(hipGetLastError());
-    hipLaunchKernelGGL(k_band_dilu_update, dim3(1), dim3(threads_for(s.g.ny)), 0, stream(), s.g, e->bd.r);
-    HIP_TRY(hipGetLastError());
+    FVH_TRY(launch_update(m, band_rows(m), e->bd.r));
+    const bool tuned = (m->cases && m->cases->band) ? m->cases->band_R != 0 : m->sched.band_R != 0;
+    if (!tuned && !forced_rows()) FVH_TRY(tune_rows(e));
     return FVH_OK;
 }
+
+int band_rows_per_lane(const fvh_mesh *m) { return band_rows(m); }
 
 int band_to32(fvh_eqn *e) {
     const int n = (int)e->m->n;
@@ -424,13 +712,15 @@ int band_to32(fvh_eqn *e) {
 template <typename T>
 static int band_apply_t(fvh_eqn *e, const T *x, T *y, const KScal *ks, Rec4T<T> *f, Rec4T<T> *bk, T *zp) {
     fvh_mesh *m = e->m;
-    const Schedule &s = m->sched;
     const int n = (int)m->n;
-    hipLaunchKernelGGL((k_band_gather<T>), grid1d(n), dim3(256), 0, stream(), n, s.b_iperm.p, x, f, ks);
+    const bool cases = m->cases && m->cases->band;
+    // per-case solves: the gather / scatter cover every case (a finished
+    // case's rows are not read again), the sweep skips finished cases
+    const KScal *ks_all = cases ? nullptr : ks;
+    hipLaunchKernelGGL((k_band_gather<T>), grid1d(n), dim3(256), 0, stream(), n, band_iperm(m), x, f, ks_all);
     HIP_TRY(hipGetLastError());
-    hipLaunchKernelGGL((k_band_dilu_apply<T>), dim3(1), dim3(threads_for(s.g.ny)), 0, stream(), s.g, f, bk, zp, ks);
-    HIP_TRY(hipGetLastError());
-    hipLaunchKernelGGL((k_band_scatter<T>), grid1d(n), dim3(256), 0, stream(), n, s.b_iperm.p, zp, y, ks);
+    FVH_TRY(launch_sweep<T>(m, band_rows(m), f, bk, zp, ks));
+    hipLaunchKernelGGL((k_band_scatter<T>), grid1d(n), dim3(256), 0, stream(), n, band_iperm(m), zp, y, ks_all);
     HIP_TRY(hipGetLastError());
     return FVH_OK;
 }
@@ -448,9 +738,15 @@ int band_dilu_apply32(fvh_eqn *e, const float *x, float *y, const KScal *ks) {
 int band_dinv_natural(fvh_eqn *e, double *out) {
     fvh_mesh *m = e->m;
     const int n = (int)m->n;
-    hipLaunchKernelGGL(k_band_dinv_natural, grid1d(n), dim3(256), 0, stream(), n, m->sched.b_iperm.p, e->bd.r.f, out);
+    hipLaunchKernelGGL(k_band_dinv_natural, grid1d(n), dim3(256), 0, stream(), n, band_iperm(m), e->bd.r.f, out);
     HIP_TRY(hipGetLastError());
     return FVH_OK;
 }
 
 }  // namespace fvh
+
+extern "C" int fvh_mesh_tuning(const fvh_mesh *m, int32_t *rows_per_lane) {
+    if (!m || !rows_per_lane) return fvh::set_error(FVH_ERR_ARG, "null argument");
+    *rows_per_lane = fvh::band_path(m) ? fvh::band_rows_per_lane(m) : 0;
+    return FVH_OK;
+}

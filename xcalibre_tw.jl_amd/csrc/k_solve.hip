@@ -18,100 +18,10 @@
 #include "fvh_eqn.h"
 #include "k_band.h"
 #include "k_sweep.h"
+#include "k_spmv.h"
 #include "k_util.h"
 
 namespace fvh {
-
-// ---------------------------------------------------------------------------
-// SpMV  y = sigma * A x  (optionally with the partial sums of w.y)
-// ---------------------------------------------------------------------------
-// Row i of A x, summed from the first entry in ascending column order.
-template <typename V>
-__device__ inline V row_ax(const int *__restrict__ rowptr, const int *__restrict__ col, const V *__restrict__ val,
-                           const V *__restrict__ x, int i) {
-    V s = 0;
-    const int p1 = rowptr[i + 1];
-    for (int p = rowptr[i]; p < p1; ++p) s = s + val[p] * x[col[p]];
-    return s;
-}
-
-// One thread per row.
-template <typename V>
-__global__ void __launch_bounds__(256) k_spmv(int n, const int *__restrict__ rowptr, const int *__restrict__ col,
-                                              const V *__restrict__ val, const V *__restrict__ x, V *__restrict__ y,
-                                              V sigma, const KScal *ks) {
-    if (ks && ks->done) return;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) y[i] = sigma * row_ax(rowptr, col, val, x, i);
-}
-
-// Canonical partial sums of one 2048-row chunk from per-row terms staged in
-// LDS (p[v][j] = term of row chunk_base + j, 0 for rows past n): lane t < 256
-// adds rows t + 256 k in ascending k, then the same tree as chunk_reduce.
-// All kWide threads of the block call it.
-constexpr int kWide = 1024;
-template <int NV>
-__device__ inline void chunk_reduce_staged(double (&p)[NV][kChunk], double *__restrict__ part, int nch) {
-    const int t = threadIdx.x;
-    __syncthreads();
-    double acc[NV];
-#pragma unroll
-    for (int v = 0; v < NV; ++v) acc[v] = 0.0;
-    if (t < kBlock) {
-#pragma unroll
-        for (int k = 0; k < kPer; ++k)
-#pragma unroll
-            for (int v = 0; v < NV; ++v) acc[v] = acc[v] + p[v][t + k * kBlock];
-    }
-    __syncthreads();
-    if (t < kBlock) {
-#pragma unroll
-        for (int v = 0; v < NV; ++v) p[v][t] = acc[v];
-    }
-    __syncthreads();
-    if (t < 128) {
-#pragma unroll
-        for (int v = 0; v < NV; ++v) p[v][t] = p[v][t] + p[v][t + 128];
-    }
-    __syncthreads();
-    if (t < 64) {
-#pragma unroll
-        for (int v = 0; v < NV; ++v) {
-            double a = p[v][t] + p[v][t + 64];
-            a = a + __shfl_down(a, 32);
-            a = a + __shfl_down(a, 16);
-            a = a + __shfl_down(a, 8);
-            a = a + __shfl_down(a, 4);
-            a = a + __shfl_down(a, 2);
-            a = a + __shfl_down(a, 1);
-            if (t == 0) part[v * nch + blockIdx.x] = a;
-        }
-    }
-}
-
-// y = sigma A x and the partial sums of w.y: one block of kWide threads per
-// chunk, two rows per thread.
-template <typename V>
-__global__ void __launch_bounds__(kWide) k_spmv_dot(int n, const int *__restrict__ rowptr, const int *__restrict__ col,
-                                                    const V *__restrict__ val, const V *__restrict__ x,
-                                                    V *__restrict__ y, V sigma, const KScal *ks,
-                                                    const V *__restrict__ w, double *part, int nch) {
-    if (ks && ks->done) return;
-    __shared__ double p[1][kChunk];
-    const int c0 = blockIdx.x * kChunk;
-#pragma unroll
-    for (int h = 0; h < kChunk / kWide; ++h) {
-        const int j = threadIdx.x + h * kWide, i = c0 + j;
-        double pr = 0.0;
-        if (i < n) {
-            const V s = sigma * row_ax(rowptr, col, val, x, i);
-            y[i] = s;
-            pr = (double)w[i] * (double)s;
-        }
-        p[0][j] = pr;
-    }
-    chunk_reduce_staged<1>(p, part, nch);
-}
 
 int launch_spmv(fvh_eqn *e, const double *x, double *y, const KScal *ks) {
     const fvh_mesh *m = e->m;
@@ -256,19 +166,22 @@ int precond_update(fvh_eqn *eq) {
             HIP_TRY(hipGetLastError());
             return FVH_OK;
         case FVH_PRECOND_DILU:
-            if (m->sched.band_nx > 0) return band_dilu_update(e, src->nz.p);
+            if (band_path(m)) return band_dilu_update(e, src->nz.p);
             return dilu_update(m, src->nz.p, e->D.p, e->Dinv.p);
         case FVH_PRECOND_ILU0: return ilu0_factor(m, src->nz.p, e->lu.p);
         default: return set_error(FVH_ERR_UNSUPPORTED, "preconditioner %d", e->pkind);
     }
 }
 
-// y = M^{-1} x
-int precond_apply(fvh_eqn *eq, double *y, const double *x, const KScal *ks, double) {
+// y = M^{-1} x.  With independent cases ks is the array of per-case Krylov
+// scalars: only the per-case sweeps look at it; the elementwise and
+// level-scheduled paths then cover every case.
+int precond_apply(fvh_eqn *eq, double *y, const double *x, const KScal *ks_in, double) {
     fvh_eqn *e = eq->pshare ? eq->pshare : eq;
     fvh_mesh *m = e->m;
     fvh_eqn *src = e->pmat ? e->pmat : e;
     const int n = (int)m->n;
+    const KScal *ks = m->cases ? nullptr : ks_in;
     switch (e->pkind) {
         case FVH_PRECOND_NONE: {
             return map(n, ks, [=] __device__(int i) { y[i] = x[i]; });
@@ -279,7 +192,7 @@ int precond_apply(fvh_eqn *eq, double *y, const double *x, const KScal *ks, doub
             return map(n, ks, [=] __device__(int i) { y[i] = s[i] * x[i]; });
         }
         case FVH_PRECOND_DILU:
-            if (m->sched.band_nx > 0) return band_dilu_apply(e, x, y, ks);
+            if (band_path(m)) return band_dilu_apply(e, x, y, ks_in);
             return dilu_apply(m, src->nz.p, e->Dinv.p, x, y, ks);
         case FVH_PRECOND_ILU0: return ilu0_apply(m, e->lu.p, x, y, ks);
         default: return set_error(FVH_ERR_UNSUPPORTED, "preconditioner %d", e->pkind);
@@ -289,8 +202,8 @@ int precond_apply(fvh_eqn *eq, double *y, const double *x, const KScal *ks, doub
 // ---------------------------------------------------------------------------
 // FP32 operators of the mixed-precision inner solves
 // ---------------------------------------------------------------------------
-static int launch_spmv32(fvh_eqn *e, const float *x, float *y, float sigma, const KScal *ks, const float *w,
-                         double *part) {
+int launch_spmv32(fvh_eqn *e, const float *x, float *y, float sigma, const KScal *ks, const float *w,
+                  double *part) {
     const fvh_mesh *m = e->m;
     if (!w) {
         hipLaunchKernelGGL((k_spmv<float>), grid1d(m->n), dim3(256), 0, stream(), (int)m->n, m->d.rowptr, m->d.col,
@@ -307,7 +220,7 @@ static int launch_spmv32(fvh_eqn *e, const float *x, float *y, float sigma, cons
 // FP32 copies read by the inner solves: A (for the SpMV) and the
 // preconditioner's data, rounded once from the FP64 values after each update.
 // The DILU reciprocals are computed in FP64 by the update itself.
-static int prepare32(fvh_eqn *e) {
+int prepare32(fvh_eqn *e) {
     const fvh_mesh *m = e->m;
     const int64_t n = m->n, nnz = m->nnz;
     KrylovWork &w = e->kw;
@@ -329,7 +242,7 @@ static int prepare32(fvh_eqn *e) {
             return map(n, nullptr, [=] __device__(int i) { sf[i] = (float)s[i]; });
         }
         case FVH_PRECOND_DILU: {
-            if (m->sched.band_nx > 0) return band_to32(P);
+            if (band_path(m)) return band_to32(P);
             if (P->p32a.n != (size_t)n) FVH_TRY(P->p32a.alloc(n));
             if (P->p32b.n != (size_t)nnz) FVH_TRY(P->p32b.alloc(nnz));
             const double *di = P->Dinv.p, *a = src->nz.p;
@@ -348,10 +261,11 @@ static int prepare32(fvh_eqn *e) {
 }
 
 // y = M^{-1} x in FP32 (same recurrences, FP32 copies of their data)
-static int precond_apply32(fvh_eqn *eq, float *y, const float *x, const KScal *ks) {
+int precond_apply32(fvh_eqn *eq, float *y, const float *x, const KScal *ks_in) {
     fvh_eqn *e = eq->pshare ? eq->pshare : eq;
     fvh_mesh *m = e->m;
     const int n = (int)m->n;
+    const KScal *ks = m->cases ? nullptr : ks_in;
     switch (e->pkind) {
         case FVH_PRECOND_NONE: return map(n, ks, [=] __device__(int i) { y[i] = x[i]; });
         case FVH_PRECOND_JACOBI:
@@ -360,7 +274,7 @@ static int precond_apply32(fvh_eqn *eq, float *y, const float *x, const KScal *k
             return map(n, ks, [=] __device__(int i) { y[i] = s[i] * x[i]; });
         }
         case FVH_PRECOND_DILU:
-            if (m->sched.band_nx > 0) return band_dilu_apply32(e, x, y, ks);
+            if (band_path(m)) return band_dilu_apply32(e, x, y, ks_in);
             return dilu_apply32(m, e->p32b.p, e->p32a.p, x, y, ks);
         case FVH_PRECOND_ILU0: return ilu0_apply32(m, e->p32b.p, x, y, ks);
         default: return set_error(FVH_ERR_UNSUPPORTED, "preconditioner %d", e->pkind);
@@ -370,19 +284,26 @@ static int precond_apply32(fvh_eqn *eq, float *y, const float *x, const KScal *k
 // ---------------------------------------------------------------------------
 // Krylov workspace
 // ---------------------------------------------------------------------------
-static int ensure_work(fvh_eqn *e, int basis) {
+int ensure_work(fvh_eqn *e, int basis) {
     KrylovWork &w = e->kw;
     const int64_t n = e->m->n;
     if (w.n != n) {
+        w.drop_graphs();  // captured kernels hold the old workspace pointers
+        // one set of Krylov scalars per independent case
+        const int K = e->m->cases ? e->m->cases->K : 1;
+        const int nred = e->m->cases ? std::max(e->m->cases->nblk, nchunks(n)) : nchunks(n);
         for (auto &v : w.v) FVH_TRY(v.alloc(n));
-        FVH_TRY(w.part.alloc(4 * (size_t)nchunks(n) + 4));
-        FVH_TRY(w.ks.alloc(1));
-        if (!w.h_ks) HIP_TRY(hipHostMalloc((void **)&w.h_ks, sizeof(KScal), hipHostMallocDefault));
+        FVH_TRY(w.part.alloc(4 * (size_t)nred + 4));
+        FVH_TRY(w.ks.alloc(K));
+        if (w.h_ks) (void)hipHostFree(w.h_ks);
+        w.h_ks = nullptr;
+        HIP_TRY(hipHostMalloc((void **)&w.h_ks, sizeof(KScal) * K, hipHostMallocDefault));
         w.n = n;
         w.cap_basis = 0;
         w.n32 = 0;
     }
     if (basis > w.cap_basis) {
+        w.drop_graphs();
         FVH_TRY(w.basis.alloc((size_t)(basis + 1) * n));
         FVH_TRY(w.small.alloc((size_t)basis * (basis + 1) / 2 + 4 * (size_t)(basis + 2)));
         w.cap_basis = basis;
@@ -390,11 +311,12 @@ static int ensure_work(fvh_eqn *e, int basis) {
     return FVH_OK;
 }
 
-static int ensure_work32(fvh_eqn *e, int basis) {
+int ensure_work32(fvh_eqn *e, int basis) {
     FVH_TRY(ensure_work(e, basis));
     KrylovWork &w = e->kw;
     const int64_t n = e->m->n;
     if (w.n32 != n) {
+        w.drop_graphs();
         for (auto &v : w.f) FVH_TRY(v.alloc(n));
         w.n32 = n;
         w.cap_basis32 = 0;
@@ -866,7 +788,7 @@ static int krylov(const Sys<T> &S, const fvh_solver_cfg &cfg, T *x, fvh_stats *s
 
 // CG runs on -A x = -b when A is negative definite and M follows its sign
 // (DILU / ILU(0)); the oracle does the same.
-static int cg_sigma(fvh_eqn *e, double *sigma) {
+int cg_sigma(fvh_eqn *e, double *sigma) {
     *sigma = 1.0;
     const fvh_eqn *P = e->pshare ? e->pshare : e;
     if (P->pkind == FVH_PRECOND_DILU || P->pkind == FVH_PRECOND_ILU0) {
@@ -899,14 +821,14 @@ static int defect64(fvh_eqn *e, const double *x, double *r, double *norm) {
 // Mixed precision: FP64 defect correction around FP32 Krylov solves.
 //   r = b - A x (FP64);  A d = r solved in FP32 from d = 0 with the same
 //   method, preconditioner and stopping rule;  x += d (FP64).
-// The correction is repeated (at most kMaxRefine times, within the itmax
-// budget) while the FP64 defect is above atol + rtol*|r0|, the last
-// correction at least halved it, and the inner solve did not simply run out
-// of iterations: when it stopped as converged, only if the FP64 defect fell
-// short of its claimed reduction by more than 100x (the sign that FP32
-// rounding, not the stopping rule, ended it); when it stopped on a
-// breakdown (GMRES' Hessenberg test, BiCGSTAB's alpha), always.
-constexpr int kMaxRefine = 4;
+// When rtol >= kInnerRtol the FP32 solve meets the stopping rule by itself
+// (its rounding is orders below rtol) and one correction is made, as the
+// FP64 solve makes one solve; the pressure-velocity loop around it is the
+// outer refinement.  A tighter rtol is beyond FP32: the inner solves then run
+// to kInnerRtol and the correction repeats until the FP64 defect meets
+// atol + rtol*|r0| (2-norm), stops halving, or the itmax budget is spent.
+constexpr double kInnerRtol = 1e-5;
+constexpr int kMaxRefine = 8;
 
 static int solve_mixed(fvh_eqn *e, const fvh_solver_cfg &cfg, double *x, fvh_stats *st) {
     const int n = (int)e->m->n;
@@ -920,33 +842,32 @@ static int solve_mixed(fvh_eqn *e, const fvh_solver_cfg &cfg, double *x, fvh_sta
     float *r32 = w.f[8].p, *d32 = w.f[9].p;
     Sys<float> S{e, n, r32, false, {w.f[0].p, w.f[1].p, w.f[2].p, w.f[3].p, w.f[4].p, w.f[5].p, w.f[6].p, w.f[7].p},
                  w.basis32.p};
+    const bool refine = cfg.rtol < kInnerRtol;
     double rn0 = 0.0, rn = 0.0;
     FVH_TRY(defect64(e, x, r, &rn0));
     rn = rn0;
     fvh_solver_cfg in = cfg;
+    if (refine) in.rtol = kInnerRtol;
     int used = 0;
-    for (int ref = 0; ref < kMaxRefine && used < cfg.itmax; ++ref) {
+    for (int ref = 0; ref < (refine ? kMaxRefine : 1) && used < cfg.itmax; ++ref) {
         if (rn == 0.0) break;
         FVH_TRY(map(n, nullptr, [=] __device__(int i) { r32[i] = (float)r[i]; }));
         in.itmax = cfg.itmax - used;
         fvh_stats is{};
         FVH_TRY(krylov(S, in, d32, &is, sigma));
-        FVH_TRY(map(n, nullptr, [=] __device__(int i) { x[i] = x[i] + (double)d32[i]; }));
-        const double rn_prev = rn;
-        FVH_TRY(defect64(e, x, r, &rn));
+        if (is.status != FVH_KRYLOV_ZERO_RES)
+            FVH_TRY(map(n, nullptr, [=] __device__(int i) { x[i] = x[i] + (double)d32[i]; }));
         used += is.niter;
         if (ref == 0) st->rnorm0 = is.rnorm0;
         st->rnorm = is.rnorm;
         st->status = is.status;
         st->refinements = ref + 1;
+        if (!refine) break;
+        const double rn_prev = rn;
+        FVH_TRY(defect64(e, x, r, &rn));
         const bool reached = rn <= cfg.atol + cfg.rtol * rn0;
         const bool progress = rn <= 0.5 * rn_prev;
-        bool again;
-        if (is.status == FVH_KRYLOV_SOLVED)
-            again = is.rnorm0 > 0.0 && rn > 100.0 * (is.rnorm / is.rnorm0) * rn_prev;
-        else
-            again = is.status == FVH_KRYLOV_BREAKDOWN;
-        if (reached || !progress || !again) break;
+        if (reached || !progress || is.status == FVH_KRYLOV_ITMAX) break;
     }
     st->niter = used;
     return FVH_OK;
@@ -958,6 +879,7 @@ int solve(fvh_eqn *e, const fvh_solver_cfg &cfg, double *x, fvh_stats *st) {
     std::memset(st, 0, sizeof *st);
     if (cfg.solver != FVH_BICGSTAB && cfg.solver != FVH_CG && cfg.solver != FVH_GMRES)
         return set_error(FVH_ERR_UNSUPPORTED, "solver %d", cfg.solver);
+    if (e->m->cases) return solve_cases(e, cfg, x, st);
     if (cfg.precision == FVH_MIXED) return solve_mixed(e, cfg, x, st);
     if (cfg.precision != FVH_FP64) return set_error(FVH_ERR_UNSUPPORTED, "precision %d", cfg.precision);
     const int basis = cfg.solver == FVH_GMRES ? std::max(cfg.itmax, 1) : 0;
@@ -1027,7 +949,7 @@ extern "C" int fvh_precond_apply(fvh_eqn *e, double *y, const double *x) {
 
 int fvh::precond_dinv_natural(fvh_eqn *e, const double **out) {
     if (e->pkind != FVH_PRECOND_DILU) return set_error(FVH_ERR_ARG, "not a DILU preconditioner");
-    if (e->m->sched.band_nx > 0) FVH_TRY(band_dinv_natural(e, e->Dinv.p));
+    if (band_path(e->m)) FVH_TRY(band_dinv_natural(e, e->Dinv.p));
     *out = e->Dinv.p;
     return FVH_OK;
 }

@@ -1,0 +1,123 @@
+// SpMV device code shared by the single-case and the per-case (batched)
+// solvers: row products in the reference's order and the canonical chunk
+// reduction of staged per-row terms.
+#pragma once
+
+#include "k_util.h"
+
+namespace fvh {
+
+// ---------------------------------------------------------------------------
+// SpMV  y = sigma * A x  (optionally with the partial sums of w.y)
+// ---------------------------------------------------------------------------
+// Row i of A x, summed from the first entry in ascending column order.  The
+// entries are taken in groups of kRowBatch (one group for the rows of 2-D
+// quad and triangle meshes) with every load of a group issued before the
+// first use, so a row costs two dependent memory latencies (col/val, then x)
+// instead of two per entry.  Slots past the row's end re-read its last entry
+// (in bounds, same cache line) and are dropped by a select.
+constexpr int kRowBatch = 5;
+template <typename V>
+__device__ inline V row_ax(const int *__restrict__ rowptr, const int *__restrict__ col, const V *__restrict__ val,
+                           const V *__restrict__ x, int i) {
+    V s = 0;
+    const int p1 = rowptr[i + 1];
+    for (int p0 = rowptr[i]; p0 < p1; p0 += kRowBatch) {
+        int c[kRowBatch];
+        V a[kRowBatch], xv[kRowBatch];
+#pragma unroll
+        for (int k = 0; k < kRowBatch; ++k) {
+            const int p = min(p0 + k, p1 - 1);
+            c[k] = col[p];
+            a[k] = val[p];
+        }
+#pragma unroll
+        for (int k = 0; k < kRowBatch; ++k) xv[k] = x[c[k]];
+#pragma unroll
+        for (int k = 0; k < kRowBatch; ++k) {
+            const V t = s + a[k] * xv[k];
+            s = (p0 + k < p1) ? t : s;
+        }
+    }
+    return s;
+}
+
+// One thread per row.
+template <typename V>
+__global__ void __launch_bounds__(256) k_spmv(int n, const int *__restrict__ rowptr, const int *__restrict__ col,
+                                              const V *__restrict__ val, const V *__restrict__ x, V *__restrict__ y,
+                                              V sigma, const KScal *ks) {
+    if (ks && ks->done) return;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) y[i] = sigma * row_ax(rowptr, col, val, x, i);
+}
+
+// Canonical partial sums of one 2048-row chunk from per-row terms staged in
+// LDS (p[v][j] = term of row chunk_base + j, 0 for rows past n): lane t < 256
+// adds rows t + 256 k in ascending k, then the same tree as chunk_reduce.
+// All kWide threads of the block call it.
+constexpr int kWide = 1024;
+template <int NV>
+__device__ inline void chunk_reduce_staged(double (&p)[NV][kChunk], double *__restrict__ part, int nch) {
+    const int t = threadIdx.x;
+    __syncthreads();
+    double acc[NV];
+#pragma unroll
+    for (int v = 0; v < NV; ++v) acc[v] = 0.0;
+    if (t < kBlock) {
+#pragma unroll
+        for (int k = 0; k < kPer; ++k)
+#pragma unroll
+            for (int v = 0; v < NV; ++v) acc[v] = acc[v] + p[v][t + k * kBlock];
+    }
+    __syncthreads();
+    if (t < kBlock) {
+#pragma unroll
+        for (int v = 0; v < NV; ++v) p[v][t] = acc[v];
+    }
+    __syncthreads();
+    if (t < 128) {
+#pragma unroll
+        for (int v = 0; v < NV; ++v) p[v][t] = p[v][t] + p[v][t + 128];
+    }
+    __syncthreads();
+    if (t < 64) {
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+            double a = p[v][t] + p[v][t + 64];
+            a = a + __shfl_down(a, 32);
+            a = a + __shfl_down(a, 16);
+            a = a + __shfl_down(a, 8);
+            a = a + __shfl_down(a, 4);
+            a = a + __shfl_down(a, 2);
+            a = a + __shfl_down(a, 1);
+            if (t == 0) part[v * nch + blockIdx.x] = a;
+        }
+    }
+}
+
+// y = sigma A x and the partial sums of w.y: one block of kWide threads per
+// chunk, two rows per thread.
+template <typename V>
+__global__ void __launch_bounds__(kWide) k_spmv_dot(int n, const int *__restrict__ rowptr, const int *__restrict__ col,
+                                                    const V *__restrict__ val, const V *__restrict__ x,
+                                                    V *__restrict__ y, V sigma, const KScal *ks,
+                                                    const V *__restrict__ w, double *part, int nch) {
+    if (ks && ks->done) return;
+    __shared__ double p[1][kChunk];
+    const int c0 = blockIdx.x * kChunk;
+#pragma unroll
+    for (int h = 0; h < kChunk / kWide; ++h) {
+        const int j = threadIdx.x + h * kWide, i = c0 + j;
+        double pr = 0.0;
+        if (i < n) {
+            const V s = sigma * row_ax(rowptr, col, val, x, i);
+            y[i] = s;
+            pr = (double)w[i] * (double)s;
+        }
+        p[0][j] = pr;
+    }
+    chunk_reduce_staged<1>(p, part, nch);
+}
+
+}  // namespace fvh

@@ -37,15 +37,17 @@ __device__ inline void chunk_reduce(double (&acc)[NV], double *__restrict__ part
     }
 }
 
-// One block of kBlock threads reduces NV partial arrays; all threads get r.
+// One block of kBlock threads reduces NV partial arrays of `cnt` entries
+// (array v at part + v*stride + base); all threads get r.
 template <int NV>
-__device__ inline void final_reduce(const double *__restrict__ part, int nch, double (&r)[NV]) {
+__device__ inline void final_reduce_at(const double *__restrict__ part, int stride, int base, int cnt,
+                                       double (&r)[NV]) {
     __shared__ double sh[NV][kBlock];
     const int t = threadIdx.x;
 #pragma unroll
     for (int v = 0; v < NV; ++v) {
         double a = 0.0;
-        for (int j = t; j < nch; j += kBlock) a = a + part[v * nch + j];
+        for (int j = t; j < cnt; j += kBlock) a = a + part[v * stride + base + j];
         sh[v][t] = a;
     }
     __syncthreads();
@@ -70,6 +72,11 @@ __device__ inline void final_reduce(const double *__restrict__ part, int nch, do
     __syncthreads();
 #pragma unroll
     for (int v = 0; v < NV; ++v) r[v] = sh[v][0];
+}
+
+template <int NV>
+__device__ inline void final_reduce(const double *__restrict__ part, int nch, double (&r)[NV]) {
+    final_reduce_at<NV>(part, nch, 0, nch, r);
 }
 
 // Elementwise work + partial sums over the canonical chunk mapping.

@@ -68,6 +68,13 @@ struct Sim {
     BcPlan *planU = nullptr, *planP = nullptr, *planK = nullptr, *planW = nullptr;
     WallPlan wK, wW, wNut;
     double ms_asm = NAN, ms_solve = NAN, ms_other = NAN;
+    // independent cases (batched mode): K > 1 or a case-split mesh
+    bool cases = false;
+    int K = 1;
+    DBuf<double> nu_face;           // viscosity of the case owning each face
+    DBuf<unsigned char> refflag;    // cells taking setReference!
+    std::vector<double> Rc;         // residual histories [K][iterations][3]
+    std::map<std::string, std::vector<fvh_stats>> cstats;
 };
 
 // ---------------------------------------------------------------------------
@@ -100,6 +107,7 @@ static int assemble(Sim &S, fvh_eqn *e, const std::vector<KTerm> &terms, const s
     a.setref = setref ? 1 : 0;
     a.pref = S.cs.pref;
     a.refcell = 0;
+    a.refflag = S.cases ? S.refflag.p : nullptr;
     return launch_assemble(e, a);
 }
 
@@ -154,6 +162,10 @@ static int update_nueff(Sim &S) {
     double nu = S.cs.nu;
     double *nueff = S.nueff.p;
     const double *nutf = S.nutf.p;
+    if (S.cases) {  // laminar, one viscosity per case
+        const double *nf = S.nu_face.p;
+        return map(S.m->nf, nullptr, [=] __device__(int f) { nueff[f] = nf[f]; });
+    }
     if (S.cs.model == FVH_KOMEGA) return map(S.m->nf, nullptr, [=] __device__(int f) { nueff[f] = nu + nutf[f]; });
     return map(S.m->nf, nullptr, [=] __device__(int f) { nueff[f] = nu; });
 }
@@ -292,7 +304,15 @@ static int run_solve(Sim &S, fvh_eqn *e, const fvh_solver_cfg &cfg, double *x, c
     fvh_stats st{};
     FVH_TRY(solve(e, cfg, x, &st));
     S.stats[name] = st;
+    if (S.cases) S.cstats[name] = e->kw.case_stats;
     return FVH_OK;
+}
+
+// residual!(R, eqn, phi, iteration) into slot comp of the device residuals
+// (per case: res_dev[3k + comp])
+static int residual_slot(Sim &S, fvh_eqn *e, const double *phi, int comp) {
+    if (S.cases) return residual_cases_async(e, phi, S.res_dev.p, 3, comp);
+    return residual_async(e, phi, S.res_dev.p + comp);
 }
 
 static int turbulence(Sim &S) {
@@ -388,7 +408,7 @@ static int turbulence(Sim &S) {
 // ---------------------------------------------------------------------------
 // Loop body
 // ---------------------------------------------------------------------------
-static int momentum(Sim &S, fvh_eqn *e, double *U, const double *gp, int comp, bool relax, const char *name, double *res) {
+static int momentum(Sim &S, fvh_eqn *e, double *U, const double *gp, int comp, bool relax, const char *name) {
     const int n = (int)S.m->n;
     FVH_TRY(copy_dev(S.prev.p, U, n));
     std::vector<KTerm> t = {term(FVH_OP_TIME, S.cs.sU.time, 1, nullptr, U),
@@ -397,7 +417,7 @@ static int momentum(Sim &S, fvh_eqn *e, double *U, const double *gp, int comp, b
     FVH_TRY(assemble(S, e, t, {{-1.0, gp}}, S.bU, comp, S.planU, relax, S.cs.vU.relax, false));
     FVH_TRY(precond_update(e));
     FVH_TRY(run_solve(S, e, S.cs.vU, U, name));
-    return residual_async(e, U, res);
+    return residual_slot(S, e, U, comp);
 }
 
 static int pressure_corrector(Sim &S) {
@@ -435,7 +455,7 @@ static int pressure_corrector(Sim &S) {
         double a = S.cs.vp.relax;
         FVH_TRY(map(n, nullptr, [=] __device__(int i) { p[i] = p0[i] + a * (p[i] - p0[i]); }));
     }
-    FVH_TRY(residual_async(S.pe.get(), S.p.p, S.res_dev.p + 2));
+    FVH_TRY(residual_slot(S, S.pe.get(), S.p.p, 2));
     FVH_TRY(launch_grad(m, S.cs.sp.gradient, S.vp, S.p.p, S.pf.p, S.gpx.p, S.gpy.p, S.gpz.p));
     {
         double *ux = S.Ux.p, *uy = S.Uy.p;
@@ -461,8 +481,8 @@ static int iterate_once(Sim &S) {
     const int n = (int)m->n;
     const int iter = S.it;
     const bool piso = S.cs.algorithm == FVH_PISO;
-    FVH_TRY(momentum(S, S.ux.get(), S.Ux.p, S.gpx.p, 0, !piso, "Ux", S.res_dev.p + 0));
-    FVH_TRY(momentum(S, S.uy.get(), S.Uy.p, S.gpy.p, 1, !piso, "Uy", S.res_dev.p + 1));
+    FVH_TRY(momentum(S, S.ux.get(), S.Ux.p, S.gpx.p, 0, !piso, "Ux"));
+    FVH_TRY(momentum(S, S.uy.get(), S.Uy.p, S.gpy.p, 1, !piso, "Uy"));
     {
         const double *vol = m->d.vol, *ax = S.ux->nz.p, *gx = S.gpx.p, *gy = S.gpy.p;
         const int *diag = m->d.diag;
@@ -477,6 +497,25 @@ static int iterate_once(Sim &S) {
     FVH_TRY(launch_interpolate(m, S.rD.p, S.rDf.p));
     const int ncorr = piso ? 2 : 1;
     for (int i = 0; i < ncorr; ++i) FVH_TRY(pressure_corrector(S));
+    const double cb = S.cs.conv_break;
+    if (S.cases) {
+        // every case advances each iteration; the loop stops once all have
+        // reached the break
+        std::vector<double> r(3 * S.K);
+        HIP_TRY(hipMemcpyAsync(r.data(), S.res_dev.p, r.size() * sizeof(double), hipMemcpyDeviceToHost, stream()));
+        HIP_TRY(hipStreamSynchronize(stream()));
+        bool all = !piso;
+        for (int k = 0; k < S.K; ++k) {
+            for (int q = 0; q < 3; ++q) S.Rc[((size_t)k * S.iterations + iter) * 3 + q] = r[3 * k + q];
+            all = all && r[3 * k] <= cb && r[3 * k + 1] <= cb && r[3 * k + 2] <= cb;
+        }
+        S.Rux[iter] = r[0];
+        S.Ruy[iter] = r[1];
+        S.Rp[iter] = r[2];
+        S.it++;
+        if (all) S.converged = true;
+        return FVH_OK;
+    }
     double r[3];
     HIP_TRY(hipMemcpyAsync(r, S.res_dev.p, 3 * sizeof(double), hipMemcpyDeviceToHost, stream()));
     HIP_TRY(hipStreamSynchronize(stream()));
@@ -484,7 +523,6 @@ static int iterate_once(Sim &S) {
     S.Ruy[iter] = r[1];
     S.Rp[iter] = r[2];
     S.it++;
-    const double cb = S.cs.conv_break;
     if (!piso && r[0] <= cb && r[1] <= cb && r[2] <= cb) S.converged = true;
     return FVH_OK;
 }
@@ -513,7 +551,7 @@ static int sim_init(Sim &S, const double *Ux, const double *Uy, const double *Uz
     for (DBuf<double> *b : {&S.mdotf, &S.nueff, &S.pf, &S.Ufx, &S.Ufy, &S.Ufz, &S.nutf, &S.nueffk, &S.nueffw, &S.nutf_tmp})
         FVH_TRY(face(*b, 0.0));
     FVH_TRY(face(S.rDf, 1.0));
-    FVH_TRY(S.res_dev.alloc(4));
+    FVH_TRY(S.res_dev.alloc(S.cases ? 3 * S.K + 1 : 4));
     FVH_TRY(S.flag.alloc(1));
     // equations and preconditioners (uy shares ux's preconditioner, which
     // reads ux's matrix; k and omega assemble into p's equation)
@@ -569,6 +607,7 @@ static int sim_init(Sim &S, const double *Ux, const double *Uy, const double *Uz
     S.Rux.assign(S.iterations, 1.0);
     S.Ruy.assign(S.iterations, 1.0);
     S.Rp.assign(S.iterations, 1.0);
+    if (S.cases) S.Rc.assign((size_t)S.K * S.iterations * 3, 1.0);
     // loop prologue (Solvers_1_SIMPLE.jl:114-119)
     FVH_TRY(velocity_faces(S, S.Ux.p, S.Uy.p, S.Uz.p));
     FVH_TRY(launch_flux(m, S.Ufx.p, S.Ufy.p, S.Ufz.p, S.mdotf.p));
@@ -586,19 +625,46 @@ struct fvh_sim {
     Sim S;
 };
 
-extern "C" int fvh_sim_create(fvh_mesh *m, const fvh_case *c, const double *Ux, const double *Uy, const double *Uz,
-                              const double *p, const double *k, const double *omega, const double *nut, fvh_sim **out) {
-    FVH_GUARD_BEGIN
+static int sim_create(fvh_mesh *m, const fvh_case *c, const double *nu_case, const int64_t *ref_cells,
+                      const double *Ux, const double *Uy, const double *Uz, const double *p, const double *k,
+                      const double *omega, const double *nut, fvh_sim **out) {
     if (!m || !c || !out) return set_error(FVH_ERR_ARG, "null argument");
     if (c->iterations < 0) return set_error(FVH_ERR_ARG, "iterations < 0");
     for (const fvh_solver_cfg *v : {&c->vU, &c->vp})
-        if (v->precision != FVH_FP64 && v->precision != FVH_MIXED && v->precision != FVH_MIXED_FP16)
-            return set_error(FVH_ERR_ARG, "precision %d", v->precision);
+        if (v->precision != FVH_FP64 && v->precision != FVH_MIXED)
+            return set_error(v->precision == FVH_MIXED_FP16 ? FVH_ERR_UNSUPPORTED : FVH_ERR_ARG, "precision %d",
+                             v->precision);
     auto h = std::make_unique<fvh_sim>();
     Sim &S = h->S;
     S.m = m;
     S.cs = *c;
     S.iterations = c->iterations;
+    if (m->cases) {
+        if (!nu_case) return set_error(FVH_ERR_ARG, "mesh has independent cases: use fvh_sim_create_cases");
+        if (c->model != FVH_LAMINAR)
+            return set_error(FVH_ERR_UNSUPPORTED, "independent cases: laminar models only");
+        const CaseLayout &L = *m->cases;
+        S.cases = true;
+        S.K = L.K;
+        DBuf<double> nuc;
+        FVH_TRY(nuc.upload(std::vector<double>(nu_case, nu_case + L.K)));
+        FVH_TRY(S.nu_face.alloc(m->nf));
+        {
+            const double *nc = nuc.p;
+            const int *cc = L.cell_case.p, *own = m->d.own1;
+            double *nf = S.nu_face.p;
+            FVH_TRY(map(m->nf, nullptr, [=] __device__(int f) { nf[f] = nc[cc[own[f]]]; }));
+        }
+        std::vector<unsigned char> flag(m->n, 0);
+        for (int q = 0; q < L.K; ++q) {
+            const int64_t rc = ref_cells ? ref_cells[q] : -1;
+            if (rc < 0) continue;
+            if (rc >= L.h_off[q + 1] - L.h_off[q]) return set_error(FVH_ERR_ARG, "reference cell of case %d", q);
+            flag[L.h_off[q] + rc] = 1;
+        }
+        FVH_TRY(S.refflag.upload(flag));
+        HIP_TRY(hipStreamSynchronize(stream()));
+    }
     auto load = [](const fvh_bcset &s, std::vector<fvh_bc> &v) {
         v.assign(s.bcs, s.bcs + s.nbc);
     };
@@ -611,7 +677,47 @@ extern "C" int fvh_sim_create(fvh_mesh *m, const fvh_case *c, const double *Ux, 
     if (rc) return rc;
     *out = h.release();
     return FVH_OK;
+}
+
+extern "C" int fvh_sim_create(fvh_mesh *m, const fvh_case *c, const double *Ux, const double *Uy, const double *Uz,
+                              const double *p, const double *k, const double *omega, const double *nut, fvh_sim **out) {
+    FVH_GUARD_BEGIN
+    return sim_create(m, c, nullptr, nullptr, Ux, Uy, Uz, p, k, omega, nut, out);
     FVH_GUARD_END
+}
+
+extern "C" int fvh_sim_create_cases(fvh_mesh *m, const fvh_case *c, const double *nu_case, const int64_t *ref_cells,
+                                    const double *Ux, const double *Uy, const double *Uz, const double *p,
+                                    fvh_sim **out) {
+    FVH_GUARD_BEGIN
+    if (!m || !m->cases) return set_error(FVH_ERR_ARG, "mesh has no independent cases (fvh_mesh_set_cases)");
+    if (!nu_case) return set_error(FVH_ERR_ARG, "null viscosities");
+    return sim_create(m, c, nu_case, ref_cells, Ux, Uy, Uz, p, nullptr, nullptr, nullptr, out);
+    FVH_GUARD_END
+}
+
+extern "C" int fvh_sim_case_residuals(fvh_sim *h, int32_t k, double *a, double *b, double *c) {
+    if (!h) return set_error(FVH_ERR_ARG, "null sim");
+    Sim &S = h->S;
+    if (!S.cases || k < 0 || k >= S.K) return set_error(FVH_ERR_ARG, "no case %d", k);
+    for (int it = 0; it < S.iterations; ++it) {
+        const double *r = &S.Rc[((size_t)k * S.iterations + it) * 3];
+        if (a) a[it] = r[0];
+        if (b) b[it] = r[1];
+        if (c) c[it] = r[2];
+    }
+    return FVH_OK;
+}
+
+extern "C" int fvh_sim_case_stats(fvh_sim *h, int32_t k, const char *field, fvh_stats *out) {
+    if (!h || !field || !out) return set_error(FVH_ERR_ARG, "null argument");
+    Sim &S = h->S;
+    if (!S.cases || k < 0 || k >= S.K) return set_error(FVH_ERR_ARG, "no case %d", k);
+    auto it = S.cstats.find(field);
+    if (it == S.cstats.end() || (int)it->second.size() != S.K)
+        return set_error(FVH_ERR_ARG, "no statistics for %s", field);
+    *out = it->second[k];
+    return FVH_OK;
 }
 
 extern "C" void fvh_sim_destroy(fvh_sim *s) { delete s; }

@@ -14,4 +14,4 @@ from .api import (  # noqa: F401
     Orthogonal, RANS, CaseDesc, Simulation, Steady, Upwind, assign, initialise, isturbulent, piso, set_runtime,
     set_schemes, set_solver, simple)
 from .mesh import (Mesh2, backward_facing_step, build_mesh, cavity, channel, delaunay_unit_square,  # noqa: F401
-                   flat_plate, from_elements, rect_grid)
+                   flat_plate, from_elements, rect_grid, union)

@@ -123,6 +123,12 @@ SIGNATURES = {
     "fvh_precond_dinv": (C.c_int, [_VP, P_D]),
     "fvh_solve": (C.c_int, [_VP, C.POINTER(SolverCfg), _VP, C.POINTER(Stats)]),
     "fvh_spmv_f32": (C.c_int, [_VP, _VP, _VP]),
+    "fvh_mesh_set_cases": (C.c_int, [_VP, C.c_int32, P_I64]),
+    "fvh_mesh_case_info": (C.c_int, [_VP, P_I32, P_I32]),
+    "fvh_mesh_tuning": (C.c_int, [_VP, P_I32]),
+    "fvh_sim_create_cases": (C.c_int, [_VP, C.POINTER(Case), P_D, P_I64, P_D, P_D, P_D, P_D, C.POINTER(_VP)]),
+    "fvh_sim_case_residuals": (C.c_int, [_VP, C.c_int32, P_D, P_D, P_D]),
+    "fvh_sim_case_stats": (C.c_int, [_VP, C.c_int32, C.c_char_p, C.POINTER(Stats)]),
     "fvh_precond_apply_f32": (C.c_int, [_VP, _VP, _VP]),
     "fvh_sim_create": (C.c_int, [_VP, C.POINTER(Case), P_D, P_D, P_D, P_D, P_D, P_D, P_D, C.POINTER(_VP)]),
     "fvh_sim_destroy": (None, [_VP]),

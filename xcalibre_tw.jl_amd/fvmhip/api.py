@@ -334,6 +334,113 @@ class Simulation:
             pass
 
 
+class BatchSimulation:
+    """simple!/piso! over many independent laminar cases advanced together:
+    one launch per phase covers every case, the Krylov solves keep one state
+    per case and the DILU sweeps run one work-group per case.
+
+    `models`: RANS{Laminar} models with their U and p BCs assigned; `prefs`:
+    the setReference! value per case (None = none; the non-None values must
+    agree).  Cases with the same patch names and the same BCs form a family
+    whose patches are merged in the union mesh.  Each case computes exactly
+    what it computes in its own Simulation (FP64: bit for bit)."""
+
+    def __init__(self, models, config, prefs=None, algorithm=L.ALG_SIMPLE, conv_break=1e-7):
+        from .mesh import union
+
+        prefs = list(prefs) if prefs is not None else [None] * len(models)
+        for mo in models:
+            if isturbulent(mo):
+                raise FVHError_unsupported("independent cases: laminar models only")
+
+        def key(mo):
+            def bck(bcs):
+                return tuple((b.ID, b.kind, repr(b.value)) for b in bcs)
+            return (tuple(mo.mesh.boundary_names), bck(mo.U.BCs), bck(mo.p.BCs))
+
+        fams, fam_of = {}, []
+        for mo in models:
+            fam_of.append(fams.setdefault(key(mo), f"f{len(fams)}"))
+        umesh = union([mo.mesh for mo in models], fam_of)
+        first = {}
+        for mo, f in zip(models, fam_of):
+            first.setdefault(f, mo)
+        ubc = dict(U=[], p=[])
+        for f, mo in first.items():
+            for name in ("U", "p"):
+                for b in getattr(mo, name).BCs:
+                    ubc[name].append(BoundaryCondition(f"{f}:{b.ID}", b.value, b.kind))
+        nus = [float(mo.nu.values if isinstance(mo.nu, ConstantScalar) else mo.nu) for mo in models]
+        um = RANS(Laminar, mesh=umesh, viscosity=ConstantScalar(nus[0]))
+        for comp in ("x", "y", "z"):
+            getattr(um.U, comp).values[:] = np.concatenate([getattr(mo.U, comp).values for mo in models])
+        um.p.values[:] = np.concatenate([mo.p.values for mo in models])
+        assign(um, "U", *ubc["U"])
+        assign(um, "p", *ubc["p"])
+        pv = {float(v) for v in prefs if v is not None}
+        if len(pv) > 1:
+            raise ValueError("independent cases: one pressure reference value for all cases")
+        pref = pv.pop() if pv else None
+        self.models, self.config, self.union_model = models, config, um
+        self.offsets = umesh.case_offsets
+        self.case = CaseDesc(um, config, pref, algorithm, conv_break)
+        self.dm = umesh.device()
+        f = _fields(um)
+        self._f = f
+        nu = np.array(nus, np.float64)
+        refs = np.array([0 if v is not None else -1 for v in prefs], np.int64)
+        self._keep = (nu, refs)
+        h = C.c_void_p()
+        L.check(L.lib().fvh_sim_create_cases(self.dm.h, C.byref(self.case.c), L.dptr(nu),
+                                             refs.ctypes.data_as(L.P_I64), L.dptr(f["Ux"]), L.dptr(f["Uy"]),
+                                             L.dptr(f["Uz"]), L.dptr(f["p"]), C.byref(h)), "fvh_sim_create_cases")
+        self.h = h
+        self.iterations = self.case.c.iterations
+
+    @property
+    def ncases(self):
+        return len(self.models)
+
+    def run(self, n):
+        done = C.c_int32()
+        L.check(L.lib().fvh_sim_iterate(self.h, int(n), C.byref(done)), "fvh_sim_iterate")
+        return done.value
+
+    def get(self, name, case=None):
+        ln = C.c_int64()
+        L.check(L.lib().fvh_sim_get(self.h, name.encode(), None, C.byref(ln)), f"fvh_sim_get({name})")
+        out = np.zeros(ln.value)
+        L.check(L.lib().fvh_sim_get(self.h, name.encode(), L.dptr(out), C.byref(ln)))
+        if case is None:
+            return out
+        if out.size != self.offsets[-1]:
+            raise ValueError(f"{name} is not a cell array")
+        return out[self.offsets[case]:self.offsets[case + 1]]
+
+    def residuals(self, case):
+        n = self.iterations
+        a, b, c = np.ones(n), np.ones(n), np.ones(n)
+        L.check(L.lib().fvh_sim_case_residuals(self.h, int(case), L.dptr(a), L.dptr(b), L.dptr(c)))
+        return a, b, c
+
+    def stats(self, field_name, case):
+        s = L.Stats()
+        L.check(L.lib().fvh_sim_case_stats(self.h, int(case), field_name.encode(), C.byref(s)))
+        return dict(niter=s.niter, status=L.KRYLOV_STATUS.get(s.status, s.status), rnorm=s.rnorm, rnorm0=s.rnorm0,
+                    refinements=s.refinements)
+
+    def __del__(self):
+        try:
+            if self.h:
+                L.lib().fvh_sim_destroy(self.h)
+        except Exception:
+            pass
+
+
+def FVHError_unsupported(msg):
+    return L.FVHError(L.FVH_ERR_UNSUPPORTED, msg)
+
+
 def simple(model, config, pref=None, conv_break=1e-7):
     """simple!(model, config; pref): returns (R_ux, R_uy, R_p); fields are
     updated in place on the model, as in the reference."""

@@ -108,6 +108,22 @@ class DeviceMesh:
         h = C.c_void_p()
         L.check(L.lib().fvh_mesh_create(C.byref(d), C.byref(h)), "fvh_mesh_create")
         self.h = h
+        off = getattr(mesh, "case_offsets", None)
+        if off is not None:
+            off = np.ascontiguousarray(off, np.int64)
+            L.check(L.lib().fvh_mesh_set_cases(self.h, len(off) - 1, off.ctypes.data_as(L.P_I64)),
+                    "fvh_mesh_set_cases")
+
+    def tuning(self):
+        """Grid rows per lane of the banded sweeps (0: level-scheduled)."""
+        r = C.c_int32()
+        L.check(L.lib().fvh_mesh_tuning(self.h, C.byref(r)))
+        return r.value
+
+    def case_info(self):
+        k, b = C.c_int32(), C.c_int32()
+        L.check(L.lib().fvh_mesh_case_info(self.h, C.byref(k), C.byref(b)))
+        return k.value, bool(b.value)
 
     def pattern(self):
         nnz = C.c_int64()
@@ -327,3 +343,83 @@ def delaunay_unit_square(npts, seed=0, nboundary=None):
     eptr = np.concatenate([np.arange(0, 2 * ne + 1, 2, dtype=np.int64), 2 * ne + 3 * np.arange(1, tri.shape[0] + 1)])
     pts = np.concatenate([P, np.zeros((P.shape[0], 1))], axis=1)
     return from_elements(pts, (eptr, everts), groups)
+
+
+def union(meshes, families):
+    """Block-diagonal union of independent meshes, for the batched mode.
+
+    Cells and internal faces of mesh k follow those of mesh k-1.  Boundary
+    patches are keyed by (families[k], patch name): meshes of one family must
+    have the same patch names in the same order, and their faces of a patch
+    are listed mesh after mesh.  The union keeps boundary faces first in patch
+    order (as the builder numbers them), so every cell sees its faces, its
+    neighbours and its boundary faces in the order it has alone.  The result
+    carries `case_offsets` (cell ranges) and `case_face_ranges`."""
+    assert len(meshes) == len(families) and meshes
+    order = []  # union patches: (family, name)
+    fam_names = {}
+    for m, fam in zip(meshes, families):
+        if fam in fam_names:
+            if fam_names[fam] != m.boundary_names:
+                raise ValueError(f"meshes of family {fam!r} differ in their patches")
+        else:
+            fam_names[fam] = list(m.boundary_names)
+            order.extend((fam, nm) for nm in m.boundary_names)
+    K = len(meshes)
+    coff = np.concatenate([[0], np.cumsum([m.ncells for m in meshes])]).astype(np.int64)
+    nbt = sum(m.nbfaces for m in meshes)
+    nint = [m.nfaces - m.nbfaces for m in meshes]
+    ioff = np.concatenate([[0], np.cumsum(nint)]).astype(np.int64)
+    # new ids of every mesh's faces
+    fmap = [np.full(m.nfaces, -1, np.int64) for m in meshes]
+    bfaces, bcells, brange = [], [], []
+    nxt = 0
+    for fam, nm in order:
+        first = nxt
+        for k, m in enumerate(meshes):
+            if families[k] != fam:
+                continue
+            f, c = m.boundary(nm)
+            fmap[k][f] = np.arange(nxt, nxt + f.size)
+            bfaces.append(np.arange(nxt, nxt + f.size))
+            bcells.append(c + coff[k])
+            nxt += f.size
+        brange.append((first, nxt - 1))
+    assert nxt == nbt
+    for k, m in enumerate(meshes):
+        fmap[k][m.nbfaces:] = nbt + ioff[k] + np.arange(nint[k])
+    nf = nbt + int(ioff[-1])
+    u = Mesh2.__new__(Mesh2)
+    u.ncells, u.nfaces = int(coff[-1]), nf
+    u.cell_centre = np.concatenate([m.cell_centre for m in meshes])
+    u.cell_volume = np.concatenate([m.cell_volume for m in meshes])
+    cfo = np.concatenate([[0], np.cumsum([m.cell_faces.size for m in meshes])]).astype(np.int64)
+    u.cell_faces_range = np.concatenate([m.cell_faces_range + cfo[k] for k, m in enumerate(meshes)])
+    u.cell_faces = np.concatenate([fmap[k][m.cell_faces] for k, m in enumerate(meshes)])
+    u.cell_neighbours = np.concatenate([m.cell_neighbours + coff[k] for k, m in enumerate(meshes)])
+    u.cell_nsign = np.concatenate([m.cell_nsign for m in meshes])
+
+    def faces(attr, shape_tail):
+        out = np.zeros((nf,) + shape_tail, getattr(meshes[0], attr).dtype)
+        for k, m in enumerate(meshes):
+            out[fmap[k]] = getattr(m, attr)
+        return out
+
+    u.face_owners = faces("face_owners", (2,))
+    for k, m in enumerate(meshes):
+        u.face_owners[fmap[k]] += coff[k]
+    u.face_centre = faces("face_centre", (3,))
+    u.face_normal = faces("face_normal", (3,))
+    u.face_e = faces("face_e", (3,))
+    u.face_area = faces("face_area", ())
+    u.face_delta = faces("face_delta", ())
+    u.face_weight = faces("face_weight", ())
+    u.boundary_range = np.array(brange, np.int64).reshape(-1, 2)
+    u.boundary_faces = np.concatenate(bfaces).astype(np.int64)
+    u.boundary_cells = np.concatenate(bcells).astype(np.int64)
+    u.boundary_names = [f"{fam}:{nm}" for fam, nm in order]
+    u._dev = None
+    u._keep = None
+    u.case_offsets = coff
+    return u
+
