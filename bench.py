@@ -103,6 +103,56 @@ def run_gpu(args, precision):
     return res
 
 
+def run_batch(args):
+    """C5: 64 independent cavity/channel cases, sizes 32^2..512^2 (seeded
+    log-uniform), Re 100..1000, advanced together (one launch per phase);
+    against the same cases run one after another as single FP64 solves."""
+    import fvmhip as F
+    from fvmhip import _lib as L
+    from tests.cases import batch_cases, config
+
+    it = args.batch_iters
+    out = {}
+    cases = batch_cases(args.batch_cases)
+    ncells = sum(m.mesh.ncells for m, _ in cases)
+    for prec, name in ((L.FP64, "fp64"), (L.MIXED, "mixed")):
+        cases = batch_cases(args.batch_cases)
+        cfg = config(it + 1, precision=prec)
+        t0 = time.time()
+        b = F.BatchSimulation([m for m, _ in cases], cfg, [p for _, p in cases])
+        setup = time.time() - t0
+        b.run(1)  # warm-up iteration
+        L.check(L.lib().fvh_synchronize())
+        t1 = time.perf_counter()
+        done = b.run(it)
+        L.check(L.lib().fvh_synchronize())
+        dt = time.perf_counter() - t1
+        out[name] = dict(iters=done, seconds=dt, setup_s=setup, cell_updates_per_s=ncells * done / dt,
+                         ms_per_iteration=1e3 * dt / done)
+        log(f"[bench] batch {name}: {1e3 * dt / done:.1f} ms/iteration over {len(cases)} cases ({ncells} cells)")
+        del b
+    # the same cases, one after another, single-case FP64
+    cases = batch_cases(args.batch_cases)
+    cfg = config(it + 1)
+    total = 0.0
+    for m, p in cases:
+        s = F.Simulation(m, cfg, p)
+        s.run(1)
+        L.check(L.lib().fvh_synchronize())
+        t1 = time.perf_counter()
+        s.run(it)
+        L.check(L.lib().fvh_synchronize())
+        total += time.perf_counter() - t1
+        del s
+    out["sequential_fp64"] = dict(iters=it, seconds=total, cell_updates_per_s=ncells * it / total,
+                                  ms_per_iteration=1e3 * total / it)
+    log(f"[bench] sequential fp64: {1e3 * total / it:.1f} ms per iteration of all cases")
+    out["cases"] = len(cases)
+    out["cells"] = ncells
+    out["speedup_vs_sequential_fp64"] = out["mixed"]["cell_updates_per_s"] / out["sequential_fp64"]["cell_updates_per_s"]
+    return out
+
+
 def run_cpu(args):
     """The oracle (1 core) on the same case: a bounded sample of iterations."""
     import fvmhip as F
@@ -119,6 +169,36 @@ def run_cpu(args):
     return dict(iters=n, seconds=dt, setup_s=setup, cell_updates_per_s=mesh.ncells * n / dt, iters_per_s=n / dt)
 
 
+def profile_ops(args):
+    """One launch of each operator on a C2-pattern matrix (diagonally dominant
+    synthetic values): the plain copy (reference for traffic counters), SpMV
+    and the DILU apply in FP64 and FP32.  Few kernels, for counter passes."""
+    import fvmhip as F
+    from fvmhip import _lib as L
+
+    mesh = F.backward_facing_step(nx=args.nx, ny=args.ny)
+    dm = mesh.device()
+    h = C.c_void_p()
+    L.check(L.lib().fvh_eqn_create(dm.h, C.byref(h)))
+    nz, b = C.c_void_p(), C.c_void_p()
+    L.check(L.lib().fvh_eqn_arrays(h, C.byref(nz), C.byref(b)))
+    rp, ci = dm.pattern()
+    rows = np.repeat(np.arange(mesh.ncells), np.diff(rp))
+    vals = np.where(rows == ci, 4.5, -0.5 - 0.1 * np.random.default_rng(1).random(ci.size))
+    L.check(L.lib().fvh_memcpy(nz, vals.ctypes.data_as(C.c_void_p), vals.nbytes))
+    L.check(L.lib().fvh_precond_set(h, L.PRECOND_DILU, None))
+    L.check(L.lib().fvh_precond_update(h))
+    n = mesh.ncells
+    x = L.DeviceArray.from_host(np.random.default_rng(0).standard_normal(n))
+    y = L.DeviceArray(n)
+    ms = C.c_double()
+    for op in (5, 0, 1, 3, 4):
+        L.check(L.lib().fvh_time_op(h, op, 1, x.vp, y.vp, C.byref(ms)))
+        log(f"[profile-ops] op {op}: {ms.value * 1e3:.1f} us")
+    L.check(L.lib().fvh_synchronize())
+    L.lib().fvh_eqn_destroy(h)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -129,6 +209,11 @@ def main():
     ap.add_argument("--ny", type=int, default=500)
     ap.add_argument("--cpu-iters", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--batch-cases", type=int, default=64)
+    ap.add_argument("--batch-iters", type=int, default=5)
+    ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--profile-ops", action="store_true",
+                    help="run each operator once on the C2 case (copy, SpMV, DILU apply; FP64 and FP32), for counters")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); the multi-GPU decomposition is design-only.",
@@ -141,6 +226,9 @@ def main():
     if F.device_count() < 1:
         print("no HIP device", file=sys.stderr)
         sys.exit(1)
+    if args.profile_ops:
+        profile_ops(args)
+        return
     forms = {"fp64": [L.FP64], "mixed": [L.MIXED], "both": [L.FP64, L.MIXED]}[args.precision]
     out = {}
     for p in forms:
@@ -160,6 +248,15 @@ def main():
     # dominant kernel of the step: the DILU apply (banded sweep), called by every Krylov iteration
     dk = "dilu_apply_fp64" if hk == "fp64" else "dilu_apply_fp32"
     d = head["ops"][dk]
+    # HBM bytes per launch measured by rocprofv3 counters on this workload
+    # (FETCH_SIZE / WRITE_SIZE passes of `bench.py --profile-ops`, scaled by a
+    # plain copy of known size); null when the summary is absent
+    traffic = None
+    tf = os.path.join(ROOT, "profiles", "c2_operators_hbm_traffic.json")
+    if os.path.exists(tf) and (args.nx, args.ny) == (2000, 500):
+        with open(tf) as f:
+            t = json.load(f)
+        traffic = t["dilu_apply_fp64_bytes" if hk == "fp64" else "dilu_apply_fp32_bytes"]
     sp = head["ops"]["spmv_fp64"]
     line = {
         "metric": METRIC,
@@ -179,7 +276,7 @@ def main():
                    "cells": head["n"], "nnz": head["nnz"], "form": hk},
         "simple_iters_per_s": head["iters_per_s"],
         "roofline": {"bound": "hbm", "achieved": d["GBps"], "peak": RL.HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": d["GBps"] / RL.HBM_PEAK_GBS, "traffic": None,
+                     "frac": d["GBps"] / RL.HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": "DILU apply: k_band_gather + k_band_dilu_apply + k_band_scatter",
                      "bytes_per_launch": d["bytes"], "ms_per_launch": d["ms"]},
         "spmv": {"kernel": "k_spmv (FP64 CSR)", "GBps": sp["GBps"], "frac": sp["GBps"] / RL.HBM_PEAK_GBS,
@@ -189,6 +286,8 @@ def main():
     }
     if "fp64" in out and "mixed" in out:
         line["mixed_vs_fp64"] = out["mixed"]["iters_per_s"] / out["fp64"]["iters_per_s"]
+    if not args.no_batch:
+        line["batch"] = run_batch(args)
     print(json.dumps(line))
 
 

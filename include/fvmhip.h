@@ -379,7 +379,9 @@ int fvh_sim_equation(fvh_sim *s, const char *name, fvh_eqn **out);
  * events on the library stream.  op: 0 = SpMV y = A x (FP64), 1 =
  * preconditioner apply y = M^{-1} x, 2 = preconditioner update, 3 = FP32
  * SpMV on the single-precision copy of A, 4 = FP32 preconditioner apply
- * (ops 3/4 round x to FP32 first, all ones when x is NULL; y is unused). */
+ * (ops 3/4 round x to FP32 first, all ones when x is NULL; y is unused),
+ * 5 = plain copy y = x of ncells doubles (16 bytes per cell moved: the
+ * reference point for hardware traffic counters). */
 int fvh_time_op(fvh_eqn *e, int32_t op, int32_t reps, const double *x, double *y, double *ms_per_launch);
 
 /* simple!(model, config; pref): whole run, fields in/out (host). */

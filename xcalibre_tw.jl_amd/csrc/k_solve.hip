@@ -998,7 +998,8 @@ extern "C" int fvh_time_op(fvh_eqn *e, int32_t op, int32_t reps, const double *x
     FVH_GUARD_BEGIN
     if (!e || !ms || reps <= 0) return set_error(FVH_ERR_ARG, "bad argument");
     if ((op == 0 || op == 1) && (!x || !y)) return set_error(FVH_ERR_ARG, "null vector");
-    if (op < 0 || op > 4) return set_error(FVH_ERR_UNSUPPORTED, "fvh_time_op: op %d", op);
+    if (op < 0 || op > 5) return set_error(FVH_ERR_UNSUPPORTED, "fvh_time_op: op %d", op);
+    if (op == 5 && (!x || !y)) return set_error(FVH_ERR_ARG, "null vector");
     float *x32 = nullptr, *y32 = nullptr;
     if (op >= 3) {  // FP32 operators on the inner-solve copies, vectors from x rounded
         FVH_TRY(ensure_work32(e, 0));
@@ -1015,7 +1016,12 @@ extern "C" int fvh_time_op(fvh_eqn *e, int32_t op, int32_t reps, const double *x
             case 1: return precond_apply(e, y, x, nullptr, 1.0);
             case 2: return precond_update(e);
             case 3: return launch_spmv32(e, x32, y32, 1.0f, nullptr, nullptr, nullptr);
-            default: return precond_apply32(e, y32, x32, nullptr);
+            case 4: return precond_apply32(e, y32, x32, nullptr);
+            default: {  // plain copy of ncells doubles: the reference point of traffic counters
+                const double *xs = x;
+                double *ys = y;
+                return map(e->m->n, nullptr, [=] __device__(int i) { ys[i] = xs[i]; });
+            }
         }
     };
     FVH_TRY(launch());  // warm
