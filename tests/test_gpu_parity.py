@@ -153,11 +153,12 @@ def test_spmv_and_dilu_full_size_bitwise(bfs_full):
         L.lib().fvh_eqn_destroy(h)
 
 
-@pytest.mark.parametrize("sweep,rows", [("levels", None), ("band", "1"), ("band", "2")])
+@pytest.mark.parametrize("sweep,rows", [("levels", None), ("band", "1"), ("band", "2"), ("band", "3")])
 def test_sweep_paths_bitwise(sweep, rows, monkeypatch):
-    """Both DILU schedules (level sets / banded anti-diagonals, one or two grid
-    rows per lane) against the oracle.  60 x 300: 5 waves of grid rows at one
-    row per lane, 3 at two; every wave boundary carries the exchange."""
+    """Both DILU schedules (level sets / banded anti-diagonals: one or two grid
+    rows per lane with a barrier per step, or pipelined waves) against the
+    oracle.  60 x 300: 5 waves of grid rows at one row per lane, 3 at two;
+    every wave boundary carries the exchange."""
     if sweep == "levels":
         monkeypatch.setenv("FVH_SWEEP", "levels")
     else:

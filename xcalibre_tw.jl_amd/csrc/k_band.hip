@@ -424,6 +424,184 @@ __device__ inline void band_apply_body(const BandGeom &g, const Rec4T<T> *__rest
     }
 }
 
+// ---------------------------------------------------------------------------
+// Pipelined sweeps (one grid row per lane, no work-group barrier per step).
+//
+// Wave w needs, at step T, one value of wave w-1: its last lane's result of
+// step T-1 (forward; backward the roles of w-1 and w+1 swap).  Instead of a
+// barrier per step, waves hand these values over through a ring in LDS with
+// two step counters per wave: prod[w] (steps wave w has published) and
+// cons[w] (steps of wave w its successor has taken).  A consumer takes kHand
+// values per wait, so the LDS round trip is paid once per kHand steps; a
+// producer may run up to kRing steps ahead of its consumer.  The waves settle
+// into a skew of a few steps and none waits for the whole work-group.
+// Termination: the first producer of the chain depends on no one, each
+// consumer's need is published by a producer that is blocked only by its own
+// consumer's lag, and kRing >= 2*kHand, so every wait is eventually met; each
+// wait is also capped (kSpinLimit sleeps), so the kernel always ends.
+// The arithmetic of every row is that of band_apply_body.
+// ---------------------------------------------------------------------------
+constexpr int kHand = 4;            // handed-over values taken per wait
+constexpr int kRing = 16;           // ring slots per wave
+constexpr int kSpinLimit = 1 << 18;
+static_assert(kPD % kHand == 0 && kRing >= 2 * kHand, "pipeline sizes");
+
+__device__ inline int lds_acquire(const int *p) {
+    return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+__device__ inline void lds_release(int *p, int v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+__device__ inline void wait_geq(const int *p, int need) {
+    for (int s = 0; s < kSpinLimit && lds_acquire(p) < need; ++s) __builtin_amdgcn_s_sleep(1);
+}
+
+template <typename T>
+__device__ inline void band_apply_pipe_body(const BandGeom &g, const Rec4T<T> *__restrict__ f, Rec4T<T> *bk,
+                                            T *__restrict__ zp) {
+    __shared__ T ring[16][kRing][2];  // [producer wave][slot][value | Dinv]
+    __shared__ int prod[16], cons[16];
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    const int n = g.nx * g.ny;
+    const bool row = t < g.ny, S = t > 0, N = t < g.ny - 1;
+    const int nwaves = blockDim.x >> 6;
+    if (t < 16) {
+        prod[t] = 0;
+        cons[t] = 0;
+    }
+    __syncthreads();
+    // ---- forward: wave w takes from w-1 ----
+    {
+        const bool take = wave > 0, give = wave + 1 < nwaves;
+        Rec4T<T> buf[kPD];  // b, dinv, A_rs, A_rw
+        Walk cw, pw;
+        cw.init(g, 0);
+        pw.init(g, 0);
+#pragma unroll
+        for (int k = 0; k < kPD; ++k) {
+            buf[k] = ldrec(f, pw.pos(t, n));
+            __builtin_amdgcn_sched_barrier(0);  // issue in consumption order
+            pw.next(g);
+        }
+        T xprev = 0, dprev = 0;
+        T cx[kHand], cd[kHand];  // handed-over values of steps base .. base+kHand-1
+#pragma unroll
+        for (int q = 0; q < kHand; ++q) cx[q] = cd[q] = 0;  // step -1: nothing below row 64w yet
+        for (int T0 = 0; T0 < g.nT; T0 += kPD) {
+#pragma unroll
+            for (int k = 0; k < kPD; ++k) {
+                const int Tk = T0 + k;
+                if (k % kHand == 0 && give) wait_geq(&cons[wave], Tk + kHand - kRing);
+                if (k % kHand == 1 && take) {
+                    wait_geq(&prod[wave - 1], Tk - 1 + kHand);
+#pragma unroll
+                    for (int q = 0; q < kHand; ++q) {
+                        cx[q] = ring[wave - 1][(Tk - 1 + q) % kRing][0];
+                        cd[q] = ring[wave - 1][(Tk - 1 + q) % kRing][1];
+                    }
+                    if (lane == 0) lds_release(&cons[wave - 1], Tk - 1 + kHand);
+                }
+                const int qi = (k + kHand - 1) % kHand;  // step Tk-1 in the handed-over block
+                const Rec4T<T> c = buf[k];
+                const T xs = dpp_shr1(xprev, cx[qi]);
+                const T ds = dpp_shr1(dprev, cd[qi]);
+                const int i = cw.T - t;
+                T x = c.a;
+                const T x1 = x - c.c * xs * ds;
+                x = S ? x1 : x;
+                const T x2 = x - c.d * xprev * dprev;
+                x = (i > 0) ? x2 : x;
+                if (row && (unsigned)i < (unsigned)g.nx) st_a(bk, cw.offj + t, x);
+                xprev = x;
+                dprev = keep(c.b);
+                if (give && lane == 63) {
+                    ring[wave][Tk % kRing][0] = xprev;
+                    ring[wave][Tk % kRing][1] = dprev;
+                    if (k % kHand == kHand - 1) lds_release(&prod[wave], Tk + 1);
+                }
+                __builtin_amdgcn_sched_barrier(0);
+                buf[k] = ldrec(f, pw.pos(t, n));
+                pw.next(g);
+                cw.next(g);
+            }
+        }
+    }
+    __syncthreads();  // forward stores complete before the backward loads; counters reset below
+    if (t < 16) {
+        prod[t] = 0;
+        cons[t] = 0;
+    }
+    __syncthreads();
+    // ---- backward: wave w takes from w+1 ----
+    {
+        const bool take = wave + 1 < nwaves, give = wave > 0;
+        Rec4T<T> buf[kPD];  // y, dinv, A_re, A_rn
+        Walk cw, pw;
+        cw.init(g, g.nT - 1);
+        pw.init(g, g.nT - 1);
+#pragma unroll
+        for (int k = 0; k < kPD; ++k) {
+            buf[k] = ldrec((const Rec4T<T> *)bk, pw.pos(t, n));
+            __builtin_amdgcn_sched_barrier(0);  // issue in consumption order
+            pw.prev(g);
+        }
+        T zprev = 0;
+        T cz[kHand];
+#pragma unroll
+        for (int q = 0; q < kHand; ++q) cz[q] = 0;
+        for (int T0 = g.nT - 1, u0 = 0; T0 >= 0; T0 -= kPD, u0 += kPD) {
+#pragma unroll
+            for (int k = 0; k < kPD; ++k) {
+                const int u = u0 + k;  // steps done in walk order
+                if (k % kHand == 0 && give) wait_geq(&cons[wave], u + kHand - kRing);
+                if (k % kHand == 1 && take) {
+                    wait_geq(&prod[wave + 1], u - 1 + kHand);
+#pragma unroll
+                    for (int q = 0; q < kHand; ++q) cz[q] = ring[wave + 1][(u - 1 + q) % kRing][0];
+                    if (lane == 0) lds_release(&cons[wave + 1], u - 1 + kHand);
+                }
+                const int qi = (k + kHand - 1) % kHand;
+                const Rec4T<T> c = buf[k];
+                const T zn = dpp_shl1(zprev, cz[qi]);
+                const int i = cw.T - t;
+                T z = c.a * c.b;
+                const T z1 = z - c.c * zprev * c.b;
+                z = (i < g.nx - 1) ? z1 : z;
+                const T z2 = z - c.d * zn * c.b;
+                z = N ? z2 : z;
+                if (row && cw.T >= 0 && (unsigned)i < (unsigned)g.nx) st_val(zp, cw.offj + t, z);
+                zprev = z;
+                if (give && lane == 0) {
+                    ring[wave][u % kRing][0] = zprev;
+                    if (k % kHand == kHand - 1) lds_release(&prod[wave], u + 1);
+                }
+                __builtin_amdgcn_sched_barrier(0);
+                buf[k] = ldrec((const Rec4T<T> *)bk, pw.pos(t, n));
+                pw.prev(g);
+                cw.prev(g);
+            }
+        }
+    }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(1024) k_band_dilu_apply_pipe(BandGeom g, const Rec4T<T> *__restrict__ f,
+                                                               Rec4T<T> *bk, T *__restrict__ zp, const KScal *ks) {
+    if (ks && ks->done) return;
+    band_apply_pipe_body<T>(g, f, bk, zp);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(1024) k_band_dilu_apply_pipe_cases(const BandGeom *__restrict__ gs,
+                                                                     const int *__restrict__ off,
+                                                                     const Rec4T<T> *__restrict__ f, Rec4T<T> *bk,
+                                                                     T *__restrict__ zp, const KScal *ks) {
+    const int k = blockIdx.x;
+    if (ks && ks[k].done) return;
+    const int o = off[k];
+    band_apply_pipe_body<T>(gs[k], f + o, bk + o, zp + o);
+}
+
 template <typename T, int R>
 __global__ void __launch_bounds__(1024 / R) k_band_dilu_apply(BandGeom g, const Rec4T<T> *__restrict__ f,
                                                               Rec4T<T> *bk, T *__restrict__ zp, const KScal *ks) {
@@ -563,26 +741,29 @@ int band_alloc(fvh_eqn *e) {
 
 static int threads_for(int ny, int R) { return (((ny + R - 1) / R + 63) / 64) * 64; }
 
-// Grid rows per lane of the sweeps: 1 or 2, chosen per mesh by timing both
-// on the mesh's own records at its first update (FVH_BAND_ROWS=1|2 forces
-// one).  Two rows per lane let four waves, one per SIMD, cover a tall grid
-// and share the per-step overhead (addressing, exchange, barrier); one row
-// per lane keeps the step shorter when the grid is low.
+// Launch variant of the sweeps, chosen per mesh by timing all on the mesh's
+// own records at its first update (FVH_BAND_ROWS=1|2|3 forces one):
+//   1, 2: one / two grid rows per lane, a work-group barrier per step (two
+//         rows let four waves, one per SIMD, share the per-step overhead);
+//   3:    one row per lane, waves pipelined through LDS without barriers.
+// The variant never changes a result: the arithmetic of every row is fixed.
 static int forced_rows() {
     const char *env = std::getenv("FVH_BAND_ROWS");
-    if (env && (env[0] == '1' || env[0] == '2') && env[1] == 0) return env[0] - '0';
+    if (env && env[0] >= '1' && env[0] <= '3' && env[1] == 0) return env[0] - '0';
     return 0;
 }
+static int lanes_rows(int variant) { return variant == 2 ? 2 : 1; }
 
 struct BandLaunch {
     bool cases;
     int R, threads, blocks;
 };
 
-static BandLaunch band_launch(const fvh_mesh *m, int R) {
+static BandLaunch band_launch(const fvh_mesh *m, int V) {
+    const int R = lanes_rows(V);
     if (m->cases && m->cases->band)
-        return BandLaunch{true, R, threads_for(m->cases->band_maxny, R), m->cases->K};
-    return BandLaunch{false, R, threads_for(m->sched.g.ny, R), 1};
+        return BandLaunch{true, V, threads_for(m->cases->band_maxny, R), m->cases->K};
+    return BandLaunch{false, V, threads_for(m->sched.g.ny, R), 1};
 }
 
 static int band_rows(const fvh_mesh *m) {
@@ -590,8 +771,7 @@ static int band_rows(const fvh_mesh *m) {
     if (f) return f;
     const int t = (m->cases && m->cases->band) ? m->cases->band_R : m->sched.band_R;
     if (t) return t;
-    const int ny = (m->cases && m->cases->band) ? m->cases->band_maxny : m->sched.g.ny;
-    return ny > 256 ? 2 : 1;  // before tuning
+    return 1;  // before tuning
 }
 
 template <typename T>
@@ -599,7 +779,10 @@ static int launch_sweep(const fvh_mesh *m, int R, const Rec4T<T> *f, Rec4T<T> *b
     const BandLaunch b = band_launch(m, R);
     if (b.cases) {
         const CaseLayout &L = *m->cases;
-        if (R == 2)
+        if (R == 3)
+            hipLaunchKernelGGL((k_band_dilu_apply_pipe_cases<T>), dim3(b.blocks), dim3(b.threads), 0, stream(),
+                               L.geoms.p, L.off.p, f, bk, zp, ks);
+        else if (R == 2)
             hipLaunchKernelGGL((k_band_dilu_apply_cases<T, 2>), dim3(b.blocks), dim3(b.threads), 0, stream(), L.geoms.p,
                                L.off.p, f, bk, zp, ks);
         else
@@ -607,7 +790,9 @@ static int launch_sweep(const fvh_mesh *m, int R, const Rec4T<T> *f, Rec4T<T> *b
                                L.off.p, f, bk, zp, ks);
     } else {
         const BandGeom &g = m->sched.g;
-        if (R == 2)
+        if (R == 3)
+            hipLaunchKernelGGL((k_band_dilu_apply_pipe<T>), dim3(1), dim3(b.threads), 0, stream(), g, f, bk, zp, ks);
+        else if (R == 2)
             hipLaunchKernelGGL((k_band_dilu_apply<T, 2>), dim3(1), dim3(b.threads), 0, stream(), g, f, bk, zp, ks);
         else
             hipLaunchKernelGGL((k_band_dilu_apply<T, 1>), dim3(1), dim3(b.threads), 0, stream(), g, f, bk, zp, ks);
@@ -616,7 +801,8 @@ static int launch_sweep(const fvh_mesh *m, int R, const Rec4T<T> *f, Rec4T<T> *b
     return FVH_OK;
 }
 
-static int launch_update(const fvh_mesh *m, int R, BandRecs r) {
+static int launch_update(const fvh_mesh *m, int V, BandRecs r) {
+    const int R = lanes_rows(V);
     const BandLaunch b = band_launch(m, R);
     if (b.cases) {
         const CaseLayout &L = *m->cases;
@@ -636,9 +822,9 @@ static int launch_update(const fvh_mesh *m, int R, BandRecs r) {
     return FVH_OK;
 }
 
-// Times the FP64 sweep at 1 and 2 rows per lane on the freshly packed
-// records (the launches only rewrite the sweep's own scratch fields) and
-// keeps the faster.
+// Times the FP64 sweep in each launch variant on the freshly packed records
+// (the launches only rewrite the sweep's own scratch fields) and keeps the
+// fastest.
 static int tune_rows(fvh_eqn *e) {
     fvh_mesh *m = e->m;
     hipEvent_t a, b;
@@ -646,7 +832,7 @@ static int tune_rows(fvh_eqn *e) {
     HIP_TRY(hipEventCreate(&b));
     float best = 0.f;
     int bestR = 1;
-    for (int R = 1; R <= 2; ++R) {
+    for (int R = 1; R <= 3; ++R) {
         FVH_TRY(launch_sweep<double>(m, R, e->bd.f.p, e->bd.b.p, e->bd.zp.p, nullptr));  // warm
         HIP_TRY(hipEventRecord(a, stream()));
         for (int rep = 0; rep < 3; ++rep)
