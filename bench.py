@@ -153,6 +153,72 @@ def run_batch(args):
     return out
 
 
+def run_c3(args):
+    """C3: steady convection-diffusion on a Delaunay triangulation of seeded
+    uniform points (1M by default, ~2M triangles), u = (sin pi x cos pi y,
+    -cos pi x sin pi y), gamma = 1e-3, phi = 1 on x = 0 and 0 elsewhere; one
+    BiCGSTAB + DILU solve to rtol 1e-8, FP64 and FP32-in-FP64."""
+    import fvmhip as F
+    from fvmhip import _lib as L
+    from tests.cases import rel_l2
+
+    t0 = time.time()
+    mesh = F.delaunay_unit_square(args.c3_points, seed=0)
+    log(f"[bench] c3: mesh {mesh.ncells} cells in {time.time() - t0:.1f}s")
+    bcs = [F.Dirichlet("left", 1.0), F.Dirichlet("right", 0.0), F.Dirichlet("bottom", 0.0), F.Dirichlet("top", 0.0)]
+    out = dict(cells=mesh.ncells, levels=mesh.device().sweep_info()[0])
+    sol = {}
+    for prec, name in ((L.FP64, "fp64"), (L.MIXED, "mixed")):
+        eq = F.ScalarEquation(mesh, F.taylor_green_flux(mesh), np.full(mesh.nfaces, 1e-3), bcs)
+        L.check(L.lib().fvh_synchronize())
+        t1 = time.perf_counter()
+        eq.discretise()
+        L.check(L.lib().fvh_synchronize())
+        t_asm = time.perf_counter() - t1
+        t1 = time.perf_counter()
+        st = eq.solve(rtol=1e-8, itmax=2000, precision=prec)
+        L.check(L.lib().fvh_synchronize())
+        dt = time.perf_counter() - t1
+        sol[name] = eq.values()
+        out[name] = dict(assembly_ms=1e3 * t_asm, solve_ms=1e3 * dt, niter=st["niter"], status=st["status"],
+                         refinements=st["refinements"])
+        log(f"[bench] c3 {name}: solve {1e3 * dt:.0f} ms, {st['niter']} iterations ({st['status']})")
+        del eq
+    out["mixed_vs_fp64"] = out["fp64"]["solve_ms"] / out["mixed"]["solve_ms"]
+    out["rel_l2_mixed_vs_fp64"] = rel_l2(sol["mixed"], sol["fp64"])
+    return out
+
+
+def run_c4(args):
+    """C4: k-omega flat plate, 1000 x 500 quads graded 1.05 from the wall,
+    U = 10 m/s, nu = 1.5e-5, k / omega from 1 % intensity and viscosity ratio
+    10; five solves per SIMPLE iteration; FP64 and mixed."""
+    import fvmhip as F
+    from fvmhip import _lib as L
+    from tests.cases import config, flatplate_model
+
+    out = {}
+    for prec, name in ((L.FP64, "fp64"), (L.MIXED, "mixed")):
+        mesh = F.flat_plate(nx=1000, ny=500, ratio=1.05)
+        model = flatplate_model(mesh)
+        cfg = config(args.warmup + args.steps + 1, schemes=dict(divergence=F.Upwind), turbulent=True,
+                     p_solver=F.CgSolver, precision=prec)
+        sim = F.Simulation(model, cfg, None, L.ALG_SIMPLE, conv_break=1e-6)
+        sim.run(args.warmup)
+        L.check(L.lib().fvh_synchronize())
+        t1 = time.perf_counter()
+        done = sim.run(args.steps)
+        L.check(L.lib().fvh_synchronize())
+        dt = time.perf_counter() - t1
+        out[name] = dict(cells=mesh.ncells, iters=done, iters_per_s=done / dt, cell_updates_per_s=mesh.ncells * done / dt,
+                         ms_per_step=1e3 * dt / max(done, 1),
+                         krylov={f: sim.stats(f) for f in ("Ux", "p", "k", "omega")})
+        log(f"[bench] c4 {name}: {done / dt:.2f} it/s")
+        del sim
+    out["mixed_vs_fp64"] = out["mixed"]["iters_per_s"] / out["fp64"]["iters_per_s"]
+    return out
+
+
 def run_cpu(args):
     """The oracle (1 core) on the same case: a bounded sample of iterations."""
     import fvmhip as F
@@ -212,6 +278,9 @@ def main():
     ap.add_argument("--batch-cases", type=int, default=64)
     ap.add_argument("--batch-iters", type=int, default=5)
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--workload", default="c2", choices=["c2", "c3", "c4"],
+                    help="c2: the headline BFS (default); c3: Delaunay transport solve; c4: k-omega flat plate")
+    ap.add_argument("--c3-points", type=int, default=1_000_000)
     ap.add_argument("--profile-ops", action="store_true",
                     help="run each operator once on the C2 case (copy, SpMV, DILU apply; FP64 and FP32), for counters")
     args = ap.parse_args()
@@ -228,6 +297,10 @@ def main():
         sys.exit(1)
     if args.profile_ops:
         profile_ops(args)
+        return
+    if args.workload != "c2":
+        res = run_c3(args) if args.workload == "c3" else run_c4(args)
+        print(json.dumps({"workload": args.workload, **res}))
         return
     forms = {"fp64": [L.FP64], "mixed": [L.MIXED], "both": [L.FP64, L.MIXED]}[args.precision]
     out = {}

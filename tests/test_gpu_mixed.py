@@ -90,7 +90,7 @@ def test_fp32_operators(monkeypatch):
     for bit equal to each other and within FP32 rounding of the FP64 apply."""
     rng = np.random.default_rng(11)
     out = {}
-    for sweep in ("band1", "band2", "band3", "levels"):
+    for sweep in ("band1", "band2", "band3", "band4", "levels"):
         if sweep == "levels":
             monkeypatch.setenv("FVH_SWEEP", "levels")
         else:
@@ -121,6 +121,7 @@ def test_fp32_operators(monkeypatch):
     assert np.array_equal(out["band1"], out["levels"])
     assert np.array_equal(out["band2"], out["levels"])
     assert np.array_equal(out["band3"], out["levels"])
+    assert np.array_equal(out["band4"], out["levels"])
 
 
 @pytest.mark.parametrize("solver,pre,sym,neg", [

@@ -153,7 +153,7 @@ def test_spmv_and_dilu_full_size_bitwise(bfs_full):
         L.lib().fvh_eqn_destroy(h)
 
 
-@pytest.mark.parametrize("sweep,rows", [("levels", None), ("band", "1"), ("band", "2"), ("band", "3")])
+@pytest.mark.parametrize("sweep,rows", [("levels", None), ("band", "1"), ("band", "2"), ("band", "3"), ("band", "4")])
 def test_sweep_paths_bitwise(sweep, rows, monkeypatch):
     """Both DILU schedules (level sets / banded anti-diagonals: one or two grid
     rows per lane with a barrier per step, or pipelined waves) against the

@@ -117,15 +117,13 @@ struct alignas(4 * sizeof(T)) Rec4T {
 };
 using Rec4 = Rec4T<double>;
 using Rec4f = Rec4T<float>;
-struct alignas(16) Rec6 {
-    double a, b, c, d, e, f;
-};
-// u: (A_rr, A_rs, A_sr, A_rw, A_wr, -) for the diagonal recurrence;
+// ud, us, uw: A_rr, A_rs*A_sr and A_rw*A_wr for the diagonal recurrence
+// (the products do not depend on D: formed once, in the recurrence's order);
 // f: (b_r, Dinv_r, A_rs, A_rw) for the forward sweep;
 // b: (y_r, Dinv_r, A_re, A_rn) for the backward sweep;
 // f32 / b32: FP32 copies of f / b for the mixed-precision inner solves.
 struct BandRecs {
-    Rec6 *u;
+    double *ud, *us, *uw;
     Rec4 *f, *b;
     Rec4f *f32, *b32;
 };

@@ -70,7 +70,7 @@ struct KrylovWork {
 
 namespace fvh {
 struct BandStore {
-    DBuf<Rec6> u;
+    DBuf<double> ud, us, uw;
     DBuf<Rec4> f, b;
     DBuf<double> zp;
     DBuf<Rec4f> f32, b32;  // FP32 copies for the mixed-precision inner solves

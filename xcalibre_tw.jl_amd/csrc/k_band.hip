@@ -86,12 +86,7 @@ __device__ inline Rec4f ldrec(const Rec4f *base, unsigned idx) {
     const float4 u = *reinterpret_cast<const float4 *>(rec_at(base, idx));
     return Rec4f{u.x, u.y, u.z, u.w};
 }
-__device__ inline Rec6 ldrec(const Rec6 *base, unsigned idx) {
-    const char *p = rec_at(base, idx);
-    const double2 u = *reinterpret_cast<const double2 *>(p), v = *reinterpret_cast<const double2 *>(p + 16),
-                  w = *reinterpret_cast<const double2 *>(p + 32);
-    return Rec6{u.x, u.y, v.x, v.y, w.x, w.y};
-}
+__device__ inline double ldval(const double *base, unsigned idx) { return *reinterpret_cast<const double *>(rec_at(base, idx)); }
 // first member of record idx
 template <typename T>
 __device__ inline void st_a(Rec4T<T> *base, unsigned idx, T v) {
@@ -123,16 +118,15 @@ __global__ void k_band_pack(int n, int nx1, const int *__restrict__ cell_case, c
     const int i = rl % nx;
     const bool S = rl >= nx, W = i > 0, E = i < nx - 1, N = rl + nx < nk;
     const int d = diag[r];
-    Rec6 u;
-    u.a = val[d];
-    u.b = S ? val[rowptr[r]] : 0.0;               // A[r, r-nx]
-    u.c = S ? val[rowptr[r - nx + 1] - 1] : 0.0;  // A[r-nx, r]
-    u.d = W ? val[d - 1] : 0.0;                   // A[r, r-1]
-    u.e = W ? val[diag[r - 1] + 1] : 0.0;         // A[r-1, r]
-    u.f = 0.0;
-    br.u[p] = u;
-    br.f[p].c = u.b;
-    br.f[p].d = u.d;
+    const double a_rs = S ? val[rowptr[r]] : 0.0;               // A[r, r-nx]
+    const double a_sr = S ? val[rowptr[r - nx + 1] - 1] : 0.0;  // A[r-nx, r]
+    const double a_rw = W ? val[d - 1] : 0.0;                   // A[r, r-1]
+    const double a_wr = W ? val[diag[r - 1] + 1] : 0.0;         // A[r-1, r]
+    br.ud[p] = val[d];
+    br.us[p] = a_rs * a_sr;
+    br.uw[p] = a_rw * a_wr;
+    br.f[p].c = a_rs;
+    br.f[p].d = a_rw;
     br.b[p].c = E ? val[d + 1] : 0.0;                  // A[r, r+1]
     br.b[p].d = N ? val[rowptr[r + 1] - 1] : 0.0;      // A[r, r+nx]
 }
@@ -214,14 +208,15 @@ struct Walk {
 // LDS) for q = 0.  Its west value is its own result of step T-1.
 // ---------------------------------------------------------------------------
 template <int R>
-__device__ inline void band_update_body(const BandGeom &g, const Rec6 *__restrict__ u, Rec4 *__restrict__ fr,
-                                        Rec4 *__restrict__ br_) {
+__device__ inline void band_update_body(const BandGeom &g, const double *__restrict__ ud,
+                                        const double *__restrict__ us, const double *__restrict__ uw,
+                                        Rec4 *__restrict__ fr, Rec4 *__restrict__ br_) {
     __shared__ double sh_d[16][2];
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
     const int wl = wave > 0 ? wave - 1 : 0;
     const int n = g.nx * g.ny;
     const int j0 = t * R;
-    Rec6 buf[kPD][R];
+    double bd[kPD][R], bs[kPD][R], bw[kPD][R];  // A_rr, A_rs*A_sr, A_rw*A_wr
     Walk cw, pw;
     cw.init(g, 0);
     pw.init(g, 0);
@@ -230,7 +225,11 @@ __device__ inline void band_update_body(const BandGeom &g, const Rec6 *__restric
         {
             const unsigned pb = pw.pos_block<R>(j0, n);
 #pragma unroll
-            for (int q = 0; q < R; ++q) buf[k][q] = ldrec(u, pb + q);
+            for (int q = 0; q < R; ++q) {
+                bd[k][q] = ldval(ud, pb + q);
+                bs[k][q] = ldval(us, pb + q);
+                bw[k][q] = ldval(uw, pb + q);
+            }
         }
         __builtin_amdgcn_sched_barrier(0);  // issue in consumption order
         pw.next(g);
@@ -245,13 +244,13 @@ __device__ inline void band_update_body(const BandGeom &g, const Rec6 *__restric
             double dn[R];
 #pragma unroll
             for (int q = 0; q < R; ++q) {
-                const Rec6 c = buf[k][q];  // ad, ls, ts, lw, tw
                 const int j = j0 + q, i = cw.T - j;
                 const double ds = q == 0 ? ds0 : dprev[q - 1];
-                double d = c.a;
-                const double d1 = d - c.b * c.c / ds;
+                // D_r = (A_rr - (A_rs A_sr) / D_s) - (A_rw A_wr) / D_w
+                double d = bd[k][q];
+                const double d1 = d - bs[k][q] / ds;
                 d = (j > 0) ? d1 : d;
-                const double d2 = d - c.d * c.e / dprev[q];
+                const double d2 = d - bw[k][q] / dprev[q];
                 d = (i > 0) ? d2 : d;
                 if (j < g.ny && (unsigned)i < (unsigned)g.nx) {
                     const double r = 1.0 / d;
@@ -270,7 +269,11 @@ __device__ inline void band_update_body(const BandGeom &g, const Rec6 *__restric
             {
                 const unsigned pb = pw.pos_block<R>(j0, n);
 #pragma unroll
-                for (int q = 0; q < R; ++q) buf[k][q] = ldrec(u, pb + q);
+                for (int q = 0; q < R; ++q) {
+                    bd[k][q] = ldval(ud, pb + q);
+                    bs[k][q] = ldval(us, pb + q);
+                    bw[k][q] = ldval(uw, pb + q);
+                }
             }
             pw.next(g);
             cw.next(g);
@@ -280,7 +283,7 @@ __device__ inline void band_update_body(const BandGeom &g, const Rec6 *__restric
 
 template <int R>
 __global__ void __launch_bounds__(1024 / R) k_band_dilu_update(BandGeom g, BandRecs br) {
-    band_update_body<R>(g, br.u, br.f, br.b);
+    band_update_body<R>(g, br.ud, br.us, br.uw, br.f, br.b);
 }
 
 // one work-group per case of a batch
@@ -288,7 +291,7 @@ template <int R>
 __global__ void __launch_bounds__(1024 / R) k_band_dilu_update_cases(const BandGeom *__restrict__ gs,
                                                                      const int *__restrict__ off, BandRecs br) {
     const int k = blockIdx.x, o = off[k];
-    band_update_body<R>(gs[k], br.u + o, br.f + o, br.b + o);
+    band_update_body<R>(gs[k], br.ud + o, br.us + o, br.uw + o, br.f + o, br.b + o);
 }
 
 // ---------------------------------------------------------------------------
@@ -296,7 +299,7 @@ __global__ void __launch_bounds__(1024 / R) k_band_dilu_update_cases(const BandG
 // reference's arithmetic) or FP32 (the inner solves of mixed precision); the
 // row ownership is that of the diagonal recurrence.
 // ---------------------------------------------------------------------------
-template <typename T, int R>
+template <typename T, int R, int PD = kPD>
 __device__ inline void band_apply_body(const BandGeom &g, const Rec4T<T> *__restrict__ f, Rec4T<T> *bk,
                                        T *__restrict__ zp) {
     // [wave][step parity][value | Dinv] of the wave's edge lane
@@ -308,12 +311,12 @@ __device__ inline void band_apply_body(const BandGeom &g, const Rec4T<T> *__rest
     const int wl = wave > 0 ? wave - 1 : 0, wr = wave + 1 < nwaves ? wave + 1 : wave;
     // ---- forward: y_r = (b_r - (A_rs y_s) Dinv_s) - (A_rw y_w) Dinv_w ----
     {
-        Rec4T<T> buf[kPD][R];  // b, dinv, A_rs, A_rw
+        Rec4T<T> buf[PD][R];  // b, dinv, A_rs, A_rw
         Walk cw, pw;
         cw.init(g, 0);
         pw.init(g, 0);
 #pragma unroll
-        for (int k = 0; k < kPD; ++k) {
+        for (int k = 0; k < PD; ++k) {
             {
                 const unsigned pb = pw.pos_block<R>(j0, n);
 #pragma unroll
@@ -325,9 +328,9 @@ __device__ inline void band_apply_body(const BandGeom &g, const Rec4T<T> *__rest
         T xprev[R], dprev[R];
 #pragma unroll
         for (int q = 0; q < R; ++q) xprev[q] = dprev[q] = 0;
-        for (int T0 = 0; T0 < g.nT; T0 += kPD) {
+        for (int T0 = 0; T0 < g.nT; T0 += PD) {
 #pragma unroll
-            for (int k = 0; k < kPD; ++k) {
+            for (int k = 0; k < PD; ++k) {
                 const int par = (cw.T + 1) & 1;
                 const T xs0 = dpp_shr1(xprev[R - 1], sh[wl][par][0]);
                 const T ds0 = dpp_shr1(dprev[R - 1], sh[wl][par][1]);
@@ -371,12 +374,12 @@ __device__ inline void band_apply_body(const BandGeom &g, const Rec4T<T> *__rest
     __syncthreads();  // forward stores complete before the backward loads
     // ---- backward: z_r = (y_r Dinv_r - (A_re z_e) Dinv_r) - (A_rn z_n) Dinv_r ----
     {
-        Rec4T<T> buf[kPD][R];  // y, dinv, A_re, A_rn
+        Rec4T<T> buf[PD][R];  // y, dinv, A_re, A_rn
         Walk cw, pw;
         cw.init(g, g.nT - 1);
         pw.init(g, g.nT - 1);
 #pragma unroll
-        for (int k = 0; k < kPD; ++k) {
+        for (int k = 0; k < PD; ++k) {
             {
                 const unsigned pb = pw.pos_block<R>(j0, n);
 #pragma unroll
@@ -388,9 +391,9 @@ __device__ inline void band_apply_body(const BandGeom &g, const Rec4T<T> *__rest
         T zprev[R];  // z of (i+1, j) per row
 #pragma unroll
         for (int q = 0; q < R; ++q) zprev[q] = 0;
-        for (int T0 = g.nT - 1; T0 >= 0; T0 -= kPD) {
+        for (int T0 = g.nT - 1; T0 >= 0; T0 -= PD) {
 #pragma unroll
-            for (int k = 0; k < kPD; ++k) {
+            for (int k = 0; k < PD; ++k) {
                 // north of the lane's last row: lane t+1's first row, step T+1
                 const T znl = dpp_shl1(zprev[0], sh[wr][(cw.T + 1) & 1][0]);
                 T zq[R];
@@ -602,6 +605,27 @@ __global__ void __launch_bounds__(1024) k_band_dilu_apply_pipe_cases(const BandG
     band_apply_pipe_body<T>(gs[k], f + o, bk + o, zp + o);
 }
 
+// one row per lane, prefetch 16 steps ahead: up to 512 rows (two waves per SIMD
+// leave room for the deeper register ring)
+constexpr int kPDDeep = 16;
+template <typename T>
+__global__ void __launch_bounds__(512) k_band_dilu_apply_deep(BandGeom g, const Rec4T<T> *__restrict__ f,
+                                                              Rec4T<T> *bk, T *__restrict__ zp, const KScal *ks) {
+    if (ks && ks->done) return;
+    band_apply_body<T, 1, kPDDeep>(g, f, bk, zp);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(512) k_band_dilu_apply_deep_cases(const BandGeom *__restrict__ gs,
+                                                                    const int *__restrict__ off,
+                                                                    const Rec4T<T> *__restrict__ f, Rec4T<T> *bk,
+                                                                    T *__restrict__ zp, const KScal *ks) {
+    const int k = blockIdx.x;
+    if (ks && ks[k].done) return;
+    const int o = off[k];
+    band_apply_body<T, 1, kPDDeep>(gs[k], f + o, bk + o, zp + o);
+}
+
 template <typename T, int R>
 __global__ void __launch_bounds__(1024 / R) k_band_dilu_apply(BandGeom g, const Rec4T<T> *__restrict__ f,
                                                               Rec4T<T> *bk, T *__restrict__ zp, const KScal *ks) {
@@ -668,8 +692,8 @@ int band_setup(fvh_mesh *m) {
     const int nx = s.band_nx;
     if (nx <= 0) return FVH_OK;
     const int n = (int)m->n, ny = n / nx;
-    // records are addressed by 32-bit byte offsets (48-byte records at most)
-    if (!full_rect(m->h_rowptr, m->h_col, 0, n, nx) || (int64_t)n * (int64_t)sizeof(Rec6) >= ((int64_t)1 << 32)) {
+    // records are addressed by 32-bit byte offsets (32-byte records at most)
+    if (!full_rect(m->h_rowptr, m->h_col, 0, n, nx) || (int64_t)n * (int64_t)sizeof(Rec4) >= ((int64_t)1 << 32)) {
         s.band_nx = 0;
         return FVH_OK;
     }
@@ -693,7 +717,7 @@ int band_cases_setup(fvh_mesh *m) {
     L.band = false;
     const char *force = std::getenv("FVH_SWEEP");
     if (force && std::strcmp(force, "levels") == 0) return FVH_OK;
-    if ((int64_t)m->n * (int64_t)sizeof(Rec6) >= ((int64_t)1 << 32)) return FVH_OK;
+    if ((int64_t)m->n * (int64_t)sizeof(Rec4) >= ((int64_t)1 << 32)) return FVH_OK;
     const auto &rp = m->h_rowptr, &col = m->h_col;
     std::vector<BandGeom> geoms(L.K);
     std::vector<int> nxs(L.K), offj_all, iperm_all(m->n), toff(L.K);
@@ -729,12 +753,14 @@ int band_cases_setup(fvh_mesh *m) {
 int band_alloc(fvh_eqn *e) {
     const int64_t n = e->m->n;
     BandStore &b = e->bd;
-    if (b.u.n != (size_t)n) {
-        FVH_TRY(b.u.alloc(n));
+    if (b.ud.n != (size_t)n) {
+        FVH_TRY(b.ud.alloc(n));
+        FVH_TRY(b.us.alloc(n));
+        FVH_TRY(b.uw.alloc(n));
         FVH_TRY(b.f.alloc(n));
         FVH_TRY(b.b.alloc(n));
         FVH_TRY(b.zp.alloc(n));
-        b.r = BandRecs{b.u.p, b.f.p, b.b.p, nullptr, nullptr};
+        b.r = BandRecs{b.ud.p, b.us.p, b.uw.p, b.f.p, b.b.p, nullptr, nullptr};
     }
     return FVH_OK;
 }
@@ -745,11 +771,13 @@ static int threads_for(int ny, int R) { return (((ny + R - 1) / R + 63) / 64) * 
 // own records at its first update (FVH_BAND_ROWS=1|2|3 forces one):
 //   1, 2: one / two grid rows per lane, a work-group barrier per step (two
 //         rows let four waves, one per SIMD, share the per-step overhead);
-//   3:    one row per lane, waves pipelined through LDS without barriers.
+//   3:    one row per lane, waves pipelined through LDS without barriers;
+//   4:    one row per lane, a barrier per step, loads issued 16 steps ahead
+//         (grids up to 512 rows).
 // The variant never changes a result: the arithmetic of every row is fixed.
 static int forced_rows() {
     const char *env = std::getenv("FVH_BAND_ROWS");
-    if (env && env[0] >= '1' && env[0] <= '3' && env[1] == 0) return env[0] - '0';
+    if (env && env[0] >= '1' && env[0] <= '4' && env[1] == 0) return env[0] - '0';
     return 0;
 }
 static int lanes_rows(int variant) { return variant == 2 ? 2 : 1; }
@@ -779,7 +807,10 @@ static int launch_sweep(const fvh_mesh *m, int R, const Rec4T<T> *f, Rec4T<T> *b
     const BandLaunch b = band_launch(m, R);
     if (b.cases) {
         const CaseLayout &L = *m->cases;
-        if (R == 3)
+        if (R == 4)
+            hipLaunchKernelGGL((k_band_dilu_apply_deep_cases<T>), dim3(b.blocks), dim3(b.threads), 0, stream(),
+                               L.geoms.p, L.off.p, f, bk, zp, ks);
+        else if (R == 3)
             hipLaunchKernelGGL((k_band_dilu_apply_pipe_cases<T>), dim3(b.blocks), dim3(b.threads), 0, stream(),
                                L.geoms.p, L.off.p, f, bk, zp, ks);
         else if (R == 2)
@@ -790,7 +821,9 @@ static int launch_sweep(const fvh_mesh *m, int R, const Rec4T<T> *f, Rec4T<T> *b
                                L.off.p, f, bk, zp, ks);
     } else {
         const BandGeom &g = m->sched.g;
-        if (R == 3)
+        if (R == 4)
+            hipLaunchKernelGGL((k_band_dilu_apply_deep<T>), dim3(1), dim3(b.threads), 0, stream(), g, f, bk, zp, ks);
+        else if (R == 3)
             hipLaunchKernelGGL((k_band_dilu_apply_pipe<T>), dim3(1), dim3(b.threads), 0, stream(), g, f, bk, zp, ks);
         else if (R == 2)
             hipLaunchKernelGGL((k_band_dilu_apply<T, 2>), dim3(1), dim3(b.threads), 0, stream(), g, f, bk, zp, ks);
@@ -832,7 +865,9 @@ static int tune_rows(fvh_eqn *e) {
     HIP_TRY(hipEventCreate(&b));
     float best = 0.f;
     int bestR = 1;
-    for (int R = 1; R <= 3; ++R) {
+    const int tallest = (m->cases && m->cases->band) ? m->cases->band_maxny : m->sched.g.ny;
+    for (int R = 1; R <= 4; ++R) {
+        if (R == 4 && tallest > 512) continue;
         FVH_TRY(launch_sweep<double>(m, R, e->bd.f.p, e->bd.b.p, e->bd.zp.p, nullptr));  // warm
         HIP_TRY(hipEventRecord(a, stream()));
         for (int rep = 0; rep < 3; ++rep)

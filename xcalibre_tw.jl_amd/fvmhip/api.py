@@ -437,6 +437,93 @@ class BatchSimulation:
             pass
 
 
+class ScalarEquation:
+    """One steady scalar transport equation on the device, driven the way the
+    reference drives one (Discretise_2_generated_distretisation.jl:3-81,
+    Discretise_4_apply_bcs.jl:3-44, Solve_1_api.jl:32-46):
+
+        div(mdotf phi) - laplacian(gamma phi) == 0
+
+    `mdotf` and `gamma` are face arrays (host, length nfaces); `bcs` are
+    Dirichlet / Neumann conditions of phi; `divergence` is Linear or Upwind.
+    discretise() assembles A and b with the BCs applied; solve() runs the
+    Krylov solve from phi (FP64 or mixed precision)."""
+
+    def __init__(self, mesh, mdotf, gamma, bcs, divergence=Linear):
+        self.mesh = mesh
+        self.dm = mesh.device()
+        nf = mesh.nfaces
+        self._mdot = L.DeviceArray.from_host(np.ascontiguousarray(mdotf, np.float64).reshape(nf))
+        self._gam = L.DeviceArray.from_host(np.ascontiguousarray(gamma, np.float64).reshape(nf))
+        self.phi = L.DeviceArray.from_host(np.zeros(mesh.ncells))
+        h = C.c_void_p()
+        L.check(L.lib().fvh_eqn_create(self.dm.h, C.byref(h)), "fvh_eqn_create")
+        self.h = h
+        terms = (L.Term * 2)()
+        terms[0].op, terms[0].scheme, terms[0].sign = L.OP_DIVERGENCE, divergence, 1
+        terms[0].flux, terms[0].phi = self._mdot.ptr, self.phi.ptr
+        terms[1].op, terms[1].scheme, terms[1].sign = L.OP_LAPLACIAN, Linear, -1
+        terms[1].flux, terms[1].phi = self._gam.ptr, self.phi.ptr
+        self._terms = terms
+        self._bcs = _bcs(mesh, bcs, False)
+        self._nbc = len(bcs)
+        self._pkind = None
+
+    def discretise(self):
+        L.check(L.lib().fvh_discretise(self.h, self._terms, 2, None, 0, None, 1.0), "fvh_discretise")
+        L.check(L.lib().fvh_apply_bcs(self.h, self._terms, 2, self._bcs, self._nbc, 0), "fvh_apply_bcs")
+
+    def solve(self, solver=BicgstabSolver, preconditioner=None, itmax=1000, atol=0.0, rtol=1e-10,
+              precision=L.FP64):
+        pre = preconditioner or DILU()
+        if self._pkind != pre.kind:
+            L.check(L.lib().fvh_precond_set(self.h, pre.kind, None), "fvh_precond_set")
+            self._pkind = pre.kind
+        L.check(L.lib().fvh_precond_update(self.h), "fvh_precond_update")
+        cfg = L.SolverCfg(solver=solver, precond=pre.kind, itmax=int(itmax), precision=int(precision),
+                          convergence=0.0, relax=1.0, atol=float(atol), rtol=float(rtol))
+        st = L.Stats()
+        L.check(L.lib().fvh_solve(self.h, C.byref(cfg), self.phi.vp, C.byref(st)), "fvh_solve")
+        return dict(niter=st.niter, status=L.KRYLOV_STATUS.get(st.status, st.status), rnorm=st.rnorm,
+                    rnorm0=st.rnorm0, refinements=st.refinements)
+
+    def residual(self):
+        r = C.c_double()
+        L.check(L.lib().fvh_residual(self.h, self.phi.vp, C.byref(r)))
+        return r.value
+
+    def values(self):
+        return self.phi.host()
+
+    def matrix(self):
+        """Host copies of A (CSR values on the mesh pattern) and b."""
+        nz, b = C.c_void_p(), C.c_void_p()
+        L.check(L.lib().fvh_eqn_arrays(self.h, C.byref(nz), C.byref(b)))
+        rp, ci = self.dm.pattern()
+        vals = np.zeros(ci.size)
+        rhs = np.zeros(self.mesh.ncells)
+        L.check(L.lib().fvh_memcpy(vals.ctypes.data_as(C.c_void_p), nz, vals.nbytes))
+        L.check(L.lib().fvh_memcpy(rhs.ctypes.data_as(C.c_void_p), b, rhs.nbytes))
+        return vals, rhs
+
+    def __del__(self):
+        try:
+            if self.h:
+                L.lib().fvh_eqn_destroy(self.h)
+        except Exception:
+            pass
+
+
+def taylor_green_flux(mesh):
+    """Face mass flux of u = (sin(pi x) cos(pi y), -cos(pi x) sin(pi y)) at the
+    face centres: mdotf = u_f . S_f."""
+    x, y = mesh.face_centre[:, 0], mesh.face_centre[:, 1]
+    u = np.sin(np.pi * x) * np.cos(np.pi * y)
+    v = -np.cos(np.pi * x) * np.sin(np.pi * y)
+    S = mesh.face_area[:, None] * mesh.face_normal
+    return u * S[:, 0] + v * S[:, 1]
+
+
 def FVHError_unsupported(msg):
     return L.FVHError(L.FVH_ERR_UNSUPPORTED, msg)
 
