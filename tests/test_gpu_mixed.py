@@ -90,12 +90,15 @@ def test_fp32_operators(monkeypatch):
     for bit equal to each other and within FP32 rounding of the FP64 apply."""
     rng = np.random.default_rng(11)
     out = {}
-    for sweep in ("band1", "band2", "band3", "band4", "levels"):
+    for sweep in ("skew", "band1", "band2", "band3", "band4", "levels"):
         if sweep == "levels":
             monkeypatch.setenv("FVH_SWEEP", "levels")
         else:
             monkeypatch.delenv("FVH_SWEEP", raising=False)
-            monkeypatch.setenv("FVH_BAND_ROWS", sweep[-1])
+            if sweep == "skew":
+                monkeypatch.delenv("FVH_BAND_ROWS", raising=False)
+            else:
+                monkeypatch.setenv("FVH_BAND_ROWS", sweep[-1])
         mesh = F.backward_facing_step(nx=150, ny=300)
         assert (mesh.device().sweep_info()[1] == 150) == (sweep != "levels")
         e = Eqn(mesh)
@@ -118,6 +121,7 @@ def test_fp32_operators(monkeypatch):
             assert rel_l2(out[sweep].astype(np.float64), y64) < 1e-5
         finally:
             e.close()
+    assert np.array_equal(out["skew"], out["levels"])
     assert np.array_equal(out["band1"], out["levels"])
     assert np.array_equal(out["band2"], out["levels"])
     assert np.array_equal(out["band3"], out["levels"])

@@ -153,22 +153,30 @@ def test_spmv_and_dilu_full_size_bitwise(bfs_full):
         L.lib().fvh_eqn_destroy(h)
 
 
-@pytest.mark.parametrize("sweep,rows", [("levels", None), ("band", "1"), ("band", "2"), ("band", "3"), ("band", "4")])
+@pytest.mark.parametrize("sweep,rows", [("levels", None), ("band", None), ("band", "1"), ("band", "2"), ("band", "3"),
+                                        ("band", "4")])
 def test_sweep_paths_bitwise(sweep, rows, monkeypatch):
-    """Both DILU schedules (level sets / banded anti-diagonals: one or two grid
-    rows per lane with a barrier per step, or pipelined waves) against the
-    oracle.  60 x 300: 5 waves of grid rows at one row per lane, 3 at two;
-    every wave boundary carries the exchange."""
+    """Every DILU schedule against the oracle: level sets; the pipelined band
+    sweeps (default: skewed slots, waves handing over through LDS and a
+    courier wave per block between blocks; two blocks of three waves here, so
+    both a wave and a block boundary carry the exchange); and the
+    single-work-group band variants (one or two grid rows per lane with a
+    barrier per step, LDS-pipelined waves, deep prefetch).  60 x 300: 5 waves
+    of grid rows at one row per lane."""
     if sweep == "levels":
         monkeypatch.setenv("FVH_SWEEP", "levels")
     else:
         monkeypatch.delenv("FVH_SWEEP", raising=False)
     if rows:
         monkeypatch.setenv("FVH_BAND_ROWS", rows)
+    else:
+        monkeypatch.delenv("FVH_BAND_ROWS", raising=False)
     mesh = F.backward_facing_step(nx=60, ny=300)
     nlev, band = mesh.device().sweep_info()
     assert nlev == 60 + 300 - 1
     assert (band == 60) == (sweep == "band")
+    if sweep == "band":
+        assert mesh.device().tuning() == (int(rows) if rows else 5)
     model = bfs_model(mesh, nu=1e-3)
     g, r = _sims(model, config(4, schemes=dict(divergence=F.Upwind)), conv_break=1e-6)
     assert g.run(4) == r.run(4)

@@ -146,6 +146,37 @@ struct CaseLayout {
     DBuf<int> nxs;
 };
 
+// Skewed (fixed-stride anti-diagonal) layout of the pipelined band sweeps.
+// Grid g (a whole mesh, or one case of a batch) stores the record of cell
+// (i, j) at slot base + (i + j + kSkewD0) * S + j: every anti-diagonal
+// T = i + j owns S consecutive slots, one per grid row, and the slots of
+// rows j >= ny, of cells with i < 0 or i >= nx and of the kSkewD0 diagonals
+// below T = 0 and the prefetch margin above the last diagonal are padding.
+// Rows are split into blocks of 64*W (one compute wave per 64 rows).
+constexpr int kSkewL = 8;                    // steps per hand-over batch
+constexpr int kSkewPD = 16;                  // record prefetch distance (steps)
+constexpr int kSkewD0 = kSkewPD + 2;         // padding diagonals below T = 0
+constexpr int kSkewTop = kSkewPD;            // padding diagonals above the last step
+struct SkewGrid {
+    int nx, ny;     // cell (i, j) is natural row o + i + j*nx
+    int o;
+    int nb;         // blocks of this grid
+    int NG;         // step groups (even): steps 0 .. NG*kSkewL-1 cover the nx+ny-1 diagonals
+    int S;          // slots per diagonal (64*W*nb)
+    unsigned base;  // first slot of the grid
+    int blk0;       // first block of the grid
+};
+struct SkewLayout {
+    int W = 0;         // compute waves per block (plus one courier wave)
+    int nblk = 0;
+    int HS = 0;        // hand-over array length (values) per block boundary
+    size_t NS = 0;     // slots over all grids
+    std::vector<SkewGrid> h_grids;
+    DBuf<SkewGrid> grids;
+    DBuf<int> blk_grid;  // block -> grid
+    DBuf<int> slot_of;   // natural row -> slot
+};
+
 // Device view of a CaseLayout, by value in kernel arguments.
 struct CaseView {
     int K, nblk;
@@ -185,6 +216,8 @@ struct fvh_mesh {
     fvh::DBuf<int> ilu_uptr, ilu_pairs;
     // independent cases (batched mode); null for a single case
     std::unique_ptr<fvh::CaseLayout> cases;
+    // pipelined band sweeps (null: the mesh is not banded, or too large for them)
+    std::unique_ptr<fvh::SkewLayout> skew;
     fvh::CaseView case_view() const {
         const fvh::CaseLayout &c = *cases;
         return fvh::CaseView{c.K, c.nblk, c.off.p, c.cb.p, c.cell_case.p, c.blk_case.p, c.blk_row0.p};

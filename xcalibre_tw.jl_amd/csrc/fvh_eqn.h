@@ -77,6 +77,25 @@ struct BandStore {
     DBuf<float> zp32;
     BandRecs r{};
 };
+// Records of the pipelined band sweeps, in the mesh's skewed slot order:
+// F (b_r, Dinv_r, A_rs, A_rw), Bk (y_r, Dinv_r, A_re, A_rn), U (A_rr,
+// A_rs*A_sr, A_rw*A_wr, 0); zp the backward sweep's output; hand the
+// global hand-over arrays between blocks ([block][forward | backward][HS]),
+// uhand those of the diagonal recurrence ([block][HS]); *h_fault (pinned
+// host memory) is set by a sweep whose hand-over wait ran out.
+struct SkewStore {
+    DBuf<Rec4> F, Bk, U;
+    DBuf<double> zp, hand, uhand;
+    DBuf<Rec4f> F32, Bk32;
+    DBuf<float> zp32, hand32;
+    int *h_fault = nullptr;
+    SkewStore() = default;
+    SkewStore(const SkewStore &) = delete;
+    SkewStore &operator=(const SkewStore &) = delete;
+    ~SkewStore() {
+        if (h_fault) (void)hipHostFree(h_fault);
+    }
+};
 }  // namespace fvh
 
 struct fvh_eqn {
@@ -89,6 +108,7 @@ struct fvh_eqn {
     fvh::DBuf<double> D, Dinv, lu, pstore;
     fvh::DBuf<float> p32a, p32b;  // FP32 copies: Dinv / Jacobi scale, and A or LU values
     fvh::BandStore bd;  // DILU on banded grids (packed order)
+    fvh::SkewStore sk;  // DILU on banded grids (pipelined sweeps)
     fvh::KrylovWork kw;
     fvh::DBuf<double> red;  // residual reductions
 };

@@ -24,4 +24,20 @@ int band_dinv_natural(fvh_eqn *e, double *out);
 // grid rows per lane the sweeps use (tuned at the first update)
 int band_rows_per_lane(const fvh_mesh *m);
 
+// Pipelined band sweeps (skewed layout, waves handing over through LDS and
+// device-coherent channels); used for banded meshes unless FVH_BAND_ROWS
+// forces one of the single-work-group variants.
+struct SkewRect {
+    int o, nx, ny;  // cells o + i + j*nx, i < nx, j < ny
+};
+int skew_setup(fvh_mesh *m, const std::vector<SkewRect> &rects);
+bool skew_active(const fvh_mesh *m);
+int skew_update(fvh_eqn *e, const double *val);
+int skew_apply(fvh_eqn *e, const double *x, double *y, const KScal *ks);
+int skew_to32(fvh_eqn *e);
+int skew_apply32(fvh_eqn *e, const float *x, float *y, const KScal *ks);
+int skew_dinv_natural(fvh_eqn *e, double *out);
+// reports (once) a sweep whose hand-over wait ran out
+int skew_check(fvh_eqn *e);
+
 }  // namespace fvh

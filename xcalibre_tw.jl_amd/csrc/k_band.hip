@@ -25,51 +25,11 @@
 
 #include "fvh_eqn.h"
 #include "k_band.h"
+#include "k_dpp.h"
 
 namespace fvh {
 
 constexpr int kPD = 8;  // prefetch distance (steps)
-
-// Work-group barrier that orders LDS only: global loads issued for later
-// steps stay in flight across it (a full __syncthreads waits for them).
-__device__ inline void lds_barrier() {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
-    __builtin_amdgcn_s_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
-}
-
-// Cross-lane shifts; the lane without a source keeps `old`.
-__device__ inline double dpp_shr1(double v, double old) {  // lane l <- lane l-1; lane 0 <- old
-    int lo = __builtin_amdgcn_update_dpp(__double2loint(old), __double2loint(v), 0x138, 0xF, 0xF, false);
-    int hi = __builtin_amdgcn_update_dpp(__double2hiint(old), __double2hiint(v), 0x138, 0xF, 0xF, false);
-    return __hiloint2double(hi, lo);
-}
-__device__ inline double dpp_shl1(double v, double old) {  // lane l <- lane l+1; lane 63 <- old
-    int lo = __builtin_amdgcn_update_dpp(__double2loint(old), __double2loint(v), 0x130, 0xF, 0xF, false);
-    int hi = __builtin_amdgcn_update_dpp(__double2hiint(old), __double2hiint(v), 0x130, 0xF, 0xF, false);
-    return __hiloint2double(hi, lo);
-}
-__device__ inline float dpp_shr1(float v, float old) {
-    return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(old), __float_as_int(v), 0x138, 0xF, 0xF, false));
-}
-__device__ inline float dpp_shl1(float v, float old) {
-    return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(old), __float_as_int(v), 0x130, 0xF, 0xF, false));
-}
-
-// A register copy the compiler keeps: a value carried to the next step is
-// copied out of its prefetch slot before the slot is refilled, so the refill
-// lands in the slot's own registers (otherwise the loop rotates the whole
-// ring through copies that wait for every load in flight).
-__device__ inline double keep(double v) {
-    double r;
-    asm volatile("v_mov_b64 %0, %1" : "=v"(r) : "v"(v));
-    return r;
-}
-__device__ inline float keep(float v) {
-    float r;
-    asm volatile("v_mov_b32 %0, %1" : "=v"(r) : "v"(v));
-    return r;
-}
 
 // Record access by 32-bit byte offset from a base pointer (one VALU add per
 // address; the packed arrays stay below 4 GiB, checked at set-up).
@@ -705,7 +665,7 @@ int band_setup(fvh_mesh *m) {
     s.g.ny = ny;
     s.g.nT = nx + ny - 1;
     s.g.offj = s.b_offj.p;
-    return FVH_OK;
+    return skew_setup(m, {SkewRect{0, nx, ny}});
 }
 
 
@@ -747,7 +707,9 @@ int band_cases_setup(fvh_mesh *m) {
     L.band_R = 0;
     L.band_maxny = maxny;
     L.band = true;
-    return FVH_OK;
+    std::vector<SkewRect> rects(L.K);
+    for (int k = 0; k < L.K; ++k) rects[k] = SkewRect{L.h_off[k], nxs[k], geoms[k].ny};
+    return skew_setup(m, rects);
 }
 
 int band_alloc(fvh_eqn *e) {
@@ -894,6 +856,7 @@ static const int *band_iperm(const fvh_mesh *m) {
 
 int band_dilu_update(fvh_eqn *e, const double *val) {
     fvh_mesh *m = e->m;
+    if (skew_active(m)) return skew_update(e, val);
     const Schedule &s = m->sched;
     FVH_TRY(band_alloc(e));
     const int n = (int)m->n;
@@ -913,9 +876,10 @@ int band_dilu_update(fvh_eqn *e, const double *val) {
     return FVH_OK;
 }
 
-int band_rows_per_lane(const fvh_mesh *m) { return band_rows(m); }
+int band_rows_per_lane(const fvh_mesh *m) { return skew_active(m) ? 5 : band_rows(m); }
 
 int band_to32(fvh_eqn *e) {
+    if (skew_active(e->m)) return skew_to32(e);
     const int n = (int)e->m->n;
     BandStore &b = e->bd;
     if (b.f32.n != (size_t)n) {
@@ -947,10 +911,12 @@ static int band_apply_t(fvh_eqn *e, const T *x, T *y, const KScal *ks, Rec4T<T> 
 }
 
 int band_dilu_apply(fvh_eqn *e, const double *x, double *y, const KScal *ks) {
+    if (skew_active(e->m)) return skew_apply(e, x, y, ks);
     return band_apply_t<double>(e, x, y, ks, e->bd.f.p, e->bd.b.p, e->bd.zp.p);
 }
 
 int band_dilu_apply32(fvh_eqn *e, const float *x, float *y, const KScal *ks) {
+    if (skew_active(e->m)) return skew_apply32(e, x, y, ks);
     if (!e->bd.f32.p) return set_error(FVH_ERR_ARG, "FP32 sweep records not prepared");
     return band_apply_t<float>(e, x, y, ks, e->bd.f32.p, e->bd.b32.p, e->bd.zp32.p);
 }
@@ -958,6 +924,7 @@ int band_dilu_apply32(fvh_eqn *e, const float *x, float *y, const KScal *ks) {
 // Natural-order copy of the packed reciprocal diagonal.
 int band_dinv_natural(fvh_eqn *e, double *out) {
     fvh_mesh *m = e->m;
+    if (skew_active(m)) return skew_dinv_natural(e, out);
     const int n = (int)m->n;
     hipLaunchKernelGGL(k_band_dinv_natural, grid1d(n), dim3(256), 0, stream(), n, band_iperm(m), e->bd.r.f, out);
     HIP_TRY(hipGetLastError());

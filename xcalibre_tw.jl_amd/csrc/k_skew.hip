@@ -1,0 +1,769 @@
+// Pipelined DILU recurrences on banded grids (cells numbered i-fastest on an
+// nx x ny rectangle, lower neighbours r-1 and r-nx only).
+//
+// Lane l of compute wave w of block rb owns grid row j = 64 (W rb + w) + l
+// and, at step T, cell (T - j, j) of anti-diagonal T.  Its west value is its
+// own result of step T-1 (a register); its south value is lane l-1's result
+// of step T-1 (DPP wave_shr:1).  Lane 0's south value is the top lane of the
+// wave below: every wave collects its top lane's results over kSkewL steps
+// (a DPP shift per step) and writes them as one batch to an LDS hand-over
+// array; the wave above reads one batch per kSkewL steps, a batch ahead of
+// use.  Between blocks (on other CUs), a courier wave per block relays the
+// batches: it copies the top wave's LDS batches to a device-coherent global
+// array and the block below's batches from theirs into its own LDS.  Only
+// the courier issues device-coherent loads and stores, so their latency never
+// sits in the compute waves' in-order memory counters.  No barrier couples
+// the waves: each waits only for the batch it needs, the waves run a few
+// batches apart, and every compute wave issues at its own rate (one per
+// SIMD).  The backward sweep runs the same pipeline from the top row block
+// down (its blocks are numbered top first, so every block waits only on
+// blocks dispatched before it).
+//
+// Hand-over arrays hold, at index s+1, the value of step s (index 0: step
+// -1, a padding value); unwritten words hold a signalling-NaN sentinel, which
+// no arithmetic produces, so a sentinel means "not yet written".
+//
+// Layout: each anti-diagonal owns S slots, one per grid row (rows rounded up
+// to whole blocks, at least one padding row), so one step's records of a wave
+// are contiguous and one step is one uniform buffer offset.  Slots outside the
+// rectangle are padding whose values make every term they enter an exact +0:
+//   F = (b, Dinv, A_rs, A_rw) = (0, 1, 0, 0),  Bk = (y, Dinv, A_re, A_rn) =
+//   (y, 1, 0, 0),  U = (A_rr, A_rs A_sr, A_rw A_wr) = (1, 0, 0);
+// a padding cell's pivot evaluates to 1 and its forward / backward values to
+// +0.  A missing neighbour of a real cell has the coefficient +0 in F and U
+// and copysign(0, Dinv_r) in Bk (set by the diagonal recurrence), so its
+// product is +0 and x - (+0) == x bit for bit, also for x = -0.  Each row
+// therefore evaluates exactly the reference's expressions over its existing
+// neighbours, in the reference's order (preconditioners_1_DILU.jl:56-162):
+//   D_r   = (A_rr - (A_rs*A_sr)/D_s) - (A_rw*A_wr)/D_w,    Dinv_r = 1/D_r
+//   y_r   = (b_r - (A_rs*y_s)*Dinv_s) - (A_rw*y_w)*Dinv_w
+//   z_r   = (y_r*Dinv_r - (A_re*z_e)*Dinv_r) - (A_rn*z_n)*Dinv_r
+// and the results are bit-identical to the level-scheduled kernels for finite
+// data.  (A zero pivot or an infinite value can carry a NaN through a padding
+// slot into an edge cell where the reference has another non-finite value.)
+//
+// Every wait is bounded: one that runs out records a fault in pinned host
+// memory (the solve reports it), and the wave stops waiting and runs to its
+// end, so every launch drains.
+#include <cstdlib>
+#include <cstring>
+
+#include "fvh_eqn.h"
+#include "k_band.h"
+#include "k_dpp.h"
+
+namespace fvh {
+
+constexpr int kL = kSkewL;
+constexpr int kPDs = kSkewPD;
+static_assert(kPDs == 2 * kL, "the record ring spans two hand-over batches");
+constexpr int kSpinLimit = 1 << 19;  // idle polls (about 30 ms) before a hand-over is declared lost
+constexpr int kWin = 32;             // hand-over words a courier moves per poll (4 batches)
+
+template <typename T>
+struct HandWord;
+template <>
+struct HandWord<double> {
+    using U = unsigned long long;
+    static constexpr U kSent = 0x7FF4DEAD5EA11A57ull;
+};
+template <>
+struct HandWord<float> {
+    using U = unsigned int;
+    static constexpr U kSent = 0x7FA11A57u;
+};
+template <typename T>
+__device__ __host__ inline T hand_sentinel() {
+    return __builtin_bit_cast(T, HandWord<T>::kSent);
+}
+template <typename T>
+__device__ inline bool is_sentinel(T v) {
+    return __builtin_bit_cast(typename HandWord<T>::U, v) == HandWord<T>::kSent;
+}
+// device-coherent global words (courier only)
+template <typename T>
+__device__ inline T gload(const T *p) {
+    using U = typename HandWord<T>::U;
+    return __builtin_bit_cast(
+        T, __hip_atomic_load(reinterpret_cast<U *>(const_cast<T *>(p)), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+}
+template <typename T>
+__device__ inline void gstore(T *p, T v) {
+    using U = typename HandWord<T>::U;
+    __hip_atomic_store(reinterpret_cast<U *>(p), __builtin_bit_cast(U, v), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+// LDS words shared by the waves of a block
+template <typename T>
+__device__ inline T lload(const T *p) {
+    using U = typename HandWord<T>::U;
+    return __builtin_bit_cast(T, __hip_atomic_load(reinterpret_cast<U *>(const_cast<T *>(p)), __ATOMIC_RELAXED,
+                                                   __HIP_MEMORY_SCOPE_WORKGROUP));
+}
+template <typename T>
+__device__ inline void lstore(T *p, T v) {
+    using U = typename HandWord<T>::U;
+    __hip_atomic_store(reinterpret_cast<U *>(p), __builtin_bit_cast(U, v), __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+__device__ inline void record_fault(int *fault) {
+    if ((threadIdx.x & 63) == 0) __hip_atomic_store(fault, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
+// Re-reads an LDS batch until the lanes `mine` hold no sentinel; bounded.
+template <typename T>
+__device__ inline T wait_batch(const T *p, bool mine, T v, int *fault, int &broken) {
+    for (int s = 0; s < kSpinLimit && __builtin_amdgcn_ballot_w64(mine && is_sentinel(v)); ++s) {
+        __builtin_amdgcn_s_sleep(1);
+        v = lload(p);
+    }
+    if (__builtin_amdgcn_ballot_w64(mine && is_sentinel(v))) {
+        record_fault(fault);
+        broken = 1;
+        v = is_sentinel(v) ? T(0) : v;
+    }
+    return v;
+}
+
+struct SkewArgs {
+    const SkewGrid *grids;
+    const int *blk_grid;
+    int nblk, HS;
+    int ks_per_grid;  // KScal k belongs to grid k (batched cases)
+    unsigned rbytes;  // bytes of one record array (buffer range)
+    int *fault;
+};
+
+// Buffer access: a per-lane byte offset (fixed for the sweep) plus a uniform
+// per-step offset.
+using Rsrc = __amdgpu_buffer_rsrc_t;
+__device__ inline Rsrc rsrc(const void *p, unsigned bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), (short)0, (int)bytes, 0x00020000);
+}
+__device__ inline Rec4 bload(Rsrc r, unsigned vo, unsigned so, Rec4 *) {
+    const auto u = __builtin_bit_cast(double2, __builtin_amdgcn_raw_buffer_load_b128(r, vo, so, 0));
+    const auto v = __builtin_bit_cast(double2, __builtin_amdgcn_raw_buffer_load_b128(r, vo + 16, so, 0));
+    return Rec4{u.x, u.y, v.x, v.y};
+}
+__device__ inline Rec4f bload(Rsrc r, unsigned vo, unsigned so, Rec4f *) {
+    const auto u = __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(r, vo, so, 0));
+    return Rec4f{u.x, u.y, u.z, u.w};
+}
+typedef unsigned u32x2 __attribute__((ext_vector_type(2)));
+__device__ inline void bstore(Rsrc r, double v, unsigned vo, unsigned so) {
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), r, vo, so, 0);
+}
+__device__ inline void bstore(Rsrc r, float v, unsigned vo, unsigned so) {
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, vo, so, 0);
+}
+
+// Whole-wave lane shifts for the hand-over batches: the next batch position
+// moves to the edge lane (the vacated lane reads 0, never used).
+__device__ inline double shl1z(double v) {
+    const int lo = __builtin_amdgcn_mov_dpp(__double2loint(v), 0x130, 0xF, 0xF, true);
+    const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), 0x130, 0xF, 0xF, true);
+    return __hiloint2double(hi, lo);
+}
+__device__ inline double shr1z(double v) {
+    const int lo = __builtin_amdgcn_mov_dpp(__double2loint(v), 0x138, 0xF, 0xF, true);
+    const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), 0x138, 0xF, 0xF, true);
+    return __hiloint2double(hi, lo);
+}
+__device__ inline float shl1z(float v) {
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x130, 0xF, 0xF, true));
+}
+__device__ inline float shr1z(float v) {
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x138, 0xF, 0xF, true));
+}
+
+// Resets the block's LDS hand-over arrays ([W+1][HS]; array c lies below
+// compute wave c): index 0 holds the padding value of step -1.
+template <typename T>
+__device__ inline void lds_reset(T *lds, int count, int HS, T first) {
+    for (int q = threadIdx.x; q < count; q += blockDim.x) lds[q] = (q % HS == 0) ? first : hand_sentinel<T>();
+}
+
+// The courier wave of a block: moves hand-over batches from the global array
+// gin (the neighbour block's, device-coherent) into LDS array lin, and from
+// LDS array lout into the block's own global array gout, kWin words per poll,
+// in whole batches, as they become available.  Indices 1 .. NG*kL.
+template <typename T>
+__device__ void courier(const T *gin, T *lin, const T *lout, T *gout, int NG, int *fault) {
+    const int lane = threadIdx.x & 63;
+    const bool act = lane < kWin;
+    int gi = gin ? 0 : NG, go = gout ? 0 : NG;  // batches moved
+    int idle = 0;
+    while ((gi < NG || go < NG) && idle < kSpinLimit) {
+        bool moved = false;
+        if (gi < NG) {
+            const int idx = gi * kL + 1 + (act ? lane : 0);
+            const T v = gload(gin + idx);
+            // whole batches at the front of the window that are complete
+            const unsigned long long bad = __builtin_amdgcn_ballot_w64(act && is_sentinel(v));
+            const int ready = bad ? (__builtin_ctzll(bad) / kL) : kWin / kL;
+            const int nb = min(ready, NG - gi);
+            if (nb > 0) {
+                if (lane < nb * kL) lstore(lin + idx, v);
+                gi += nb;
+                moved = true;
+            }
+        }
+        if (go < NG) {
+            const int idx = go * kL + 1 + (act ? lane : 0);
+            const T v = lload(lout + idx);
+            const unsigned long long bad = __builtin_amdgcn_ballot_w64(act && is_sentinel(v));
+            const int ready = bad ? (__builtin_ctzll(bad) / kL) : kWin / kL;
+            const int nb = min(ready, NG - go);
+            if (nb > 0) {
+                if (lane < nb * kL) gstore(gout + idx, v);
+                go += nb;
+                moved = true;
+            }
+        }
+        if (moved) {
+            idle = 0;
+        } else {
+            ++idle;
+            __builtin_amdgcn_s_sleep(1);
+        }
+    }
+    if (gi < NG || go < NG) record_fault(fault);
+}
+
+// ---------------------------------------------------------------------------
+// Diagonal recurrence (update_dilu_diagonal!): Dinv into F.b and Bk.b
+// ---------------------------------------------------------------------------
+template <int W>
+__global__ void __launch_bounds__(64 * (W + 1))
+    k_skew_update(SkewArgs a, const Rec4 *__restrict__ U, Rec4 *__restrict__ F, Rec4 *__restrict__ Bk,
+                  double *__restrict__ hand) {
+    extern __shared__ unsigned char skew_lds[];
+    double *lds = reinterpret_cast<double *>(skew_lds);
+    const int b = blockIdx.x;
+    const SkewGrid g = a.grids[a.blk_grid[b]];
+    const int HS = a.HS, rb = b - g.blk0;
+    lds_reset(lds, (W + 1) * HS, HS, 1.0);
+    __syncthreads();
+    const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    if (w == W) {  // courier: D of the row below from block rb-1, the top row's D to block rb+1
+        courier(rb > 0 ? hand + (size_t)(b - 1) * HS : nullptr, lds, lds + W * HS,
+                rb < g.nb - 1 ? hand + (size_t)b * HS : nullptr, g.NG, a.fault);
+        return;
+    }
+    const int lane = threadIdx.x & 63, wg = rb * W + w;  // wave of the grid
+    const unsigned S = g.S, j0 = wg * 64, j = j0 + lane;
+    const bool src = wg > 0;
+    const double *lin = lds + w * HS;
+    double *lout = lds + (w + 1) * HS;
+    const Rsrc ru = rsrc(U, a.rbytes), rf = rsrc(F, a.rbytes), rbk = rsrc(Bk, a.rbytes);
+    const unsigned vo = (g.base + j) * (unsigned)sizeof(Rec4), step = S * (unsigned)sizeof(Rec4);
+    auto so = [&](int T) { return (unsigned)(T + kSkewD0) * step; };
+    const int ql = lane < kL ? lane : kL - 1;
+    const int Teast = (int)j + g.nx - 1;  // step of the row's east-edge cell
+    const bool row = (int)j < g.ny, top = (int)j == g.ny - 1;
+
+    double hb[2] = {src ? lload(lin + ql) : 1.0, src ? lload(lin + kL + ql) : 1.0};
+    Rec4 ring[kPDs];
+#pragma unroll
+    for (int k = 0; k < kPDs; ++k) {
+        ring[k] = bload(ru, vo, so(k), (Rec4 *)nullptr);
+        __builtin_amdgcn_sched_barrier(0);  // issue in consumption order
+    }
+    double dprev = 1.0, pub = 1.0;
+    int broken = 0;
+    for (int G0 = 0; G0 < g.NG; G0 += 2) {
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const int G = G0 + h;
+            double cur = hb[h];
+            if (src && !broken && __builtin_amdgcn_ballot_w64(lane < kL && is_sentinel(cur)))
+                cur = wait_batch(lin + G * kL + ql, lane < kL, cur, a.fault, broken);
+#pragma unroll
+            for (int k = 0; k < kL; ++k) {
+                const int r = h * kL + k, T = G * kL + k;
+                const Rec4 u = ring[r];
+                const double nxt = shl1z(cur);
+                const double ds = dpp_shr1(dprev, cur);
+                cur = nxt;
+                const double d = (u.a - u.b / ds) - u.c / dprev;
+                const double di = 1.0 / d;
+                bstore(rf, di, vo + 8, so(T));
+                bstore(rbk, di, vo + 8, so(T));
+                if (row && T == Teast) bstore(rbk, copysign(0.0, di), vo + 16, so(T));
+                if (top && (unsigned)(T - (int)j) < (unsigned)g.nx) bstore(rbk, copysign(0.0, di), vo + 24, so(T));
+                pub = dpp_shl1(pub, d);
+                dprev = d;
+                __builtin_amdgcn_sched_barrier(0);  // the slot is consumed: refill it below
+                ring[r] = bload(ru, vo, so(T + kPDs), (Rec4 *)nullptr);
+            }
+            if (lane >= 64 - kL) lstore(lout + G * kL + 1 + lane - (64 - kL), pub);
+            hb[h] = src ? lload(lin + (G + 2) * kL + ql) : 1.0;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// ldiv!: forward sweep into Bk.a, backward sweep into zp (FP64: the
+// reference's arithmetic; FP32: the inner solves of mixed precision)
+// ---------------------------------------------------------------------------
+template <typename T, int W>
+__global__ void __launch_bounds__(64 * (W + 1))
+    k_skew_fwd(SkewArgs a, const Rec4T<T> *__restrict__ F, Rec4T<T> *__restrict__ Bk, T *__restrict__ hand,
+               const KScal *ks) {
+    using R = Rec4T<T>;
+    extern __shared__ unsigned char skew_lds[];
+    T *lds = reinterpret_cast<T *>(skew_lds);
+    const int b = blockIdx.x;
+    const int gi = a.blk_grid[b];
+    if (ks && ks[a.ks_per_grid ? gi : 0].done) return;
+    const SkewGrid g = a.grids[gi];
+    const int HS = a.HS, rb = b - g.blk0;
+    lds_reset(lds, (W + 1) * HS, HS, T(0));
+    __syncthreads();
+    const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    if (w == W) {  // courier: the forward channel of block b is hand[2b]
+        courier(rb > 0 ? hand + (size_t)(b - 1) * 2 * HS : nullptr, lds, lds + W * HS,
+                rb < g.nb - 1 ? hand + (size_t)b * 2 * HS : nullptr, g.NG, a.fault);
+        return;
+    }
+    const int lane = threadIdx.x & 63, wg = rb * W + w;
+    const unsigned S = g.S, j0 = wg * 64;
+    const bool src = wg > 0;
+    const T *lin = lds + w * HS;
+    T *lout = lds + (w + 1) * HS;
+    const Rsrc rf = rsrc(F, a.rbytes), rbk = rsrc(Bk, a.rbytes);
+    const unsigned vo = (g.base + j0 + lane) * (unsigned)sizeof(R), step = S * (unsigned)sizeof(R);
+    auto so = [&](int t) { return (unsigned)(t + kSkewD0) * step; };
+    // lane 0's south pivot: Dinv of (step T-1, row j0-1), static; lane q < kL
+    // of the batch of group G holds step G*kL + q
+    const int ql = lane < kL ? lane : kL - 1;
+    const T *dp = &F[g.base + (size_t)(ql - 1 + kSkewD0) * S + j0 - 1].b;
+    const size_t hstep = (size_t)kL * S * 4;  // one batch of steps, in T
+
+    T yb[2] = {src ? lload(lin + ql) : T(0), src ? lload(lin + kL + ql) : T(0)};
+    T db[2] = {dp[0], dp[hstep]};
+    dp += 2 * hstep;
+    // The prologue stores +0 into the wave's own pre-start padding (its value
+    // already), so that it issues what a step issues and the loop's memory
+    // waits count the same outstanding operations on every iteration.
+    R ring[kPDs];
+#pragma unroll
+    for (int k = 0; k < kPDs; ++k) {
+        bstore(rbk, T(0), vo, so(k - kPDs));
+        ring[k] = bload(rf, vo, so(k), (R *)nullptr);
+        __builtin_amdgcn_sched_barrier(0);
+    }
+    T xprev = 0, dprev = 1, pub = 0;
+    int broken = 0;
+    for (int G0 = 0; G0 < g.NG; G0 += 2) {
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const int G = G0 + h;
+            T yc = yb[h], dc = db[h];
+            if (src && !broken && __builtin_amdgcn_ballot_w64(lane < kL && is_sentinel(yc)))
+                yc = wait_batch(lin + G * kL + ql, lane < kL, yc, a.fault, broken);
+#pragma unroll
+            for (int k = 0; k < kL; ++k) {
+                const int r = h * kL + k, Ts = G * kL + k;
+                const R c = ring[r];
+                const T yn = shl1z(yc), dn = shl1z(dc);
+                const T xs = dpp_shr1(xprev, yc);
+                const T ds = dpp_shr1(dprev, dc);
+                yc = yn;
+                dc = dn;
+                const T x = (c.a - c.c * xs * ds) - c.d * xprev * dprev;
+                bstore(rbk, x, vo, so(Ts));
+                pub = dpp_shl1(pub, x);
+                xprev = x;
+                dprev = keep(c.b);
+                __builtin_amdgcn_sched_barrier(0);
+                ring[r] = bload(rf, vo, so(Ts + kPDs), (R *)nullptr);
+            }
+            if (lane >= 64 - kL) lstore(lout + G * kL + 1 + lane - (64 - kL), pub);
+            yb[h] = src ? lload(lin + (G + 2) * kL + ql) : T(0);
+            db[h] = *dp;
+            dp += hstep;
+        }
+    }
+}
+
+template <typename T, int W>
+__global__ void __launch_bounds__(64 * (W + 1))
+    k_skew_bwd(SkewArgs a, const Rec4T<T> *__restrict__ Bk, T *__restrict__ zp, T *__restrict__ hand,
+               const KScal *ks) {
+    using R = Rec4T<T>;
+    extern __shared__ unsigned char skew_lds[];
+    T *lds = reinterpret_cast<T *>(skew_lds);
+    const int b = a.nblk - 1 - (int)blockIdx.x;  // top row blocks first
+    const int gi = a.blk_grid[b];
+    if (ks && ks[a.ks_per_grid ? gi : 0].done) return;
+    const SkewGrid g = a.grids[gi];
+    const int HS = a.HS, rb = b - g.blk0;
+    lds_reset(lds, (W + 1) * HS, HS, T(0));
+    __syncthreads();
+    const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    if (w == W) {  // courier: the backward channel of block b is hand[2b+1]
+        courier(rb < g.nb - 1 ? hand + ((size_t)(b + 1) * 2 + 1) * HS : nullptr, lds + W * HS, lds,
+                rb > 0 ? hand + ((size_t)b * 2 + 1) * HS : nullptr, g.NG, a.fault);
+        return;
+    }
+    const int lane = threadIdx.x & 63, wg = rb * W + w;
+    const unsigned S = g.S, j0 = wg * 64;
+    const bool src = wg < g.nb * W - 1;  // a wave above exists
+    const int Ttop = g.NG * kL - 1;      // step u handles anti-diagonal Ttop - u
+    const T *lin = lds + (w + 1) * HS;
+    T *lout = lds + w * HS;
+    const Rsrc rbk = rsrc(Bk, a.rbytes), rz = rsrc(zp, a.rbytes / 4);
+    const unsigned vo = (g.base + j0 + lane) * (unsigned)sizeof(R), step = S * (unsigned)sizeof(R);
+    const unsigned voz = (g.base + j0 + lane) * (unsigned)sizeof(T), stepz = S * (unsigned)sizeof(T);
+    auto so = [&](int t) { return (unsigned)(t + kSkewD0) * step; };
+    auto soz = [&](int t) { return (unsigned)(t + kSkewD0) * stepz; };
+    // lane 63-q of the batch of group G holds step u = G*kL + q (index u)
+    const int qh = 63 - lane < kL ? 63 - lane : kL - 1;
+
+    T zb[2] = {src ? lload(lin + qh) : T(0), src ? lload(lin + kL + qh) : T(0)};
+    // prologue stores: +0 into the padding diagonals above the top (see the
+    // forward sweep)
+    R ring[kPDs];
+#pragma unroll
+    for (int k = 0; k < kPDs; ++k) {
+        bstore(rz, T(0), voz, soz(Ttop + kPDs - k));
+        ring[k] = bload(rbk, vo, so(Ttop - k), (R *)nullptr);
+        __builtin_amdgcn_sched_barrier(0);
+    }
+    T zprev = 0, pub = 0;
+    int broken = 0;
+    for (int G0 = 0; G0 < g.NG; G0 += 2) {
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const int G = G0 + h;
+            T zc = zb[h];
+            if (src && !broken && __builtin_amdgcn_ballot_w64(lane >= 64 - kL && is_sentinel(zc)))
+                zc = wait_batch(lin + G * kL + qh, lane >= 64 - kL, zc, a.fault, broken);
+#pragma unroll
+            for (int k = 0; k < kL; ++k) {
+                const int r = h * kL + k, Tb = Ttop - (G * kL + k);
+                const R c = ring[r];
+                const T zn_next = shr1z(zc);
+                const T zn = dpp_shl1(zprev, zc);
+                zc = zn_next;
+                const T z = ((c.a * c.b) - c.c * zprev * c.b) - c.d * zn * c.b;
+                bstore(rz, z, voz, soz(Tb));
+                pub = dpp_shr1(pub, z);
+                zprev = z;
+                __builtin_amdgcn_sched_barrier(0);
+                ring[r] = bload(rbk, vo, so(Tb - kPDs), (R *)nullptr);
+            }
+            if (lane < kL) lstore(lout + G * kL + kL - lane, pub);
+            zb[h] = src ? lload(lin + (G + 2) * kL + qh) : T(0);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// All-CU helpers: padding, packing, gather / scatter, FP32 copies
+// ---------------------------------------------------------------------------
+__global__ void k_skew_init(size_t ns, Rec4 *F, Rec4 *Bk, Rec4 *U, double *zp) {
+    const size_t s = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= ns) return;
+    F[s] = Rec4{0.0, 1.0, 0.0, 0.0};
+    Bk[s] = Rec4{0.0, 1.0, 0.0, 0.0};
+    U[s] = Rec4{1.0, 0.0, 0.0, 0.0};
+    zp[s] = 0.0;
+}
+
+template <typename T>
+__global__ void k_skew_fill(size_t ns, T *x, T v) {
+    const size_t s = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s < ns) x[s] = v;
+}
+
+// Natural CSR values of the full-rectangle pattern -> skewed records; also
+// resets the diagonal recurrence's global hand-over arrays.
+__global__ void k_skew_pack(int n, const SkewGrid *__restrict__ grids, const int *__restrict__ cell_grid,
+                            const int *__restrict__ slot_of, const int *__restrict__ rowptr,
+                            const int *__restrict__ diag, const double *__restrict__ val, Rec4 *__restrict__ F,
+                            Rec4 *__restrict__ Bk, Rec4 *__restrict__ U, double *__restrict__ hand, int nhand,
+                            int HS) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p < nhand) hand[p] = (p % HS == 0) ? 1.0 : hand_sentinel<double>();
+    if (p >= n) return;
+    const SkewGrid g = grids[cell_grid ? cell_grid[p] : 0];
+    const int rl = p - g.o, nx = g.nx;
+    const int i = rl % nx, j = rl / nx;
+    const bool S = j > 0, Wn = i > 0, E = i < nx - 1, N = j < g.ny - 1;
+    const int d = diag[p];
+    const double a_rs = S ? val[rowptr[p]] : 0.0;               // A[r, r-nx]
+    const double a_sr = S ? val[rowptr[p - nx + 1] - 1] : 0.0;  // A[r-nx, r]
+    const double a_rw = Wn ? val[d - 1] : 0.0;                  // A[r, r-1]
+    const double a_wr = Wn ? val[diag[p - 1] + 1] : 0.0;        // A[r-1, r]
+    const int s = slot_of[p];
+    U[s] = Rec4{val[d], a_rs * a_sr, a_rw * a_wr, 0.0};
+    F[s].c = a_rs;
+    F[s].d = a_rw;
+    Bk[s].c = E ? val[d + 1] : 0.0;                // A[r, r+1]
+    Bk[s].d = N ? val[rowptr[p + 1] - 1] : 0.0;    // A[r, r+nx]
+}
+
+// x (natural) -> F.a; also resets the sweeps' global hand-over arrays.
+template <typename T>
+__global__ void k_skew_gather(int n, const int *__restrict__ slot_of, const T *__restrict__ x,
+                              Rec4T<T> *__restrict__ F, T *__restrict__ hand, int nhand, int HS, const KScal *ks) {
+    if (ks && ks->done) return;
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p < nhand) hand[p] = (p % HS == 0) ? T(0) : hand_sentinel<T>();
+    if (p < n) F[slot_of[p]].a = x[p];
+}
+
+template <typename T>
+__global__ void k_skew_scatter(int n, const int *__restrict__ slot_of, const T *__restrict__ zp, T *__restrict__ y,
+                               const KScal *ks) {
+    if (ks && ks->done) return;
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p < n) y[p] = zp[slot_of[p]];
+}
+
+__global__ void k_skew_to32(size_t ns, const Rec4 *__restrict__ F, const Rec4 *__restrict__ Bk,
+                            Rec4f *__restrict__ F32, Rec4f *__restrict__ Bk32) {
+    const size_t s = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= ns) return;
+    const Rec4 u = F[s], v = Bk[s];
+    F32[s] = Rec4f{0.f, (float)u.b, (float)u.c, (float)u.d};
+    Bk32[s] = Rec4f{0.f, (float)v.b, (float)v.c, (float)v.d};
+}
+
+__global__ void k_skew_dinv_natural(int n, const int *__restrict__ slot_of, const Rec4 *__restrict__ F,
+                                    double *__restrict__ out) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p < n) out[p] = F[slot_of[p]].b;
+}
+
+// ---------------------------------------------------------------------------
+// Host side
+// ---------------------------------------------------------------------------
+static dim3 grid_for(size_t n) { return dim3((unsigned)((n + 255) / 256)); }
+
+static int max_lds_bytes() {
+    static int v = -1;
+    if (v < 0) {
+        int dev = 0, b = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&b, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess || b <= 0)
+            b = 64 * 1024;
+        v = b;
+    }
+    return v;
+}
+
+int skew_setup(fvh_mesh *m, const std::vector<SkewRect> &rects) {
+    m->skew.reset();
+    const char *force = std::getenv("FVH_SWEEP");
+    if (force && std::strcmp(force, "levels") == 0) return FVH_OK;
+    if (rects.empty()) return FVH_OK;
+    int maxny = 1, maxNG = 0;
+    auto ng_of = [](const SkewRect &r) { return 2 * ((r.nx + r.ny - 1 + 2 * kL - 1) / (2 * kL)); };
+    for (const SkewRect &r : rects) {
+        maxny = std::max(maxny, r.ny);
+        maxNG = std::max(maxNG, ng_of(r));
+    }
+    // hand-over arrays: indices up to NG*kL, read a window (kWin) or two
+    // batches past the last one
+    const int HS = (maxNG * kL + kWin + 2 * kL + 64 + 63) / 64 * 64;
+    // up to three compute waves and the courier: one wave per SIMD
+    int W = std::min(3, (maxny + 1 + 63) / 64);
+    while (W > 1 && (size_t)(W + 1) * HS * sizeof(double) > (size_t)max_lds_bytes()) --W;
+    if ((size_t)(W + 1) * HS * sizeof(double) > (size_t)max_lds_bytes()) return FVH_OK;  // too long for LDS
+    auto L = std::make_unique<SkewLayout>();
+    L->W = W;
+    L->HS = HS;
+    size_t base = 0;
+    int blk = 0;
+    std::vector<int> blk_grid;
+    for (size_t k = 0; k < rects.size(); ++k) {
+        const SkewRect &r = rects[k];
+        SkewGrid g{};
+        g.nx = r.nx;
+        g.ny = r.ny;
+        g.o = r.o;
+        g.nb = (r.ny + 1 + 64 * W - 1) / (64 * W);
+        g.NG = ng_of(r);
+        g.S = 64 * W * g.nb;
+        g.base = (unsigned)base;
+        g.blk0 = blk;
+        const size_t nd = (size_t)g.NG * kL + kSkewD0 + kSkewTop;
+        base += nd * (size_t)g.S;
+        // records are addressed by 32-bit byte offsets
+        if (base * sizeof(Rec4) >= ((size_t)1 << 31)) return FVH_OK;
+        for (int q = 0; q < g.nb; ++q) blk_grid.push_back((int)k);
+        blk += g.nb;
+        L->h_grids.push_back(g);
+    }
+    L->NS = base;
+    L->nblk = blk;
+    std::vector<int> slot_of(m->n, 0);
+    for (const SkewGrid &g : L->h_grids)
+        for (int j = 0; j < g.ny; ++j)
+            for (int i = 0; i < g.nx; ++i)
+                slot_of[(size_t)g.o + i + (size_t)j * g.nx] =
+                    (int)(g.base + (size_t)(i + j + kSkewD0) * g.S + j);
+    FVH_TRY(L->grids.upload(L->h_grids));
+    FVH_TRY(L->blk_grid.upload(blk_grid));
+    FVH_TRY(L->slot_of.upload(slot_of));
+    m->skew = std::move(L);
+    return FVH_OK;
+}
+
+bool skew_active(const fvh_mesh *m) {
+    if (!m->skew) return false;
+    const char *env = std::getenv("FVH_BAND_ROWS");
+    return !(env && env[0] >= '1' && env[0] <= '4' && env[1] == 0);
+}
+
+static const int *cell_grid(const fvh_mesh *m) { return m->cases ? m->cases->cell_case.p : nullptr; }
+
+static int skew_alloc(fvh_eqn *e) {
+    const SkewLayout &L = *e->m->skew;
+    SkewStore &s = e->sk;
+    if (s.F.n == L.NS) return FVH_OK;
+    FVH_TRY(s.F.alloc(L.NS));
+    FVH_TRY(s.Bk.alloc(L.NS));
+    FVH_TRY(s.U.alloc(L.NS));
+    FVH_TRY(s.zp.alloc(L.NS));
+    FVH_TRY(s.hand.alloc((size_t)L.nblk * 2 * L.HS));
+    FVH_TRY(s.uhand.alloc((size_t)L.nblk * L.HS));
+    if (!s.h_fault) {
+        HIP_TRY(hipHostMalloc((void **)&s.h_fault, sizeof(int), hipHostMallocDefault));
+        *s.h_fault = 0;
+    }
+    hipLaunchKernelGGL(k_skew_init, grid_for(L.NS), dim3(256), 0, stream(), L.NS, s.F.p, s.Bk.p, s.U.p, s.zp.p);
+    HIP_TRY(hipGetLastError());
+    return FVH_OK;
+}
+
+template <typename R>
+static SkewArgs skew_args(const fvh_eqn *e) {
+    const SkewLayout &L = *e->m->skew;
+    return SkewArgs{L.grids.p, L.blk_grid.p, L.nblk, L.HS, e->m->cases ? 1 : 0, (unsigned)(L.NS * sizeof(R)),
+                    e->sk.h_fault};
+}
+
+// Dynamic LDS beyond 64 KiB has to be allowed per kernel.
+template <typename K>
+static int allow_lds(K kernel, size_t bytes) {
+    if (bytes > 64 * 1024)
+        HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void *>(kernel), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    (int)bytes));
+    return FVH_OK;
+}
+
+template <int W>
+static int launch_update_w(fvh_eqn *e) {
+    const SkewLayout &L = *e->m->skew;
+    const size_t lds = (size_t)(W + 1) * L.HS * sizeof(double);
+    FVH_TRY(allow_lds(k_skew_update<W>, lds));
+    hipLaunchKernelGGL(k_skew_update<W>, dim3(L.nblk), dim3(64 * (W + 1)), lds, stream(), skew_args<Rec4>(e),
+                       e->sk.U.p, e->sk.F.p, e->sk.Bk.p, e->sk.uhand.p);
+    HIP_TRY(hipGetLastError());
+    return FVH_OK;
+}
+
+int skew_update(fvh_eqn *e, const double *val) {
+    fvh_mesh *m = e->m;
+    const SkewLayout &L = *m->skew;
+    FVH_TRY(skew_alloc(e));
+    const int n = (int)m->n, nhand = L.nblk * L.HS;
+    hipLaunchKernelGGL(k_skew_pack, grid_for(std::max(n, nhand)), dim3(256), 0, stream(), n, L.grids.p, cell_grid(m),
+                       L.slot_of.p, m->d.rowptr, m->d.diag, val, e->sk.F.p, e->sk.Bk.p, e->sk.U.p, e->sk.uhand.p,
+                       nhand, L.HS);
+    HIP_TRY(hipGetLastError());
+    switch (L.W) {
+        case 1: return launch_update_w<1>(e);
+        case 2: return launch_update_w<2>(e);
+        case 3: return launch_update_w<3>(e);
+        default: return launch_update_w<4>(e);
+    }
+}
+
+template <typename T, int W>
+static int launch_sweeps_w(fvh_eqn *e, Rec4T<T> *F, Rec4T<T> *Bk, T *zp, T *hand, const KScal *ks) {
+    const SkewLayout &L = *e->m->skew;
+    const SkewArgs a = skew_args<Rec4T<T>>(e);
+    const size_t lds = (size_t)(W + 1) * L.HS * sizeof(T);
+    FVH_TRY(allow_lds(k_skew_fwd<T, W>, lds));
+    FVH_TRY(allow_lds(k_skew_bwd<T, W>, lds));
+    hipLaunchKernelGGL((k_skew_fwd<T, W>), dim3(L.nblk), dim3(64 * (W + 1)), lds, stream(), a, F, Bk, hand, ks);
+    HIP_TRY(hipGetLastError());
+    hipLaunchKernelGGL((k_skew_bwd<T, W>), dim3(L.nblk), dim3(64 * (W + 1)), lds, stream(), a, Bk, zp, hand, ks);
+    HIP_TRY(hipGetLastError());
+    return FVH_OK;
+}
+
+template <typename T>
+static int skew_apply_t(fvh_eqn *e, const T *x, T *y, const KScal *ks, Rec4T<T> *F, Rec4T<T> *Bk, T *zp, T *hand) {
+    fvh_mesh *m = e->m;
+    const SkewLayout &L = *m->skew;
+    const int n = (int)m->n, nhand = 2 * L.nblk * L.HS;
+    // per-case solves: gather / scatter cover every case, the sweeps skip
+    // finished cases
+    const KScal *ks_all = m->cases ? nullptr : ks;
+    hipLaunchKernelGGL((k_skew_gather<T>), grid_for(std::max(n, nhand)), dim3(256), 0, stream(), n, L.slot_of.p, x, F,
+                       hand, nhand, L.HS, ks_all);
+    HIP_TRY(hipGetLastError());
+    switch (L.W) {
+        case 1: FVH_TRY((launch_sweeps_w<T, 1>(e, F, Bk, zp, hand, ks))); break;
+        case 2: FVH_TRY((launch_sweeps_w<T, 2>(e, F, Bk, zp, hand, ks))); break;
+        case 3: FVH_TRY((launch_sweeps_w<T, 3>(e, F, Bk, zp, hand, ks))); break;
+        default: FVH_TRY((launch_sweeps_w<T, 4>(e, F, Bk, zp, hand, ks))); break;
+    }
+    hipLaunchKernelGGL((k_skew_scatter<T>), grid_for(n), dim3(256), 0, stream(), n, L.slot_of.p, zp, y, ks_all);
+    HIP_TRY(hipGetLastError());
+    return FVH_OK;
+}
+
+int skew_apply(fvh_eqn *e, const double *x, double *y, const KScal *ks) {
+    SkewStore &s = e->sk;
+    if (!s.F.p) return set_error(FVH_ERR_ARG, "DILU records not prepared");
+    return skew_apply_t<double>(e, x, y, ks, s.F.p, s.Bk.p, s.zp.p, s.hand.p);
+}
+
+int skew_to32(fvh_eqn *e) {
+    const SkewLayout &L = *e->m->skew;
+    SkewStore &s = e->sk;
+    if (s.F32.n != L.NS) {
+        FVH_TRY(s.F32.alloc(L.NS));
+        FVH_TRY(s.Bk32.alloc(L.NS));
+        FVH_TRY(s.zp32.alloc(L.NS));
+        FVH_TRY(s.hand32.alloc((size_t)L.nblk * 2 * L.HS));
+        hipLaunchKernelGGL(k_skew_fill<float>, grid_for(L.NS), dim3(256), 0, stream(), L.NS, s.zp32.p, 0.f);
+        HIP_TRY(hipGetLastError());
+    }
+    hipLaunchKernelGGL(k_skew_to32, grid_for(L.NS), dim3(256), 0, stream(), L.NS, s.F.p, s.Bk.p, s.F32.p, s.Bk32.p);
+    HIP_TRY(hipGetLastError());
+    return FVH_OK;
+}
+
+int skew_apply32(fvh_eqn *e, const float *x, float *y, const KScal *ks) {
+    SkewStore &s = e->sk;
+    if (!s.F32.p) return set_error(FVH_ERR_ARG, "FP32 sweep records not prepared");
+    return skew_apply_t<float>(e, x, y, ks, s.F32.p, s.Bk32.p, s.zp32.p, s.hand32.p);
+}
+
+int skew_dinv_natural(fvh_eqn *e, double *out) {
+    const int n = (int)e->m->n;
+    hipLaunchKernelGGL(k_skew_dinv_natural, grid_for(n), dim3(256), 0, stream(), n, e->m->skew->slot_of.p, e->sk.F.p,
+                       out);
+    HIP_TRY(hipGetLastError());
+    return FVH_OK;
+}
+
+// A hand-over wait of a sweep ran out since the last check (reported once).
+int skew_check(fvh_eqn *e) {
+    int *f = e->sk.h_fault;
+    if (f && __atomic_load_n(f, __ATOMIC_ACQUIRE)) {
+        *f = 0;
+        return set_error(FVH_ERR_HIP, "band sweep: a hand-over between waves did not arrive in time");
+    }
+    return FVH_OK;
+}
+
+}  // namespace fvh

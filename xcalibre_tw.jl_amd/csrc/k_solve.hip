@@ -873,7 +873,7 @@ static int solve_mixed(fvh_eqn *e, const fvh_solver_cfg &cfg, double *x, fvh_sta
     return FVH_OK;
 }
 
-int solve(fvh_eqn *e, const fvh_solver_cfg &cfg, double *x, fvh_stats *st) {
+static int solve_any(fvh_eqn *e, const fvh_solver_cfg &cfg, double *x, fvh_stats *st) {
     fvh_stats local{};
     if (!st) st = &local;
     std::memset(st, 0, sizeof *st);
@@ -890,6 +890,12 @@ int solve(fvh_eqn *e, const fvh_solver_cfg &cfg, double *x, fvh_stats *st) {
     double sigma = 1.0;
     if (cfg.solver == FVH_CG) FVH_TRY(cg_sigma(e, &sigma));
     return krylov(S, cfg, x, st, sigma);
+}
+
+int solve(fvh_eqn *e, const fvh_solver_cfg &cfg, double *x, fvh_stats *st) {
+    const int rc = solve_any(e, cfg, x, st);
+    const int fault = skew_check(e);
+    return rc != FVH_OK ? rc : fault;
 }
 
 }  // namespace fvh
