@@ -153,30 +153,32 @@ def test_spmv_and_dilu_full_size_bitwise(bfs_full):
         L.lib().fvh_eqn_destroy(h)
 
 
-@pytest.mark.parametrize("sweep,rows", [("levels", None), ("band", None), ("band", "1"), ("band", "2"), ("band", "3"),
-                                        ("band", "4")])
+@pytest.mark.parametrize("sweep,rows", [("levels", None), ("band", None), ("band", "w1"), ("band", "w2"),
+                                        ("band", "w3"), ("band", "1"), ("band", "2"), ("band", "3"), ("band", "4")])
 def test_sweep_paths_bitwise(sweep, rows, monkeypatch):
     """Every DILU schedule against the oracle: level sets; the pipelined band
-    sweeps (default: skewed slots, waves handing over through LDS and a
-    courier wave per block between blocks; two blocks of three waves here, so
-    both a wave and a block boundary carry the exchange); and the
-    single-work-group band variants (one or two grid rows per lane with a
-    barrier per step, LDS-pipelined waves, deep prefetch).  60 x 300: 5 waves
-    of grid rows at one row per lane."""
+    sweeps (default: skewed slots, waves handing over through LDS within a
+    block and through a courier wave per block between blocks; tuned, or one,
+    two or three waves per block, so block boundaries fall between different
+    waves); and the single-work-group band variants (one or two grid rows per
+    lane with a barrier per step, LDS-pipelined waves, deep prefetch).
+    60 x 300: 5 waves of grid rows at one row per lane."""
     if sweep == "levels":
         monkeypatch.setenv("FVH_SWEEP", "levels")
     else:
         monkeypatch.delenv("FVH_SWEEP", raising=False)
-    if rows:
+    monkeypatch.delenv("FVH_BAND_ROWS", raising=False)
+    monkeypatch.delenv("FVH_SKEW_WAVES", raising=False)
+    if rows and rows.startswith("w"):
+        monkeypatch.setenv("FVH_SKEW_WAVES", rows[1:])
+    elif rows:
         monkeypatch.setenv("FVH_BAND_ROWS", rows)
-    else:
-        monkeypatch.delenv("FVH_BAND_ROWS", raising=False)
     mesh = F.backward_facing_step(nx=60, ny=300)
     nlev, band = mesh.device().sweep_info()
     assert nlev == 60 + 300 - 1
     assert (band == 60) == (sweep == "band")
     if sweep == "band":
-        assert mesh.device().tuning() == (int(rows) if rows else 5)
+        assert mesh.device().tuning() == (int(rows) if rows and not rows.startswith("w") else 5)
     model = bfs_model(mesh, nu=1e-3)
     g, r = _sims(model, config(4, schemes=dict(divergence=F.Upwind)), conv_break=1e-6)
     assert g.run(4) == r.run(4)

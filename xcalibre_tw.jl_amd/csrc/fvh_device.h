@@ -149,32 +149,38 @@ struct CaseLayout {
 // Skewed (fixed-stride anti-diagonal) layout of the pipelined band sweeps.
 // Grid g (a whole mesh, or one case of a batch) stores the record of cell
 // (i, j) at slot base + (i + j + kSkewD0) * S + j: every anti-diagonal
-// T = i + j owns S consecutive slots, one per grid row, and the slots of
-// rows j >= ny, of cells with i < 0 or i >= nx and of the kSkewD0 diagonals
-// below T = 0 and the prefetch margin above the last diagonal are padding.
-// Rows are split into blocks of 64*W (one compute wave per 64 rows).
+// T = i + j owns S consecutive slots, one per grid row (S a multiple of 64
+// above ny), and the slots of rows j >= ny, of cells with i < 0 or i >= nx
+// and of the kSkewD0 diagonals below T = 0 and the prefetch margin above the
+// last diagonal are padding.  One compute wave owns 64 rows; a block runs W
+// consecutive waves (W is a launch parameter, tuned per mesh).
 constexpr int kSkewL = 8;                    // steps per hand-over batch
 constexpr int kSkewPD = 16;                  // record prefetch distance (steps)
 constexpr int kSkewD0 = kSkewPD + 2;         // padding diagonals below T = 0
 constexpr int kSkewTop = kSkewPD;            // padding diagonals above the last step
+constexpr int kSkewMaxW = 3;                 // compute waves per block (plus one courier wave)
 struct SkewGrid {
     int nx, ny;     // cell (i, j) is natural row o + i + j*nx
     int o;
-    int nb;         // blocks of this grid
+    int nw;         // compute waves (S / 64)
     int NG;         // step groups (even): steps 0 .. NG*kSkewL-1 cover the nx+ny-1 diagonals
-    int S;          // slots per diagonal (64*W*nb)
+    int S;          // slots per diagonal
     unsigned base;  // first slot of the grid
-    int blk0;       // first block of the grid
+    int blk0[kSkewMaxW];  // first block of the grid, W = 1 .. kSkewMaxW waves per block
+};
+struct SkewTile {
+    int grid, i0, j0;
 };
 struct SkewLayout {
-    int W = 0;         // compute waves per block (plus one courier wave)
-    int nblk = 0;
-    int HS = 0;        // hand-over array length (values) per block boundary
-    size_t NS = 0;     // slots over all grids
+    int W = 0;                 // waves per block in use (0: not tuned yet)
+    int HS = 0;                // hand-over array length (values) per block boundary
+    size_t NS = 0;             // slots over all grids
+    int nblk[kSkewMaxW] = {};  // blocks over all grids, per W
     std::vector<SkewGrid> h_grids;
     DBuf<SkewGrid> grids;
-    DBuf<int> blk_grid;  // block -> grid
-    DBuf<int> slot_of;   // natural row -> slot
+    DBuf<int> blk_grid[kSkewMaxW];  // block -> grid, per W
+    DBuf<int> slot_of;              // natural row -> slot
+    DBuf<SkewTile> tiles;           // 64 x 64 tiles of the grids (gather / scatter)
 };
 
 // Device view of a CaseLayout, by value in kernel arguments.

@@ -241,15 +241,16 @@ __global__ void __launch_bounds__(64 * (W + 1))
     double *lds = reinterpret_cast<double *>(skew_lds);
     const int b = blockIdx.x;
     const SkewGrid g = a.grids[a.blk_grid[b]];
-    const int HS = a.HS, rb = b - g.blk0;
+    const int HS = a.HS, rb = b - g.blk0[W - 1], nb = (g.nw + W - 1) / W;
     lds_reset(lds, (W + 1) * HS, HS, 1.0);
     __syncthreads();
     const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
     if (w == W) {  // courier: D of the row below from block rb-1, the top row's D to block rb+1
         courier(rb > 0 ? hand + (size_t)(b - 1) * HS : nullptr, lds, lds + W * HS,
-                rb < g.nb - 1 ? hand + (size_t)b * HS : nullptr, g.NG, a.fault);
+                rb < nb - 1 ? hand + (size_t)b * HS : nullptr, g.NG, a.fault);
         return;
     }
+    if (rb * W + w >= g.nw) return;  // past the grid's last wave (last block)
     const int lane = threadIdx.x & 63, wg = rb * W + w;  // wave of the grid
     const unsigned S = g.S, j0 = wg * 64, j = j0 + lane;
     const bool src = wg > 0;
@@ -317,15 +318,16 @@ __global__ void __launch_bounds__(64 * (W + 1))
     const int gi = a.blk_grid[b];
     if (ks && ks[a.ks_per_grid ? gi : 0].done) return;
     const SkewGrid g = a.grids[gi];
-    const int HS = a.HS, rb = b - g.blk0;
+    const int HS = a.HS, rb = b - g.blk0[W - 1], nb = (g.nw + W - 1) / W;
     lds_reset(lds, (W + 1) * HS, HS, T(0));
     __syncthreads();
     const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
     if (w == W) {  // courier: the forward channel of block b is hand[2b]
         courier(rb > 0 ? hand + (size_t)(b - 1) * 2 * HS : nullptr, lds, lds + W * HS,
-                rb < g.nb - 1 ? hand + (size_t)b * 2 * HS : nullptr, g.NG, a.fault);
+                rb < nb - 1 ? hand + (size_t)b * 2 * HS : nullptr, g.NG, a.fault);
         return;
     }
+    if (rb * W + w >= g.nw) return;  // past the grid's last wave (last block)
     const int lane = threadIdx.x & 63, wg = rb * W + w;
     const unsigned S = g.S, j0 = wg * 64;
     const bool src = wg > 0;
@@ -398,18 +400,19 @@ __global__ void __launch_bounds__(64 * (W + 1))
     const int gi = a.blk_grid[b];
     if (ks && ks[a.ks_per_grid ? gi : 0].done) return;
     const SkewGrid g = a.grids[gi];
-    const int HS = a.HS, rb = b - g.blk0;
+    const int HS = a.HS, rb = b - g.blk0[W - 1], nb = (g.nw + W - 1) / W;
     lds_reset(lds, (W + 1) * HS, HS, T(0));
     __syncthreads();
     const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
     if (w == W) {  // courier: the backward channel of block b is hand[2b+1]
-        courier(rb < g.nb - 1 ? hand + ((size_t)(b + 1) * 2 + 1) * HS : nullptr, lds + W * HS, lds,
+        courier(rb < nb - 1 ? hand + ((size_t)(b + 1) * 2 + 1) * HS : nullptr, lds + W * HS, lds,
                 rb > 0 ? hand + ((size_t)b * 2 + 1) * HS : nullptr, g.NG, a.fault);
         return;
     }
+    if (rb * W + w >= g.nw) return;  // past the grid's last wave (last block)
     const int lane = threadIdx.x & 63, wg = rb * W + w;
     const unsigned S = g.S, j0 = wg * 64;
-    const bool src = wg < g.nb * W - 1;  // a wave above exists
+    const bool src = wg < g.nw - 1;  // a wave above exists
     const int Ttop = g.NG * kL - 1;      // step u handles anti-diagonal Ttop - u
     const T *lin = lds + (w + 1) * HS;
     T *lout = lds + w * HS;
@@ -505,22 +508,68 @@ __global__ void k_skew_pack(int n, const SkewGrid *__restrict__ grids, const int
     Bk[s].d = N ? val[rowptr[p + 1] - 1] : 0.0;    // A[r, r+nx]
 }
 
-// x (natural) -> F.a; also resets the sweeps' global hand-over arrays.
-template <typename T>
-__global__ void k_skew_gather(int n, const int *__restrict__ slot_of, const T *__restrict__ x,
-                              Rec4T<T> *__restrict__ F, T *__restrict__ hand, int nhand, int HS, const KScal *ks) {
-    if (ks && ks->done) return;
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p < nhand) hand[p] = (p % HS == 0) ? T(0) : hand_sentinel<T>();
-    if (p < n) F[slot_of[p]].a = x[p];
-}
+// Natural order <-> slot order in 64 x 64 tiles of one grid (rows j0..j0+63,
+// columns i0..i0+63), through LDS: the natural side is read / written along
+// grid rows, the slot side along anti-diagonals (consecutive rows of one
+// diagonal are consecutive slots), so both sides are coalesced.
+constexpr int kTile = 64;
 
 template <typename T>
-__global__ void k_skew_scatter(int n, const int *__restrict__ slot_of, const T *__restrict__ zp, T *__restrict__ y,
-                               const KScal *ks) {
+struct TileLds {
+    T v[kTile][kTile + 1];
+};
+
+// x (natural) -> F.a; blocks beyond the tiles reset the sweeps' global
+// hand-over arrays.
+template <typename T>
+__global__ void __launch_bounds__(256)
+    k_skew_gather(int ntiles, const SkewTile *__restrict__ tiles, const SkewGrid *__restrict__ grids,
+                  const T *__restrict__ x, Rec4T<T> *__restrict__ F, T *__restrict__ hand, int nhand, int HS,
+                  const KScal *ks) {
     if (ks && ks->done) return;
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p < n) y[p] = zp[slot_of[p]];
+    if ((int)blockIdx.x >= ntiles) {
+        const int p = ((int)blockIdx.x - ntiles) * 256 + threadIdx.x;
+        if (p < nhand) hand[p] = (p % HS == 0) ? T(0) : hand_sentinel<T>();
+        return;
+    }
+    __shared__ TileLds<T> t;
+    const SkewTile tl = tiles[blockIdx.x];
+    const SkewGrid g = grids[tl.grid];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int i = tl.i0 + lane;
+    for (int r = wv; r < kTile; r += 4) {
+        const int j = tl.j0 + r;
+        if (i < g.nx && j < g.ny) t.v[r][lane] = x[(size_t)g.o + i + (size_t)j * g.nx];
+    }
+    __syncthreads();
+    for (int d = wv; d < 2 * kTile - 1; d += 4) {  // diagonal T = i0 + j0 + d, row j0 + lane
+        const int j = tl.j0 + lane, il = d - lane;
+        if (il >= 0 && il < kTile && tl.i0 + il < g.nx && j < g.ny)
+            F[g.base + (size_t)(tl.i0 + tl.j0 + d + kSkewD0) * g.S + j].a = t.v[lane][il];
+    }
+}
+
+// zp (slots) -> y (natural)
+template <typename T>
+__global__ void __launch_bounds__(256)
+    k_skew_scatter(const SkewTile *__restrict__ tiles, const SkewGrid *__restrict__ grids, const T *__restrict__ zp,
+                   T *__restrict__ y, const KScal *ks) {
+    if (ks && ks->done) return;
+    __shared__ TileLds<T> t;
+    const SkewTile tl = tiles[blockIdx.x];
+    const SkewGrid g = grids[tl.grid];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    for (int d = wv; d < 2 * kTile - 1; d += 4) {
+        const int j = tl.j0 + lane, il = d - lane;
+        if (il >= 0 && il < kTile && tl.i0 + il < g.nx && j < g.ny)
+            t.v[lane][il] = zp[g.base + (size_t)(tl.i0 + tl.j0 + d + kSkewD0) * g.S + j];
+    }
+    __syncthreads();
+    const int i = tl.i0 + lane;
+    for (int r = wv; r < kTile; r += 4) {
+        const int j = tl.j0 + r;
+        if (i < g.nx && j < g.ny) y[(size_t)g.o + i + (size_t)j * g.nx] = t.v[r][lane];
+    }
 }
 
 __global__ void k_skew_to32(size_t ns, const Rec4 *__restrict__ F, const Rec4 *__restrict__ Bk,
@@ -560,55 +609,54 @@ int skew_setup(fvh_mesh *m, const std::vector<SkewRect> &rects) {
     const char *force = std::getenv("FVH_SWEEP");
     if (force && std::strcmp(force, "levels") == 0) return FVH_OK;
     if (rects.empty()) return FVH_OK;
-    int maxny = 1, maxNG = 0;
+    int maxNG = 0;
     auto ng_of = [](const SkewRect &r) { return 2 * ((r.nx + r.ny - 1 + 2 * kL - 1) / (2 * kL)); };
-    for (const SkewRect &r : rects) {
-        maxny = std::max(maxny, r.ny);
-        maxNG = std::max(maxNG, ng_of(r));
-    }
+    for (const SkewRect &r : rects) maxNG = std::max(maxNG, ng_of(r));
     // hand-over arrays: indices up to NG*kL, read a window (kWin) or two
     // batches past the last one
     const int HS = (maxNG * kL + kWin + 2 * kL + 64 + 63) / 64 * 64;
-    // up to three compute waves and the courier: one wave per SIMD
-    int W = std::min(3, (maxny + 1 + 63) / 64);
-    while (W > 1 && (size_t)(W + 1) * HS * sizeof(double) > (size_t)max_lds_bytes()) --W;
-    if ((size_t)(W + 1) * HS * sizeof(double) > (size_t)max_lds_bytes()) return FVH_OK;  // too long for LDS
+    if ((size_t)2 * HS * sizeof(double) > (size_t)max_lds_bytes()) return FVH_OK;  // too long even for W = 1
     auto L = std::make_unique<SkewLayout>();
-    L->W = W;
     L->HS = HS;
     size_t base = 0;
-    int blk = 0;
-    std::vector<int> blk_grid;
+    std::vector<int> blk_grid[kSkewMaxW];
     for (size_t k = 0; k < rects.size(); ++k) {
         const SkewRect &r = rects[k];
         SkewGrid g{};
         g.nx = r.nx;
         g.ny = r.ny;
         g.o = r.o;
-        g.nb = (r.ny + 1 + 64 * W - 1) / (64 * W);
+        g.nw = (r.ny + 1 + 63) / 64;  // at least one padding row
         g.NG = ng_of(r);
-        g.S = 64 * W * g.nb;
+        g.S = 64 * g.nw;
         g.base = (unsigned)base;
-        g.blk0 = blk;
+        for (int W = 1; W <= kSkewMaxW; ++W) {
+            g.blk0[W - 1] = L->nblk[W - 1];
+            const int nb = (g.nw + W - 1) / W;
+            for (int q = 0; q < nb; ++q) blk_grid[W - 1].push_back((int)k);
+            L->nblk[W - 1] += nb;
+        }
         const size_t nd = (size_t)g.NG * kL + kSkewD0 + kSkewTop;
         base += nd * (size_t)g.S;
         // records are addressed by 32-bit byte offsets
         if (base * sizeof(Rec4) >= ((size_t)1 << 31)) return FVH_OK;
-        for (int q = 0; q < g.nb; ++q) blk_grid.push_back((int)k);
-        blk += g.nb;
         L->h_grids.push_back(g);
     }
     L->NS = base;
-    L->nblk = blk;
     std::vector<int> slot_of(m->n, 0);
     for (const SkewGrid &g : L->h_grids)
         for (int j = 0; j < g.ny; ++j)
             for (int i = 0; i < g.nx; ++i)
                 slot_of[(size_t)g.o + i + (size_t)j * g.nx] =
                     (int)(g.base + (size_t)(i + j + kSkewD0) * g.S + j);
+    std::vector<SkewTile> tiles;
+    for (size_t k = 0; k < L->h_grids.size(); ++k)
+        for (int j0 = 0; j0 < L->h_grids[k].ny; j0 += kTile)
+            for (int i0 = 0; i0 < L->h_grids[k].nx; i0 += kTile) tiles.push_back(SkewTile{(int)k, i0, j0});
     FVH_TRY(L->grids.upload(L->h_grids));
-    FVH_TRY(L->blk_grid.upload(blk_grid));
+    for (int W = 1; W <= kSkewMaxW; ++W) FVH_TRY(L->blk_grid[W - 1].upload(blk_grid[W - 1]));
     FVH_TRY(L->slot_of.upload(slot_of));
+    FVH_TRY(L->tiles.upload(tiles));
     m->skew = std::move(L);
     return FVH_OK;
 }
@@ -617,6 +665,24 @@ bool skew_active(const fvh_mesh *m) {
     if (!m->skew) return false;
     const char *env = std::getenv("FVH_BAND_ROWS");
     return !(env && env[0] >= '1' && env[0] <= '4' && env[1] == 0);
+}
+
+// Waves per block: FVH_SKEW_WAVES=1..3 forces one, otherwise the tuned value
+// (3 before tuning), never more than the LDS holds.
+static int forced_waves() {
+    const char *env = std::getenv("FVH_SKEW_WAVES");
+    if (env && env[0] >= '1' && env[0] <= '0' + kSkewMaxW && env[1] == 0) return env[0] - '0';
+    return 0;
+}
+static int lds_waves_max(const SkewLayout &L) {
+    int W = kSkewMaxW;
+    while (W > 1 && (size_t)(W + 1) * L.HS * sizeof(double) > (size_t)max_lds_bytes()) --W;
+    return W;
+}
+static int skew_waves(const SkewLayout &L) {
+    const int f = forced_waves();
+    const int w = f ? f : (L.W ? L.W : kSkewMaxW);
+    return std::min(w, lds_waves_max(L));
 }
 
 static const int *cell_grid(const fvh_mesh *m) { return m->cases ? m->cases->cell_case.p : nullptr; }
@@ -629,8 +695,8 @@ static int skew_alloc(fvh_eqn *e) {
     FVH_TRY(s.Bk.alloc(L.NS));
     FVH_TRY(s.U.alloc(L.NS));
     FVH_TRY(s.zp.alloc(L.NS));
-    FVH_TRY(s.hand.alloc((size_t)L.nblk * 2 * L.HS));
-    FVH_TRY(s.uhand.alloc((size_t)L.nblk * L.HS));
+    FVH_TRY(s.hand.alloc((size_t)L.nblk[0] * 2 * L.HS));  // W = 1 has the most blocks
+    FVH_TRY(s.uhand.alloc((size_t)L.nblk[0] * L.HS));
     if (!s.h_fault) {
         HIP_TRY(hipHostMalloc((void **)&s.h_fault, sizeof(int), hipHostMallocDefault));
         *s.h_fault = 0;
@@ -641,10 +707,10 @@ static int skew_alloc(fvh_eqn *e) {
 }
 
 template <typename R>
-static SkewArgs skew_args(const fvh_eqn *e) {
+static SkewArgs skew_args(const fvh_eqn *e, int W) {
     const SkewLayout &L = *e->m->skew;
-    return SkewArgs{L.grids.p, L.blk_grid.p, L.nblk, L.HS, e->m->cases ? 1 : 0, (unsigned)(L.NS * sizeof(R)),
-                    e->sk.h_fault};
+    return SkewArgs{L.grids.p, L.blk_grid[W - 1].p, L.nblk[W - 1], L.HS, e->m->cases ? 1 : 0,
+                    (unsigned)(L.NS * sizeof(R)), e->sk.h_fault};
 }
 
 // Dynamic LDS beyond 64 KiB has to be allowed per kernel.
@@ -661,39 +727,42 @@ static int launch_update_w(fvh_eqn *e) {
     const SkewLayout &L = *e->m->skew;
     const size_t lds = (size_t)(W + 1) * L.HS * sizeof(double);
     FVH_TRY(allow_lds(k_skew_update<W>, lds));
-    hipLaunchKernelGGL(k_skew_update<W>, dim3(L.nblk), dim3(64 * (W + 1)), lds, stream(), skew_args<Rec4>(e),
+    hipLaunchKernelGGL(k_skew_update<W>, dim3(L.nblk[W - 1]), dim3(64 * (W + 1)), lds, stream(), skew_args<Rec4>(e, W),
                        e->sk.U.p, e->sk.F.p, e->sk.Bk.p, e->sk.uhand.p);
     HIP_TRY(hipGetLastError());
     return FVH_OK;
 }
 
+static int skew_tune(fvh_eqn *e);
+
 int skew_update(fvh_eqn *e, const double *val) {
     fvh_mesh *m = e->m;
     const SkewLayout &L = *m->skew;
     FVH_TRY(skew_alloc(e));
-    const int n = (int)m->n, nhand = L.nblk * L.HS;
+    const int n = (int)m->n, nhand = L.nblk[0] * L.HS;
     hipLaunchKernelGGL(k_skew_pack, grid_for(std::max(n, nhand)), dim3(256), 0, stream(), n, L.grids.p, cell_grid(m),
                        L.slot_of.p, m->d.rowptr, m->d.diag, val, e->sk.F.p, e->sk.Bk.p, e->sk.U.p, e->sk.uhand.p,
                        nhand, L.HS);
     HIP_TRY(hipGetLastError());
-    switch (L.W) {
-        case 1: return launch_update_w<1>(e);
-        case 2: return launch_update_w<2>(e);
-        case 3: return launch_update_w<3>(e);
-        default: return launch_update_w<4>(e);
+    switch (skew_waves(L)) {
+        case 1: FVH_TRY(launch_update_w<1>(e)); break;
+        case 2: FVH_TRY(launch_update_w<2>(e)); break;
+        default: FVH_TRY(launch_update_w<3>(e)); break;
     }
+    if (!m->skew->W && !forced_waves()) FVH_TRY(skew_tune(e));
+    return FVH_OK;
 }
 
 template <typename T, int W>
 static int launch_sweeps_w(fvh_eqn *e, Rec4T<T> *F, Rec4T<T> *Bk, T *zp, T *hand, const KScal *ks) {
     const SkewLayout &L = *e->m->skew;
-    const SkewArgs a = skew_args<Rec4T<T>>(e);
+    const SkewArgs a = skew_args<Rec4T<T>>(e, W);
     const size_t lds = (size_t)(W + 1) * L.HS * sizeof(T);
     FVH_TRY(allow_lds(k_skew_fwd<T, W>, lds));
     FVH_TRY(allow_lds(k_skew_bwd<T, W>, lds));
-    hipLaunchKernelGGL((k_skew_fwd<T, W>), dim3(L.nblk), dim3(64 * (W + 1)), lds, stream(), a, F, Bk, hand, ks);
+    hipLaunchKernelGGL((k_skew_fwd<T, W>), dim3(a.nblk), dim3(64 * (W + 1)), lds, stream(), a, F, Bk, hand, ks);
     HIP_TRY(hipGetLastError());
-    hipLaunchKernelGGL((k_skew_bwd<T, W>), dim3(L.nblk), dim3(64 * (W + 1)), lds, stream(), a, Bk, zp, hand, ks);
+    hipLaunchKernelGGL((k_skew_bwd<T, W>), dim3(a.nblk), dim3(64 * (W + 1)), lds, stream(), a, Bk, zp, hand, ks);
     HIP_TRY(hipGetLastError());
     return FVH_OK;
 }
@@ -702,21 +771,58 @@ template <typename T>
 static int skew_apply_t(fvh_eqn *e, const T *x, T *y, const KScal *ks, Rec4T<T> *F, Rec4T<T> *Bk, T *zp, T *hand) {
     fvh_mesh *m = e->m;
     const SkewLayout &L = *m->skew;
-    const int n = (int)m->n, nhand = 2 * L.nblk * L.HS;
+    const int nhand = 2 * L.nblk[0] * L.HS;
     // per-case solves: gather / scatter cover every case, the sweeps skip
     // finished cases
     const KScal *ks_all = m->cases ? nullptr : ks;
-    hipLaunchKernelGGL((k_skew_gather<T>), grid_for(std::max(n, nhand)), dim3(256), 0, stream(), n, L.slot_of.p, x, F,
-                       hand, nhand, L.HS, ks_all);
+    const int ntiles = (int)L.tiles.n;
+    hipLaunchKernelGGL((k_skew_gather<T>), dim3(ntiles + (nhand + 255) / 256), dim3(256), 0, stream(), ntiles,
+                       L.tiles.p, L.grids.p, x, F, hand, nhand, L.HS, ks_all);
     HIP_TRY(hipGetLastError());
-    switch (L.W) {
+    switch (skew_waves(L)) {
         case 1: FVH_TRY((launch_sweeps_w<T, 1>(e, F, Bk, zp, hand, ks))); break;
         case 2: FVH_TRY((launch_sweeps_w<T, 2>(e, F, Bk, zp, hand, ks))); break;
-        case 3: FVH_TRY((launch_sweeps_w<T, 3>(e, F, Bk, zp, hand, ks))); break;
-        default: FVH_TRY((launch_sweeps_w<T, 4>(e, F, Bk, zp, hand, ks))); break;
+        default: FVH_TRY((launch_sweeps_w<T, 3>(e, F, Bk, zp, hand, ks))); break;
     }
-    hipLaunchKernelGGL((k_skew_scatter<T>), grid_for(n), dim3(256), 0, stream(), n, L.slot_of.p, zp, y, ks_all);
+    hipLaunchKernelGGL((k_skew_scatter<T>), dim3(ntiles), dim3(256), 0, stream(), L.tiles.p, L.grids.p, zp, y, ks_all);
     HIP_TRY(hipGetLastError());
+    return FVH_OK;
+}
+
+// Waves per block, chosen per mesh by timing the FP64 apply in each width on
+// the freshly packed records (the sweeps only rewrite their own scratch
+// fields); the width never changes a result.
+static int skew_tune(fvh_eqn *e) {
+    SkewLayout &L = *e->m->skew;
+    SkewStore &s = e->sk;
+    const int n = (int)e->m->n;
+    DBuf<double> x, y;
+    FVH_TRY(x.alloc(n));
+    FVH_TRY(y.alloc(n));
+    FVH_TRY(x.zero());
+    hipEvent_t a, b;
+    HIP_TRY(hipEventCreate(&a));
+    HIP_TRY(hipEventCreate(&b));
+    float best = 0.f;
+    int bestW = 1;
+    for (int W = 1; W <= lds_waves_max(L); ++W) {
+        L.W = W;
+        FVH_TRY(skew_apply_t<double>(e, x.p, y.p, nullptr, s.F.p, s.Bk.p, s.zp.p, s.hand.p));  // warm
+        HIP_TRY(hipEventRecord(a, stream()));
+        for (int rep = 0; rep < 3; ++rep)
+            FVH_TRY(skew_apply_t<double>(e, x.p, y.p, nullptr, s.F.p, s.Bk.p, s.zp.p, s.hand.p));
+        HIP_TRY(hipEventRecord(b, stream()));
+        HIP_TRY(hipEventSynchronize(b));
+        float t = 0.f;
+        HIP_TRY(hipEventElapsedTime(&t, a, b));
+        if (W == 1 || t < best) {
+            best = t;
+            bestW = W;
+        }
+    }
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    L.W = bestW;
     return FVH_OK;
 }
 
@@ -733,7 +839,7 @@ int skew_to32(fvh_eqn *e) {
         FVH_TRY(s.F32.alloc(L.NS));
         FVH_TRY(s.Bk32.alloc(L.NS));
         FVH_TRY(s.zp32.alloc(L.NS));
-        FVH_TRY(s.hand32.alloc((size_t)L.nblk * 2 * L.HS));
+        FVH_TRY(s.hand32.alloc((size_t)L.nblk[0] * 2 * L.HS));
         hipLaunchKernelGGL(k_skew_fill<float>, grid_for(L.NS), dim3(256), 0, stream(), L.NS, s.zp32.p, 0.f);
         HIP_TRY(hipGetLastError());
     }
