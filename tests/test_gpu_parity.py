@@ -169,8 +169,8 @@ def test_sweep_paths_bitwise(sweep, rows, monkeypatch):
         monkeypatch.delenv("FVH_SWEEP", raising=False)
     monkeypatch.delenv("FVH_BAND_ROWS", raising=False)
     monkeypatch.delenv("FVH_SKEW_WAVES", raising=False)
-    if rows and rows.startswith("w"):
-        monkeypatch.setenv("FVH_SKEW_WAVES", rows[1:])
+    if rows and rows.startswith("w"):  # waves per block
+        monkeypatch.setenv("FVH_SKEW_WAVES", rows[1])
     elif rows:
         monkeypatch.setenv("FVH_BAND_ROWS", rows)
     mesh = F.backward_facing_step(nx=60, ny=300)

@@ -155,7 +155,7 @@ struct CaseLayout {
 // last diagonal are padding.  One compute wave owns 64 rows; a block runs W
 // consecutive waves (W is a launch parameter, tuned per mesh).
 constexpr int kSkewL = 8;                    // steps per hand-over batch
-constexpr int kSkewPD = 16;                  // record prefetch distance (steps)
+constexpr int kSkewPD = 32;                  // deepest record prefetch (steps)
 constexpr int kSkewD0 = kSkewPD + 2;         // padding diagonals below T = 0
 constexpr int kSkewTop = kSkewPD;            // padding diagonals above the last step
 constexpr int kSkewMaxW = 3;                 // compute waves per block (plus one courier wave)
@@ -163,7 +163,7 @@ struct SkewGrid {
     int nx, ny;     // cell (i, j) is natural row o + i + j*nx
     int o;
     int nw;         // compute waves (S / 64)
-    int NG;         // step groups (even): steps 0 .. NG*kSkewL-1 cover the nx+ny-1 diagonals
+    int NG;         // step groups (multiple of 4): steps 0 .. NG*kSkewL-1 cover the nx+ny-1 diagonals
     int S;          // slots per diagonal
     unsigned base;  // first slot of the grid
     int blk0[kSkewMaxW];  // first block of the grid, W = 1 .. kSkewMaxW waves per block

@@ -55,8 +55,15 @@
 namespace fvh {
 
 constexpr int kL = kSkewL;
-constexpr int kPDs = kSkewPD;
-static_assert(kPDs == 2 * kL, "the record ring spans two hand-over batches");
+// The record ring holds M hand-over batches of steps (prefetch distance
+// M*kL): as deep as the vector-memory counter allows for the operations a
+// step issues (FP64: two 16-byte loads and a store; FP32: one load and a
+// store; at most 63 in flight).
+template <typename T>
+constexpr int ring_batches() {
+    return sizeof(T) == 4 ? 4 : 2;
+}
+static_assert(4 * kL <= kSkewPD, "padding diagonals cover the deepest ring");
 constexpr int kSpinLimit = 1 << 19;  // idle polls (about 30 ms) before a hand-over is declared lost
 constexpr int kWin = 32;             // hand-over words a courier moves per poll (4 batches)
 
@@ -263,18 +270,21 @@ __global__ void __launch_bounds__(64 * (W + 1))
     const int Teast = (int)j + g.nx - 1;  // step of the row's east-edge cell
     const bool row = (int)j < g.ny, top = (int)j == g.ny - 1;
 
-    double hb[2] = {src ? lload(lin + ql) : 1.0, src ? lload(lin + kL + ql) : 1.0};
-    Rec4 ring[kPDs];
+    constexpr int M = ring_batches<double>(), PD = M * kL;
+    double hb[M];
 #pragma unroll
-    for (int k = 0; k < kPDs; ++k) {
+    for (int h = 0; h < M; ++h) hb[h] = src ? lload(lin + h * kL + ql) : 1.0;
+    Rec4 ring[PD];
+#pragma unroll
+    for (int k = 0; k < PD; ++k) {
         ring[k] = bload(ru, vo, so(k), (Rec4 *)nullptr);
         __builtin_amdgcn_sched_barrier(0);  // issue in consumption order
     }
     double dprev = 1.0, pub = 1.0;
     int broken = 0;
-    for (int G0 = 0; G0 < g.NG; G0 += 2) {
+    for (int G0 = 0; G0 < g.NG; G0 += M) {
 #pragma unroll
-        for (int h = 0; h < 2; ++h) {
+        for (int h = 0; h < M; ++h) {
             const int G = G0 + h;
             double cur = hb[h];
             if (src && !broken && __builtin_amdgcn_ballot_w64(lane < kL && is_sentinel(cur)))
@@ -295,10 +305,10 @@ __global__ void __launch_bounds__(64 * (W + 1))
                 pub = dpp_shl1(pub, d);
                 dprev = d;
                 __builtin_amdgcn_sched_barrier(0);  // the slot is consumed: refill it below
-                ring[r] = bload(ru, vo, so(T + kPDs), (Rec4 *)nullptr);
+                ring[r] = bload(ru, vo, so(T + PD), (Rec4 *)nullptr);
             }
             if (lane >= 64 - kL) lstore(lout + G * kL + 1 + lane - (64 - kL), pub);
-            hb[h] = src ? lload(lin + (G + 2) * kL + ql) : 1.0;
+            hb[h] = src ? lload(lin + (G + M) * kL + ql) : 1.0;
         }
     }
 }
@@ -342,24 +352,29 @@ __global__ void __launch_bounds__(64 * (W + 1))
     const T *dp = &F[g.base + (size_t)(ql - 1 + kSkewD0) * S + j0 - 1].b;
     const size_t hstep = (size_t)kL * S * 4;  // one batch of steps, in T
 
-    T yb[2] = {src ? lload(lin + ql) : T(0), src ? lload(lin + kL + ql) : T(0)};
-    T db[2] = {dp[0], dp[hstep]};
-    dp += 2 * hstep;
+    constexpr int M = ring_batches<T>(), PD = M * kL;
+    T yb[M], db[M];
+#pragma unroll
+    for (int h = 0; h < M; ++h) {
+        yb[h] = src ? lload(lin + h * kL + ql) : T(0);
+        db[h] = dp[h * hstep];
+    }
+    dp += M * hstep;
     // The prologue stores +0 into the wave's own pre-start padding (its value
     // already), so that it issues what a step issues and the loop's memory
     // waits count the same outstanding operations on every iteration.
-    R ring[kPDs];
+    R ring[PD];
 #pragma unroll
-    for (int k = 0; k < kPDs; ++k) {
-        bstore(rbk, T(0), vo, so(k - kPDs));
+    for (int k = 0; k < PD; ++k) {
+        bstore(rbk, T(0), vo, so(k - PD));
         ring[k] = bload(rf, vo, so(k), (R *)nullptr);
         __builtin_amdgcn_sched_barrier(0);
     }
     T xprev = 0, dprev = 1, pub = 0;
     int broken = 0;
-    for (int G0 = 0; G0 < g.NG; G0 += 2) {
+    for (int G0 = 0; G0 < g.NG; G0 += M) {
 #pragma unroll
-        for (int h = 0; h < 2; ++h) {
+        for (int h = 0; h < M; ++h) {
             const int G = G0 + h;
             T yc = yb[h], dc = db[h];
             if (src && !broken && __builtin_amdgcn_ballot_w64(lane < kL && is_sentinel(yc)))
@@ -379,10 +394,10 @@ __global__ void __launch_bounds__(64 * (W + 1))
                 xprev = x;
                 dprev = keep(c.b);
                 __builtin_amdgcn_sched_barrier(0);
-                ring[r] = bload(rf, vo, so(Ts + kPDs), (R *)nullptr);
+                ring[r] = bload(rf, vo, so(Ts + PD), (R *)nullptr);
             }
             if (lane >= 64 - kL) lstore(lout + G * kL + 1 + lane - (64 - kL), pub);
-            yb[h] = src ? lload(lin + (G + 2) * kL + ql) : T(0);
+            yb[h] = src ? lload(lin + (G + M) * kL + ql) : T(0);
             db[h] = *dp;
             dp += hstep;
         }
@@ -424,21 +439,24 @@ __global__ void __launch_bounds__(64 * (W + 1))
     // lane 63-q of the batch of group G holds step u = G*kL + q (index u)
     const int qh = 63 - lane < kL ? 63 - lane : kL - 1;
 
-    T zb[2] = {src ? lload(lin + qh) : T(0), src ? lload(lin + kL + qh) : T(0)};
+    constexpr int M = ring_batches<T>(), PD = M * kL;
+    T zb[M];
+#pragma unroll
+    for (int h = 0; h < M; ++h) zb[h] = src ? lload(lin + h * kL + qh) : T(0);
     // prologue stores: +0 into the padding diagonals above the top (see the
     // forward sweep)
-    R ring[kPDs];
+    R ring[PD];
 #pragma unroll
-    for (int k = 0; k < kPDs; ++k) {
-        bstore(rz, T(0), voz, soz(Ttop + kPDs - k));
+    for (int k = 0; k < PD; ++k) {
+        bstore(rz, T(0), voz, soz(Ttop + PD - k));
         ring[k] = bload(rbk, vo, so(Ttop - k), (R *)nullptr);
         __builtin_amdgcn_sched_barrier(0);
     }
     T zprev = 0, pub = 0;
     int broken = 0;
-    for (int G0 = 0; G0 < g.NG; G0 += 2) {
+    for (int G0 = 0; G0 < g.NG; G0 += M) {
 #pragma unroll
-        for (int h = 0; h < 2; ++h) {
+        for (int h = 0; h < M; ++h) {
             const int G = G0 + h;
             T zc = zb[h];
             if (src && !broken && __builtin_amdgcn_ballot_w64(lane >= 64 - kL && is_sentinel(zc)))
@@ -455,10 +473,10 @@ __global__ void __launch_bounds__(64 * (W + 1))
                 pub = dpp_shr1(pub, z);
                 zprev = z;
                 __builtin_amdgcn_sched_barrier(0);
-                ring[r] = bload(rbk, vo, so(Tb - kPDs), (R *)nullptr);
+                ring[r] = bload(rbk, vo, so(Tb - PD), (R *)nullptr);
             }
             if (lane < kL) lstore(lout + G * kL + kL - lane, pub);
-            zb[h] = src ? lload(lin + (G + 2) * kL + qh) : T(0);
+            zb[h] = src ? lload(lin + (G + M) * kL + qh) : T(0);
         }
     }
 }
@@ -537,13 +555,18 @@ __global__ void __launch_bounds__(256)
     const SkewGrid g = grids[tl.grid];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int i = tl.i0 + lane;
-    for (int r = wv; r < kTile; r += 4) {
-        const int j = tl.j0 + r;
-        if (i < g.nx && j < g.ny) t.v[r][lane] = x[(size_t)g.o + i + (size_t)j * g.nx];
+    T v[kTile / 4];  // all loads in flight before the LDS stores
+#pragma unroll
+    for (int it = 0; it < kTile / 4; ++it) {
+        const int j = tl.j0 + wv + 4 * it;
+        v[it] = (i < g.nx && j < g.ny) ? x[(size_t)g.o + i + (size_t)j * g.nx] : T(0);
     }
+#pragma unroll
+    for (int it = 0; it < kTile / 4; ++it) t.v[wv + 4 * it][lane] = v[it];
     __syncthreads();
-    for (int d = wv; d < 2 * kTile - 1; d += 4) {  // diagonal T = i0 + j0 + d, row j0 + lane
-        const int j = tl.j0 + lane, il = d - lane;
+#pragma unroll
+    for (int it = 0; it < kTile / 2; ++it) {  // diagonal T = i0 + j0 + d, row j0 + lane
+        const int d = wv + 4 * it, j = tl.j0 + lane, il = d - lane;
         if (il >= 0 && il < kTile && tl.i0 + il < g.nx && j < g.ny)
             F[g.base + (size_t)(tl.i0 + tl.j0 + d + kSkewD0) * g.S + j].a = t.v[lane][il];
     }
@@ -559,15 +582,23 @@ __global__ void __launch_bounds__(256)
     const SkewTile tl = tiles[blockIdx.x];
     const SkewGrid g = grids[tl.grid];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    for (int d = wv; d < 2 * kTile - 1; d += 4) {
-        const int j = tl.j0 + lane, il = d - lane;
-        if (il >= 0 && il < kTile && tl.i0 + il < g.nx && j < g.ny)
-            t.v[lane][il] = zp[g.base + (size_t)(tl.i0 + tl.j0 + d + kSkewD0) * g.S + j];
+    T v[kTile / 2];  // all loads in flight before the LDS stores
+#pragma unroll
+    for (int it = 0; it < kTile / 2; ++it) {
+        const int d = wv + 4 * it, j = tl.j0 + lane, il = d - lane;
+        const bool in = il >= 0 && il < kTile && tl.i0 + il < g.nx && j < g.ny;
+        v[it] = in ? zp[g.base + (size_t)(tl.i0 + tl.j0 + d + kSkewD0) * g.S + j] : T(0);
+    }
+#pragma unroll
+    for (int it = 0; it < kTile / 2; ++it) {
+        const int il = wv + 4 * it - lane;
+        if (il >= 0 && il < kTile) t.v[lane][il] = v[it];
     }
     __syncthreads();
     const int i = tl.i0 + lane;
-    for (int r = wv; r < kTile; r += 4) {
-        const int j = tl.j0 + r;
+#pragma unroll
+    for (int it = 0; it < kTile / 4; ++it) {
+        const int r = wv + 4 * it, j = tl.j0 + r;
         if (i < g.nx && j < g.ny) y[(size_t)g.o + i + (size_t)j * g.nx] = t.v[r][lane];
     }
 }
@@ -610,11 +641,12 @@ int skew_setup(fvh_mesh *m, const std::vector<SkewRect> &rects) {
     if (force && std::strcmp(force, "levels") == 0) return FVH_OK;
     if (rects.empty()) return FVH_OK;
     int maxNG = 0;
-    auto ng_of = [](const SkewRect &r) { return 2 * ((r.nx + r.ny - 1 + 2 * kL - 1) / (2 * kL)); };
+    // step groups: a whole number of rings of every depth (4 batches)
+    auto ng_of = [](const SkewRect &r) { return 4 * ((r.nx + r.ny - 1 + 4 * kL - 1) / (4 * kL)); };
     for (const SkewRect &r : rects) maxNG = std::max(maxNG, ng_of(r));
-    // hand-over arrays: indices up to NG*kL, read a window (kWin) or two
-    // batches past the last one
-    const int HS = (maxNG * kL + kWin + 2 * kL + 64 + 63) / 64 * 64;
+    // hand-over arrays: indices up to NG*kL, read a window (kWin) or a ring of
+    // batches (at most 4) past the last one
+    const int HS = (maxNG * kL + kWin + 4 * kL + 64 + 63) / 64 * 64;
     if ((size_t)2 * HS * sizeof(double) > (size_t)max_lds_bytes()) return FVH_OK;  // too long even for W = 1
     auto L = std::make_unique<SkewLayout>();
     L->HS = HS;
@@ -685,6 +717,7 @@ static int skew_waves(const SkewLayout &L) {
     return std::min(w, lds_waves_max(L));
 }
 
+
 static const int *cell_grid(const fvh_mesh *m) { return m->cases ? m->cases->cell_case.p : nullptr; }
 
 static int skew_alloc(fvh_eqn *e) {
@@ -727,8 +760,8 @@ static int launch_update_w(fvh_eqn *e) {
     const SkewLayout &L = *e->m->skew;
     const size_t lds = (size_t)(W + 1) * L.HS * sizeof(double);
     FVH_TRY(allow_lds(k_skew_update<W>, lds));
-    hipLaunchKernelGGL(k_skew_update<W>, dim3(L.nblk[W - 1]), dim3(64 * (W + 1)), lds, stream(), skew_args<Rec4>(e, W),
-                       e->sk.U.p, e->sk.F.p, e->sk.Bk.p, e->sk.uhand.p);
+    hipLaunchKernelGGL((k_skew_update<W>), dim3(L.nblk[W - 1]), dim3(64 * (W + 1)), lds, stream(),
+                       skew_args<Rec4>(e, W), e->sk.U.p, e->sk.F.p, e->sk.Bk.p, e->sk.uhand.p);
     HIP_TRY(hipGetLastError());
     return FVH_OK;
 }
@@ -803,7 +836,7 @@ static int skew_tune(fvh_eqn *e) {
     hipEvent_t a, b;
     HIP_TRY(hipEventCreate(&a));
     HIP_TRY(hipEventCreate(&b));
-    float best = 0.f;
+    float best = -1.f;
     int bestW = 1;
     for (int W = 1; W <= lds_waves_max(L); ++W) {
         L.W = W;
@@ -815,7 +848,7 @@ static int skew_tune(fvh_eqn *e) {
         HIP_TRY(hipEventSynchronize(b));
         float t = 0.f;
         HIP_TRY(hipEventElapsedTime(&t, a, b));
-        if (W == 1 || t < best) {
+        if (best < 0.f || t < best) {
             best = t;
             bestW = W;
         }
