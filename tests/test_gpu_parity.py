@@ -153,10 +153,12 @@ def test_spmv_and_dilu_full_size_bitwise(bfs_full):
         L.lib().fvh_eqn_destroy(h)
 
 
-@pytest.mark.parametrize("sweep,rows", [("levels", None), ("band", None), ("band", "w1"), ("band", "w2"),
-                                        ("band", "w3"), ("band", "1"), ("band", "2"), ("band", "3"), ("band", "4")])
+@pytest.mark.parametrize("sweep,rows", [("levels", None), ("levels", "rowwise"), ("band", None), ("band", "w1"),
+                                        ("band", "w2"), ("band", "w3"), ("band", "1"), ("band", "2"), ("band", "3"),
+                                        ("band", "4")])
 def test_sweep_paths_bitwise(sweep, rows, monkeypatch):
-    """Every DILU schedule against the oracle: level sets; the pipelined band
+    """Every DILU schedule against the oracle: level sets (per-row records
+    prefetched a level ahead, or the row-by-row kernels); the pipelined band
     sweeps (default: skewed slots, waves handing over through LDS within a
     block and through a courier wave per block between blocks; tuned, or one,
     two or three waves per block, so block boundaries fall between different
@@ -169,7 +171,10 @@ def test_sweep_paths_bitwise(sweep, rows, monkeypatch):
         monkeypatch.delenv("FVH_SWEEP", raising=False)
     monkeypatch.delenv("FVH_BAND_ROWS", raising=False)
     monkeypatch.delenv("FVH_SKEW_WAVES", raising=False)
-    if rows and rows.startswith("w"):  # waves per block
+    monkeypatch.delenv("FVH_LEVEL_RECORDS", raising=False)
+    if rows == "rowwise":
+        monkeypatch.setenv("FVH_LEVEL_RECORDS", "0")
+    elif rows and rows.startswith("w"):  # waves per block
         monkeypatch.setenv("FVH_SKEW_WAVES", rows[1])
     elif rows:
         monkeypatch.setenv("FVH_BAND_ROWS", rows)
@@ -178,7 +183,7 @@ def test_sweep_paths_bitwise(sweep, rows, monkeypatch):
     assert nlev == 60 + 300 - 1
     assert (band == 60) == (sweep == "band")
     if sweep == "band":
-        assert mesh.device().tuning() == (int(rows) if rows and not rows.startswith("w") else 5)
+        assert mesh.device().tuning() == (int(rows) if rows and rows[0].isdigit() else 5)
     model = bfs_model(mesh, nu=1e-3)
     g, r = _sims(model, config(4, schemes=dict(divergence=F.Upwind)), conv_break=1e-6)
     assert g.run(4) == r.run(4)

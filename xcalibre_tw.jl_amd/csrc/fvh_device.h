@@ -172,7 +172,7 @@ struct SkewTile {
     int grid, i0, j0;
 };
 struct SkewLayout {
-    int W = 0;                 // waves per block in use (0: not tuned yet)
+    int W = 0, W32 = 0;        // waves per block of the FP64 / FP32 sweeps (0: not tuned yet)
     int HS = 0;                // hand-over array length (values) per block boundary
     size_t NS = 0;             // slots over all grids
     int nblk[kSkewMaxW] = {};  // blocks over all grids, per W
@@ -192,6 +192,7 @@ struct CaseView {
 // Sweep schedule of the lower (forward) and upper (backward) triangles.
 struct Schedule {
     int nlev = 0;
+    int max_deps = 0;  // most lower (or upper) neighbours of a row
     DBuf<int> fptr, frows, bptr, brows;
     std::vector<int> h_fptr, h_bptr;
     int band_nx = 0;  // >0: A is a banded grid (deps only r-1, r-nx)

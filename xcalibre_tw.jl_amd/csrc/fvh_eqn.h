@@ -96,6 +96,21 @@ struct SkewStore {
         if (h_fault) (void)hipHostFree(h_fault);
     }
 };
+// Per-row records of the level-scheduled DILU (update, forward, backward;
+// FP32 copies of the sweep records), stored as opaque blobs of the record
+// size: at most kLevDeps neighbours per row.
+constexpr int kLevDeps = 4;
+constexpr int kLevRecBytes64 = 96, kLevRecBytes32 = 64;
+struct alignas(16) LevBlob64 {
+    unsigned char b[kLevRecBytes64];
+};
+struct alignas(16) LevBlob32 {
+    unsigned char b[kLevRecBytes32];
+};
+struct LevelStore {
+    DBuf<LevBlob64> upd, fwd, bwd;
+    DBuf<LevBlob32> fwd32, bwd32;
+};
 }  // namespace fvh
 
 struct fvh_eqn {
@@ -109,6 +124,7 @@ struct fvh_eqn {
     fvh::DBuf<float> p32a, p32b;  // FP32 copies: Dinv / Jacobi scale, and A or LU values
     fvh::BandStore bd;  // DILU on banded grids (packed order)
     fvh::SkewStore sk;  // DILU on banded grids (pipelined sweeps)
+    fvh::LevelStore lv; // DILU on unstructured meshes (level-scheduled records)
     fvh::KrylovWork kw;
     fvh::DBuf<double> red;  // residual reductions
 };

@@ -117,15 +117,15 @@ __device__ inline void record_fault(int *fault) {
     if ((threadIdx.x & 63) == 0) __hip_atomic_store(fault, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
 }
 
-// Re-reads an LDS batch until the lanes `mine` hold no sentinel; bounded.
+// Re-reads an LDS batch until the lanes `mine` hold no sentinel; bounded
+// (a batch that never comes sets `broken`).
 template <typename T>
-__device__ inline T wait_batch(const T *p, bool mine, T v, int *fault, int &broken) {
+__device__ inline T wait_batch(const T *p, bool mine, T v, int &broken) {
     for (int s = 0; s < kSpinLimit && __builtin_amdgcn_ballot_w64(mine && is_sentinel(v)); ++s) {
         __builtin_amdgcn_s_sleep(1);
         v = lload(p);
     }
-    if (__builtin_amdgcn_ballot_w64(mine && is_sentinel(v))) {
-        record_fault(fault);
+    if (__builtin_amdgcn_ballot_w64(mine && is_sentinel(v))) {  // the fault is recorded after the walk
         broken = 1;
         v = is_sentinel(v) ? T(0) : v;
     }
@@ -288,7 +288,7 @@ __global__ void __launch_bounds__(64 * (W + 1))
             const int G = G0 + h;
             double cur = hb[h];
             if (src && !broken && __builtin_amdgcn_ballot_w64(lane < kL && is_sentinel(cur)))
-                cur = wait_batch(lin + G * kL + ql, lane < kL, cur, a.fault, broken);
+                cur = wait_batch(lin + G * kL + ql, lane < kL, cur, broken);
 #pragma unroll
             for (int k = 0; k < kL; ++k) {
                 const int r = h * kL + k, T = G * kL + k;
@@ -311,6 +311,7 @@ __global__ void __launch_bounds__(64 * (W + 1))
             hb[h] = src ? lload(lin + (G + M) * kL + ql) : 1.0;
         }
     }
+    if (broken) record_fault(a.fault);
 }
 
 // ---------------------------------------------------------------------------
@@ -353,13 +354,20 @@ __global__ void __launch_bounds__(64 * (W + 1))
     const size_t hstep = (size_t)kL * S * 4;  // one batch of steps, in T
 
     constexpr int M = ring_batches<T>(), PD = M * kL;
-    T yb[M], db[M];
+    // FP32 (the inner solves of mixed precision) works on premultiplied
+    // coefficients with fused multiply-adds: no pivots travel with y
+    constexpr bool kFused = sizeof(T) == 4;
+    // hand-over batches are read U batches ahead (the lane-0 pivots come from
+    // memory, which needs the distance)
+    constexpr int U = 4;
+    static_assert(U % M == 0, "whole rings per unrolled block");
+    T yb[U], db[U];
 #pragma unroll
-    for (int h = 0; h < M; ++h) {
+    for (int h = 0; h < U; ++h) {
         yb[h] = src ? lload(lin + h * kL + ql) : T(0);
-        db[h] = dp[h * hstep];
+        db[h] = kFused ? T(0) : dp[h * hstep];
     }
-    dp += M * hstep;
+    dp += U * hstep;
     // The prologue stores +0 into the wave's own pre-start padding (its value
     // already), so that it issues what a step issues and the loop's memory
     // waits count the same outstanding operations on every iteration.
@@ -372,36 +380,45 @@ __global__ void __launch_bounds__(64 * (W + 1))
     }
     T xprev = 0, dprev = 1, pub = 0;
     int broken = 0;
-    for (int G0 = 0; G0 < g.NG; G0 += M) {
+    for (int G0 = 0; G0 < g.NG; G0 += U) {
 #pragma unroll
-        for (int h = 0; h < M; ++h) {
+        for (int h = 0; h < U; ++h) {
             const int G = G0 + h;
             T yc = yb[h], dc = db[h];
             if (src && !broken && __builtin_amdgcn_ballot_w64(lane < kL && is_sentinel(yc)))
-                yc = wait_batch(lin + G * kL + ql, lane < kL, yc, a.fault, broken);
+                yc = wait_batch(lin + G * kL + ql, lane < kL, yc, broken);
 #pragma unroll
             for (int k = 0; k < kL; ++k) {
-                const int r = h * kL + k, Ts = G * kL + k;
+                const int r = (h % M) * kL + k, Ts = G * kL + k;
                 const R c = ring[r];
-                const T yn = shl1z(yc), dn = shl1z(dc);
-                const T xs = dpp_shr1(xprev, yc);
-                const T ds = dpp_shr1(dprev, dc);
-                yc = yn;
-                dc = dn;
-                const T x = (c.a - c.c * xs * ds) - c.d * xprev * dprev;
+                T x;
+                if constexpr (kFused) {  // (b, A_rs Dinv_s, A_rw Dinv_w)
+                    const T yn = shl1z(yc);
+                    const T xs = dpp_shr1(xprev, yc);
+                    yc = yn;
+                    x = __builtin_fmaf(-c.c, xprev, __builtin_fmaf(-c.b, xs, c.a));
+                } else {  // (b, Dinv, A_rs, A_rw)
+                    const T yn = shl1z(yc), dn = shl1z(dc);
+                    const T xs = dpp_shr1(xprev, yc);
+                    const T ds = dpp_shr1(dprev, dc);
+                    yc = yn;
+                    dc = dn;
+                    x = (c.a - c.c * xs * ds) - c.d * xprev * dprev;
+                    dprev = keep(c.b);
+                }
                 bstore(rbk, x, vo, so(Ts));
                 pub = dpp_shl1(pub, x);
                 xprev = x;
-                dprev = keep(c.b);
                 __builtin_amdgcn_sched_barrier(0);
                 ring[r] = bload(rf, vo, so(Ts + PD), (R *)nullptr);
             }
             if (lane >= 64 - kL) lstore(lout + G * kL + 1 + lane - (64 - kL), pub);
-            yb[h] = src ? lload(lin + (G + M) * kL + ql) : T(0);
-            db[h] = *dp;
+            yb[h] = src ? lload(lin + (G + U) * kL + ql) : T(0);
+            if constexpr (!kFused) db[h] = *dp;
             dp += hstep;
         }
     }
+    if (broken) record_fault(a.fault);
 }
 
 template <typename T, int W>
@@ -460,7 +477,7 @@ __global__ void __launch_bounds__(64 * (W + 1))
             const int G = G0 + h;
             T zc = zb[h];
             if (src && !broken && __builtin_amdgcn_ballot_w64(lane >= 64 - kL && is_sentinel(zc)))
-                zc = wait_batch(lin + G * kL + qh, lane >= 64 - kL, zc, a.fault, broken);
+                zc = wait_batch(lin + G * kL + qh, lane >= 64 - kL, zc, broken);
 #pragma unroll
             for (int k = 0; k < kL; ++k) {
                 const int r = h * kL + k, Tb = Ttop - (G * kL + k);
@@ -468,7 +485,11 @@ __global__ void __launch_bounds__(64 * (W + 1))
                 const T zn_next = shr1z(zc);
                 const T zn = dpp_shl1(zprev, zc);
                 zc = zn_next;
-                const T z = ((c.a * c.b) - c.c * zprev * c.b) - c.d * zn * c.b;
+                T z;
+                if constexpr (sizeof(T) == 4)  // (y, Dinv, A_re Dinv, A_rn Dinv), fused
+                    z = __builtin_fmaf(-c.d, zn, __builtin_fmaf(-c.c, zprev, c.a * c.b));
+                else
+                    z = ((c.a * c.b) - c.c * zprev * c.b) - c.d * zn * c.b;
                 bstore(rz, z, voz, soz(Tb));
                 pub = dpp_shr1(pub, z);
                 zprev = z;
@@ -479,6 +500,7 @@ __global__ void __launch_bounds__(64 * (W + 1))
             zb[h] = src ? lload(lin + (G + M) * kL + qh) : T(0);
         }
     }
+    if (broken) record_fault(a.fault);
 }
 
 // ---------------------------------------------------------------------------
@@ -603,13 +625,23 @@ __global__ void __launch_bounds__(256)
     }
 }
 
-__global__ void k_skew_to32(size_t ns, const Rec4 *__restrict__ F, const Rec4 *__restrict__ Bk,
-                            Rec4f *__restrict__ F32, Rec4f *__restrict__ Bk32) {
-    const size_t s = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= ns) return;
+// FP32 records for the inner solves of mixed precision: the coefficients
+// premultiplied by their pivots in FP64 and rounded once,
+//   F32 = (b, A_rs Dinv_s, A_rw Dinv_w, 0),  Bk32 = (y, Dinv_r, A_re Dinv_r, A_rn Dinv_r),
+// the south / west pivots being those of slots s - S - 1 / s - S (grid
+// blockIdx.y; its first diagonal is padding).
+__global__ void k_skew_to32(const SkewGrid *__restrict__ grids, const Rec4 *__restrict__ F,
+                            const Rec4 *__restrict__ Bk, Rec4f *__restrict__ F32, Rec4f *__restrict__ Bk32) {
+    const SkewGrid g = grids[blockIdx.y];
+    const size_t nslot = ((size_t)g.NG * kSkewL + kSkewD0 + kSkewTop) * g.S;
+    const size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= nslot) return;
+    const size_t s = g.base + q;
     const Rec4 u = F[s], v = Bk[s];
-    F32[s] = Rec4f{0.f, (float)u.b, (float)u.c, (float)u.d};
-    Bk32[s] = Rec4f{0.f, (float)v.b, (float)v.c, (float)v.d};
+    const double ds = q >= (size_t)g.S + 1 ? F[s - g.S - 1].b : 1.0;
+    const double dw = q >= (size_t)g.S ? F[s - g.S].b : 1.0;
+    F32[s] = Rec4f{0.f, (float)(u.c * ds), (float)(u.d * dw), 0.f};
+    Bk32[s] = Rec4f{0.f, (float)v.b, (float)(v.c * v.b), (float)(v.d * v.b)};
 }
 
 __global__ void k_skew_dinv_natural(int n, const int *__restrict__ slot_of, const Rec4 *__restrict__ F,
@@ -711,9 +743,10 @@ static int lds_waves_max(const SkewLayout &L) {
     while (W > 1 && (size_t)(W + 1) * L.HS * sizeof(double) > (size_t)max_lds_bytes()) --W;
     return W;
 }
-static int skew_waves(const SkewLayout &L) {
+static int skew_waves(const SkewLayout &L, bool fp32 = false) {
     const int f = forced_waves();
-    const int w = f ? f : (L.W ? L.W : kSkewMaxW);
+    const int tuned = fp32 ? L.W32 : L.W;
+    const int w = f ? f : (tuned ? tuned : kSkewMaxW);
     return std::min(w, lds_waves_max(L));
 }
 
@@ -812,7 +845,7 @@ static int skew_apply_t(fvh_eqn *e, const T *x, T *y, const KScal *ks, Rec4T<T> 
     hipLaunchKernelGGL((k_skew_gather<T>), dim3(ntiles + (nhand + 255) / 256), dim3(256), 0, stream(), ntiles,
                        L.tiles.p, L.grids.p, x, F, hand, nhand, L.HS, ks_all);
     HIP_TRY(hipGetLastError());
-    switch (skew_waves(L)) {
+    switch (skew_waves(L, sizeof(T) == 4)) {
         case 1: FVH_TRY((launch_sweeps_w<T, 1>(e, F, Bk, zp, hand, ks))); break;
         case 2: FVH_TRY((launch_sweeps_w<T, 2>(e, F, Bk, zp, hand, ks))); break;
         default: FVH_TRY((launch_sweeps_w<T, 3>(e, F, Bk, zp, hand, ks))); break;
@@ -822,14 +855,14 @@ static int skew_apply_t(fvh_eqn *e, const T *x, T *y, const KScal *ks, Rec4T<T> 
     return FVH_OK;
 }
 
-// Waves per block, chosen per mesh by timing the FP64 apply in each width on
-// the freshly packed records (the sweeps only rewrite their own scratch
-// fields); the width never changes a result.
-static int skew_tune(fvh_eqn *e) {
+// Waves per block, chosen per mesh and precision by timing the apply in each
+// width on the freshly packed records (the sweeps only rewrite their own
+// scratch fields); the width never changes a result.
+template <typename T>
+static int skew_tune_t(fvh_eqn *e, Rec4T<T> *F, Rec4T<T> *Bk, T *zp, T *hand, int &W_out) {
     SkewLayout &L = *e->m->skew;
-    SkewStore &s = e->sk;
     const int n = (int)e->m->n;
-    DBuf<double> x, y;
+    DBuf<T> x, y;
     FVH_TRY(x.alloc(n));
     FVH_TRY(y.alloc(n));
     FVH_TRY(x.zero());
@@ -839,11 +872,10 @@ static int skew_tune(fvh_eqn *e) {
     float best = -1.f;
     int bestW = 1;
     for (int W = 1; W <= lds_waves_max(L); ++W) {
-        L.W = W;
-        FVH_TRY(skew_apply_t<double>(e, x.p, y.p, nullptr, s.F.p, s.Bk.p, s.zp.p, s.hand.p));  // warm
+        W_out = W;
+        FVH_TRY(skew_apply_t<T>(e, x.p, y.p, nullptr, F, Bk, zp, hand));  // warm
         HIP_TRY(hipEventRecord(a, stream()));
-        for (int rep = 0; rep < 3; ++rep)
-            FVH_TRY(skew_apply_t<double>(e, x.p, y.p, nullptr, s.F.p, s.Bk.p, s.zp.p, s.hand.p));
+        for (int rep = 0; rep < 3; ++rep) FVH_TRY(skew_apply_t<T>(e, x.p, y.p, nullptr, F, Bk, zp, hand));
         HIP_TRY(hipEventRecord(b, stream()));
         HIP_TRY(hipEventSynchronize(b));
         float t = 0.f;
@@ -855,8 +887,14 @@ static int skew_tune(fvh_eqn *e) {
     }
     (void)hipEventDestroy(a);
     (void)hipEventDestroy(b);
-    L.W = bestW;
+    W_out = bestW;
+    (void)L;
     return FVH_OK;
+}
+
+static int skew_tune(fvh_eqn *e) {
+    SkewStore &s = e->sk;
+    return skew_tune_t<double>(e, s.F.p, s.Bk.p, s.zp.p, s.hand.p, e->m->skew->W);
 }
 
 int skew_apply(fvh_eqn *e, const double *x, double *y, const KScal *ks) {
@@ -866,7 +904,7 @@ int skew_apply(fvh_eqn *e, const double *x, double *y, const KScal *ks) {
 }
 
 int skew_to32(fvh_eqn *e) {
-    const SkewLayout &L = *e->m->skew;
+    SkewLayout &L = *e->m->skew;
     SkewStore &s = e->sk;
     if (s.F32.n != L.NS) {
         FVH_TRY(s.F32.alloc(L.NS));
@@ -876,8 +914,13 @@ int skew_to32(fvh_eqn *e) {
         hipLaunchKernelGGL(k_skew_fill<float>, grid_for(L.NS), dim3(256), 0, stream(), L.NS, s.zp32.p, 0.f);
         HIP_TRY(hipGetLastError());
     }
-    hipLaunchKernelGGL(k_skew_to32, grid_for(L.NS), dim3(256), 0, stream(), L.NS, s.F.p, s.Bk.p, s.F32.p, s.Bk32.p);
+    size_t most = 0;
+    for (const SkewGrid &g : L.h_grids)
+        most = std::max(most, ((size_t)g.NG * kSkewL + kSkewD0 + kSkewTop) * g.S);
+    hipLaunchKernelGGL(k_skew_to32, dim3((unsigned)((most + 255) / 256), (unsigned)L.h_grids.size()), dim3(256), 0,
+                       stream(), L.grids.p, s.F.p, s.Bk.p, s.F32.p, s.Bk32.p);
     HIP_TRY(hipGetLastError());
+    if (!L.W32 && !forced_waves()) FVH_TRY(skew_tune_t<float>(e, s.F32.p, s.Bk32.p, s.zp32.p, s.hand32.p, e->m->skew->W32));
     return FVH_OK;
 }
 

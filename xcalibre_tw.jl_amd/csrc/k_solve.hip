@@ -17,6 +17,7 @@
 
 #include "fvh_eqn.h"
 #include "k_band.h"
+#include "k_level.h"
 #include "k_sweep.h"
 #include "k_spmv.h"
 #include "k_util.h"
@@ -167,6 +168,7 @@ int precond_update(fvh_eqn *eq) {
             return FVH_OK;
         case FVH_PRECOND_DILU:
             if (band_path(m)) return band_dilu_update(e, src->nz.p);
+            if (lev_path(m)) return lev_dilu_update(e, src->nz.p);
             return dilu_update(m, src->nz.p, e->D.p, e->Dinv.p);
         case FVH_PRECOND_ILU0: return ilu0_factor(m, src->nz.p, e->lu.p);
         default: return set_error(FVH_ERR_UNSUPPORTED, "preconditioner %d", e->pkind);
@@ -193,6 +195,7 @@ int precond_apply(fvh_eqn *eq, double *y, const double *x, const KScal *ks_in, d
         }
         case FVH_PRECOND_DILU:
             if (band_path(m)) return band_dilu_apply(e, x, y, ks_in);
+            if (lev_path(m) && e->lv.fwd.p) return lev_dilu_apply(e, x, y, ks);
             return dilu_apply(m, src->nz.p, e->Dinv.p, x, y, ks);
         case FVH_PRECOND_ILU0: return ilu0_apply(m, e->lu.p, x, y, ks);
         default: return set_error(FVH_ERR_UNSUPPORTED, "preconditioner %d", e->pkind);
@@ -243,12 +246,22 @@ int prepare32(fvh_eqn *e) {
         }
         case FVH_PRECOND_DILU: {
             if (band_path(m)) return band_to32(P);
+            if (lev_path(m) && P->lv.fwd.p) FVH_TRY(lev_to32(P));
             if (P->p32a.n != (size_t)n) FVH_TRY(P->p32a.alloc(n));
             if (P->p32b.n != (size_t)nnz) FVH_TRY(P->p32b.alloc(nnz));
             const double *di = P->Dinv.p, *a = src->nz.p;
             float *df = P->p32a.p, *af = P->p32b.p;
+            const int *rp = m->d.rowptr, *cl = m->d.col;
             FVH_TRY(map(n, nullptr, [=] __device__(int i) { df[i] = (float)di[i]; }));
-            return map(nnz, nullptr, [=] __device__(int i) { af[i] = (float)a[i]; });
+            // each off-diagonal premultiplied by the pivot it meets in the
+            // sweeps (its column's below the diagonal, its row's above),
+            // rounded once
+            return map(n, nullptr, [=] __device__(int i) {
+                for (int p = rp[i]; p < rp[i + 1]; ++p) {
+                    const int j = cl[p];
+                    af[p] = (float)(a[p] * (j < i ? di[j] : di[i]));
+                }
+            });
         }
         case FVH_PRECOND_ILU0: {
             if (P->p32b.n != (size_t)nnz) FVH_TRY(P->p32b.alloc(nnz));
@@ -275,6 +288,7 @@ int precond_apply32(fvh_eqn *eq, float *y, const float *x, const KScal *ks_in) {
         }
         case FVH_PRECOND_DILU:
             if (band_path(m)) return band_dilu_apply32(e, x, y, ks_in);
+            if (lev_path(m) && e->lv.fwd32.p) return lev_dilu_apply32(e, x, y, ks);
             return dilu_apply32(m, e->p32b.p, e->p32a.p, x, y, ks);
         case FVH_PRECOND_ILU0: return ilu0_apply32(m, e->p32b.p, x, y, ks);
         default: return set_error(FVH_ERR_UNSUPPORTED, "preconditioner %d", e->pkind);

@@ -38,13 +38,16 @@ __global__ void __launch_bounds__(kSweepBlock)
                         const T *__restrict__ Dinv, const T *__restrict__ x, T *y, const KScal *ks) {
     if (ks && ks->done) return;
     // forward: y_i = x_i - sum_{j<i} (A_ij y_j) Dinv_j, j ascending
+    // (FP32: val holds A_ij Dinv_j below and A_ij Dinv_i above the diagonal,
+    // rounded once, and the terms are fused multiply-adds)
     for (int l = 0; l < nlev; ++l) {
         for (int k = fptr[l] + threadIdx.x; k < fptr[l + 1]; k += kSweepBlock) {
             const int i = frows[k];
             T s = x[i];
             for (int p = rowptr[i]; p < diag[i]; ++p) {
                 const int j = col[p];
-                s = s - val[p] * y[j] * Dinv[j];
+                if constexpr (sizeof(T) == 4) s = __builtin_fmaf(-val[p], y[j], s);
+                else s = s - val[p] * y[j] * Dinv[j];
             }
             y[i] = s;
         }
@@ -56,7 +59,10 @@ __global__ void __launch_bounds__(kSweepBlock)
             const int i = brows[k];
             const T di = Dinv[i];
             T s = y[i] * di;
-            for (int p = diag[i] + 1; p < rowptr[i + 1]; ++p) s = s - val[p] * y[col[p]] * di;
+            for (int p = diag[i] + 1; p < rowptr[i + 1]; ++p) {
+                if constexpr (sizeof(T) == 4) s = __builtin_fmaf(-val[p], y[col[p]], s);
+                else s = s - val[p] * y[col[p]] * di;
+            }
             y[i] = s;
         }
         __syncthreads();

@@ -183,6 +183,9 @@ int build_schedule(fvh_mesh *m) {
     group(lf, nlev, m->sched.h_fptr, frows, false);
     group(lb, nlevb, m->sched.h_bptr, brows, true);
     m->sched.nlev = nlev;
+    int deg = 0;  // most lower / upper neighbours of a row
+    for (int i = 0; i < n; ++i) deg = std::max(deg, std::max(dg[i] - rp[i], rp[i + 1] - dg[i] - 1));
+    m->sched.max_deps = deg;
     FVH_TRY(m->sched.fptr.upload(m->sched.h_fptr));
     FVH_TRY(m->sched.frows.upload(frows));
     FVH_TRY(m->sched.bptr.upload(m->sched.h_bptr));
