@@ -117,17 +117,15 @@ __device__ inline void record_fault(int *fault) {
     if ((threadIdx.x & 63) == 0) __hip_atomic_store(fault, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
 }
 
-// Re-reads an LDS batch until the lanes `mine` hold no sentinel; bounded
-// (a batch that never comes sets `broken`).
+// Re-reads an LDS batch until no lane holds a sentinel; bounded: returns
+// with sentinels still in it when the batch never comes (or at once when the
+// wave is already `broken`).  Out of line: the sweeps branch here only when a
+// batch was not yet written.
 template <typename T>
-__device__ inline T wait_batch(const T *p, bool mine, T v, int &broken) {
-    for (int s = 0; s < kSpinLimit && __builtin_amdgcn_ballot_w64(mine && is_sentinel(v)); ++s) {
+__device__ __attribute__((noinline)) T wait_batch(const T *p, T v, int broken) {
+    for (int s = 0; !broken && s < kSpinLimit && __builtin_amdgcn_ballot_w64(is_sentinel(v)); ++s) {
         __builtin_amdgcn_s_sleep(1);
         v = lload(p);
-    }
-    if (__builtin_amdgcn_ballot_w64(mine && is_sentinel(v))) {  // the fault is recorded after the walk
-        broken = 1;
-        v = is_sentinel(v) ? T(0) : v;
     }
     return v;
 }
@@ -184,10 +182,16 @@ __device__ inline float shr1z(float v) {
 }
 
 // Resets the block's LDS hand-over arrays ([W+1][HS]; array c lies below
-// compute wave c): index 0 holds the padding value of step -1.
+// compute wave c): index 0 holds the padding value of step -1, the others
+// the sentinel, except array `open`, which no wave or courier writes (the
+// source of the grid's first wave): it holds the padding value throughout.
+// Words [HS-64, HS) of each array are a dummy area beyond every batch index:
+// the lanes that write no batch word write there, so that the hand-over
+// store needs no lane mask.
 template <typename T>
-__device__ inline void lds_reset(T *lds, int count, int HS, T first) {
-    for (int q = threadIdx.x; q < count; q += blockDim.x) lds[q] = (q % HS == 0) ? first : hand_sentinel<T>();
+__device__ inline void lds_reset(T *lds, int count, int HS, T first, int open) {
+    for (int q = threadIdx.x; q < count; q += blockDim.x)
+        lds[q] = (q % HS == 0 || q / HS == open) ? first : hand_sentinel<T>();
 }
 
 // The courier wave of a block: moves hand-over batches from the global array
@@ -238,6 +242,63 @@ __device__ void courier(const T *gin, T *lin, const T *lout, T *gout, int NG, in
 }
 
 // ---------------------------------------------------------------------------
+// Per-group hand-over, shared by the three sweeps.  A compute wave reads the
+// batch of group G+kLook at the end of group G (kLook groups ahead of use),
+// tests the batch of group G+1 for sentinels at the start of group G (the
+// compare's latency then hides behind a group of steps; the branch to the
+// out-of-line wait is rarely taken), and writes its own batch at the end of
+// each group with every lane storing (lanes without a batch word store into
+// the dummy area), so no lane mask or branch sits in the step sequence.
+// The prologue issues the same LDS pattern as a group end, which keeps the
+// compiler's LDS waits at the true distance.
+// ---------------------------------------------------------------------------
+// Batch reads reach index (NG + kLook) * kL - 1, courier windows
+// NG * kL + kWin - kL; the arrays are sized (HS, in skew_setup) to keep
+// both below the dummy area [HS-64, HS).
+constexpr int kLook = 4;
+
+template <typename T>
+struct Hand {
+    const T *rd;       // this lane's read word of group 0 (lin + ql)
+    T *wr;             // this lane's write word of group 0 (batch word or dummy)
+    int wstep;         // words the write word advances per group (kL or 0)
+    T yb[kLook];       // batches of the next kLook groups
+    unsigned long long bad;  // lanes of the next group's batch still holding a sentinel
+
+    __device__ inline void start(const T *lin, T *lout, int HS, int q, bool writer, int widx) {
+        const int lane = threadIdx.x & 63;
+        rd = lin + q;
+        wr = writer ? lout + widx : lout + HS - 64 + lane;
+        wstep = writer ? kL : 0;
+        T *dummy = lout + HS - 64 + lane;
+#pragma unroll
+        for (int h = 0; h < kLook; ++h) {
+            lstore(dummy, T(0));
+            yb[h] = lload(rd + h * kL);
+        }
+        bad = __builtin_amdgcn_ballot_w64(is_sentinel(yb[0]));
+    }
+    // the batch of group G (slot h = G % kLook), waited for if needed
+    __device__ inline T take(int G, int h, int &broken) {
+        T v = yb[h];
+        if (bad) {
+            v = wait_batch(rd + G * kL, v, broken);
+            if (__builtin_amdgcn_ballot_w64(is_sentinel(v))) {  // lost: the fault is recorded after the walk
+                broken = 1;
+                v = is_sentinel(v) ? T(0) : v;
+            }
+        }
+        bad = __builtin_amdgcn_ballot_w64(is_sentinel(yb[(h + 1) % kLook]));
+        return v;
+    }
+    __device__ inline void finish(int G, int h, T pub) {
+        lstore(wr, pub);
+        wr += wstep;
+        yb[h] = lload(rd + (G + kLook) * kL);
+    }
+};
+
+// ---------------------------------------------------------------------------
 // Diagonal recurrence (update_dilu_diagonal!): Dinv into F.b and Bk.b
 // ---------------------------------------------------------------------------
 template <int W>
@@ -249,7 +310,7 @@ __global__ void __launch_bounds__(64 * (W + 1))
     const int b = blockIdx.x;
     const SkewGrid g = a.grids[a.blk_grid[b]];
     const int HS = a.HS, rb = b - g.blk0[W - 1], nb = (g.nw + W - 1) / W;
-    lds_reset(lds, (W + 1) * HS, HS, 1.0);
+    lds_reset(lds, (W + 1) * HS, HS, 1.0, rb == 0 ? 0 : -1);
     __syncthreads();
     const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
     if (w == W) {  // courier: D of the row below from block rb-1, the top row's D to block rb+1
@@ -259,39 +320,37 @@ __global__ void __launch_bounds__(64 * (W + 1))
     }
     if (rb * W + w >= g.nw) return;  // past the grid's last wave (last block)
     const int lane = threadIdx.x & 63, wg = rb * W + w;  // wave of the grid
-    const unsigned S = g.S, j0 = wg * 64, j = j0 + lane;
-    const bool src = wg > 0;
-    const double *lin = lds + w * HS;
-    double *lout = lds + (w + 1) * HS;
+    const unsigned S = g.S, j = wg * 64 + lane;
     const Rsrc ru = rsrc(U, a.rbytes), rf = rsrc(F, a.rbytes), rbk = rsrc(Bk, a.rbytes);
     const unsigned vo = (g.base + j) * (unsigned)sizeof(Rec4), step = S * (unsigned)sizeof(Rec4);
     auto so = [&](int T) { return (unsigned)(T + kSkewD0) * step; };
-    const int ql = lane < kL ? lane : kL - 1;
-    const int Teast = (int)j + g.nx - 1;  // step of the row's east-edge cell
-    const bool row = (int)j < g.ny, top = (int)j == g.ny - 1;
 
     constexpr int M = ring_batches<double>(), PD = M * kL;
-    double hb[M];
-#pragma unroll
-    for (int h = 0; h < M; ++h) hb[h] = src ? lload(lin + h * kL + ql) : 1.0;
+    static_assert(kLook % M == 0, "whole rings per unrolled block");
+    Hand<double> hd;
+    hd.start(lds + w * HS, lds + (w + 1) * HS, HS, min(lane, kL - 1), lane >= 64 - kL, 1 + lane - (64 - kL));
+    // the prologue issues what a step issues (two stores of the pivot 1.0
+    // that the pre-start padding holds already, one record load), so that the
+    // loop's memory waits count the same outstanding operations every step
     Rec4 ring[PD];
 #pragma unroll
     for (int k = 0; k < PD; ++k) {
+        bstore(rf, 1.0, vo + 8, so(k - PD));
+        bstore(rbk, 1.0, vo + 8, so(k - PD));
         ring[k] = bload(ru, vo, so(k), (Rec4 *)nullptr);
         __builtin_amdgcn_sched_barrier(0);  // issue in consumption order
     }
     double dprev = 1.0, pub = 1.0;
     int broken = 0;
-    for (int G0 = 0; G0 < g.NG; G0 += M) {
+    int G0 = 0;
+    do {
 #pragma unroll
-        for (int h = 0; h < M; ++h) {
+        for (int h = 0; h < kLook; ++h) {
             const int G = G0 + h;
-            double cur = hb[h];
-            if (src && !broken && __builtin_amdgcn_ballot_w64(lane < kL && is_sentinel(cur)))
-                cur = wait_batch(lin + G * kL + ql, lane < kL, cur, broken);
+            double cur = hd.take(G, h, broken);
 #pragma unroll
             for (int k = 0; k < kL; ++k) {
-                const int r = h * kL + k, T = G * kL + k;
+                const int r = (h % M) * kL + k, T = G * kL + k;
                 const Rec4 u = ring[r];
                 const double nxt = shl1z(cur);
                 const double ds = dpp_shr1(dprev, cur);
@@ -300,17 +359,16 @@ __global__ void __launch_bounds__(64 * (W + 1))
                 const double di = 1.0 / d;
                 bstore(rf, di, vo + 8, so(T));
                 bstore(rbk, di, vo + 8, so(T));
-                if (row && T == Teast) bstore(rbk, copysign(0.0, di), vo + 16, so(T));
-                if (top && (unsigned)(T - (int)j) < (unsigned)g.nx) bstore(rbk, copysign(0.0, di), vo + 24, so(T));
-                pub = dpp_shl1(pub, d);
+                if (k > 0) pub = dpp_shl1(pub, dprev);  // the previous step's D (its register is free now)
                 dprev = d;
                 __builtin_amdgcn_sched_barrier(0);  // the slot is consumed: refill it below
                 ring[r] = bload(ru, vo, so(T + PD), (Rec4 *)nullptr);
             }
-            if (lane >= 64 - kL) lstore(lout + G * kL + 1 + lane - (64 - kL), pub);
-            hb[h] = src ? lload(lin + (G + M) * kL + ql) : 1.0;
+            pub = dpp_shl1(pub, dprev);
+            hd.finish(G, h, pub);
         }
-    }
+        G0 += kLook;
+    } while (G0 < g.NG);  // NG >= 1: no guard between the prologue and the loop
     if (broken) record_fault(a.fault);
 }
 
@@ -330,7 +388,7 @@ __global__ void __launch_bounds__(64 * (W + 1))
     if (ks && ks[a.ks_per_grid ? gi : 0].done) return;
     const SkewGrid g = a.grids[gi];
     const int HS = a.HS, rb = b - g.blk0[W - 1], nb = (g.nw + W - 1) / W;
-    lds_reset(lds, (W + 1) * HS, HS, T(0));
+    lds_reset(lds, (W + 1) * HS, HS, T(0), rb == 0 ? 0 : -1);
     __syncthreads();
     const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
     if (w == W) {  // courier: the forward channel of block b is hand[2b]
@@ -341,33 +399,26 @@ __global__ void __launch_bounds__(64 * (W + 1))
     if (rb * W + w >= g.nw) return;  // past the grid's last wave (last block)
     const int lane = threadIdx.x & 63, wg = rb * W + w;
     const unsigned S = g.S, j0 = wg * 64;
-    const bool src = wg > 0;
-    const T *lin = lds + w * HS;
-    T *lout = lds + (w + 1) * HS;
     const Rsrc rf = rsrc(F, a.rbytes), rbk = rsrc(Bk, a.rbytes);
     const unsigned vo = (g.base + j0 + lane) * (unsigned)sizeof(R), step = S * (unsigned)sizeof(R);
     auto so = [&](int t) { return (unsigned)(t + kSkewD0) * step; };
     // lane 0's south pivot: Dinv of (step T-1, row j0-1), static; lane q < kL
     // of the batch of group G holds step G*kL + q
-    const int ql = lane < kL ? lane : kL - 1;
+    const int ql = min(lane, kL - 1);
     const T *dp = &F[g.base + (size_t)(ql - 1 + kSkewD0) * S + j0 - 1].b;
     const size_t hstep = (size_t)kL * S * 4;  // one batch of steps, in T
 
     constexpr int M = ring_batches<T>(), PD = M * kL;
+    static_assert(kLook % M == 0, "whole rings per unrolled block");
     // FP32 (the inner solves of mixed precision) works on premultiplied
     // coefficients with fused multiply-adds: no pivots travel with y
     constexpr bool kFused = sizeof(T) == 4;
-    // hand-over batches are read U batches ahead (the lane-0 pivots come from
-    // memory, which needs the distance)
-    constexpr int U = 4;
-    static_assert(U % M == 0, "whole rings per unrolled block");
-    T yb[U], db[U];
+    Hand<T> hd;
+    hd.start(lds + w * HS, lds + (w + 1) * HS, HS, ql, lane >= 64 - kL, 1 + lane - (64 - kL));
+    T db[kLook];  // lane-0 pivots of the next kLook groups (FP64)
 #pragma unroll
-    for (int h = 0; h < U; ++h) {
-        yb[h] = src ? lload(lin + h * kL + ql) : T(0);
-        db[h] = kFused ? T(0) : dp[h * hstep];
-    }
-    dp += U * hstep;
+    for (int h = 0; h < kLook; ++h) db[h] = kFused ? T(0) : dp[h * hstep];
+    dp += kLook * hstep;
     // The prologue stores +0 into the wave's own pre-start padding (its value
     // already), so that it issues what a step issues and the loop's memory
     // waits count the same outstanding operations on every iteration.
@@ -380,13 +431,12 @@ __global__ void __launch_bounds__(64 * (W + 1))
     }
     T xprev = 0, dprev = 1, pub = 0;
     int broken = 0;
-    for (int G0 = 0; G0 < g.NG; G0 += U) {
+    int G0 = 0;
+    do {
 #pragma unroll
-        for (int h = 0; h < U; ++h) {
+        for (int h = 0; h < kLook; ++h) {
             const int G = G0 + h;
-            T yc = yb[h], dc = db[h];
-            if (src && !broken && __builtin_amdgcn_ballot_w64(lane < kL && is_sentinel(yc)))
-                yc = wait_batch(lin + G * kL + ql, lane < kL, yc, broken);
+            T yc = hd.take(G, h, broken), dc = db[h];
 #pragma unroll
             for (int k = 0; k < kL; ++k) {
                 const int r = (h % M) * kL + k, Ts = G * kL + k;
@@ -407,17 +457,18 @@ __global__ void __launch_bounds__(64 * (W + 1))
                     dprev = keep(c.b);
                 }
                 bstore(rbk, x, vo, so(Ts));
-                pub = dpp_shl1(pub, x);
+                if (k > 0) pub = dpp_shl1(pub, xprev);  // the previous step's value (its register is free now)
                 xprev = x;
                 __builtin_amdgcn_sched_barrier(0);
                 ring[r] = bload(rf, vo, so(Ts + PD), (R *)nullptr);
             }
-            if (lane >= 64 - kL) lstore(lout + G * kL + 1 + lane - (64 - kL), pub);
-            yb[h] = src ? lload(lin + (G + U) * kL + ql) : T(0);
+            pub = dpp_shl1(pub, xprev);
+            hd.finish(G, h, pub);
             if constexpr (!kFused) db[h] = *dp;
             dp += hstep;
         }
-    }
+        G0 += kLook;
+    } while (G0 < g.NG);
     if (broken) record_fault(a.fault);
 }
 
@@ -433,7 +484,8 @@ __global__ void __launch_bounds__(64 * (W + 1))
     if (ks && ks[a.ks_per_grid ? gi : 0].done) return;
     const SkewGrid g = a.grids[gi];
     const int HS = a.HS, rb = b - g.blk0[W - 1], nb = (g.nw + W - 1) / W;
-    lds_reset(lds, (W + 1) * HS, HS, T(0));
+    // the grid's top wave reads the array above it, which nobody writes
+    lds_reset(lds, (W + 1) * HS, HS, T(0), rb == nb - 1 ? g.nw - rb * W : -1);
     __syncthreads();
     const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
     if (w == W) {  // courier: the backward channel of block b is hand[2b+1]
@@ -444,22 +496,19 @@ __global__ void __launch_bounds__(64 * (W + 1))
     if (rb * W + w >= g.nw) return;  // past the grid's last wave (last block)
     const int lane = threadIdx.x & 63, wg = rb * W + w;
     const unsigned S = g.S, j0 = wg * 64;
-    const bool src = wg < g.nw - 1;  // a wave above exists
-    const int Ttop = g.NG * kL - 1;      // step u handles anti-diagonal Ttop - u
-    const T *lin = lds + (w + 1) * HS;
-    T *lout = lds + w * HS;
+    const int Ttop = g.NG * kL - 1;  // step u handles anti-diagonal Ttop - u
     const Rsrc rbk = rsrc(Bk, a.rbytes), rz = rsrc(zp, a.rbytes / 4);
     const unsigned vo = (g.base + j0 + lane) * (unsigned)sizeof(R), step = S * (unsigned)sizeof(R);
     const unsigned voz = (g.base + j0 + lane) * (unsigned)sizeof(T), stepz = S * (unsigned)sizeof(T);
     auto so = [&](int t) { return (unsigned)(t + kSkewD0) * step; };
     auto soz = [&](int t) { return (unsigned)(t + kSkewD0) * stepz; };
     // lane 63-q of the batch of group G holds step u = G*kL + q (index u)
-    const int qh = 63 - lane < kL ? 63 - lane : kL - 1;
+    const int qh = min(63 - lane, kL - 1);
 
     constexpr int M = ring_batches<T>(), PD = M * kL;
-    T zb[M];
-#pragma unroll
-    for (int h = 0; h < M; ++h) zb[h] = src ? lload(lin + h * kL + qh) : T(0);
+    static_assert(kLook % M == 0, "whole rings per unrolled block");
+    Hand<T> hd;
+    hd.start(lds + (w + 1) * HS, lds + w * HS, HS, qh, lane < kL, kL - lane);
     // prologue stores: +0 into the padding diagonals above the top (see the
     // forward sweep)
     R ring[PD];
@@ -471,16 +520,15 @@ __global__ void __launch_bounds__(64 * (W + 1))
     }
     T zprev = 0, pub = 0;
     int broken = 0;
-    for (int G0 = 0; G0 < g.NG; G0 += M) {
+    int G0 = 0;
+    do {
 #pragma unroll
-        for (int h = 0; h < M; ++h) {
+        for (int h = 0; h < kLook; ++h) {
             const int G = G0 + h;
-            T zc = zb[h];
-            if (src && !broken && __builtin_amdgcn_ballot_w64(lane >= 64 - kL && is_sentinel(zc)))
-                zc = wait_batch(lin + G * kL + qh, lane >= 64 - kL, zc, broken);
+            T zc = hd.take(G, h, broken);
 #pragma unroll
             for (int k = 0; k < kL; ++k) {
-                const int r = h * kL + k, Tb = Ttop - (G * kL + k);
+                const int r = (h % M) * kL + k, Tb = Ttop - (G * kL + k);
                 const R c = ring[r];
                 const T zn_next = shr1z(zc);
                 const T zn = dpp_shl1(zprev, zc);
@@ -491,15 +539,16 @@ __global__ void __launch_bounds__(64 * (W + 1))
                 else
                     z = ((c.a * c.b) - c.c * zprev * c.b) - c.d * zn * c.b;
                 bstore(rz, z, voz, soz(Tb));
-                pub = dpp_shr1(pub, z);
+                if (k > 0) pub = dpp_shr1(pub, zprev);  // the previous step's value (its register is free now)
                 zprev = z;
                 __builtin_amdgcn_sched_barrier(0);
                 ring[r] = bload(rbk, vo, so(Tb - PD), (R *)nullptr);
             }
-            if (lane < kL) lstore(lout + G * kL + kL - lane, pub);
-            zb[h] = src ? lload(lin + (G + M) * kL + qh) : T(0);
+            pub = dpp_shr1(pub, zprev);
+            hd.finish(G, h, pub);
         }
-    }
+        G0 += kLook;
+    } while (G0 < g.NG);
     if (broken) record_fault(a.fault);
 }
 
@@ -644,6 +693,24 @@ __global__ void k_skew_to32(const SkewGrid *__restrict__ grids, const Rec4 *__re
     Bk32[s] = Rec4f{0.f, (float)v.b, (float)(v.c * v.b), (float)(v.d * v.b)};
 }
 
+// The coefficients a cell does not have in the backward sweep (east of each
+// row's last cell, north of the top row) become copysign(0, Dinv), so that
+// their products vanish exactly (x - (+0) == x, x == -0 included).  One
+// thread per edge cell, blockIdx.y: the grid.
+__global__ void k_skew_edges(const SkewGrid *__restrict__ grids, Rec4 *__restrict__ Bk) {
+    const SkewGrid g = grids[blockIdx.y];
+    const int q = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    auto slot = [&](int T, int j) { return g.base + (size_t)(T + kSkewD0) * g.S + (size_t)j; };
+    if (q < g.ny) {  // row q's east-edge cell, step q + nx - 1
+        Rec4 &r = Bk[slot(q + g.nx - 1, q)];
+        r.c = copysign(0.0, r.b);
+    } else if (q < g.ny + g.nx) {  // top-row cell of column q - ny
+        const int j = g.ny - 1;
+        Rec4 &r = Bk[slot(j + q - g.ny, j)];
+        r.d = copysign(0.0, r.b);
+    }
+}
+
 __global__ void k_skew_dinv_natural(int n, const int *__restrict__ slot_of, const Rec4 *__restrict__ F,
                                     double *__restrict__ out) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
@@ -678,7 +745,7 @@ int skew_setup(fvh_mesh *m, const std::vector<SkewRect> &rects) {
     for (const SkewRect &r : rects) maxNG = std::max(maxNG, ng_of(r));
     // hand-over arrays: indices up to NG*kL, read a window (kWin) or a ring of
     // batches (at most 4) past the last one
-    const int HS = (maxNG * kL + kWin + 4 * kL + 64 + 63) / 64 * 64;
+    const int HS = (maxNG * kL + kWin + kLook * kL + 64 + 63) / 64 * 64;
     if ((size_t)2 * HS * sizeof(double) > (size_t)max_lds_bytes()) return FVH_OK;  // too long even for W = 1
     auto L = std::make_unique<SkewLayout>();
     L->HS = HS;
@@ -815,6 +882,11 @@ int skew_update(fvh_eqn *e, const double *val) {
         case 2: FVH_TRY(launch_update_w<2>(e)); break;
         default: FVH_TRY(launch_update_w<3>(e)); break;
     }
+    int edge = 0;
+    for (const SkewGrid &g : L.h_grids) edge = std::max(edge, g.nx + g.ny);
+    hipLaunchKernelGGL(k_skew_edges, dim3((unsigned)((edge + 255) / 256), (unsigned)L.h_grids.size()), dim3(256), 0,
+                       stream(), L.grids.p, e->sk.Bk.p);
+    HIP_TRY(hipGetLastError());
     if (!m->skew->W && !forced_waves()) FVH_TRY(skew_tune(e));
     return FVH_OK;
 }
