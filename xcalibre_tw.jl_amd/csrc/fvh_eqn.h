@@ -79,7 +79,9 @@ struct BandStore {
 };
 // Records of the pipelined band sweeps, in the mesh's skewed slot order:
 // F (b_r, Dinv_r, A_rs, A_rw), Bk (y_r, Dinv_r, A_re, A_rn), U (A_rr,
-// A_rs*A_sr, A_rw*A_wr, 0); zp the backward sweep's output; hand the
+// A_rs*A_sr, A_rw*A_wr, 0); zp the backward sweep's output, one row of S
+// words per anti-diagonal at the records' stride (4S words: slot base + q of
+// a grid at zp[4 base + q' ] with q' = 4 (T + kSkewD0) S + j); hand the
 // global hand-over arrays between blocks ([block][forward | backward][HS]),
 // uhand those of the diagonal recurrence ([block][HS]); *h_fault (pinned
 // host memory) is set by a sweep whose hand-over wait ran out.

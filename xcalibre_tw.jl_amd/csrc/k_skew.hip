@@ -145,6 +145,14 @@ using Rsrc = __amdgpu_buffer_rsrc_t;
 __device__ inline Rsrc rsrc(const void *p, unsigned bytes) {
     return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), (short)0, (int)bytes, 0x00020000);
 }
+// The buffer from the records of step t on (t + kSkewD0 >= 0; `step` bytes
+// per step): the sweeps move their buffers once per loop iteration, so every
+// step's offset is a constant multiple of the step, held in a scalar register
+// for the whole sweep, and a step issues no address arithmetic.
+__device__ inline Rsrc rsrc_at(const void *p, unsigned bytes, int t, unsigned step) {
+    const unsigned off = (unsigned)(t + kSkewD0) * step;
+    return rsrc(static_cast<const char *>(p) + off, bytes - off);
+}
 __device__ inline Rec4 bload(Rsrc r, unsigned vo, unsigned so, Rec4 *) {
     const auto u = __builtin_bit_cast(double2, __builtin_amdgcn_raw_buffer_load_b128(r, vo, so, 0));
     const auto v = __builtin_bit_cast(double2, __builtin_amdgcn_raw_buffer_load_b128(r, vo + 16, so, 0));
@@ -321,9 +329,7 @@ __global__ void __launch_bounds__(64 * (W + 1))
     if (rb * W + w >= g.nw) return;  // past the grid's last wave (last block)
     const int lane = threadIdx.x & 63, wg = rb * W + w;  // wave of the grid
     const unsigned S = g.S, j = wg * 64 + lane;
-    const Rsrc ru = rsrc(U, a.rbytes), rf = rsrc(F, a.rbytes), rbk = rsrc(Bk, a.rbytes);
     const unsigned vo = (g.base + j) * (unsigned)sizeof(Rec4), step = S * (unsigned)sizeof(Rec4);
-    auto so = [&](int T) { return (unsigned)(T + kSkewD0) * step; };
 
     constexpr int M = ring_batches<double>(), PD = M * kL;
     static_assert(kLook % M == 0, "whole rings per unrolled block");
@@ -333,36 +339,44 @@ __global__ void __launch_bounds__(64 * (W + 1))
     // that the pre-start padding holds already, one record load), so that the
     // loop's memory waits count the same outstanding operations every step
     Rec4 ring[PD];
+    {
+        const Rsrc ru = rsrc_at(U, a.rbytes, 0, step), rf = rsrc_at(F, a.rbytes, -PD, step),
+                   rbk = rsrc_at(Bk, a.rbytes, -PD, step);
 #pragma unroll
-    for (int k = 0; k < PD; ++k) {
-        bstore(rf, 1.0, vo + 8, so(k - PD));
-        bstore(rbk, 1.0, vo + 8, so(k - PD));
-        ring[k] = bload(ru, vo, so(k), (Rec4 *)nullptr);
-        __builtin_amdgcn_sched_barrier(0);  // issue in consumption order
+        for (int k = 0; k < PD; ++k) {
+            bstore(rf, 1.0, vo + 8, k * step);
+            bstore(rbk, 1.0, vo + 8, k * step);
+            ring[k] = bload(ru, vo, k * step, (Rec4 *)nullptr);
+            __builtin_amdgcn_sched_barrier(0);  // issue in consumption order
+        }
     }
     double dprev = 1.0, pub = 1.0;
     int broken = 0;
     int G0 = 0;
     do {
+        // this iteration's steps G0*kL + i, i < kLook*kL: stores at i*step, refills PD steps on
+        const Rsrc ru = rsrc_at(U, a.rbytes, G0 * kL + PD, step), rf = rsrc_at(F, a.rbytes, G0 * kL, step),
+                   rbk = rsrc_at(Bk, a.rbytes, G0 * kL, step);
 #pragma unroll
         for (int h = 0; h < kLook; ++h) {
             const int G = G0 + h;
             double cur = hd.take(G, h, broken);
 #pragma unroll
             for (int k = 0; k < kL; ++k) {
-                const int r = (h % M) * kL + k, T = G * kL + k;
+                const int r = (h % M) * kL + k;
+                const unsigned si = (unsigned)(h * kL + k) * step;
                 const Rec4 u = ring[r];
                 const double nxt = shl1z(cur);
                 const double ds = dpp_shr1(dprev, cur);
                 cur = nxt;
                 const double d = (u.a - u.b / ds) - u.c / dprev;
                 const double di = 1.0 / d;
-                bstore(rf, di, vo + 8, so(T));
-                bstore(rbk, di, vo + 8, so(T));
+                bstore(rf, di, vo + 8, si);
+                bstore(rbk, di, vo + 8, si);
                 if (k > 0) pub = dpp_shl1(pub, dprev);  // the previous step's D (its register is free now)
                 dprev = d;
                 __builtin_amdgcn_sched_barrier(0);  // the slot is consumed: refill it below
-                ring[r] = bload(ru, vo, so(T + PD), (Rec4 *)nullptr);
+                ring[r] = bload(ru, vo, si, (Rec4 *)nullptr);
             }
             pub = dpp_shl1(pub, dprev);
             hd.finish(G, h, pub);
@@ -399,9 +413,7 @@ __global__ void __launch_bounds__(64 * (W + 1))
     if (rb * W + w >= g.nw) return;  // past the grid's last wave (last block)
     const int lane = threadIdx.x & 63, wg = rb * W + w;
     const unsigned S = g.S, j0 = wg * 64;
-    const Rsrc rf = rsrc(F, a.rbytes), rbk = rsrc(Bk, a.rbytes);
     const unsigned vo = (g.base + j0 + lane) * (unsigned)sizeof(R), step = S * (unsigned)sizeof(R);
-    auto so = [&](int t) { return (unsigned)(t + kSkewD0) * step; };
     // lane 0's south pivot: Dinv of (step T-1, row j0-1), static; lane q < kL
     // of the batch of group G holds step G*kL + q
     const int ql = min(lane, kL - 1);
@@ -423,23 +435,29 @@ __global__ void __launch_bounds__(64 * (W + 1))
     // already), so that it issues what a step issues and the loop's memory
     // waits count the same outstanding operations on every iteration.
     R ring[PD];
+    {
+        const Rsrc rf = rsrc_at(F, a.rbytes, 0, step), rbk = rsrc_at(Bk, a.rbytes, -PD, step);
 #pragma unroll
-    for (int k = 0; k < PD; ++k) {
-        bstore(rbk, T(0), vo, so(k - PD));
-        ring[k] = bload(rf, vo, so(k), (R *)nullptr);
-        __builtin_amdgcn_sched_barrier(0);
+        for (int k = 0; k < PD; ++k) {
+            bstore(rbk, T(0), vo, k * step);
+            ring[k] = bload(rf, vo, k * step, (R *)nullptr);
+            __builtin_amdgcn_sched_barrier(0);
+        }
     }
     T xprev = 0, dprev = 1, pub = 0;
     int broken = 0;
     int G0 = 0;
     do {
+        // this iteration's steps G0*kL + i, i < kLook*kL: stores at i*step, refills PD steps on
+        const Rsrc rf = rsrc_at(F, a.rbytes, G0 * kL + PD, step), rbk = rsrc_at(Bk, a.rbytes, G0 * kL, step);
 #pragma unroll
         for (int h = 0; h < kLook; ++h) {
             const int G = G0 + h;
             T yc = hd.take(G, h, broken), dc = db[h];
 #pragma unroll
             for (int k = 0; k < kL; ++k) {
-                const int r = (h % M) * kL + k, Ts = G * kL + k;
+                const int r = (h % M) * kL + k;
+                const unsigned si = (unsigned)(h * kL + k) * step;
                 const R c = ring[r];
                 T x;
                 if constexpr (kFused) {  // (b, A_rs Dinv_s, A_rw Dinv_w)
@@ -456,11 +474,11 @@ __global__ void __launch_bounds__(64 * (W + 1))
                     x = (c.a - c.c * xs * ds) - c.d * xprev * dprev;
                     dprev = keep(c.b);
                 }
-                bstore(rbk, x, vo, so(Ts));
+                bstore(rbk, x, vo, si);
                 if (k > 0) pub = dpp_shl1(pub, xprev);  // the previous step's value (its register is free now)
                 xprev = x;
                 __builtin_amdgcn_sched_barrier(0);
-                ring[r] = bload(rf, vo, so(Ts + PD), (R *)nullptr);
+                ring[r] = bload(rf, vo, si, (R *)nullptr);
             }
             pub = dpp_shl1(pub, xprev);
             hd.finish(G, h, pub);
@@ -497,11 +515,10 @@ __global__ void __launch_bounds__(64 * (W + 1))
     const int lane = threadIdx.x & 63, wg = rb * W + w;
     const unsigned S = g.S, j0 = wg * 64;
     const int Ttop = g.NG * kL - 1;  // step u handles anti-diagonal Ttop - u
-    const Rsrc rbk = rsrc(Bk, a.rbytes), rz = rsrc(zp, a.rbytes / 4);
     const unsigned vo = (g.base + j0 + lane) * (unsigned)sizeof(R), step = S * (unsigned)sizeof(R);
-    const unsigned voz = (g.base + j0 + lane) * (unsigned)sizeof(T), stepz = S * (unsigned)sizeof(T);
-    auto so = [&](int t) { return (unsigned)(t + kSkewD0) * step; };
-    auto soz = [&](int t) { return (unsigned)(t + kSkewD0) * stepz; };
+    // z rows lie at the records' stride: the same step offsets as the records
+    const unsigned voz = (4 * g.base + j0 + lane) * (unsigned)sizeof(T);
+    constexpr int NI = kLook * kL;  // steps per loop iteration
     // lane 63-q of the batch of group G holds step u = G*kL + q (index u)
     const int qh = min(63 - lane, kL - 1);
 
@@ -511,24 +528,33 @@ __global__ void __launch_bounds__(64 * (W + 1))
     hd.start(lds + (w + 1) * HS, lds + w * HS, HS, qh, lane < kL, kL - lane);
     // prologue stores: +0 into the padding diagonals above the top (see the
     // forward sweep)
+    // (offsets count down from the top of each range: step Ttop - u sits at
+    // (NI - 1 - i) strides above the range's lowest step, u = G0*kL + i)
     R ring[PD];
+    {
+        const Rsrc rbk = rsrc_at(Bk, a.rbytes, Ttop - (NI - 1), step);
+        const Rsrc rz = rsrc_at(zp, a.rbytes, Ttop + PD - (NI - 1), step);
 #pragma unroll
-    for (int k = 0; k < PD; ++k) {
-        bstore(rz, T(0), voz, soz(Ttop + PD - k));
-        ring[k] = bload(rbk, vo, so(Ttop - k), (R *)nullptr);
-        __builtin_amdgcn_sched_barrier(0);
+        for (int k = 0; k < PD; ++k) {
+            bstore(rz, T(0), voz, (unsigned)(NI - 1 - k) * step);
+            ring[k] = bload(rbk, vo, (unsigned)(NI - 1 - k) * step, (R *)nullptr);
+            __builtin_amdgcn_sched_barrier(0);
+        }
     }
     T zprev = 0, pub = 0;
     int broken = 0;
     int G0 = 0;
     do {
+        // this iteration's steps u = G0*kL + i: z of diagonal Ttop - u, refills of Ttop - u - PD
+        const int tl = Ttop - G0 * kL - (NI - 1);  // lowest diagonal written this iteration
+        const Rsrc rbk = rsrc_at(Bk, a.rbytes, tl - PD, step), rz = rsrc_at(zp, a.rbytes, tl, step);
 #pragma unroll
         for (int h = 0; h < kLook; ++h) {
             const int G = G0 + h;
             T zc = hd.take(G, h, broken);
 #pragma unroll
             for (int k = 0; k < kL; ++k) {
-                const int r = (h % M) * kL + k, Tb = Ttop - (G * kL + k);
+                const int r = (h % M) * kL + k, i = h * kL + k;
                 const R c = ring[r];
                 const T zn_next = shr1z(zc);
                 const T zn = dpp_shl1(zprev, zc);
@@ -538,11 +564,11 @@ __global__ void __launch_bounds__(64 * (W + 1))
                     z = __builtin_fmaf(-c.d, zn, __builtin_fmaf(-c.c, zprev, c.a * c.b));
                 else
                     z = ((c.a * c.b) - c.c * zprev * c.b) - c.d * zn * c.b;
-                bstore(rz, z, voz, soz(Tb));
+                bstore(rz, z, voz, (unsigned)(NI - 1 - i) * step);
                 if (k > 0) pub = dpp_shr1(pub, zprev);  // the previous step's value (its register is free now)
                 zprev = z;
                 __builtin_amdgcn_sched_barrier(0);
-                ring[r] = bload(rbk, vo, so(Tb - PD), (R *)nullptr);
+                ring[r] = bload(rbk, vo, (unsigned)(NI - 1 - i) * step, (R *)nullptr);
             }
             pub = dpp_shr1(pub, zprev);
             hd.finish(G, h, pub);
@@ -561,7 +587,8 @@ __global__ void k_skew_init(size_t ns, Rec4 *F, Rec4 *Bk, Rec4 *U, double *zp) {
     F[s] = Rec4{0.0, 1.0, 0.0, 0.0};
     Bk[s] = Rec4{0.0, 1.0, 0.0, 0.0};
     U[s] = Rec4{1.0, 0.0, 0.0, 0.0};
-    zp[s] = 0.0;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) zp[4 * s + q] = 0.0;
 }
 
 template <typename T>
@@ -658,7 +685,7 @@ __global__ void __launch_bounds__(256)
     for (int it = 0; it < kTile / 2; ++it) {
         const int d = wv + 4 * it, j = tl.j0 + lane, il = d - lane;
         const bool in = il >= 0 && il < kTile && tl.i0 + il < g.nx && j < g.ny;
-        v[it] = in ? zp[g.base + (size_t)(tl.i0 + tl.j0 + d + kSkewD0) * g.S + j] : T(0);
+        v[it] = in ? zp[4 * (g.base + (size_t)(tl.i0 + tl.j0 + d + kSkewD0) * g.S) + j] : T(0);
     }
 #pragma unroll
     for (int it = 0; it < kTile / 2; ++it) {
@@ -827,7 +854,7 @@ static int skew_alloc(fvh_eqn *e) {
     FVH_TRY(s.F.alloc(L.NS));
     FVH_TRY(s.Bk.alloc(L.NS));
     FVH_TRY(s.U.alloc(L.NS));
-    FVH_TRY(s.zp.alloc(L.NS));
+    FVH_TRY(s.zp.alloc(4 * L.NS));  // rows at the records' stride
     FVH_TRY(s.hand.alloc((size_t)L.nblk[0] * 2 * L.HS));  // W = 1 has the most blocks
     FVH_TRY(s.uhand.alloc((size_t)L.nblk[0] * L.HS));
     if (!s.h_fault) {
@@ -981,9 +1008,9 @@ int skew_to32(fvh_eqn *e) {
     if (s.F32.n != L.NS) {
         FVH_TRY(s.F32.alloc(L.NS));
         FVH_TRY(s.Bk32.alloc(L.NS));
-        FVH_TRY(s.zp32.alloc(L.NS));
+        FVH_TRY(s.zp32.alloc(4 * L.NS));
         FVH_TRY(s.hand32.alloc((size_t)L.nblk[0] * 2 * L.HS));
-        hipLaunchKernelGGL(k_skew_fill<float>, grid_for(L.NS), dim3(256), 0, stream(), L.NS, s.zp32.p, 0.f);
+        hipLaunchKernelGGL(k_skew_fill<float>, grid_for(4 * L.NS), dim3(256), 0, stream(), 4 * L.NS, s.zp32.p, 0.f);
         HIP_TRY(hipGetLastError());
     }
     size_t most = 0;
