@@ -8,11 +8,11 @@
 // wave below: every wave collects its top lane's results over kSkewL steps
 // (a DPP shift per step) and writes them as one batch to an LDS hand-over
 // array; the wave above reads one batch per kSkewL steps, a batch ahead of
-// use.  Between blocks (on other CUs), a courier wave per block relays the
-// batches: it copies the top wave's LDS batches to a device-coherent global
-// array and the block below's batches from theirs into its own LDS.  Only
-// the courier issues device-coherent loads and stores, so their latency never
-// sits in the compute waves' in-order memory counters.  No barrier couples
+// use.  Between blocks (on other CUs), two courier waves per block relay the
+// batches: one copies the top wave's LDS batches to a device-coherent global
+// array, the other the block below's batches from theirs into its own LDS.
+// Only the couriers issue device-coherent loads and stores, so their latency
+// never sits in the compute waves' in-order memory counters.  No barrier couples
 // the waves: each waits only for the batch it needs, the waves run a few
 // batches apart, and every compute wave issues at its own rate (one per
 // SIMD).  The backward sweep runs the same pipeline from the top row block
@@ -202,51 +202,75 @@ __device__ inline void lds_reset(T *lds, int count, int HS, T first, int open) {
         lds[q] = (q % HS == 0 || q / HS == open) ? first : hand_sentinel<T>();
 }
 
-// The courier wave of a block: moves hand-over batches from the global array
-// gin (the neighbour block's, device-coherent) into LDS array lin, and from
-// LDS array lout into the block's own global array gout, kWin words per poll,
-// in whole batches, as they become available.  Indices 1 .. NG*kL.
+// The courier waves of a block relay hand-over batches between blocks, in
+// whole batches, kWin words per poll.  The incoming courier copies the
+// neighbour block's device-coherent global array gin into LDS array lin; it
+// keeps kPolls loads of its window in flight, so a batch that lands is seen
+// about one load latency later (each sample carries the window it read, and
+// the batches it shows complete beyond those already moved are stored).  The
+// outgoing courier copies LDS array lout into the block's own global array
+// gout as the compute wave writes it (its stores are never waited for).
+// Indices 1 .. NG*kL; every loop is bounded and records a fault when a batch
+// never comes.
+constexpr int kPolls = 4;
+constexpr int kPollLimit = 1 << 17;  // samples without progress (tens of ms) before a hand-over is declared lost
+
 template <typename T>
-__device__ void courier(const T *gin, T *lin, const T *lout, T *gout, int NG, int *fault) {
+__device__ void courier_in(const T *gin, T *lin, int NG, int *fault) {
+    if (!gin) return;
+    const int lane = threadIdx.x & 63;
+    const int l = lane < kWin ? lane : kWin - 1;
+    T val[kPolls];
+    int base[kPolls];
+#pragma unroll
+    for (int p = 0; p < kPolls; ++p) {
+        base[p] = 0;
+        val[p] = gload(gin + 1 + l);
+    }
+    int gi = 0, idle = 0;
+    while (gi < NG && idle < kPollLimit) {
+#pragma unroll
+        for (int p = 0; p < kPolls; ++p) {
+            const int b0 = base[p];
+            const unsigned long long bad = __builtin_amdgcn_ballot_w64(lane < kWin && is_sentinel(val[p]));
+            const int upto = min(b0 + (bad ? (int)(__builtin_ctzll(bad) / kL) : kWin / kL), NG);
+            if (upto > gi) {  // batches [gi, upto) are new in this sample
+                const int idx = b0 * kL + 1 + lane;
+                if (lane < kWin && idx > gi * kL && idx <= upto * kL) lstore(lin + idx, val[p]);
+                gi = upto;
+                idle = 0;
+            } else {
+                ++idle;
+            }
+            base[p] = gi;
+            val[p] = gload(gin + gi * kL + 1 + l);  // gi <= NG: inside the array
+        }
+    }
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");  // the samples still in flight
+    if (gi < NG) record_fault(fault);
+}
+
+template <typename T>
+__device__ void courier_out(const T *lout, T *gout, int NG, int *fault) {
+    if (!gout) return;
     const int lane = threadIdx.x & 63;
     const bool act = lane < kWin;
-    int gi = gin ? 0 : NG, go = gout ? 0 : NG;  // batches moved
-    int idle = 0;
-    while ((gi < NG || go < NG) && idle < kSpinLimit) {
-        bool moved = false;
-        if (gi < NG) {
-            const int idx = gi * kL + 1 + (act ? lane : 0);
-            const T v = gload(gin + idx);
-            // whole batches at the front of the window that are complete
-            const unsigned long long bad = __builtin_amdgcn_ballot_w64(act && is_sentinel(v));
-            const int ready = bad ? (__builtin_ctzll(bad) / kL) : kWin / kL;
-            const int nb = min(ready, NG - gi);
-            if (nb > 0) {
-                if (lane < nb * kL) lstore(lin + idx, v);
-                gi += nb;
-                moved = true;
-            }
-        }
-        if (go < NG) {
-            const int idx = go * kL + 1 + (act ? lane : 0);
-            const T v = lload(lout + idx);
-            const unsigned long long bad = __builtin_amdgcn_ballot_w64(act && is_sentinel(v));
-            const int ready = bad ? (__builtin_ctzll(bad) / kL) : kWin / kL;
-            const int nb = min(ready, NG - go);
-            if (nb > 0) {
-                if (lane < nb * kL) gstore(gout + idx, v);
-                go += nb;
-                moved = true;
-            }
-        }
-        if (moved) {
+    int go = 0, idle = 0;
+    while (go < NG && idle < kSpinLimit) {
+        const int idx = go * kL + 1 + (act ? lane : 0);
+        const T v = lload(lout + idx);
+        const unsigned long long bad = __builtin_amdgcn_ballot_w64(act && is_sentinel(v));
+        const int nb = min(bad ? (int)(__builtin_ctzll(bad) / kL) : kWin / kL, NG - go);
+        if (nb > 0) {
+            if (lane < nb * kL) gstore(gout + idx, v);
+            go += nb;
             idle = 0;
         } else {
             ++idle;
             __builtin_amdgcn_s_sleep(1);
         }
     }
-    if (gi < NG || go < NG) record_fault(fault);
+    if (go < NG) record_fault(fault);
 }
 
 // ---------------------------------------------------------------------------
@@ -310,7 +334,7 @@ struct Hand {
 // Diagonal recurrence (update_dilu_diagonal!): Dinv into F.b and Bk.b
 // ---------------------------------------------------------------------------
 template <int W>
-__global__ void __launch_bounds__(64 * (W + 1))
+__global__ void __launch_bounds__(64 * (W + 2))
     k_skew_update(SkewArgs a, const Rec4 *__restrict__ U, Rec4 *__restrict__ F, Rec4 *__restrict__ Bk,
                   double *__restrict__ hand) {
     extern __shared__ unsigned char skew_lds[];
@@ -321,9 +345,12 @@ __global__ void __launch_bounds__(64 * (W + 1))
     lds_reset(lds, (W + 1) * HS, HS, 1.0, rb == 0 ? 0 : -1);
     __syncthreads();
     const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
-    if (w == W) {  // courier: D of the row below from block rb-1, the top row's D to block rb+1
-        courier(rb > 0 ? hand + (size_t)(b - 1) * HS : nullptr, lds, lds + W * HS,
-                rb < nb - 1 ? hand + (size_t)b * HS : nullptr, g.NG, a.fault);
+    if (w == W) {  // incoming courier: D of the row below from block rb-1
+        courier_in(rb > 0 ? hand + (size_t)(b - 1) * HS : nullptr, lds, g.NG, a.fault);
+        return;
+    }
+    if (w == W + 1) {  // outgoing courier: the top row's D to block rb+1
+        courier_out(lds + W * HS, rb < nb - 1 ? hand + (size_t)b * HS : nullptr, g.NG, a.fault);
         return;
     }
     if (rb * W + w >= g.nw) return;  // past the grid's last wave (last block)
@@ -391,7 +418,7 @@ __global__ void __launch_bounds__(64 * (W + 1))
 // reference's arithmetic; FP32: the inner solves of mixed precision)
 // ---------------------------------------------------------------------------
 template <typename T, int W>
-__global__ void __launch_bounds__(64 * (W + 1))
+__global__ void __launch_bounds__(64 * (W + 2))
     k_skew_fwd(SkewArgs a, const Rec4T<T> *__restrict__ F, Rec4T<T> *__restrict__ Bk, T *__restrict__ hand,
                const KScal *ks) {
     using R = Rec4T<T>;
@@ -405,9 +432,12 @@ __global__ void __launch_bounds__(64 * (W + 1))
     lds_reset(lds, (W + 1) * HS, HS, T(0), rb == 0 ? 0 : -1);
     __syncthreads();
     const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
-    if (w == W) {  // courier: the forward channel of block b is hand[2b]
-        courier(rb > 0 ? hand + (size_t)(b - 1) * 2 * HS : nullptr, lds, lds + W * HS,
-                rb < nb - 1 ? hand + (size_t)b * 2 * HS : nullptr, g.NG, a.fault);
+    if (w == W) {  // couriers: the forward channel of block b is hand[2b]
+        courier_in(rb > 0 ? hand + (size_t)(b - 1) * 2 * HS : nullptr, lds, g.NG, a.fault);
+        return;
+    }
+    if (w == W + 1) {
+        courier_out(lds + W * HS, rb < nb - 1 ? hand + (size_t)b * 2 * HS : nullptr, g.NG, a.fault);
         return;
     }
     if (rb * W + w >= g.nw) return;  // past the grid's last wave (last block)
@@ -491,7 +521,7 @@ __global__ void __launch_bounds__(64 * (W + 1))
 }
 
 template <typename T, int W>
-__global__ void __launch_bounds__(64 * (W + 1))
+__global__ void __launch_bounds__(64 * (W + 2))
     k_skew_bwd(SkewArgs a, const Rec4T<T> *__restrict__ Bk, T *__restrict__ zp, T *__restrict__ hand,
                const KScal *ks) {
     using R = Rec4T<T>;
@@ -506,9 +536,12 @@ __global__ void __launch_bounds__(64 * (W + 1))
     lds_reset(lds, (W + 1) * HS, HS, T(0), rb == nb - 1 ? g.nw - rb * W : -1);
     __syncthreads();
     const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
-    if (w == W) {  // courier: the backward channel of block b is hand[2b+1]
-        courier(rb < nb - 1 ? hand + ((size_t)(b + 1) * 2 + 1) * HS : nullptr, lds + W * HS, lds,
-                rb > 0 ? hand + ((size_t)b * 2 + 1) * HS : nullptr, g.NG, a.fault);
+    if (w == W) {  // couriers: the backward channel of block b is hand[2b+1]
+        courier_in(rb < nb - 1 ? hand + ((size_t)(b + 1) * 2 + 1) * HS : nullptr, lds + W * HS, g.NG, a.fault);
+        return;
+    }
+    if (w == W + 1) {
+        courier_out(lds, rb > 0 ? hand + ((size_t)b * 2 + 1) * HS : nullptr, g.NG, a.fault);
         return;
     }
     if (rb * W + w >= g.nw) return;  // past the grid's last wave (last block)
@@ -887,7 +920,7 @@ static int launch_update_w(fvh_eqn *e) {
     const SkewLayout &L = *e->m->skew;
     const size_t lds = (size_t)(W + 1) * L.HS * sizeof(double);
     FVH_TRY(allow_lds(k_skew_update<W>, lds));
-    hipLaunchKernelGGL((k_skew_update<W>), dim3(L.nblk[W - 1]), dim3(64 * (W + 1)), lds, stream(),
+    hipLaunchKernelGGL((k_skew_update<W>), dim3(L.nblk[W - 1]), dim3(64 * (W + 2)), lds, stream(),
                        skew_args<Rec4>(e, W), e->sk.U.p, e->sk.F.p, e->sk.Bk.p, e->sk.uhand.p);
     HIP_TRY(hipGetLastError());
     return FVH_OK;
@@ -925,9 +958,9 @@ static int launch_sweeps_w(fvh_eqn *e, Rec4T<T> *F, Rec4T<T> *Bk, T *zp, T *hand
     const size_t lds = (size_t)(W + 1) * L.HS * sizeof(T);
     FVH_TRY(allow_lds(k_skew_fwd<T, W>, lds));
     FVH_TRY(allow_lds(k_skew_bwd<T, W>, lds));
-    hipLaunchKernelGGL((k_skew_fwd<T, W>), dim3(a.nblk), dim3(64 * (W + 1)), lds, stream(), a, F, Bk, hand, ks);
+    hipLaunchKernelGGL((k_skew_fwd<T, W>), dim3(a.nblk), dim3(64 * (W + 2)), lds, stream(), a, F, Bk, hand, ks);
     HIP_TRY(hipGetLastError());
-    hipLaunchKernelGGL((k_skew_bwd<T, W>), dim3(a.nblk), dim3(64 * (W + 1)), lds, stream(), a, Bk, zp, hand, ks);
+    hipLaunchKernelGGL((k_skew_bwd<T, W>), dim3(a.nblk), dim3(64 * (W + 2)), lds, stream(), a, Bk, zp, hand, ks);
     HIP_TRY(hipGetLastError());
     return FVH_OK;
 }
