@@ -154,6 +154,8 @@ struct CaseLayout {
 // and of the kSkewD0 diagonals below T = 0 and the prefetch margin above the
 // last diagonal are padding.  One compute wave owns 64 rows; a block runs W
 // consecutive waves (W is a launch parameter, tuned per mesh).
+constexpr int kLevRing = 16384;    // results the level walk keeps in LDS (by schedule position)
+constexpr int kLevLate = 1 << 30;  // flag of a neighbour read after the level barrier
 constexpr int kSkewL = 8;                    // steps per hand-over batch
 constexpr int kSkewPD = 32;                  // deepest record prefetch (steps)
 constexpr int kSkewD0 = kSkewPD + 2;         // padding diagonals below T = 0
@@ -195,6 +197,14 @@ struct Schedule {
     int max_deps = 0;  // most lower (or upper) neighbours of a row
     DBuf<int> fptr, frows, bptr, brows;
     std::vector<int> h_fptr, h_bptr;
+    // Per matrix entry (lower entries: forward schedule, upper: backward),
+    // where the level walk finds the neighbour's result: -(1 + k) for word k
+    // of the LDS ring of results (the neighbour's schedule position p, word
+    // p mod kLevRing, still held when the row's level ends), j for a result
+    // two or more levels back (prefetched a level ahead), j | kLevLate for
+    // one a level back that the ring no longer holds (read after the level
+    // barrier; only next to levels wider than the ring).
+    DBuf<int> lcode;
     int band_nx = 0;  // >0: A is a banded grid (deps only r-1, r-nx)
     int band_R = 0;   // grid rows per lane of the sweeps (0: not tuned yet)
     BandGeom g{};

@@ -10,9 +10,8 @@
 // column, value, then the neighbour's result).  Here each row's static data
 // - its index, its lower (or upper) neighbours, their coefficients and the
 // neighbours' reciprocal pivots - sits in one record at its level position,
-// and the records (and the row's own input) of the next level are loaded
-// while the current level computes.  After each barrier a level waits only
-// for its neighbours' fresh results, one memory round trip.
+// loaded ahead of use, and each neighbour's result is found where it is
+// cheapest (below): after each barrier a level mostly waits for LDS only.
 //
 // A record holds at most kLevDeps neighbours; meshes with more per row keep
 // the row-by-row kernels.  The FP32 sweeps (inner solves of mixed precision)
@@ -45,8 +44,8 @@ struct alignas(16) LevRec {
 // ---------------------------------------------------------------------------
 __global__ void k_lev_pack(int n, const int *__restrict__ rows, const int *__restrict__ rowptr,
                            const int *__restrict__ col, const int *__restrict__ diag, const int *__restrict__ tslot,
-                           const double *__restrict__ val, const double *__restrict__ Dinv, int mode,
-                           LevRec<double> *__restrict__ R) {
+                           const int *__restrict__ lcode, const double *__restrict__ val,
+                           const double *__restrict__ Dinv, int mode, LevRec<double> *__restrict__ R) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n) return;
     const int i = rows[k];
@@ -58,7 +57,8 @@ __global__ void k_lev_pack(int n, const int *__restrict__ rows, const int *__res
     for (int q = 0; q < kLevDeps; ++q) r.j[q] = i;  // unused slots point at the row itself (never read)
     for (int q = 0; q < r.m; ++q) {
         const int p = p0 + q, j = col[p];
-        r.j[q] = j;
+        r.j[q] = lcode[p];  // where the walk finds neighbour j's result
+        if (r.j[q] >= 0 && (r.j[q] & kLevLate)) r.pad[0] = 1;
         if (mode == 0) {
             r.a[q] = val[p] * val[tslot[p]];
         } else {
@@ -82,6 +82,7 @@ __global__ void k_lev_to32(int n, int mode, const LevRec<double> *__restrict__ R
     LevRec<float> s{};
     s.i = r.i;
     s.m = r.m;
+    s.pad[0] = r.pad[0];
     for (int q = 0; q < kLevDeps; ++q) {
         s.j[q] = r.j[q];
         s.a[q] = (float)(r.a[q] * (mode == 1 ? r.d[q] : r.d[0]));
@@ -93,73 +94,136 @@ __global__ void k_lev_to32(int n, int mode, const LevRec<double> *__restrict__ R
 // ---------------------------------------------------------------------------
 // The level walk.  MODE 0: diagonal recurrence (y = D, y2 = Dinv); 1: forward
 // sweep y = L-solve of x; 2: backward sweep in place on y.
+//
+// One work-group walks the levels, thread t taking row t of each level (and
+// every kLevBlock-th row after it in wider levels), with a barrier between
+// levels.  Each result also goes to an LDS ring at its schedule position
+// (mod kLevRing), which still holds most neighbours when their rows come
+// (neighbours are mostly a few levels back).  A neighbour the ring has lost
+// is two or more levels back, so final when the walk passes the level before
+// its row's: its value is loaded then, one level ahead of use, with the row's
+// input; the records are loaded two levels ahead.  Three register sets take
+// turns (the loop is unrolled by three), so no load is waited for before its
+// use.  A level's critical path holds LDS reads and the barrier only, except
+// for neighbours one level back that a ring narrower than their level lost
+// (read from global memory after the barrier).
 // ---------------------------------------------------------------------------
 template <typename T, int MODE>
-__device__ inline void lev_row(const LevRec<T> &r, T in, const T *y, T *out, T *out2) {
+__device__ inline T lev_eval(const LevRec<T> &r, T in, const T (&nb)[kLevDeps]) {
     if (MODE == 0) {  // D_i = A_ii - sum (A_ij A_ji) / D_j
         T d = r.d[0];
 #pragma unroll
         for (int q = 0; q < kLevDeps; ++q)
-            if (q < r.m) d = d - r.a[q] / y[r.j[q]];
-        out[r.i] = d;
-        out2[r.i] = T(1) / d;
+            if (q < r.m) d = d - r.a[q] / nb[q];
+        return d;
     } else if (MODE == 1) {  // y_i = x_i - sum (A_ij y_j) Dinv_j
         T s = in;
 #pragma unroll
         for (int q = 0; q < kLevDeps; ++q)
             if (q < r.m) {
-                if constexpr (sizeof(T) == 4) s = __builtin_fmaf(-r.a[q], y[r.j[q]], s);  // a = A_ij Dinv_j
-                else s = s - r.a[q] * y[r.j[q]] * r.d[q];
+                if constexpr (sizeof(T) == 4) s = __builtin_fmaf(-r.a[q], nb[q], s);  // a = A_ij Dinv_j
+                else s = s - r.a[q] * nb[q] * r.d[q];
             }
-        out[r.i] = s;
+        return s;
     } else {  // z_i = y_i Dinv_i - sum (A_ij z_j) Dinv_i
         const T di = r.d[0];
         T s = in * di;
 #pragma unroll
         for (int q = 0; q < kLevDeps; ++q)
             if (q < r.m) {
-                if constexpr (sizeof(T) == 4) s = __builtin_fmaf(-r.a[q], y[r.j[q]], s);  // a = A_ij Dinv_i
-                else s = s - r.a[q] * y[r.j[q]] * di;
+                if constexpr (sizeof(T) == 4) s = __builtin_fmaf(-r.a[q], nb[q], s);  // a = A_ij Dinv_i
+                else s = s - r.a[q] * nb[q] * di;
             }
-        out[r.i] = s;
+        return s;
     }
 }
 
+template <typename T>
+struct LevSet {  // one level's first-pass row of this thread
+    LevRec<T> r;
+    T p[kLevDeps];  // neighbours the ring has lost (prefetched)
+    T v;            // the row input
+};
+
+// Loads are issued only for present rows and for neighbours the ring lost
+// (the walk is bound by the one CU's load throughput); the ring is read with
+// a select rather than a branch.  Rows with a neighbour to read after the
+// barrier (flag in pad[0]) take a per-wave slow path that waits for all.
 template <typename T, int MODE>
 __global__ void __launch_bounds__(kLevBlock)
     k_lev_walk(int nlev, const int *__restrict__ lptr, const LevRec<T> *__restrict__ R, const T *__restrict__ x,
                T *y, T *y2, const KScal *ks) {
     if (ks && ks->done) return;
+    extern __shared__ unsigned char lev_lds[];
+    T *ring = reinterpret_cast<T *>(lev_lds);  // [kLevRing] results by schedule position
     const int t = threadIdx.x;
     // the row input (x for the forward sweep, the forward result for the
     // backward sweep) is final before the walk starts
     const T *in = MODE == 1 ? x : y;
-    int beg = lptr[0], end = lptr[1];
-    LevRec<T> r{};
-    T v = T(0);
-    if (beg + t < end) {
-        r = R[beg + t];
-        if (MODE != 0) v = in[r.i];
-    }
-    for (int l = 0; l < nlev; ++l) {
-        // next level's first records, loaded while this level computes
-        const int nend = l + 1 < nlev ? lptr[l + 2] : end;
-        LevRec<T> rn{};
-        T vn = T(0);
-        if (l + 1 < nlev && end + t < nend) {
-            rn = R[end + t];
-            if (MODE != 0) vn = in[rn.i];
+    auto load_rec = [&](LevSet<T> &S, int l) {
+        S.r = LevRec<T>{};
+        S.r.m = -1;  // absent row
+        if (l < nlev && lptr[l] + t < lptr[l + 1]) S.r = R[lptr[l] + t];
+    };
+    auto prefetch = [&](LevSet<T> &S) {
+#pragma unroll
+        for (int q = 0; q < kLevDeps; ++q) {
+            const int c = S.r.j[q];
+            S.p[q] = (q < S.r.m && c >= 0 && !(c & kLevLate)) ? y[c] : T(0);
         }
-        if (beg + t < end) lev_row<T, MODE>(r, v, y, y, y2);
+        S.v = (MODE != 0 && S.r.m >= 0) ? in[S.r.i] : T(0);
+    };
+    auto finish = [&](const LevRec<T> &r, T v, int pos) {
+        ring[pos & (kLevRing - 1)] = v;
+        y[r.i] = v;
+        if (MODE == 0) y2[r.i] = T(1) / v;
+    };
+    auto level = [&](int l, LevSet<T> &S) {
+        const int beg = lptr[l], end = lptr[l + 1];
+        T nb[kLevDeps];
+#pragma unroll
+        for (int q = 0; q < kLevDeps; ++q) {
+            const int c = S.r.j[q];
+            const T rv = ring[c < 0 ? -1 - c : 0];
+            nb[q] = c < 0 ? rv : S.p[q];
+        }
+        if (__builtin_amdgcn_ballot_w64(beg + t < end && S.r.pad[0] != 0)) {  // neighbours one level back, beyond the ring
+            asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+#pragma unroll
+            for (int q = 0; q < kLevDeps; ++q) {
+                const int c = S.r.j[q];
+                if (q < S.r.m && c >= 0 && (c & kLevLate)) nb[q] = y[c & ~kLevLate];
+            }
+        }
+        if (beg + t < end) finish(S.r, lev_eval<T, MODE>(S.r, S.v, nb), beg + t);
         for (int k = beg + t + kLevBlock; k < end; k += kLevBlock) {  // levels wider than the block
             const LevRec<T> rk = R[k];
-            lev_row<T, MODE>(rk, MODE != 0 ? in[rk.i] : T(0), y, y, y2);
+            T nk[kLevDeps];
+#pragma unroll
+            for (int q = 0; q < kLevDeps; ++q) {
+                const int c = rk.j[q];
+                nk[q] = q >= rk.m ? T(0) : c < 0 ? ring[-1 - c] : y[c & ~kLevLate];
+            }
+            finish(rk, lev_eval<T, MODE>(rk, MODE != 0 ? in[rk.i] : T(0), nk), k);
         }
         __syncthreads();
-        r = rn;
-        v = vn;
-        beg = end;
-        end = nend;
+    };
+    LevSet<T> A, B, C;  // levels l, l+1, l+2 at the step of level l (roles rotate)
+    load_rec(A, 0);
+    load_rec(B, 1);
+    prefetch(A);  // level 0 has no neighbours: its inputs only
+    for (int l = 0; l < nlev; l += 3) {
+        load_rec(C, l + 2);
+        prefetch(B);
+        level(l, A);
+        if (l + 1 >= nlev) break;
+        load_rec(A, l + 3);
+        prefetch(C);
+        level(l + 1, B);
+        if (l + 2 >= nlev) break;
+        load_rec(B, l + 4);
+        prefetch(A);
+        level(l + 2, C);
     }
 }
 
@@ -168,8 +232,24 @@ __global__ void __launch_bounds__(kLevBlock)
 // ---------------------------------------------------------------------------
 static dim3 lev_grid(int n) { return dim3((unsigned)((n + 255) / 256)); }
 
+// The walk's LDS ring of results (FP64: 128 KiB) needs the larger dynamic
+// LDS allowance, set once per instantiation.
+template <typename T, int MODE>
+static int launch_walk(int nlev, const int *lptr, const LevRec<T> *R, const T *x, T *y, T *y2, const KScal *ks) {
+    constexpr size_t lds = kLevRing * sizeof(T);
+    static bool allowed = false;
+    if (!allowed) {
+        HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void *>(k_lev_walk<T, MODE>),
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        allowed = true;
+    }
+    hipLaunchKernelGGL((k_lev_walk<T, MODE>), dim3(1), dim3(kLevBlock), lds, stream(), nlev, lptr, R, x, y, y2, ks);
+    HIP_TRY(hipGetLastError());
+    return FVH_OK;
+}
+
 bool lev_path(const fvh_mesh *m) {
-    if (m->sched.nlev <= 0 || m->sched.max_deps > kLevDeps) return false;
+    if (m->sched.nlev <= 0 || m->sched.max_deps > kLevDeps || m->n >= kLevLate) return false;
     const char *env = std::getenv("FVH_LEVEL_RECORDS");
     return !(env && env[0] == '0' && env[1] == 0);
 }
@@ -188,16 +268,15 @@ int lev_dilu_update(fvh_eqn *e, const double *val) {
     auto *Fw = reinterpret_cast<LevRec<double> *>(L.fwd.p);
     auto *Bw = reinterpret_cast<LevRec<double> *>(L.bwd.p);
     hipLaunchKernelGGL(k_lev_pack, lev_grid(n), dim3(256), 0, stream(), n, s.frows.p, m->d.rowptr, m->d.col, m->d.diag,
-                       m->d.tslot, val, (const double *)nullptr, 0, U);
+                       m->d.tslot, s.lcode.p, val, (const double *)nullptr, 0, U);
     HIP_TRY(hipGetLastError());
-    hipLaunchKernelGGL((k_lev_walk<double, 0>), dim3(1), dim3(kLevBlock), 0, stream(), s.nlev, s.fptr.p, U,
-                       (const double *)nullptr, e->D.p, e->Dinv.p, (const KScal *)nullptr);
-    HIP_TRY(hipGetLastError());
+    FVH_TRY((launch_walk<double, 0>(s.nlev, s.fptr.p, U, (const double *)nullptr, e->D.p, e->Dinv.p,
+                                    (const KScal *)nullptr)));
     hipLaunchKernelGGL(k_lev_pack, lev_grid(n), dim3(256), 0, stream(), n, s.frows.p, m->d.rowptr, m->d.col, m->d.diag,
-                       m->d.tslot, val, e->Dinv.p, 1, Fw);
+                       m->d.tslot, s.lcode.p, val, e->Dinv.p, 1, Fw);
     HIP_TRY(hipGetLastError());
     hipLaunchKernelGGL(k_lev_pack, lev_grid(n), dim3(256), 0, stream(), n, s.brows.p, m->d.rowptr, m->d.col, m->d.diag,
-                       m->d.tslot, val, e->Dinv.p, 2, Bw);
+                       m->d.tslot, s.lcode.p, val, e->Dinv.p, 2, Bw);
     HIP_TRY(hipGetLastError());
     return FVH_OK;
 }
@@ -205,12 +284,8 @@ int lev_dilu_update(fvh_eqn *e, const double *val) {
 template <typename T>
 static int lev_apply_t(fvh_eqn *e, const LevRec<T> *Fw, const LevRec<T> *Bw, const T *x, T *y, const KScal *ks) {
     const Schedule &s = e->m->sched;
-    hipLaunchKernelGGL((k_lev_walk<T, 1>), dim3(1), dim3(kLevBlock), 0, stream(), s.nlev, s.fptr.p, Fw, x, y,
-                       (T *)nullptr, ks);
-    HIP_TRY(hipGetLastError());
-    hipLaunchKernelGGL((k_lev_walk<T, 2>), dim3(1), dim3(kLevBlock), 0, stream(), s.nlev, s.bptr.p, Bw, x, y,
-                       (T *)nullptr, ks);
-    HIP_TRY(hipGetLastError());
+    FVH_TRY((launch_walk<T, 1>(s.nlev, s.fptr.p, Fw, x, y, (T *)nullptr, ks)));
+    FVH_TRY((launch_walk<T, 2>(s.nlev, s.bptr.p, Bw, x, y, (T *)nullptr, ks)));
     return FVH_OK;
 }
 
