@@ -198,13 +198,14 @@ struct Schedule {
     DBuf<int> fptr, frows, bptr, brows;
     std::vector<int> h_fptr, h_bptr;
     // Per matrix entry (lower entries: forward schedule, upper: backward),
-    // where the level walk finds the neighbour's result: -(1 + k) for word k
-    // of the LDS ring of results (the neighbour's schedule position p, word
-    // p mod kLevRing, still held when the row's level ends), j for a result
-    // two or more levels back (prefetched a level ahead), j | kLevLate for
-    // one a level back that the ring no longer holds (read after the level
-    // barrier; only next to levels wider than the ring).
-    DBuf<int> lcode;
+    // where the level walk finds the neighbour's result, by the neighbour's
+    // position p in the walk's order: -(1 + k) for word k = p mod kLevRing of
+    // the LDS ring of results (still held when the row's level ends), p for
+    // a result two or more levels back (prefetched a level ahead), p |
+    // kLevLate for one a level back that the ring no longer holds (read after
+    // the level barrier; only next to levels wider than the ring).  f2b[k]:
+    // the forward position of the row at backward position k.
+    DBuf<int> lcode, f2b;
     int band_nx = 0;  // >0: A is a banded grid (deps only r-1, r-nx)
     int band_R = 0;   // grid rows per lane of the sweeps (0: not tuned yet)
     BandGeom g{};

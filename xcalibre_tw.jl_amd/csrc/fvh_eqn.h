@@ -109,9 +109,13 @@ struct alignas(16) LevBlob64 {
 struct alignas(16) LevBlob32 {
     unsigned char b[kLevRecBytes32];
 };
+// Records of the level walks and their vectors in walk order (a: input,
+// b: results).
 struct LevelStore {
     DBuf<LevBlob64> upd, fwd, bwd;
     DBuf<LevBlob32> fwd32, bwd32;
+    DBuf<double> a, b;
+    DBuf<float> a32, b32;
 };
 }  // namespace fvh
 

@@ -145,38 +145,38 @@ struct LevSet {  // one level's first-pass row of this thread
     T v;            // the row input
 };
 
-// Loads are issued only for present rows and for neighbours the ring lost
-// (the walk is bound by the one CU's load throughput); the ring is read with
-// a select rather than a branch.  Rows with a neighbour to read after the
-// barrier (flag in pad[0]) take a per-wave slow path that waits for all.
+// The walk reads its input and writes its results in walk order (position
+// k = row k of the walk), so both are contiguous per level; MODE 0 also
+// writes Dinv in row order.  Loads are issued only for present rows and for
+// neighbours the ring lost (the walk is bound by the one CU's load
+// throughput); the ring is read with a select rather than a branch.  Rows
+// with a neighbour to read after the barrier (flag in pad[0]) take a
+// per-wave slow path that waits for all.
 template <typename T, int MODE>
 __global__ void __launch_bounds__(kLevBlock)
-    k_lev_walk(int nlev, const int *__restrict__ lptr, const LevRec<T> *__restrict__ R, const T *__restrict__ x,
-               T *y, T *y2, const KScal *ks) {
+    k_lev_walk(int nlev, const int *__restrict__ lptr, const LevRec<T> *__restrict__ R, const T *__restrict__ in,
+               T *out, T *dinv, const KScal *ks) {
     if (ks && ks->done) return;
     extern __shared__ unsigned char lev_lds[];
-    T *ring = reinterpret_cast<T *>(lev_lds);  // [kLevRing] results by schedule position
+    T *ring = reinterpret_cast<T *>(lev_lds);  // [kLevRing] results by position
     const int t = threadIdx.x;
-    // the row input (x for the forward sweep, the forward result for the
-    // backward sweep) is final before the walk starts
-    const T *in = MODE == 1 ? x : y;
     auto load_rec = [&](LevSet<T> &S, int l) {
         S.r = LevRec<T>{};
         S.r.m = -1;  // absent row
         if (l < nlev && lptr[l] + t < lptr[l + 1]) S.r = R[lptr[l] + t];
     };
-    auto prefetch = [&](LevSet<T> &S) {
+    auto prefetch = [&](LevSet<T> &S, int l) {
 #pragma unroll
         for (int q = 0; q < kLevDeps; ++q) {
             const int c = S.r.j[q];
-            S.p[q] = (q < S.r.m && c >= 0 && !(c & kLevLate)) ? y[c] : T(0);
+            S.p[q] = (q < S.r.m && c >= 0 && !(c & kLevLate)) ? out[c] : T(0);
         }
-        S.v = (MODE != 0 && S.r.m >= 0) ? in[S.r.i] : T(0);
+        S.v = (MODE != 0 && S.r.m >= 0) ? in[lptr[l] + t] : T(0);
     };
     auto finish = [&](const LevRec<T> &r, T v, int pos) {
         ring[pos & (kLevRing - 1)] = v;
-        y[r.i] = v;
-        if (MODE == 0) y2[r.i] = T(1) / v;
+        out[pos] = v;
+        if (MODE == 0) dinv[r.i] = T(1) / v;
     };
     auto level = [&](int l, LevSet<T> &S) {
         const int beg = lptr[l], end = lptr[l + 1];
@@ -192,7 +192,7 @@ __global__ void __launch_bounds__(kLevBlock)
 #pragma unroll
             for (int q = 0; q < kLevDeps; ++q) {
                 const int c = S.r.j[q];
-                if (q < S.r.m && c >= 0 && (c & kLevLate)) nb[q] = y[c & ~kLevLate];
+                if (q < S.r.m && c >= 0 && (c & kLevLate)) nb[q] = out[c & ~kLevLate];
             }
         }
         if (beg + t < end) finish(S.r, lev_eval<T, MODE>(S.r, S.v, nb), beg + t);
@@ -202,29 +202,41 @@ __global__ void __launch_bounds__(kLevBlock)
 #pragma unroll
             for (int q = 0; q < kLevDeps; ++q) {
                 const int c = rk.j[q];
-                nk[q] = q >= rk.m ? T(0) : c < 0 ? ring[-1 - c] : y[c & ~kLevLate];
+                nk[q] = q >= rk.m ? T(0) : c < 0 ? ring[-1 - c] : out[c & ~kLevLate];
             }
-            finish(rk, lev_eval<T, MODE>(rk, MODE != 0 ? in[rk.i] : T(0), nk), k);
+            finish(rk, lev_eval<T, MODE>(rk, MODE != 0 ? in[k] : T(0), nk), k);
         }
         __syncthreads();
     };
     LevSet<T> A, B, C;  // levels l, l+1, l+2 at the step of level l (roles rotate)
     load_rec(A, 0);
     load_rec(B, 1);
-    prefetch(A);  // level 0 has no neighbours: its inputs only
+    prefetch(A, 0);  // level 0 has no neighbours: its inputs only
     for (int l = 0; l < nlev; l += 3) {
         load_rec(C, l + 2);
-        prefetch(B);
+        prefetch(B, l + 1);
         level(l, A);
         if (l + 1 >= nlev) break;
         load_rec(A, l + 3);
-        prefetch(C);
+        prefetch(C, l + 2);
         level(l + 1, B);
         if (l + 2 >= nlev) break;
         load_rec(B, l + 4);
-        prefetch(A);
+        prefetch(A, l + 3);
         level(l + 2, C);
     }
+}
+
+// Vectors between row order and walk order (all CUs): to[k] = from[idx[k]]
+// (gather) or to[idx[k]] = from[k] (scatter).
+template <typename T, bool SCATTER>
+__global__ void k_lev_perm(int n, const int *__restrict__ idx, const T *__restrict__ from, T *__restrict__ to,
+                           const KScal *ks) {
+    if (ks && ks->done) return;
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    if (SCATTER) to[idx[k]] = from[k];
+    else to[k] = from[idx[k]];
 }
 
 // ---------------------------------------------------------------------------
@@ -235,7 +247,7 @@ static dim3 lev_grid(int n) { return dim3((unsigned)((n + 255) / 256)); }
 // The walk's LDS ring of results (FP64: 128 KiB) needs the larger dynamic
 // LDS allowance, set once per instantiation.
 template <typename T, int MODE>
-static int launch_walk(int nlev, const int *lptr, const LevRec<T> *R, const T *x, T *y, T *y2, const KScal *ks) {
+static int launch_walk(int nlev, const int *lptr, const LevRec<T> *R, const T *in, T *out, T *dinv, const KScal *ks) {
     constexpr size_t lds = kLevRing * sizeof(T);
     static bool allowed = false;
     if (!allowed) {
@@ -243,7 +255,7 @@ static int launch_walk(int nlev, const int *lptr, const LevRec<T> *R, const T *x
                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
         allowed = true;
     }
-    hipLaunchKernelGGL((k_lev_walk<T, MODE>), dim3(1), dim3(kLevBlock), lds, stream(), nlev, lptr, R, x, y, y2, ks);
+    hipLaunchKernelGGL((k_lev_walk<T, MODE>), dim3(1), dim3(kLevBlock), lds, stream(), nlev, lptr, R, in, out, dinv, ks);
     HIP_TRY(hipGetLastError());
     return FVH_OK;
 }
@@ -263,6 +275,8 @@ int lev_dilu_update(fvh_eqn *e, const double *val) {
         FVH_TRY(L.upd.alloc(n));
         FVH_TRY(L.fwd.alloc(n));
         FVH_TRY(L.bwd.alloc(n));
+        FVH_TRY(L.a.alloc(n));
+        FVH_TRY(L.b.alloc(n));
     }
     auto *U = reinterpret_cast<LevRec<double> *>(L.upd.p);
     auto *Fw = reinterpret_cast<LevRec<double> *>(L.fwd.p);
@@ -270,7 +284,7 @@ int lev_dilu_update(fvh_eqn *e, const double *val) {
     hipLaunchKernelGGL(k_lev_pack, lev_grid(n), dim3(256), 0, stream(), n, s.frows.p, m->d.rowptr, m->d.col, m->d.diag,
                        m->d.tslot, s.lcode.p, val, (const double *)nullptr, 0, U);
     HIP_TRY(hipGetLastError());
-    FVH_TRY((launch_walk<double, 0>(s.nlev, s.fptr.p, U, (const double *)nullptr, e->D.p, e->Dinv.p,
+    FVH_TRY((launch_walk<double, 0>(s.nlev, s.fptr.p, U, (const double *)nullptr, L.b.p, e->Dinv.p,
                                     (const KScal *)nullptr)));
     hipLaunchKernelGGL(k_lev_pack, lev_grid(n), dim3(256), 0, stream(), n, s.frows.p, m->d.rowptr, m->d.col, m->d.diag,
                        m->d.tslot, s.lcode.p, val, e->Dinv.p, 1, Fw);
@@ -281,19 +295,29 @@ int lev_dilu_update(fvh_eqn *e, const double *val) {
     return FVH_OK;
 }
 
+// x (row order) -> forward walk order -> forward walk -> backward walk order
+// -> backward walk -> y (row order); wa / wb: two vectors in walk order.
 template <typename T>
-static int lev_apply_t(fvh_eqn *e, const LevRec<T> *Fw, const LevRec<T> *Bw, const T *x, T *y, const KScal *ks) {
+static int lev_apply_t(fvh_eqn *e, const LevRec<T> *Fw, const LevRec<T> *Bw, const T *x, T *y, T *wa, T *wb,
+                       const KScal *ks) {
     const Schedule &s = e->m->sched;
-    FVH_TRY((launch_walk<T, 1>(s.nlev, s.fptr.p, Fw, x, y, (T *)nullptr, ks)));
-    FVH_TRY((launch_walk<T, 2>(s.nlev, s.bptr.p, Bw, x, y, (T *)nullptr, ks)));
+    const int n = (int)e->m->n;
+    hipLaunchKernelGGL((k_lev_perm<T, false>), lev_grid(n), dim3(256), 0, stream(), n, s.frows.p, x, wa, ks);
+    HIP_TRY(hipGetLastError());
+    FVH_TRY((launch_walk<T, 1>(s.nlev, s.fptr.p, Fw, wa, wb, (T *)nullptr, ks)));
+    hipLaunchKernelGGL((k_lev_perm<T, false>), lev_grid(n), dim3(256), 0, stream(), n, s.f2b.p, wb, wa, ks);
+    HIP_TRY(hipGetLastError());
+    FVH_TRY((launch_walk<T, 2>(s.nlev, s.bptr.p, Bw, wa, wb, (T *)nullptr, ks)));
+    hipLaunchKernelGGL((k_lev_perm<T, true>), lev_grid(n), dim3(256), 0, stream(), n, s.brows.p, wb, y, ks);
+    HIP_TRY(hipGetLastError());
     return FVH_OK;
 }
 
 int lev_dilu_apply(fvh_eqn *e, const double *x, double *y, const KScal *ks) {
-    const LevelStore &L = e->lv;
+    LevelStore &L = e->lv;
     if (!L.fwd.p) return set_error(FVH_ERR_ARG, "DILU records not prepared");
     return lev_apply_t<double>(e, reinterpret_cast<const LevRec<double> *>(L.fwd.p),
-                               reinterpret_cast<const LevRec<double> *>(L.bwd.p), x, y, ks);
+                               reinterpret_cast<const LevRec<double> *>(L.bwd.p), x, y, L.a.p, L.b.p, ks);
 }
 
 int lev_to32(fvh_eqn *e) {
@@ -302,6 +326,8 @@ int lev_to32(fvh_eqn *e) {
     if (L.fwd32.n != (size_t)n) {
         FVH_TRY(L.fwd32.alloc(n));
         FVH_TRY(L.bwd32.alloc(n));
+        FVH_TRY(L.a32.alloc(n));
+        FVH_TRY(L.b32.alloc(n));
     }
     hipLaunchKernelGGL(k_lev_to32, lev_grid(n), dim3(256), 0, stream(), n, 1,
                        reinterpret_cast<const LevRec<double> *>(L.fwd.p), reinterpret_cast<LevRec<float> *>(L.fwd32.p));
@@ -313,10 +339,10 @@ int lev_to32(fvh_eqn *e) {
 }
 
 int lev_dilu_apply32(fvh_eqn *e, const float *x, float *y, const KScal *ks) {
-    const LevelStore &L = e->lv;
+    LevelStore &L = e->lv;
     if (!L.fwd32.p) return set_error(FVH_ERR_ARG, "FP32 DILU records not prepared");
     return lev_apply_t<float>(e, reinterpret_cast<const LevRec<float> *>(L.fwd32.p),
-                              reinterpret_cast<const LevRec<float> *>(L.bwd32.p), x, y, ks);
+                              reinterpret_cast<const LevRec<float> *>(L.bwd32.p), x, y, L.a32.p, L.b32.p, ks);
 }
 
 static_assert(sizeof(LevRec<double>) == kLevRecBytes64 && sizeof(LevRec<float>) == kLevRecBytes32,

@@ -185,8 +185,8 @@ int build_schedule(fvh_mesh *m) {
     // a structurally symmetric pattern: the longest dependency chain is the same both ways
     if (nlevb != nlev) return set_error(FVH_ERR_ARG, "matrix pattern is not structurally symmetric");
     m->sched.nlev = nlev;
-    {  // where the level walk finds each neighbour's result
-        std::vector<int> pos(n), code(rp[n]);
+    {  // where the level walk finds each neighbour's result (positions in the walk's order)
+        std::vector<int> pos(n), posf(n), code(rp[n]), f2b(n);
         auto codes = [&](const std::vector<int> &lev, const std::vector<int> &ptr, const std::vector<int> &rows,
                          bool upper) {
             for (int k = 0; k < n; ++k) pos[rows[k]] = k;
@@ -195,15 +195,18 @@ int build_schedule(fvh_mesh *m) {
                 for (int p = upper ? dg[i] + 1 : rp[i]; p < (upper ? rp[i + 1] : dg[i]); ++p) {
                     const int j = col[p];
                     if (pos[j] >= end - kLevRing) code[p] = -(1 + (pos[j] & (kLevRing - 1)));
-                    else if (lev[i] - lev[j] >= 2) code[p] = j;
-                    else code[p] = j | kLevLate;
+                    else if (lev[i] - lev[j] >= 2) code[p] = pos[j];
+                    else code[p] = pos[j] | kLevLate;
                 }
             }
         };
         codes(lf, m->sched.h_fptr, frows, false);
+        posf = pos;
         codes(lb, m->sched.h_bptr, brows, true);
         for (int i = 0; i < n; ++i) code[dg[i]] = i;
+        for (int k = 0; k < n; ++k) f2b[k] = posf[brows[k]];
         FVH_TRY(m->sched.lcode.upload(code));
+        FVH_TRY(m->sched.f2b.upload(f2b));
     }
     int deg = 0;  // most lower / upper neighbours of a row
     for (int i = 0; i < n; ++i) deg = std::max(deg, std::max(dg[i] - rp[i], rp[i + 1] - dg[i] - 1));
