@@ -350,8 +350,10 @@ static int read_ks(fvh_eqn *e) {
 
 __device__ inline double dsign(double x) { return x > 0.0 ? 1.0 : (x < 0.0 ? -1.0 : x); }
 
-// Iterations are enqueued in batches between host checks of ks->done.
-static int next_batch(int done_iters) { return done_iters < 8 ? 2 : (done_iters < 32 ? 4 : 8); }
+// Iterations are enqueued in batches between host checks of ks->done: 2, 2,
+// then as many as are done, up to 16 (each check idles the device for a
+// host round trip; iterations enqueued past the end skip at once).
+static int next_batch(int done_iters) { return done_iters < 4 ? 2 : std::min(16, done_iters); }
 
 // A linear system as the Krylov recurrences see it.  T is the storage type
 // of the vectors: double for the reference path (start value in x, A and M
