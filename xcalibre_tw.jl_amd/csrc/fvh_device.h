@@ -207,6 +207,7 @@ struct Schedule {
     // the forward position of the row at backward position k.
     DBuf<int> lcode, f2b;
     int band_nx = 0;  // >0: A is a banded grid (deps only r-1, r-nx)
+    DBuf<unsigned char> bmask;  // banded grid: bit k of row r = entry r + (-nx, -1, 0, 1, nx)[k] present
     int band_R = 0;   // grid rows per lane of the sweeps (0: not tuned yet)
     BandGeom g{};
     DBuf<int> b_offj, b_iperm;

@@ -54,6 +54,11 @@ struct KrylovWork {
     DBuf<float> f[10];           // FP32 vectors
     DBuf<float> basis32;         // GMRES basis of the inner solves
     DBuf<float> valf;            // FP32 copy of A
+    // banded grids: A's values by band position ([5][n], 0 where a row has
+    // no entry), packed for the duration of a solve (band_live)
+    DBuf<double> bval;
+    DBuf<float> bval32;
+    bool band_live = false;
     std::vector<fvh_stats> case_stats;  // last solve, per independent case
     std::map<std::string, hipGraphExec_t> graphs;  // captured iteration batches (independent cases)
     void drop_graphs() {

@@ -24,10 +24,28 @@
 
 namespace fvh {
 
+// Banded grids (single case): during a solve the SpMV reads A by band
+// position (five values per row, no column indices), bit-identical to CSR.
+static bool band_spmv(const fvh_mesh *m) { return !m->cases && m->sched.band_nx > 0; }
+
+static int band_pack(fvh_eqn *e) {
+    const fvh_mesh *m = e->m;
+    const int n = (int)m->n;
+    if (e->kw.bval.n != 5 * (size_t)n) FVH_TRY(e->kw.bval.alloc(5 * (size_t)n));
+    hipLaunchKernelGGL((k_band_values<double>), grid1d(n), dim3(256), 0, stream(), n, m->d.rowptr, m->sched.bmask.p,
+                       e->nz.p, e->kw.bval.p);
+    HIP_TRY(hipGetLastError());
+    return FVH_OK;
+}
+
 int launch_spmv(fvh_eqn *e, const double *x, double *y, const KScal *ks) {
     const fvh_mesh *m = e->m;
-    hipLaunchKernelGGL((k_spmv<double>), grid1d(m->n), dim3(256), 0, stream(), (int)m->n, m->d.rowptr, m->d.col, e->nz.p,
-                       x, y, 1.0, ks);
+    if (e->kw.band_live)
+        hipLaunchKernelGGL((k_spmv_band<double>), grid1d(m->n), dim3(256), 0, stream(), (int)m->n, m->sched.band_nx,
+                           m->sched.bmask.p, e->kw.bval.p, x, y, 1.0, ks);
+    else
+        hipLaunchKernelGGL((k_spmv<double>), grid1d(m->n), dim3(256), 0, stream(), (int)m->n, m->d.rowptr, m->d.col,
+                           e->nz.p, x, y, 1.0, ks);
     HIP_TRY(hipGetLastError());
     return FVH_OK;
 }
@@ -36,8 +54,12 @@ static int launch_spmv_dot(fvh_eqn *e, const double *x, double *y, double sigma,
                            double *part) {
     const fvh_mesh *m = e->m;
     int nch = nchunks(m->n);
-    hipLaunchKernelGGL((k_spmv_dot<double>), dim3(nch), dim3(kWide), 0, stream(), (int)m->n, m->d.rowptr, m->d.col,
-                       e->nz.p, x, y, sigma, ks, w, part, nch);
+    if (e->kw.band_live)
+        hipLaunchKernelGGL((k_spmv_dot_band<double>), dim3(nch), dim3(kWide), 0, stream(), (int)m->n,
+                           m->sched.band_nx, m->sched.bmask.p, e->kw.bval.p, x, y, sigma, ks, w, part, nch);
+    else
+        hipLaunchKernelGGL((k_spmv_dot<double>), dim3(nch), dim3(kWide), 0, stream(), (int)m->n, m->d.rowptr,
+                           m->d.col, e->nz.p, x, y, sigma, ks, w, part, nch);
     HIP_TRY(hipGetLastError());
     return FVH_OK;
 }
@@ -208,6 +230,18 @@ int precond_apply(fvh_eqn *eq, double *y, const double *x, const KScal *ks_in, d
 int launch_spmv32(fvh_eqn *e, const float *x, float *y, float sigma, const KScal *ks, const float *w,
                   double *part) {
     const fvh_mesh *m = e->m;
+    if (e->kw.band_live && e->kw.bval32.p) {
+        if (!w) {
+            hipLaunchKernelGGL((k_spmv_band<float>), grid1d(m->n), dim3(256), 0, stream(), (int)m->n,
+                               m->sched.band_nx, m->sched.bmask.p, e->kw.bval32.p, x, y, sigma, ks);
+        } else {
+            const int nch = nchunks(m->n);
+            hipLaunchKernelGGL((k_spmv_dot_band<float>), dim3(nch), dim3(kWide), 0, stream(), (int)m->n,
+                               m->sched.band_nx, m->sched.bmask.p, e->kw.bval32.p, x, y, sigma, ks, w, part, nch);
+        }
+        HIP_TRY(hipGetLastError());
+        return FVH_OK;
+    }
     if (!w) {
         hipLaunchKernelGGL((k_spmv<float>), grid1d(m->n), dim3(256), 0, stream(), (int)m->n, m->d.rowptr, m->d.col,
                            e->kw.valf.p, x, y, sigma, ks);
@@ -232,6 +266,12 @@ int prepare32(fvh_eqn *e) {
         const double *a = e->nz.p;
         float *af = w.valf.p;
         FVH_TRY(map(nnz, nullptr, [=] __device__(int i) { af[i] = (float)a[i]; }));
+    }
+    if (band_spmv(m)) {  // the same values by band position (each rounded once, as in valf)
+        if (w.bval32.n != 5 * (size_t)n) FVH_TRY(w.bval32.alloc(5 * (size_t)n));
+        hipLaunchKernelGGL((k_band_values<float>), grid1d(n), dim3(256), 0, stream(), (int)n, m->d.rowptr,
+                           m->sched.bmask.p, e->nz.p, w.bval32.p);
+        HIP_TRY(hipGetLastError());
     }
     fvh_eqn *P = e->pshare ? e->pshare : e;
     fvh_eqn *src = P->pmat ? P->pmat : P;
@@ -909,7 +949,12 @@ static int solve_any(fvh_eqn *e, const fvh_solver_cfg &cfg, double *x, fvh_stats
 }
 
 int solve(fvh_eqn *e, const fvh_solver_cfg &cfg, double *x, fvh_stats *st) {
+    if (band_spmv(e->m)) {  // A is fixed for the solve: its SpMVs read it by band position
+        FVH_TRY(band_pack(e));
+        e->kw.band_live = true;
+    }
     const int rc = solve_any(e, cfg, x, st);
+    e->kw.band_live = false;
     const int fault = skew_check(e);
     return rc != FVH_OK ? rc : fault;
 }

@@ -42,6 +42,37 @@ __device__ inline V row_ax(const int *__restrict__ rowptr, const int *__restrict
     return s;
 }
 
+// Row i of a banded grid's A x: entries among i-nx, i-1, i, i+1, i+nx (bit k
+// of mask[i]), values C[k n + i]; the present ones summed from 0 in
+// ascending column order, as row_ax does, so the two agree bit for bit.
+template <typename V>
+__device__ inline V band_ax(int n, int nx, const unsigned char *__restrict__ mask, const V *__restrict__ C,
+                            const V *__restrict__ x, int i) {
+    const int mk = mask[i];
+    const int off[5] = {-nx, -1, 0, 1, nx};
+    V a[5], xv[5];
+#pragma unroll
+    for (int k = 0; k < 5; ++k) a[k] = C[(size_t)k * n + i];
+#pragma unroll
+    for (int k = 0; k < 5; ++k) xv[k] = x[((mk >> k) & 1) ? i + off[k] : i];
+    V s = 0;
+#pragma unroll
+    for (int k = 0; k < 5; ++k) {
+        const V t = s + a[k] * xv[k];
+        s = ((mk >> k) & 1) ? t : s;
+    }
+    return s;
+}
+
+template <typename V>
+__global__ void __launch_bounds__(256) k_spmv_band(int n, int nx, const unsigned char *__restrict__ mask,
+                                                   const V *__restrict__ C, const V *__restrict__ x,
+                                                   V *__restrict__ y, V sigma, const KScal *ks) {
+    if (ks && ks->done) return;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) y[i] = sigma * band_ax(n, nx, mask, C, x, i);
+}
+
 // One thread per row.
 template <typename V>
 __global__ void __launch_bounds__(256) k_spmv(int n, const int *__restrict__ rowptr, const int *__restrict__ col,
@@ -118,6 +149,45 @@ __global__ void __launch_bounds__(kWide) k_spmv_dot(int n, const int *__restrict
         p[0][j] = pr;
     }
     chunk_reduce_staged<1>(p, part, nch);
+}
+
+// The banded k_spmv_dot.
+template <typename V>
+__global__ void __launch_bounds__(kWide) k_spmv_dot_band(int n, int nx, const unsigned char *__restrict__ mask,
+                                                         const V *__restrict__ C, const V *__restrict__ x,
+                                                         V *__restrict__ y, V sigma, const KScal *ks,
+                                                         const V *__restrict__ w, double *part, int nch) {
+    if (ks && ks->done) return;
+    __shared__ double p[1][kChunk];
+    const int c0 = blockIdx.x * kChunk;
+#pragma unroll
+    for (int h = 0; h < kChunk / kWide; ++h) {
+        const int j = threadIdx.x + h * kWide, i = c0 + j;
+        double pr = 0.0;
+        if (i < n) {
+            const V s = sigma * band_ax(n, nx, mask, C, x, i);
+            y[i] = s;
+            pr = (double)w[i] * (double)s;
+        }
+        p[0][j] = pr;
+    }
+    chunk_reduce_staged<1>(p, part, nch);
+}
+
+// A's values by band position (see band_ax) from the CSR values.
+template <typename V>
+__global__ void k_band_values(int n, const int *__restrict__ rowptr, const unsigned char *__restrict__ mask,
+                              const double *__restrict__ val, V *__restrict__ C) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int mk = mask[i];
+    int p = rowptr[i];
+#pragma unroll
+    for (int k = 0; k < 5; ++k) {
+        const bool in = (mk >> k) & 1;
+        C[(size_t)k * n + i] = in ? (V)val[p] : V(0);
+        p += in ? 1 : 0;
+    }
 }
 
 }  // namespace fvh

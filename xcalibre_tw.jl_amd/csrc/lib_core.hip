@@ -235,6 +235,18 @@ int build_schedule(fvh_mesh *m) {
     } else {
         m->sched.band_nx = 0;
     }
+    if (m->sched.band_nx > 0) {  // which of r-nx, r-1, r, r+1, r+nx each row holds (the banded SpMV)
+        const int bnx = m->sched.band_nx;
+        std::vector<unsigned char> mask(n, 0);
+        for (int i = 0; i < n; ++i)
+            for (int p = rp[i]; p < rp[i + 1]; ++p) {
+                const int d = col[p] - i;
+                const int k = d == -bnx ? 0 : d == -1 ? 1 : d == 0 ? 2 : d == 1 ? 3 : d == bnx ? 4 : -1;
+                if (k < 0) return set_error(FVH_ERR_ARG, "banded pattern with an entry off the band");
+                mask[i] |= (unsigned char)(1u << k);
+            }
+        FVH_TRY(m->sched.bmask.upload(mask));
+    }
     return FVH_OK;
 }
 
