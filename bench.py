@@ -85,6 +85,11 @@ def run_gpu(args, precision):
     ops = {}
     t = timed(0, 200)
     ops["spmv_fp64"] = dict(ms=t, bytes=RL.spmv(n, nnz), GBps=RL.gbps(RL.spmv(n, nnz), t))
+    banded = mesh.device().sweep_info()[1] > 0
+    if banded:  # the SpMV the solves use on a structured grid
+        t = timed(6, 200)
+        b = RL.spmv_band(n)
+        ops["spmv_band_fp64"] = dict(ms=t, bytes=b, GBps=RL.gbps(b, t))
     t = timed(1, 20)
     ops["dilu_apply_fp64"] = dict(ms=t, bytes=RL.dilu_apply(n, nnz), GBps=RL.gbps(RL.dilu_apply(n, nnz), t))
     t = timed(2, 10, None, None)
@@ -93,6 +98,10 @@ def run_gpu(args, precision):
         b = RL.spmv(n, nnz, 4, 4, 4)
         t = timed(3, 200)
         ops["spmv_fp32"] = dict(ms=t, bytes=b, GBps=RL.gbps(b, t))
+        if banded:
+            b = RL.spmv_band(n, 4, 4, 4)
+            t = timed(7, 200)
+            ops["spmv_band_fp32"] = dict(ms=t, bytes=b, GBps=RL.gbps(b, t))
         b = RL.dilu_apply(n, nnz, 4, 4, 4, 4, 4)
         t = timed(4, 20)
         ops["dilu_apply_fp32"] = dict(ms=t, bytes=b, GBps=RL.gbps(b, t))
@@ -358,6 +367,12 @@ def main():
                      "bytes_per_launch": d["bytes"], "ms_per_launch": d["ms"]},
         "spmv": {"kernel": "k_spmv (FP64 CSR)", "GBps": sp["GBps"], "frac": sp["GBps"] / RL.HBM_PEAK_GBS,
                  "bytes_per_launch": sp["bytes"], "ms_per_launch": sp["ms"]},
+        "spmv_band": ({"kernel": "k_spmv_band (FP64, A by band position: the SpMV of the solves on this grid)",
+                       "GBps": head["ops"]["spmv_band_fp64"]["GBps"],
+                       "frac": head["ops"]["spmv_band_fp64"]["GBps"] / RL.HBM_PEAK_GBS,
+                       "bytes_per_launch": head["ops"]["spmv_band_fp64"]["bytes"],
+                       "ms_per_launch": head["ops"]["spmv_band_fp64"]["ms"]}
+                      if "spmv_band_fp64" in head["ops"] else None),
         "cpu_baseline": cpu,
         "forms": out,
     }

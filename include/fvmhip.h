@@ -381,7 +381,9 @@ int fvh_sim_equation(fvh_sim *s, const char *name, fvh_eqn **out);
  * SpMV on the single-precision copy of A, 4 = FP32 preconditioner apply
  * (ops 3/4 round x to FP32 first, all ones when x is NULL; y is unused),
  * 5 = plain copy y = x of ncells doubles (16 bytes per cell moved: the
- * reference point for hardware traffic counters). */
+ * reference point for hardware traffic counters), 6 / 7 = the banded SpMV
+ * that solves on a structured grid use (FP64 / FP32, A read as five values
+ * per row; FVH_ERR_UNSUPPORTED on other meshes; op 7 rounds x like op 3). */
 int fvh_time_op(fvh_eqn *e, int32_t op, int32_t reps, const double *x, double *y, double *ms_per_launch);
 
 /* simple!(model, config; pref): whole run, fields in/out (host). */

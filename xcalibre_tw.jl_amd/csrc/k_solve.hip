@@ -1065,10 +1065,12 @@ extern "C" int fvh_time_op(fvh_eqn *e, int32_t op, int32_t reps, const double *x
     FVH_GUARD_BEGIN
     if (!e || !ms || reps <= 0) return set_error(FVH_ERR_ARG, "bad argument");
     if ((op == 0 || op == 1) && (!x || !y)) return set_error(FVH_ERR_ARG, "null vector");
-    if (op < 0 || op > 5) return set_error(FVH_ERR_UNSUPPORTED, "fvh_time_op: op %d", op);
-    if (op == 5 && (!x || !y)) return set_error(FVH_ERR_ARG, "null vector");
+    if (op < 0 || op > 7) return set_error(FVH_ERR_UNSUPPORTED, "fvh_time_op: op %d", op);
+    if ((op == 5 || op == 6) && (!x || !y)) return set_error(FVH_ERR_ARG, "null vector");
+    if (op >= 6 && !band_spmv(e->m)) return set_error(FVH_ERR_UNSUPPORTED, "fvh_time_op: the mesh is not a banded grid");
     float *x32 = nullptr, *y32 = nullptr;
-    if (op >= 3) {  // FP32 operators on the inner-solve copies, vectors from x rounded
+    if (op == 6) FVH_TRY(band_pack(e));
+    if (op >= 3 && op != 5 && op != 6) {  // FP32 operators on the inner-solve copies, vectors from x rounded
         FVH_TRY(ensure_work32(e, 0));
         FVH_TRY(prepare32(e));
         x32 = e->kw.f[8].p;
@@ -1084,6 +1086,14 @@ extern "C" int fvh_time_op(fvh_eqn *e, int32_t op, int32_t reps, const double *x
             case 2: return precond_update(e);
             case 3: return launch_spmv32(e, x32, y32, 1.0f, nullptr, nullptr, nullptr);
             case 4: return precond_apply32(e, y32, x32, nullptr);
+            case 6:
+            case 7: {  // the banded SpMV of the solves (FP64 / FP32)
+                e->kw.band_live = true;
+                const int rc = op == 6 ? launch_spmv(e, x, y, nullptr)
+                                       : launch_spmv32(e, x32, y32, 1.0f, nullptr, nullptr, nullptr);
+                e->kw.band_live = false;
+                return rc;
+            }
             default: {  // plain copy of ncells doubles: the reference point of traffic counters
                 const double *xs = x;
                 double *ys = y;

@@ -14,6 +14,12 @@ def spmv(n, nnz, s_a=8, s_x=8, s_y=8):
     return 4 * (n + 1) + nnz * (4 + s_a) + n * s_x + n * s_y
 
 
+def spmv_band(n, s_a=8, s_x=8, s_y=8):
+    """y = A x on a structured grid, A by band position: five values and a
+    one-byte entry mask per row, x and y once."""
+    return n * (5 * s_a + 1) + n * s_x + n * s_y
+
+
 def dilu_apply(n, nnz, s_a=8, s_d=8, s_b=8, s_y=8, s_x=8):
     """ldiv! with DILU, forward then backward sweep: row pointers (both
     sweeps), the strictly lower / upper entries with their columns, Dinv in
