@@ -250,7 +250,8 @@ def run_cpu(args):
 def profile_ops(args):
     """One launch of each operator on a C2-pattern matrix (diagonally dominant
     synthetic values): the plain copy (reference for traffic counters), SpMV
-    and the DILU apply in FP64 and FP32.  Few kernels, for counter passes."""
+    (CSR, and banded on a structured grid) and the DILU apply in FP64 and
+    FP32.  Few kernels, for counter passes."""
     import fvmhip as F
     from fvmhip import _lib as L
 
@@ -270,7 +271,8 @@ def profile_ops(args):
     x = L.DeviceArray.from_host(np.random.default_rng(0).standard_normal(n))
     y = L.DeviceArray(n)
     ms = C.c_double()
-    for op in (5, 0, 1, 3, 4):
+    ops = (5, 0, 1, 3, 4) + ((6, 7) if dm.sweep_info()[1] > 0 else ())
+    for op in ops:
         L.check(L.lib().fvh_time_op(h, op, 1, x.vp, y.vp, C.byref(ms)))
         log(f"[profile-ops] op {op}: {ms.value * 1e3:.1f} us")
     L.check(L.lib().fvh_synchronize())
