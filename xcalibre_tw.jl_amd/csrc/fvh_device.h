@@ -153,13 +153,18 @@ struct CaseLayout {
 // above ny), and the slots of rows j >= ny, of cells with i < 0 or i >= nx
 // and of the kSkewD0 diagonals below T = 0 and the prefetch margin above the
 // last diagonal are padding.  One compute wave owns 64 rows; a block runs W
-// consecutive waves (W is a launch parameter, tuned per mesh).
+// consecutive waves (W is a launch parameter, tuned per mesh).  The FP32
+// sweeps keep their results in quads of four consecutive diagonals per row
+// (one 16-byte store per four steps): the value of (T, j) at base +
+// ((T + kSkewQ0) & ~3) * S + 4 j + ((T + kSkewQ0) & 3), inside the grid's
+// slots for every T the sweeps touch.
 constexpr int kLevRing = 16384;    // results the level walk keeps in LDS (by schedule position)
 constexpr int kLevLate = 1 << 30;  // flag of a neighbour read after the level barrier
 constexpr int kSkewL = 8;                    // steps per hand-over batch
 constexpr int kSkewPD = 32;                  // deepest record prefetch (steps)
 constexpr int kSkewD0 = kSkewPD + 2;         // padding diagonals below T = 0
-constexpr int kSkewTop = kSkewPD;            // padding diagonals above the last step
+constexpr int kSkewTop = kSkewPD + 4;        // padding diagonals above the last step (+ the quads' margin)
+constexpr int kSkewQ0 = kSkewD0 + 2;         // quad origin: T + kSkewQ0 is a multiple of 4 when T is
 constexpr int kSkewMaxW = 3;                 // compute waves per block (plus one courier wave)
 struct SkewGrid {
     int nx, ny;     // cell (i, j) is natural row o + i + j*nx

@@ -95,6 +95,7 @@ struct SkewStore {
     DBuf<double> zp, hand, uhand;
     DBuf<Rec4f> F32, Bk32;
     DBuf<float> zp32, hand32;
+    DBuf<float> yq32;  // the FP32 forward sweep's results in quads (zp32 holds the backward's in quads)
     int *h_fault = nullptr;
     SkewStore() = default;
     SkewStore(const SkewStore &) = delete;

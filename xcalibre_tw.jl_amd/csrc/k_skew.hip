@@ -170,6 +170,27 @@ __device__ inline void bstore(Rsrc r, float v, unsigned vo, unsigned so) {
     __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, vo, so, 0);
 }
 
+// Quads of FP32 results (four consecutive diagonals of one row, 16 bytes):
+// the buffer from the quad of diagonal t on (t + kSkewQ0 a multiple of 4;
+// `step` bytes per four diagonals, which is also the FP32 records' stride).
+typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
+__device__ inline Rsrc rsrc_q(const void *p, unsigned bytes, int t, unsigned step) {
+    const unsigned off = (unsigned)((t + kSkewQ0) / 4) * step;
+    return rsrc(static_cast<const char *>(p) + off, bytes - off);
+}
+__device__ inline void bstore4(Rsrc r, const float (&q)[4], unsigned vo, unsigned so) {
+    const u32x4 v = {__float_as_uint(q[0]), __float_as_uint(q[1]), __float_as_uint(q[2]), __float_as_uint(q[3])};
+    __builtin_amdgcn_raw_buffer_store_b128(v, r, vo, so, 0);
+}
+__device__ inline float4 bload4(Rsrc r, unsigned vo, unsigned so) {
+    return __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(r, vo, so, 0));
+}
+// (Dinv, A_re Dinv, A_rn Dinv) of an FP32 backward record: its last 12 bytes
+__device__ inline Rec4f bload_bk3(Rsrc r, unsigned vo, unsigned so) {
+    const auto v = __builtin_amdgcn_raw_buffer_load_b96(r, vo + 4, so, 0);
+    return Rec4f{0.f, __uint_as_float(v[0]), __uint_as_float(v[1]), __uint_as_float(v[2])};
+}
+
 // Whole-wave lane shifts for the hand-over batches: the next batch position
 // moves to the edge lane (the vacated lane reads 0, never used).
 __device__ inline double shl1z(double v) {
@@ -419,8 +440,8 @@ __global__ void __launch_bounds__(64 * (W + 2))
 // ---------------------------------------------------------------------------
 template <typename T, int W>
 __global__ void __launch_bounds__(64 * (W + 2))
-    k_skew_fwd(SkewArgs a, const Rec4T<T> *__restrict__ F, Rec4T<T> *__restrict__ Bk, T *__restrict__ hand,
-               const KScal *ks) {
+    k_skew_fwd(SkewArgs a, const Rec4T<T> *__restrict__ F, Rec4T<T> *__restrict__ Bk, T *__restrict__ yq,
+               T *__restrict__ hand, const KScal *ks) {
     using R = Rec4T<T>;
     extern __shared__ unsigned char skew_lds[];
     T *lds = reinterpret_cast<T *>(skew_lds);
@@ -453,8 +474,11 @@ __global__ void __launch_bounds__(64 * (W + 2))
     constexpr int M = ring_batches<T>(), PD = M * kL;
     static_assert(kLook % M == 0, "whole rings per unrolled block");
     // FP32 (the inner solves of mixed precision) works on premultiplied
-    // coefficients with fused multiply-adds: no pivots travel with y
+    // coefficients with fused multiply-adds: no pivots travel with y; its
+    // results go out in quads (one store per four steps)
     constexpr bool kFused = sizeof(T) == 4;
+    const unsigned vq = (g.base + 4 * (j0 + lane)) * 4u, qbytes = a.rbytes / 4;
+    float q[4] = {0.f, 0.f, 0.f, 0.f};
     Hand<T> hd;
     hd.start(lds + w * HS, lds + (w + 1) * HS, HS, ql, lane >= 64 - kL, 1 + lane - (64 - kL));
     T db[kLook];  // lane-0 pivots of the next kLook groups (FP64)
@@ -467,9 +491,14 @@ __global__ void __launch_bounds__(64 * (W + 2))
     R ring[PD];
     {
         const Rsrc rf = rsrc_at(F, a.rbytes, 0, step), rbk = rsrc_at(Bk, a.rbytes, -PD, step);
+        const Rsrc rq = rsrc_q(yq, qbytes, -PD, step);
 #pragma unroll
         for (int k = 0; k < PD; ++k) {
-            bstore(rbk, T(0), vo, k * step);
+            if constexpr (kFused) {
+                if (k % 4 == 3) bstore4(rq, q, vq, (unsigned)(k / 4) * step);
+            } else {
+                bstore(rbk, T(0), vo, k * step);
+            }
             ring[k] = bload(rf, vo, k * step, (R *)nullptr);
             __builtin_amdgcn_sched_barrier(0);
         }
@@ -480,6 +509,7 @@ __global__ void __launch_bounds__(64 * (W + 2))
     do {
         // this iteration's steps G0*kL + i, i < kLook*kL: stores at i*step, refills PD steps on
         const Rsrc rf = rsrc_at(F, a.rbytes, G0 * kL + PD, step), rbk = rsrc_at(Bk, a.rbytes, G0 * kL, step);
+        const Rsrc rq = rsrc_q(yq, qbytes, G0 * kL, step);
 #pragma unroll
         for (int h = 0; h < kLook; ++h) {
             const int G = G0 + h;
@@ -504,7 +534,12 @@ __global__ void __launch_bounds__(64 * (W + 2))
                     x = (c.a - c.c * xs * ds) - c.d * xprev * dprev;
                     dprev = keep(c.b);
                 }
-                bstore(rbk, x, vo, si);
+                if constexpr (kFused) {
+                    q[k % 4] = x;
+                    if (k % 4 == 3) bstore4(rq, q, vq, (unsigned)((h * kL + k) / 4) * step);
+                } else {
+                    bstore(rbk, x, vo, si);
+                }
                 if (k > 0) pub = dpp_shl1(pub, xprev);  // the previous step's value (its register is free now)
                 xprev = x;
                 __builtin_amdgcn_sched_barrier(0);
@@ -522,8 +557,8 @@ __global__ void __launch_bounds__(64 * (W + 2))
 
 template <typename T, int W>
 __global__ void __launch_bounds__(64 * (W + 2))
-    k_skew_bwd(SkewArgs a, const Rec4T<T> *__restrict__ Bk, T *__restrict__ zp, T *__restrict__ hand,
-               const KScal *ks) {
+    k_skew_bwd(SkewArgs a, const Rec4T<T> *__restrict__ Bk, const T *__restrict__ yq, T *__restrict__ zp,
+               T *__restrict__ hand, const KScal *ks) {
     using R = Rec4T<T>;
     extern __shared__ unsigned char skew_lds[];
     T *lds = reinterpret_cast<T *>(skew_lds);
@@ -563,14 +598,33 @@ __global__ void __launch_bounds__(64 * (W + 2))
     // forward sweep)
     // (offsets count down from the top of each range: step Ttop - u sits at
     // (NI - 1 - i) strides above the range's lowest step, u = G0*kL + i)
+    // FP32: y comes in quads (a ring of PD/4 quads, one load per four steps),
+    // z goes out in quads, and the records' y field is not read
+    constexpr bool kQuad = sizeof(T) == 4;
+    constexpr int NQ = PD / 4;  // quads per loop iteration (NI == PD for FP32)
+    static_assert(!kQuad || NI == PD, "FP32: one ring of quads per loop iteration");
+    const unsigned vq = (g.base + 4 * (j0 + lane)) * 4u, qbytes = a.rbytes / 4;
+    float4 yr[kQuad ? NQ : 1];
+    float q[4] = {0.f, 0.f, 0.f, 0.f};
+    auto rec = [&](Rsrc r, unsigned so) {
+        if constexpr (kQuad) return bload_bk3(r, vo, so);
+        else return bload(r, vo, so, (R *)nullptr);
+    };
     R ring[PD];
     {
         const Rsrc rbk = rsrc_at(Bk, a.rbytes, Ttop - (NI - 1), step);
         const Rsrc rz = rsrc_at(zp, a.rbytes, Ttop + PD - (NI - 1), step);
+        const Rsrc ry = rsrc_q(yq, qbytes, Ttop - (NI - 1), step), rzq = rsrc_q(zp, a.rbytes, Ttop + 1, step);
 #pragma unroll
         for (int k = 0; k < PD; ++k) {
-            bstore(rz, T(0), voz, (unsigned)(NI - 1 - k) * step);
-            ring[k] = bload(rbk, vo, (unsigned)(NI - 1 - k) * step, (R *)nullptr);
+            if constexpr (kQuad) {
+                if (k % 4 == 3) bstore4(rzq, q, vq, (unsigned)(k / 4) * step);  // +0 above the top
+            } else {
+                bstore(rz, T(0), voz, (unsigned)(NI - 1 - k) * step);
+            }
+            ring[k] = rec(rbk, (unsigned)(NI - 1 - k) * step);
+            if constexpr (kQuad)
+                if (k % 4 == 3) yr[k / 4] = bload4(ry, vq, (unsigned)(NQ - 1 - k / 4) * step);
             __builtin_amdgcn_sched_barrier(0);
         }
     }
@@ -581,6 +635,7 @@ __global__ void __launch_bounds__(64 * (W + 2))
         // this iteration's steps u = G0*kL + i: z of diagonal Ttop - u, refills of Ttop - u - PD
         const int tl = Ttop - G0 * kL - (NI - 1);  // lowest diagonal written this iteration
         const Rsrc rbk = rsrc_at(Bk, a.rbytes, tl - PD, step), rz = rsrc_at(zp, a.rbytes, tl, step);
+        const Rsrc ry = rsrc_q(yq, qbytes, tl - PD, step), rzq = rsrc_q(zp, a.rbytes, tl, step);
 #pragma unroll
         for (int h = 0; h < kLook; ++h) {
             const int G = G0 + h;
@@ -588,20 +643,29 @@ __global__ void __launch_bounds__(64 * (W + 2))
 #pragma unroll
             for (int k = 0; k < kL; ++k) {
                 const int r = (h % M) * kL + k, i = h * kL + k;
+                // FP32: step i is word 3 - i%4 of quad i/4 (four diagonals, descending)
+                const int mq = i / 4, wq = 3 - i % 4;
                 const R c = ring[r];
                 const T zn_next = shr1z(zc);
                 const T zn = dpp_shl1(zprev, zc);
                 zc = zn_next;
                 T z;
-                if constexpr (sizeof(T) == 4)  // (y, Dinv, A_re Dinv, A_rn Dinv), fused
-                    z = __builtin_fmaf(-c.d, zn, __builtin_fmaf(-c.c, zprev, c.a * c.b));
-                else
+                if constexpr (kQuad) {  // (-, Dinv, A_re Dinv, A_rn Dinv), fused; y from its quad
+                    const float4 yv = yr[mq];
+                    const float y = wq == 0 ? yv.x : wq == 1 ? yv.y : wq == 2 ? yv.z : yv.w;
+                    z = __builtin_fmaf(-c.d, zn, __builtin_fmaf(-c.c, zprev, y * c.b));
+                    q[wq] = z;
+                    if (wq == 0) bstore4(rzq, q, vq, (unsigned)(NQ - 1 - mq) * step);
+                } else {
                     z = ((c.a * c.b) - c.c * zprev * c.b) - c.d * zn * c.b;
-                bstore(rz, z, voz, (unsigned)(NI - 1 - i) * step);
+                    bstore(rz, z, voz, (unsigned)(NI - 1 - i) * step);
+                }
                 if (k > 0) pub = dpp_shr1(pub, zprev);  // the previous step's value (its register is free now)
                 zprev = z;
                 __builtin_amdgcn_sched_barrier(0);
-                ring[r] = bload(rbk, vo, (unsigned)(NI - 1 - i) * step, (R *)nullptr);
+                ring[r] = rec(rbk, (unsigned)(NI - 1 - i) * step);
+                if constexpr (kQuad)
+                    if (wq == 0) yr[mq] = bload4(ry, vq, (unsigned)(NQ - 1 - mq) * step);
             }
             pub = dpp_shr1(pub, zprev);
             hd.finish(G, h, pub);
@@ -719,6 +783,37 @@ __global__ void __launch_bounds__(256)
         const int d = wv + 4 * it, j = tl.j0 + lane, il = d - lane;
         const bool in = il >= 0 && il < kTile && tl.i0 + il < g.nx && j < g.ny;
         v[it] = in ? zp[4 * (g.base + (size_t)(tl.i0 + tl.j0 + d + kSkewD0) * g.S) + j] : T(0);
+    }
+#pragma unroll
+    for (int it = 0; it < kTile / 2; ++it) {
+        const int il = wv + 4 * it - lane;
+        if (il >= 0 && il < kTile) t.v[lane][il] = v[it];
+    }
+    __syncthreads();
+    const int i = tl.i0 + lane;
+#pragma unroll
+    for (int it = 0; it < kTile / 4; ++it) {
+        const int r = wv + 4 * it, j = tl.j0 + r;
+        if (i < g.nx && j < g.ny) y[(size_t)g.o + i + (size_t)j * g.nx] = t.v[r][lane];
+    }
+}
+
+// The FP32 backward sweep's quads (zq) -> y (natural).
+__global__ void __launch_bounds__(256)
+    k_skew_scatter_q(const SkewTile *__restrict__ tiles, const SkewGrid *__restrict__ grids,
+                     const float *__restrict__ zq, float *__restrict__ y, const KScal *ks) {
+    if (ks && ks->done) return;
+    __shared__ TileLds<float> t;
+    const SkewTile tl = tiles[blockIdx.x];
+    const SkewGrid g = grids[tl.grid];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    float v[kTile / 2];  // all loads in flight before the LDS stores
+#pragma unroll
+    for (int it = 0; it < kTile / 2; ++it) {
+        const int d = wv + 4 * it, j = tl.j0 + lane, il = d - lane;
+        const bool in = il >= 0 && il < kTile && tl.i0 + il < g.nx && j < g.ny;
+        const int Tq = tl.i0 + tl.j0 + d + kSkewQ0;
+        v[it] = in ? zq[g.base + (size_t)(Tq & ~3) * g.S + 4 * (size_t)j + (Tq & 3)] : 0.f;
     }
 #pragma unroll
     for (int it = 0; it < kTile / 2; ++it) {
@@ -952,21 +1047,23 @@ int skew_update(fvh_eqn *e, const double *val) {
 }
 
 template <typename T, int W>
-static int launch_sweeps_w(fvh_eqn *e, Rec4T<T> *F, Rec4T<T> *Bk, T *zp, T *hand, const KScal *ks) {
+static int launch_sweeps_w(fvh_eqn *e, Rec4T<T> *F, Rec4T<T> *Bk, T *yq, T *zp, T *hand, const KScal *ks) {
     const SkewLayout &L = *e->m->skew;
     const SkewArgs a = skew_args<Rec4T<T>>(e, W);
     const size_t lds = (size_t)(W + 1) * L.HS * sizeof(T);
     FVH_TRY(allow_lds(k_skew_fwd<T, W>, lds));
     FVH_TRY(allow_lds(k_skew_bwd<T, W>, lds));
-    hipLaunchKernelGGL((k_skew_fwd<T, W>), dim3(a.nblk), dim3(64 * (W + 2)), lds, stream(), a, F, Bk, hand, ks);
+    hipLaunchKernelGGL((k_skew_fwd<T, W>), dim3(a.nblk), dim3(64 * (W + 2)), lds, stream(), a, F, Bk, yq, hand, ks);
     HIP_TRY(hipGetLastError());
-    hipLaunchKernelGGL((k_skew_bwd<T, W>), dim3(a.nblk), dim3(64 * (W + 2)), lds, stream(), a, Bk, zp, hand, ks);
+    hipLaunchKernelGGL((k_skew_bwd<T, W>), dim3(a.nblk), dim3(64 * (W + 2)), lds, stream(), a, Bk, (const T *)yq, zp,
+                       hand, ks);
     HIP_TRY(hipGetLastError());
     return FVH_OK;
 }
 
 template <typename T>
-static int skew_apply_t(fvh_eqn *e, const T *x, T *y, const KScal *ks, Rec4T<T> *F, Rec4T<T> *Bk, T *zp, T *hand) {
+static int skew_apply_t(fvh_eqn *e, const T *x, T *y, const KScal *ks, Rec4T<T> *F, Rec4T<T> *Bk, T *yq, T *zp,
+                        T *hand) {
     fvh_mesh *m = e->m;
     const SkewLayout &L = *m->skew;
     const int nhand = 2 * L.nblk[0] * L.HS;
@@ -978,11 +1075,15 @@ static int skew_apply_t(fvh_eqn *e, const T *x, T *y, const KScal *ks, Rec4T<T> 
                        L.tiles.p, L.grids.p, x, F, hand, nhand, L.HS, ks_all);
     HIP_TRY(hipGetLastError());
     switch (skew_waves(L, sizeof(T) == 4)) {
-        case 1: FVH_TRY((launch_sweeps_w<T, 1>(e, F, Bk, zp, hand, ks))); break;
-        case 2: FVH_TRY((launch_sweeps_w<T, 2>(e, F, Bk, zp, hand, ks))); break;
-        default: FVH_TRY((launch_sweeps_w<T, 3>(e, F, Bk, zp, hand, ks))); break;
+        case 1: FVH_TRY((launch_sweeps_w<T, 1>(e, F, Bk, yq, zp, hand, ks))); break;
+        case 2: FVH_TRY((launch_sweeps_w<T, 2>(e, F, Bk, yq, zp, hand, ks))); break;
+        default: FVH_TRY((launch_sweeps_w<T, 3>(e, F, Bk, yq, zp, hand, ks))); break;
     }
-    hipLaunchKernelGGL((k_skew_scatter<T>), dim3(ntiles), dim3(256), 0, stream(), L.tiles.p, L.grids.p, zp, y, ks_all);
+    if constexpr (sizeof(T) == 4)
+        hipLaunchKernelGGL(k_skew_scatter_q, dim3(ntiles), dim3(256), 0, stream(), L.tiles.p, L.grids.p, zp, y, ks_all);
+    else
+        hipLaunchKernelGGL((k_skew_scatter<T>), dim3(ntiles), dim3(256), 0, stream(), L.tiles.p, L.grids.p, zp, y,
+                           ks_all);
     HIP_TRY(hipGetLastError());
     return FVH_OK;
 }
@@ -991,7 +1092,7 @@ static int skew_apply_t(fvh_eqn *e, const T *x, T *y, const KScal *ks, Rec4T<T> 
 // width on the freshly packed records (the sweeps only rewrite their own
 // scratch fields); the width never changes a result.
 template <typename T>
-static int skew_tune_t(fvh_eqn *e, Rec4T<T> *F, Rec4T<T> *Bk, T *zp, T *hand, int &W_out) {
+static int skew_tune_t(fvh_eqn *e, Rec4T<T> *F, Rec4T<T> *Bk, T *yq, T *zp, T *hand, int &W_out) {
     SkewLayout &L = *e->m->skew;
     const int n = (int)e->m->n;
     DBuf<T> x, y;
@@ -1005,9 +1106,9 @@ static int skew_tune_t(fvh_eqn *e, Rec4T<T> *F, Rec4T<T> *Bk, T *zp, T *hand, in
     int bestW = 1;
     for (int W = 1; W <= lds_waves_max(L); ++W) {
         W_out = W;
-        FVH_TRY(skew_apply_t<T>(e, x.p, y.p, nullptr, F, Bk, zp, hand));  // warm
+        FVH_TRY(skew_apply_t<T>(e, x.p, y.p, nullptr, F, Bk, yq, zp, hand));  // warm
         HIP_TRY(hipEventRecord(a, stream()));
-        for (int rep = 0; rep < 3; ++rep) FVH_TRY(skew_apply_t<T>(e, x.p, y.p, nullptr, F, Bk, zp, hand));
+        for (int rep = 0; rep < 3; ++rep) FVH_TRY(skew_apply_t<T>(e, x.p, y.p, nullptr, F, Bk, yq, zp, hand));
         HIP_TRY(hipEventRecord(b, stream()));
         HIP_TRY(hipEventSynchronize(b));
         float t = 0.f;
@@ -1026,13 +1127,13 @@ static int skew_tune_t(fvh_eqn *e, Rec4T<T> *F, Rec4T<T> *Bk, T *zp, T *hand, in
 
 static int skew_tune(fvh_eqn *e) {
     SkewStore &s = e->sk;
-    return skew_tune_t<double>(e, s.F.p, s.Bk.p, s.zp.p, s.hand.p, e->m->skew->W);
+    return skew_tune_t<double>(e, s.F.p, s.Bk.p, (double *)nullptr, s.zp.p, s.hand.p, e->m->skew->W);
 }
 
 int skew_apply(fvh_eqn *e, const double *x, double *y, const KScal *ks) {
     SkewStore &s = e->sk;
     if (!s.F.p) return set_error(FVH_ERR_ARG, "DILU records not prepared");
-    return skew_apply_t<double>(e, x, y, ks, s.F.p, s.Bk.p, s.zp.p, s.hand.p);
+    return skew_apply_t<double>(e, x, y, ks, s.F.p, s.Bk.p, (double *)nullptr, s.zp.p, s.hand.p);
 }
 
 int skew_to32(fvh_eqn *e) {
@@ -1042,6 +1143,8 @@ int skew_to32(fvh_eqn *e) {
         FVH_TRY(s.F32.alloc(L.NS));
         FVH_TRY(s.Bk32.alloc(L.NS));
         FVH_TRY(s.zp32.alloc(4 * L.NS));
+        FVH_TRY(s.yq32.alloc(L.NS));
+        hipLaunchKernelGGL(k_skew_fill<float>, grid_for(L.NS), dim3(256), 0, stream(), L.NS, s.yq32.p, 0.f);
         FVH_TRY(s.hand32.alloc((size_t)L.nblk[0] * 2 * L.HS));
         hipLaunchKernelGGL(k_skew_fill<float>, grid_for(4 * L.NS), dim3(256), 0, stream(), 4 * L.NS, s.zp32.p, 0.f);
         HIP_TRY(hipGetLastError());
@@ -1052,14 +1155,15 @@ int skew_to32(fvh_eqn *e) {
     hipLaunchKernelGGL(k_skew_to32, dim3((unsigned)((most + 255) / 256), (unsigned)L.h_grids.size()), dim3(256), 0,
                        stream(), L.grids.p, s.F.p, s.Bk.p, s.F32.p, s.Bk32.p);
     HIP_TRY(hipGetLastError());
-    if (!L.W32 && !forced_waves()) FVH_TRY(skew_tune_t<float>(e, s.F32.p, s.Bk32.p, s.zp32.p, s.hand32.p, e->m->skew->W32));
+    if (!L.W32 && !forced_waves())
+        FVH_TRY(skew_tune_t<float>(e, s.F32.p, s.Bk32.p, s.yq32.p, s.zp32.p, s.hand32.p, e->m->skew->W32));
     return FVH_OK;
 }
 
 int skew_apply32(fvh_eqn *e, const float *x, float *y, const KScal *ks) {
     SkewStore &s = e->sk;
     if (!s.F32.p) return set_error(FVH_ERR_ARG, "FP32 sweep records not prepared");
-    return skew_apply_t<float>(e, x, y, ks, s.F32.p, s.Bk32.p, s.zp32.p, s.hand32.p);
+    return skew_apply_t<float>(e, x, y, ks, s.F32.p, s.Bk32.p, s.yq32.p, s.zp32.p, s.hand32.p);
 }
 
 int skew_dinv_natural(fvh_eqn *e, double *out) {
