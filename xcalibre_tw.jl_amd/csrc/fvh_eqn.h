@@ -84,9 +84,8 @@ struct BandStore {
 };
 // Records of the pipelined band sweeps, in the mesh's skewed slot order:
 // F (b_r, Dinv_r, A_rs, A_rw), Bk (y_r, Dinv_r, A_re, A_rn), U (A_rr,
-// A_rs*A_sr, A_rw*A_wr, 0); zp the backward sweep's output, one row of S
-// words per anti-diagonal at the records' stride (4S words: slot base + q of
-// a grid at zp[4 base + q' ] with q' = 4 (T + kSkewD0) S + j); hand the
+// A_rs*A_sr, A_rw*A_wr, 0); yq / zp the forward / backward sweep's results
+// in quads (four diagonals of a row, see kSkewQ0); hand the
 // global hand-over arrays between blocks ([block][forward | backward][HS]),
 // uhand those of the diagonal recurrence ([block][HS]); *h_fault (pinned
 // host memory) is set by a sweep whose hand-over wait ran out.
@@ -95,7 +94,11 @@ struct SkewStore {
     DBuf<double> zp, hand, uhand;
     DBuf<Rec4f> F32, Bk32;
     DBuf<float> zp32, hand32;
-    DBuf<float> yq32;  // the FP32 forward sweep's results in quads (zp32 holds the backward's in quads)
+    DBuf<float> yq32;
+    DBuf<double> yq;
+    // the FP64 backward sweep's coefficients: (Dinv, A_re, A_rn) of the four
+    // diagonals of a quad in one 96-byte record per row
+    DBuf<double> bq;
     int *h_fault = nullptr;
     SkewStore() = default;
     SkewStore(const SkewStore &) = delete;

@@ -170,10 +170,14 @@ __device__ inline void bstore(Rsrc r, float v, unsigned vo, unsigned so) {
     __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, vo, so, 0);
 }
 
-// Quads of FP32 results (four consecutive diagonals of one row, 16 bytes):
+// Quads of results (four consecutive diagonals of one row, 16 or 32 bytes):
 // the buffer from the quad of diagonal t on (t + kSkewQ0 a multiple of 4;
-// `step` bytes per four diagonals, which is also the FP32 records' stride).
+// `step` bytes per four diagonals, which is also the records' stride).
 typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
+template <typename T>
+struct QuadT {
+    T v[4];
+};
 __device__ inline Rsrc rsrc_q(const void *p, unsigned bytes, int t, unsigned step) {
     const unsigned off = (unsigned)((t + kSkewQ0) / 4) * step;
     return rsrc(static_cast<const char *>(p) + off, bytes - off);
@@ -182,8 +186,38 @@ __device__ inline void bstore4(Rsrc r, const float (&q)[4], unsigned vo, unsigne
     const u32x4 v = {__float_as_uint(q[0]), __float_as_uint(q[1]), __float_as_uint(q[2]), __float_as_uint(q[3])};
     __builtin_amdgcn_raw_buffer_store_b128(v, r, vo, so, 0);
 }
-__device__ inline float4 bload4(Rsrc r, unsigned vo, unsigned so) {
-    return __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(r, vo, so, 0));
+__device__ inline void bstore4(Rsrc r, const double (&q)[4], unsigned vo, unsigned so) {
+    __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u32x4, double2{q[0], q[1]}), r, vo, so, 0);
+    __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u32x4, double2{q[2], q[3]}), r, vo + 16, so, 0);
+}
+__device__ inline QuadT<float> bload_q(Rsrc r, unsigned vo, unsigned so, float *) {
+    const auto u = __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(r, vo, so, 0));
+    return QuadT<float>{{u.x, u.y, u.z, u.w}};
+}
+__device__ inline QuadT<double> bload_q(Rsrc r, unsigned vo, unsigned so, double *) {
+    const auto u = __builtin_bit_cast(double2, __builtin_amdgcn_raw_buffer_load_b128(r, vo, so, 0));
+    const auto v = __builtin_bit_cast(double2, __builtin_amdgcn_raw_buffer_load_b128(r, vo + 16, so, 0));
+    return QuadT<double>{{u.x, u.y, v.x, v.y}};
+}
+// FP64 backward quad records: (Dinv, A_re, A_rn) of the four diagonals of a
+// quad, 96 bytes per row and quad (`3 step` bytes per quad); the buffer from
+// the quad of diagonal t on.
+struct RecQ {
+    double v[12];
+};
+__device__ inline Rsrc rsrc_bq(const void *p, unsigned bytes, int t, unsigned step) {
+    const unsigned off = (unsigned)((t + kSkewQ0) / 4) * 3 * step;
+    return rsrc(static_cast<const char *>(p) + off, bytes - off);
+}
+__device__ inline RecQ bload_rq(Rsrc r, unsigned vo, unsigned so) {
+    RecQ c;
+#pragma unroll
+    for (int t = 0; t < 6; ++t) {
+        const auto u = __builtin_bit_cast(double2, __builtin_amdgcn_raw_buffer_load_b128(r, vo + 16 * t, so, 0));
+        c.v[2 * t] = u.x;
+        c.v[2 * t + 1] = u.y;
+    }
+    return c;
 }
 // (Dinv, A_re Dinv, A_rn Dinv) of an FP32 backward record: its last 12 bytes
 __device__ inline Rec4f bload_bk3(Rsrc r, unsigned vo, unsigned so) {
@@ -435,13 +469,14 @@ __global__ void __launch_bounds__(64 * (W + 2))
 }
 
 // ---------------------------------------------------------------------------
-// ldiv!: forward sweep into Bk.a, backward sweep into zp (FP64: the
-// reference's arithmetic; FP32: the inner solves of mixed precision)
+// ldiv!: forward sweep into quads of y (yq), backward sweep into quads of z
+// (zp) (FP64: the reference's arithmetic; FP32: the inner solves of mixed
+// precision)
 // ---------------------------------------------------------------------------
 template <typename T, int W>
 __global__ void __launch_bounds__(64 * (W + 2))
-    k_skew_fwd(SkewArgs a, const Rec4T<T> *__restrict__ F, Rec4T<T> *__restrict__ Bk, T *__restrict__ yq,
-               T *__restrict__ hand, const KScal *ks) {
+    k_skew_fwd(SkewArgs a, const Rec4T<T> *__restrict__ F, T *__restrict__ yq, T *__restrict__ hand,
+               const KScal *ks) {
     using R = Rec4T<T>;
     extern __shared__ unsigned char skew_lds[];
     T *lds = reinterpret_cast<T *>(skew_lds);
@@ -474,11 +509,11 @@ __global__ void __launch_bounds__(64 * (W + 2))
     constexpr int M = ring_batches<T>(), PD = M * kL;
     static_assert(kLook % M == 0, "whole rings per unrolled block");
     // FP32 (the inner solves of mixed precision) works on premultiplied
-    // coefficients with fused multiply-adds: no pivots travel with y; its
-    // results go out in quads (one store per four steps)
+    // coefficients with fused multiply-adds: no pivots travel with y.  The
+    // results go out in quads (16 or 32 bytes per four steps).
     constexpr bool kFused = sizeof(T) == 4;
-    const unsigned vq = (g.base + 4 * (j0 + lane)) * 4u, qbytes = a.rbytes / 4;
-    float q[4] = {0.f, 0.f, 0.f, 0.f};
+    const unsigned vq = (g.base + 4 * (j0 + lane)) * (unsigned)sizeof(T), qbytes = a.rbytes / 4;
+    T q[4] = {T(0), T(0), T(0), T(0)};
     Hand<T> hd;
     hd.start(lds + w * HS, lds + (w + 1) * HS, HS, ql, lane >= 64 - kL, 1 + lane - (64 - kL));
     T db[kLook];  // lane-0 pivots of the next kLook groups (FP64)
@@ -490,15 +525,11 @@ __global__ void __launch_bounds__(64 * (W + 2))
     // waits count the same outstanding operations on every iteration.
     R ring[PD];
     {
-        const Rsrc rf = rsrc_at(F, a.rbytes, 0, step), rbk = rsrc_at(Bk, a.rbytes, -PD, step);
+        const Rsrc rf = rsrc_at(F, a.rbytes, 0, step);
         const Rsrc rq = rsrc_q(yq, qbytes, -PD, step);
 #pragma unroll
         for (int k = 0; k < PD; ++k) {
-            if constexpr (kFused) {
-                if (k % 4 == 3) bstore4(rq, q, vq, (unsigned)(k / 4) * step);
-            } else {
-                bstore(rbk, T(0), vo, k * step);
-            }
+            if (k % 4 == 3) bstore4(rq, q, vq, (unsigned)(k / 4) * step);
             ring[k] = bload(rf, vo, k * step, (R *)nullptr);
             __builtin_amdgcn_sched_barrier(0);
         }
@@ -508,7 +539,7 @@ __global__ void __launch_bounds__(64 * (W + 2))
     int G0 = 0;
     do {
         // this iteration's steps G0*kL + i, i < kLook*kL: stores at i*step, refills PD steps on
-        const Rsrc rf = rsrc_at(F, a.rbytes, G0 * kL + PD, step), rbk = rsrc_at(Bk, a.rbytes, G0 * kL, step);
+        const Rsrc rf = rsrc_at(F, a.rbytes, G0 * kL + PD, step);
         const Rsrc rq = rsrc_q(yq, qbytes, G0 * kL, step);
 #pragma unroll
         for (int h = 0; h < kLook; ++h) {
@@ -534,12 +565,8 @@ __global__ void __launch_bounds__(64 * (W + 2))
                     x = (c.a - c.c * xs * ds) - c.d * xprev * dprev;
                     dprev = keep(c.b);
                 }
-                if constexpr (kFused) {
-                    q[k % 4] = x;
-                    if (k % 4 == 3) bstore4(rq, q, vq, (unsigned)((h * kL + k) / 4) * step);
-                } else {
-                    bstore(rbk, x, vo, si);
-                }
+                q[k % 4] = x;
+                if (k % 4 == 3) bstore4(rq, q, vq, (unsigned)((h * kL + k) / 4) * step);
                 if (k > 0) pub = dpp_shl1(pub, xprev);  // the previous step's value (its register is free now)
                 xprev = x;
                 __builtin_amdgcn_sched_barrier(0);
@@ -557,7 +584,7 @@ __global__ void __launch_bounds__(64 * (W + 2))
 
 template <typename T, int W>
 __global__ void __launch_bounds__(64 * (W + 2))
-    k_skew_bwd(SkewArgs a, const Rec4T<T> *__restrict__ Bk, const T *__restrict__ yq, T *__restrict__ zp,
+    k_skew_bwd(SkewArgs a, const void *__restrict__ Brec, const T *__restrict__ yq, T *__restrict__ zp,
                T *__restrict__ hand, const KScal *ks) {
     using R = Rec4T<T>;
     extern __shared__ unsigned char skew_lds[];
@@ -584,8 +611,6 @@ __global__ void __launch_bounds__(64 * (W + 2))
     const unsigned S = g.S, j0 = wg * 64;
     const int Ttop = g.NG * kL - 1;  // step u handles anti-diagonal Ttop - u
     const unsigned vo = (g.base + j0 + lane) * (unsigned)sizeof(R), step = S * (unsigned)sizeof(R);
-    // z rows lie at the records' stride: the same step offsets as the records
-    const unsigned voz = (4 * g.base + j0 + lane) * (unsigned)sizeof(T);
     constexpr int NI = kLook * kL;  // steps per loop iteration
     // lane 63-q of the batch of group G holds step u = G*kL + q (index u)
     const int qh = min(63 - lane, kL - 1);
@@ -594,38 +619,40 @@ __global__ void __launch_bounds__(64 * (W + 2))
     static_assert(kLook % M == 0, "whole rings per unrolled block");
     Hand<T> hd;
     hd.start(lds + (w + 1) * HS, lds + w * HS, HS, qh, lane < kL, kL - lane);
-    // prologue stores: +0 into the padding diagonals above the top (see the
-    // forward sweep)
-    // (offsets count down from the top of each range: step Ttop - u sits at
-    // (NI - 1 - i) strides above the range's lowest step, u = G0*kL + i)
-    // FP32: y comes in quads (a ring of PD/4 quads, one load per four steps),
-    // z goes out in quads, and the records' y field is not read
-    constexpr bool kQuad = sizeof(T) == 4;
-    constexpr int NQ = PD / 4;  // quads per loop iteration (NI == PD for FP32)
-    static_assert(!kQuad || NI == PD, "FP32: one ring of quads per loop iteration");
-    const unsigned vq = (g.base + 4 * (j0 + lane)) * 4u, qbytes = a.rbytes / 4;
-    float4 yr[kQuad ? NQ : 1];
-    float q[4] = {0.f, 0.f, 0.f, 0.f};
-    auto rec = [&](Rsrc r, unsigned so) {
-        if constexpr (kQuad) return bload_bk3(r, vo, so);
-        else return bload(r, vo, so, (R *)nullptr);
-    };
-    R ring[PD];
+    // y comes in quads (a ring of PD/4 quads, one refill per four steps) and
+    // z goes out in quads; Ttop + 1 + kSkewQ0 is a multiple of 4, so step i
+    // of an iteration is word 3 - i%4 of the iteration's quad i/4 (four
+    // diagonals, descending).  The coefficients: FP32 reads the last 12 bytes
+    // of each step's record (Dinv, A_re Dinv, A_rn Dinv), premultiplied for
+    // fused multiply-adds; FP64 reads quad records (Dinv, A_re, A_rn for four
+    // diagonals, 96 bytes) with the quads of y.
+    constexpr bool kFused = sizeof(T) == 4;
+    constexpr int NQ = NI / 4, NR = PD / 4;  // quads per loop iteration, quads in the rings
+    static_assert(NQ % NR == 0, "whole rings per loop iteration");
+    const unsigned vq = (g.base + 4 * (j0 + lane)) * (unsigned)sizeof(T), qbytes = a.rbytes / 4;
+    const unsigned vb = (3 * g.base + 12 * (j0 + lane)) * 8u;
+    const R *B32 = static_cast<const R *>(Brec);
+    QuadT<T> yr[NR];
+    RecQ rr[kFused ? 1 : NR];
+    R ring[kFused ? PD : 1];
+    T q[4] = {T(0), T(0), T(0), T(0)};
+    // prologue: the first PD steps' coefficients and y quads, and quads of
+    // +0 into the padding diagonals above the top (see the forward sweep),
+    // in the loop's order; offsets count down from the top of each range
     {
-        const Rsrc rbk = rsrc_at(Bk, a.rbytes, Ttop - (NI - 1), step);
-        const Rsrc rz = rsrc_at(zp, a.rbytes, Ttop + PD - (NI - 1), step);
-        const Rsrc ry = rsrc_q(yq, qbytes, Ttop - (NI - 1), step), rzq = rsrc_q(zp, a.rbytes, Ttop + 1, step);
+        const int tl = Ttop - (NI - 1);
+        const Rsrc ry = rsrc_q(yq, qbytes, tl, step), rzq = rsrc_q(zp, qbytes, Ttop + 1, step);
 #pragma unroll
         for (int k = 0; k < PD; ++k) {
-            if constexpr (kQuad) {
-                if (k % 4 == 3) bstore4(rzq, q, vq, (unsigned)(k / 4) * step);  // +0 above the top
-            } else {
-                bstore(rz, T(0), voz, (unsigned)(NI - 1 - k) * step);
-            }
-            ring[k] = rec(rbk, (unsigned)(NI - 1 - k) * step);
-            if constexpr (kQuad)
-                if (k % 4 == 3) yr[k / 4] = bload4(ry, vq, (unsigned)(NQ - 1 - k / 4) * step);
+            const int mq = k / 4;
+            if (k % 4 == 3) bstore4(rzq, q, vq, (unsigned)mq * step);
             __builtin_amdgcn_sched_barrier(0);
+            if constexpr (kFused) {
+                ring[k] = bload_bk3(rsrc_at(B32, a.rbytes, tl, step), vo, (unsigned)(NI - 1 - k) * step);
+            } else if (k % 4 == 3) {
+                rr[mq] = bload_rq(rsrc_bq(Brec, 3 * qbytes, tl, step), vb, (unsigned)(NQ - 1 - mq) * 3 * step);
+            }
+            if (k % 4 == 3) yr[mq] = bload_q(ry, vq, (unsigned)(NQ - 1 - mq) * step, (T *)nullptr);
         }
     }
     T zprev = 0, pub = 0;
@@ -634,38 +661,37 @@ __global__ void __launch_bounds__(64 * (W + 2))
     do {
         // this iteration's steps u = G0*kL + i: z of diagonal Ttop - u, refills of Ttop - u - PD
         const int tl = Ttop - G0 * kL - (NI - 1);  // lowest diagonal written this iteration
-        const Rsrc rbk = rsrc_at(Bk, a.rbytes, tl - PD, step), rz = rsrc_at(zp, a.rbytes, tl, step);
-        const Rsrc ry = rsrc_q(yq, qbytes, tl - PD, step), rzq = rsrc_q(zp, a.rbytes, tl, step);
+        const Rsrc ry = rsrc_q(yq, qbytes, tl - PD, step), rzq = rsrc_q(zp, qbytes, tl, step);
 #pragma unroll
         for (int h = 0; h < kLook; ++h) {
             const int G = G0 + h;
             T zc = hd.take(G, h, broken);
 #pragma unroll
             for (int k = 0; k < kL; ++k) {
-                const int r = (h % M) * kL + k, i = h * kL + k;
-                // FP32: step i is word 3 - i%4 of quad i/4 (four diagonals, descending)
-                const int mq = i / 4, wq = 3 - i % 4;
-                const R c = ring[r];
+                const int i = h * kL + k, mq = i / 4, wq = 3 - i % 4, rq = mq % NR;
                 const T zn_next = shr1z(zc);
                 const T zn = dpp_shl1(zprev, zc);
                 zc = zn_next;
+                const T y = yr[rq].v[wq];
                 T z;
-                if constexpr (kQuad) {  // (-, Dinv, A_re Dinv, A_rn Dinv), fused; y from its quad
-                    const float4 yv = yr[mq];
-                    const float y = wq == 0 ? yv.x : wq == 1 ? yv.y : wq == 2 ? yv.z : yv.w;
+                if constexpr (kFused) {  // (-, Dinv, A_re Dinv, A_rn Dinv)
+                    const R c = ring[i % PD];
                     z = __builtin_fmaf(-c.d, zn, __builtin_fmaf(-c.c, zprev, y * c.b));
-                    q[wq] = z;
-                    if (wq == 0) bstore4(rzq, q, vq, (unsigned)(NQ - 1 - mq) * step);
-                } else {
-                    z = ((c.a * c.b) - c.c * zprev * c.b) - c.d * zn * c.b;
-                    bstore(rz, z, voz, (unsigned)(NI - 1 - i) * step);
+                } else {  // the reference's order of operations
+                    const double di = rr[rq].v[3 * wq], ae = rr[rq].v[3 * wq + 1], an = rr[rq].v[3 * wq + 2];
+                    z = ((y * di) - ae * zprev * di) - an * zn * di;
                 }
+                q[wq] = z;
+                if (wq == 0) bstore4(rzq, q, vq, (unsigned)(NQ - 1 - mq) * step);
                 if (k > 0) pub = dpp_shr1(pub, zprev);  // the previous step's value (its register is free now)
                 zprev = z;
                 __builtin_amdgcn_sched_barrier(0);
-                ring[r] = rec(rbk, (unsigned)(NI - 1 - i) * step);
-                if constexpr (kQuad)
-                    if (wq == 0) yr[mq] = bload4(ry, vq, (unsigned)(NQ - 1 - mq) * step);
+                if constexpr (kFused) {
+                    ring[i % PD] = bload_bk3(rsrc_at(B32, a.rbytes, tl - PD, step), vo, (unsigned)(NI - 1 - i) * step);
+                } else if (wq == 0) {
+                    rr[rq] = bload_rq(rsrc_bq(Brec, 3 * qbytes, tl - PD, step), vb, (unsigned)(NQ - 1 - mq) * 3 * step);
+                }
+                if (wq == 0) yr[rq] = bload_q(ry, vq, (unsigned)(NQ - 1 - mq) * step, (T *)nullptr);
             }
             pub = dpp_shr1(pub, zprev);
             hd.finish(G, h, pub);
@@ -678,14 +704,16 @@ __global__ void __launch_bounds__(64 * (W + 2))
 // ---------------------------------------------------------------------------
 // All-CU helpers: padding, packing, gather / scatter, FP32 copies
 // ---------------------------------------------------------------------------
-__global__ void k_skew_init(size_t ns, Rec4 *F, Rec4 *Bk, Rec4 *U, double *zp) {
+__global__ void k_skew_init(size_t ns, Rec4 *F, Rec4 *Bk, Rec4 *U, double *zp, double *yq, double *bq) {
     const size_t s = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= ns) return;
     F[s] = Rec4{0.0, 1.0, 0.0, 0.0};
     Bk[s] = Rec4{0.0, 1.0, 0.0, 0.0};
     U[s] = Rec4{1.0, 0.0, 0.0, 0.0};
+    zp[s] = 0.0;
+    yq[s] = 0.0;
 #pragma unroll
-    for (int q = 0; q < 4; ++q) zp[4 * s + q] = 0.0;
+    for (int q = 0; q < 3; ++q) bq[3 * s + q] = 0.0;
 }
 
 template <typename T>
@@ -767,11 +795,11 @@ __global__ void __launch_bounds__(256)
     }
 }
 
-// zp (slots) -> y (natural)
+// The backward sweep's quads (zq) -> y (natural).
 template <typename T>
 __global__ void __launch_bounds__(256)
-    k_skew_scatter(const SkewTile *__restrict__ tiles, const SkewGrid *__restrict__ grids, const T *__restrict__ zp,
-                   T *__restrict__ y, const KScal *ks) {
+    k_skew_scatter_q(const SkewTile *__restrict__ tiles, const SkewGrid *__restrict__ grids,
+                     const T *__restrict__ zq, T *__restrict__ y, const KScal *ks) {
     if (ks && ks->done) return;
     __shared__ TileLds<T> t;
     const SkewTile tl = tiles[blockIdx.x];
@@ -782,38 +810,8 @@ __global__ void __launch_bounds__(256)
     for (int it = 0; it < kTile / 2; ++it) {
         const int d = wv + 4 * it, j = tl.j0 + lane, il = d - lane;
         const bool in = il >= 0 && il < kTile && tl.i0 + il < g.nx && j < g.ny;
-        v[it] = in ? zp[4 * (g.base + (size_t)(tl.i0 + tl.j0 + d + kSkewD0) * g.S) + j] : T(0);
-    }
-#pragma unroll
-    for (int it = 0; it < kTile / 2; ++it) {
-        const int il = wv + 4 * it - lane;
-        if (il >= 0 && il < kTile) t.v[lane][il] = v[it];
-    }
-    __syncthreads();
-    const int i = tl.i0 + lane;
-#pragma unroll
-    for (int it = 0; it < kTile / 4; ++it) {
-        const int r = wv + 4 * it, j = tl.j0 + r;
-        if (i < g.nx && j < g.ny) y[(size_t)g.o + i + (size_t)j * g.nx] = t.v[r][lane];
-    }
-}
-
-// The FP32 backward sweep's quads (zq) -> y (natural).
-__global__ void __launch_bounds__(256)
-    k_skew_scatter_q(const SkewTile *__restrict__ tiles, const SkewGrid *__restrict__ grids,
-                     const float *__restrict__ zq, float *__restrict__ y, const KScal *ks) {
-    if (ks && ks->done) return;
-    __shared__ TileLds<float> t;
-    const SkewTile tl = tiles[blockIdx.x];
-    const SkewGrid g = grids[tl.grid];
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    float v[kTile / 2];  // all loads in flight before the LDS stores
-#pragma unroll
-    for (int it = 0; it < kTile / 2; ++it) {
-        const int d = wv + 4 * it, j = tl.j0 + lane, il = d - lane;
-        const bool in = il >= 0 && il < kTile && tl.i0 + il < g.nx && j < g.ny;
         const int Tq = tl.i0 + tl.j0 + d + kSkewQ0;
-        v[it] = in ? zq[g.base + (size_t)(Tq & ~3) * g.S + 4 * (size_t)j + (Tq & 3)] : 0.f;
+        v[it] = in ? zq[g.base + (size_t)(Tq & ~3) * g.S + 4 * (size_t)j + (Tq & 3)] : T(0);
     }
 #pragma unroll
     for (int it = 0; it < kTile / 2; ++it) {
@@ -864,6 +862,30 @@ __global__ void k_skew_edges(const SkewGrid *__restrict__ grids, Rec4 *__restric
         Rec4 &r = Bk[slot(j + q - g.ny, j)];
         r.d = copysign(0.0, r.b);
     }
+}
+
+// FP64 backward quad records: (Dinv, A_re, A_rn) of Bk, twelve doubles per
+// row and quad of diagonals (quad (T + kSkewQ0)/4, word (T + kSkewQ0)%4),
+// for the diagonals T >= -kSkewPD the sweep reads.  One thread per row and
+// quad (four coalesced record loads, one contiguous 96-byte record);
+// blockIdx.y: the grid.
+static_assert(kSkewPD % 4 == 0 && (kSkewQ0 - kSkewPD) % 4 == 0, "quads start at a padding diagonal");
+__global__ void k_skew_bq(const SkewGrid *__restrict__ grids, const Rec4 *__restrict__ Bk, double *__restrict__ bq) {
+    const SkewGrid g = grids[blockIdx.y];
+    const size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= ((size_t)g.NG * kSkewL + kSkewPD) / 4 * g.S) return;
+    const int qi = (int)(q / g.S) + (kSkewQ0 - kSkewPD) / 4, j = (int)(q % g.S);
+    double c[12];
+#pragma unroll
+    for (int w = 0; w < 4; ++w) {  // diagonal T = 4 qi + w - kSkewQ0
+        const Rec4 r = Bk[g.base + (size_t)(4 * qi + w - kSkewQ0 + kSkewD0) * g.S + j];
+        c[3 * w] = r.b;
+        c[3 * w + 1] = r.c;
+        c[3 * w + 2] = r.d;
+    }
+    double2 *o = reinterpret_cast<double2 *>(bq + 3 * (g.base + (size_t)qi * 4 * g.S + 4 * (size_t)j));
+#pragma unroll
+    for (int t = 0; t < 6; ++t) o[t] = double2{c[2 * t], c[2 * t + 1]};
 }
 
 __global__ void k_skew_dinv_natural(int n, const int *__restrict__ slot_of, const Rec4 *__restrict__ F,
@@ -982,14 +1004,17 @@ static int skew_alloc(fvh_eqn *e) {
     FVH_TRY(s.F.alloc(L.NS));
     FVH_TRY(s.Bk.alloc(L.NS));
     FVH_TRY(s.U.alloc(L.NS));
-    FVH_TRY(s.zp.alloc(4 * L.NS));  // rows at the records' stride
+    FVH_TRY(s.zp.alloc(L.NS));  // quads
+    FVH_TRY(s.yq.alloc(L.NS));
+    FVH_TRY(s.bq.alloc(3 * L.NS));
     FVH_TRY(s.hand.alloc((size_t)L.nblk[0] * 2 * L.HS));  // W = 1 has the most blocks
     FVH_TRY(s.uhand.alloc((size_t)L.nblk[0] * L.HS));
     if (!s.h_fault) {
         HIP_TRY(hipHostMalloc((void **)&s.h_fault, sizeof(int), hipHostMallocDefault));
         *s.h_fault = 0;
     }
-    hipLaunchKernelGGL(k_skew_init, grid_for(L.NS), dim3(256), 0, stream(), L.NS, s.F.p, s.Bk.p, s.U.p, s.zp.p);
+    hipLaunchKernelGGL(k_skew_init, grid_for(L.NS), dim3(256), 0, stream(), L.NS, s.F.p, s.Bk.p, s.U.p, s.zp.p,
+                       s.yq.p, s.bq.p);
     HIP_TRY(hipGetLastError());
     return FVH_OK;
 }
@@ -1042,27 +1067,32 @@ int skew_update(fvh_eqn *e, const double *val) {
     hipLaunchKernelGGL(k_skew_edges, dim3((unsigned)((edge + 255) / 256), (unsigned)L.h_grids.size()), dim3(256), 0,
                        stream(), L.grids.p, e->sk.Bk.p);
     HIP_TRY(hipGetLastError());
+    size_t most = 0;
+    for (const SkewGrid &g : L.h_grids) most = std::max(most, ((size_t)g.NG * kSkewL + kSkewPD) / 4 * g.S);
+    hipLaunchKernelGGL(k_skew_bq, dim3((unsigned)((most + 255) / 256), (unsigned)L.h_grids.size()), dim3(256), 0,
+                       stream(), L.grids.p, e->sk.Bk.p, e->sk.bq.p);
+    HIP_TRY(hipGetLastError());
     if (!m->skew->W && !forced_waves()) FVH_TRY(skew_tune(e));
     return FVH_OK;
 }
 
 template <typename T, int W>
-static int launch_sweeps_w(fvh_eqn *e, Rec4T<T> *F, Rec4T<T> *Bk, T *yq, T *zp, T *hand, const KScal *ks) {
+static int launch_sweeps_w(fvh_eqn *e, Rec4T<T> *F, const void *Brec, T *yq, T *zp, T *hand, const KScal *ks) {
     const SkewLayout &L = *e->m->skew;
     const SkewArgs a = skew_args<Rec4T<T>>(e, W);
     const size_t lds = (size_t)(W + 1) * L.HS * sizeof(T);
     FVH_TRY(allow_lds(k_skew_fwd<T, W>, lds));
     FVH_TRY(allow_lds(k_skew_bwd<T, W>, lds));
-    hipLaunchKernelGGL((k_skew_fwd<T, W>), dim3(a.nblk), dim3(64 * (W + 2)), lds, stream(), a, F, Bk, yq, hand, ks);
+    hipLaunchKernelGGL((k_skew_fwd<T, W>), dim3(a.nblk), dim3(64 * (W + 2)), lds, stream(), a, F, yq, hand, ks);
     HIP_TRY(hipGetLastError());
-    hipLaunchKernelGGL((k_skew_bwd<T, W>), dim3(a.nblk), dim3(64 * (W + 2)), lds, stream(), a, Bk, (const T *)yq, zp,
+    hipLaunchKernelGGL((k_skew_bwd<T, W>), dim3(a.nblk), dim3(64 * (W + 2)), lds, stream(), a, Brec, (const T *)yq, zp,
                        hand, ks);
     HIP_TRY(hipGetLastError());
     return FVH_OK;
 }
 
 template <typename T>
-static int skew_apply_t(fvh_eqn *e, const T *x, T *y, const KScal *ks, Rec4T<T> *F, Rec4T<T> *Bk, T *yq, T *zp,
+static int skew_apply_t(fvh_eqn *e, const T *x, T *y, const KScal *ks, Rec4T<T> *F, const void *Brec, T *yq, T *zp,
                         T *hand) {
     fvh_mesh *m = e->m;
     const SkewLayout &L = *m->skew;
@@ -1075,15 +1105,12 @@ static int skew_apply_t(fvh_eqn *e, const T *x, T *y, const KScal *ks, Rec4T<T> 
                        L.tiles.p, L.grids.p, x, F, hand, nhand, L.HS, ks_all);
     HIP_TRY(hipGetLastError());
     switch (skew_waves(L, sizeof(T) == 4)) {
-        case 1: FVH_TRY((launch_sweeps_w<T, 1>(e, F, Bk, yq, zp, hand, ks))); break;
-        case 2: FVH_TRY((launch_sweeps_w<T, 2>(e, F, Bk, yq, zp, hand, ks))); break;
-        default: FVH_TRY((launch_sweeps_w<T, 3>(e, F, Bk, yq, zp, hand, ks))); break;
+        case 1: FVH_TRY((launch_sweeps_w<T, 1>(e, F, Brec, yq, zp, hand, ks))); break;
+        case 2: FVH_TRY((launch_sweeps_w<T, 2>(e, F, Brec, yq, zp, hand, ks))); break;
+        default: FVH_TRY((launch_sweeps_w<T, 3>(e, F, Brec, yq, zp, hand, ks))); break;
     }
-    if constexpr (sizeof(T) == 4)
-        hipLaunchKernelGGL(k_skew_scatter_q, dim3(ntiles), dim3(256), 0, stream(), L.tiles.p, L.grids.p, zp, y, ks_all);
-    else
-        hipLaunchKernelGGL((k_skew_scatter<T>), dim3(ntiles), dim3(256), 0, stream(), L.tiles.p, L.grids.p, zp, y,
-                           ks_all);
+    hipLaunchKernelGGL((k_skew_scatter_q<T>), dim3(ntiles), dim3(256), 0, stream(), L.tiles.p, L.grids.p, zp, y,
+                       ks_all);
     HIP_TRY(hipGetLastError());
     return FVH_OK;
 }
@@ -1092,7 +1119,7 @@ static int skew_apply_t(fvh_eqn *e, const T *x, T *y, const KScal *ks, Rec4T<T> 
 // width on the freshly packed records (the sweeps only rewrite their own
 // scratch fields); the width never changes a result.
 template <typename T>
-static int skew_tune_t(fvh_eqn *e, Rec4T<T> *F, Rec4T<T> *Bk, T *yq, T *zp, T *hand, int &W_out) {
+static int skew_tune_t(fvh_eqn *e, Rec4T<T> *F, const void *Brec, T *yq, T *zp, T *hand, int &W_out) {
     SkewLayout &L = *e->m->skew;
     const int n = (int)e->m->n;
     DBuf<T> x, y;
@@ -1106,9 +1133,9 @@ static int skew_tune_t(fvh_eqn *e, Rec4T<T> *F, Rec4T<T> *Bk, T *yq, T *zp, T *h
     int bestW = 1;
     for (int W = 1; W <= lds_waves_max(L); ++W) {
         W_out = W;
-        FVH_TRY(skew_apply_t<T>(e, x.p, y.p, nullptr, F, Bk, yq, zp, hand));  // warm
+        FVH_TRY(skew_apply_t<T>(e, x.p, y.p, nullptr, F, Brec, yq, zp, hand));  // warm
         HIP_TRY(hipEventRecord(a, stream()));
-        for (int rep = 0; rep < 3; ++rep) FVH_TRY(skew_apply_t<T>(e, x.p, y.p, nullptr, F, Bk, yq, zp, hand));
+        for (int rep = 0; rep < 3; ++rep) FVH_TRY(skew_apply_t<T>(e, x.p, y.p, nullptr, F, Brec, yq, zp, hand));
         HIP_TRY(hipEventRecord(b, stream()));
         HIP_TRY(hipEventSynchronize(b));
         float t = 0.f;
@@ -1127,13 +1154,13 @@ static int skew_tune_t(fvh_eqn *e, Rec4T<T> *F, Rec4T<T> *Bk, T *yq, T *zp, T *h
 
 static int skew_tune(fvh_eqn *e) {
     SkewStore &s = e->sk;
-    return skew_tune_t<double>(e, s.F.p, s.Bk.p, (double *)nullptr, s.zp.p, s.hand.p, e->m->skew->W);
+    return skew_tune_t<double>(e, s.F.p, s.bq.p, s.yq.p, s.zp.p, s.hand.p, e->m->skew->W);
 }
 
 int skew_apply(fvh_eqn *e, const double *x, double *y, const KScal *ks) {
     SkewStore &s = e->sk;
     if (!s.F.p) return set_error(FVH_ERR_ARG, "DILU records not prepared");
-    return skew_apply_t<double>(e, x, y, ks, s.F.p, s.Bk.p, (double *)nullptr, s.zp.p, s.hand.p);
+    return skew_apply_t<double>(e, x, y, ks, s.F.p, s.bq.p, s.yq.p, s.zp.p, s.hand.p);
 }
 
 int skew_to32(fvh_eqn *e) {
@@ -1142,11 +1169,11 @@ int skew_to32(fvh_eqn *e) {
     if (s.F32.n != L.NS) {
         FVH_TRY(s.F32.alloc(L.NS));
         FVH_TRY(s.Bk32.alloc(L.NS));
-        FVH_TRY(s.zp32.alloc(4 * L.NS));
+        FVH_TRY(s.zp32.alloc(L.NS));
         FVH_TRY(s.yq32.alloc(L.NS));
         hipLaunchKernelGGL(k_skew_fill<float>, grid_for(L.NS), dim3(256), 0, stream(), L.NS, s.yq32.p, 0.f);
         FVH_TRY(s.hand32.alloc((size_t)L.nblk[0] * 2 * L.HS));
-        hipLaunchKernelGGL(k_skew_fill<float>, grid_for(4 * L.NS), dim3(256), 0, stream(), 4 * L.NS, s.zp32.p, 0.f);
+        hipLaunchKernelGGL(k_skew_fill<float>, grid_for(L.NS), dim3(256), 0, stream(), L.NS, s.zp32.p, 0.f);
         HIP_TRY(hipGetLastError());
     }
     size_t most = 0;
