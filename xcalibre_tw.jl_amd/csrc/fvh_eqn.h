@@ -82,23 +82,25 @@ struct BandStore {
     DBuf<float> zp32;
     BandRecs r{};
 };
-// Records of the pipelined band sweeps, in the mesh's skewed slot order:
-// F (b_r, Dinv_r, A_rs, A_rw), Bk (y_r, Dinv_r, A_re, A_rn), U (A_rr,
-// A_rs*A_sr, A_rw*A_wr, 0); yq / zp the forward / backward sweep's results
-// in quads (four diagonals of a row, see kSkewQ0); hand the
-// global hand-over arrays between blocks ([block][forward | backward][HS]),
-// uhand those of the diagonal recurrence ([block][HS]); *h_fault (pinned
-// host memory) is set by a sweep whose hand-over wait ran out.
+// The pipelined band sweeps' buffers of one precision, in the mesh's skewed
+// slot order: kf / kb the forward / backward quad records (coefficients of
+// four diagonals of a row together), bq / yq / zp quads of the right-hand
+// side, the forward and the backward results (see kSkewQ0), hand the global
+// hand-over arrays between blocks ([block][forward | backward][HS]).
+template <typename T>
+struct SkewSweep {
+    DBuf<T> kf, kb, bq, yq, zp, hand;
+};
+// The diagonal recurrence's records F (0, Dinv_r, A_rs, A_rw), Bk (0,
+// Dinv_r, A_re, A_rn), U (A_rr, A_rs*A_sr, A_rw*A_wr, 0) and its global
+// hand-over arrays uhand ([block][HS]); the sweeps' FP64 and FP32 buffers;
+// *h_fault (pinned host memory) is set by a sweep whose hand-over wait ran
+// out.
 struct SkewStore {
     DBuf<Rec4> F, Bk, U;
-    DBuf<double> zp, hand, uhand;
-    DBuf<Rec4f> F32, Bk32;
-    DBuf<float> zp32, hand32;
-    DBuf<float> yq32;
-    DBuf<double> yq;
-    // the FP64 backward sweep's coefficients: (Dinv, A_re, A_rn) of the four
-    // diagonals of a quad in one 96-byte record per row
-    DBuf<double> bq;
+    DBuf<double> uhand;
+    SkewSweep<double> s64;
+    SkewSweep<float> s32;
     int *h_fault = nullptr;
     SkewStore() = default;
     SkewStore(const SkewStore &) = delete;

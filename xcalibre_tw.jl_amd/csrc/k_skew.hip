@@ -170,16 +170,20 @@ __device__ inline void bstore(Rsrc r, float v, unsigned vo, unsigned so) {
     __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, vo, so, 0);
 }
 
-// Quads of results (four consecutive diagonals of one row, 16 or 32 bytes):
-// the buffer from the quad of diagonal t on (t + kSkewQ0 a multiple of 4;
-// `step` bytes per four diagonals, which is also the records' stride).
+// Quads: the values of four consecutive diagonals of one row, stored
+// together (results, right-hand sides) or N coefficients per diagonal
+// (the sweeps' quad records, 4N values per row and quad).  The buffer from
+// the quad of diagonal t on (t + kSkewQ0 a multiple of 4), `qstride` bytes
+// per quad of diagonals: a quad of values takes S*4*sizeof(T) bytes, which
+// is the Rec4 records' stride per diagonal (`step`); a quad record N times
+// that.
 typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
-template <typename T>
-struct QuadT {
-    T v[4];
+template <typename T, int N>
+struct Vec {
+    T v[N];
 };
-__device__ inline Rsrc rsrc_q(const void *p, unsigned bytes, int t, unsigned step) {
-    const unsigned off = (unsigned)((t + kSkewQ0) / 4) * step;
+__device__ inline Rsrc rsrc_q(const void *p, unsigned bytes, int t, unsigned qstride) {
+    const unsigned off = (unsigned)((t + kSkewQ0) / 4) * qstride;
     return rsrc(static_cast<const char *>(p) + off, bytes - off);
 }
 __device__ inline void bstore4(Rsrc r, const float (&q)[4], unsigned vo, unsigned so) {
@@ -190,39 +194,29 @@ __device__ inline void bstore4(Rsrc r, const double (&q)[4], unsigned vo, unsign
     __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u32x4, double2{q[0], q[1]}), r, vo, so, 0);
     __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u32x4, double2{q[2], q[3]}), r, vo + 16, so, 0);
 }
-__device__ inline QuadT<float> bload_q(Rsrc r, unsigned vo, unsigned so, float *) {
-    const auto u = __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(r, vo, so, 0));
-    return QuadT<float>{{u.x, u.y, u.z, u.w}};
-}
-__device__ inline QuadT<double> bload_q(Rsrc r, unsigned vo, unsigned so, double *) {
-    const auto u = __builtin_bit_cast(double2, __builtin_amdgcn_raw_buffer_load_b128(r, vo, so, 0));
-    const auto v = __builtin_bit_cast(double2, __builtin_amdgcn_raw_buffer_load_b128(r, vo + 16, so, 0));
-    return QuadT<double>{{u.x, u.y, v.x, v.y}};
-}
-// FP64 backward quad records: (Dinv, A_re, A_rn) of the four diagonals of a
-// quad, 96 bytes per row and quad (`3 step` bytes per quad); the buffer from
-// the quad of diagonal t on.
-struct RecQ {
-    double v[12];
-};
-__device__ inline Rsrc rsrc_bq(const void *p, unsigned bytes, int t, unsigned step) {
-    const unsigned off = (unsigned)((t + kSkewQ0) / 4) * 3 * step;
-    return rsrc(static_cast<const char *>(p) + off, bytes - off);
-}
-__device__ inline RecQ bload_rq(Rsrc r, unsigned vo, unsigned so) {
-    RecQ c;
-#pragma unroll
-    for (int t = 0; t < 6; ++t) {
-        const auto u = __builtin_bit_cast(double2, __builtin_amdgcn_raw_buffer_load_b128(r, vo + 16 * t, so, 0));
-        c.v[2 * t] = u.x;
-        c.v[2 * t + 1] = u.y;
+// Piece t (16 bytes) of N values into c
+template <typename T, int N>
+__device__ inline void bload_piece(Vec<T, N> &c, int t, Rsrc r, unsigned vo, unsigned so) {
+    const u32x4 u = __builtin_amdgcn_raw_buffer_load_b128(r, vo + 16 * t, so, 0);
+    if constexpr (sizeof(T) == 8) {
+        const auto d = __builtin_bit_cast(double2, u);
+        c.v[2 * t] = d.x;
+        c.v[2 * t + 1] = d.y;
+    } else {
+        const auto f = __builtin_bit_cast(float4, u);
+        c.v[4 * t] = f.x;
+        c.v[4 * t + 1] = f.y;
+        c.v[4 * t + 2] = f.z;
+        c.v[4 * t + 3] = f.w;
     }
-    return c;
 }
-// (Dinv, A_re Dinv, A_rn Dinv) of an FP32 backward record: its last 12 bytes
-__device__ inline Rec4f bload_bk3(Rsrc r, unsigned vo, unsigned so) {
-    const auto v = __builtin_amdgcn_raw_buffer_load_b96(r, vo + 4, so, 0);
-    return Rec4f{0.f, __uint_as_float(v[0]), __uint_as_float(v[1]), __uint_as_float(v[2])};
+// A whole quad of N values per diagonal (4N values) in 16-byte loads.  The
+// sweeps refill a quad after its last step: spreading the pieces over the
+// quad's steps (each right after its last reader) measured slower.
+template <typename T, int N>
+__device__ inline void bload_quad(Vec<T, 4 * N> &c, Rsrc r, unsigned vo, unsigned so) {
+#pragma unroll
+    for (int t = 0; t < 4 * N * (int)sizeof(T) / 16; ++t) bload_piece(c, t, r, vo, so);
 }
 
 // Whole-wave lane shifts for the hand-over batches: the next batch position
@@ -470,13 +464,29 @@ __global__ void __launch_bounds__(64 * (W + 2))
 
 // ---------------------------------------------------------------------------
 // ldiv!: forward sweep into quads of y (yq), backward sweep into quads of z
-// (zp) (FP64: the reference's arithmetic; FP32: the inner solves of mixed
-// precision)
+// (zp).  FP64 evaluates the reference's expressions; FP32 (the inner solves
+// of mixed precision) works on coefficients premultiplied by their pivots,
+// with fused multiply-adds.
+//
+// FP64 forward: one record per step, F = (b, Dinv, A_rs, A_rw); the west
+// pivot is the wave's own previous Dinv, the south pivot lane l-1's (lane
+// 0's comes with the hand-over: the wave below's top row pivots, read from
+// F kLook groups ahead).  The other sweeps read quad records, N values per
+// diagonal for four diagonals at once (one refill per four steps):
+//   forward  FP32 (b quads apart)   (A_rs Dinv_s, A_rw Dinv_w)
+//   backward FP64 (Dinv, A_re, A_rn),   FP32 (Dinv, A_re Dinv, A_rn Dinv)
+// with Dinv_s / Dinv_w the pivots of the south / west neighbours (slots
+// (T-1, j-1) and (T-1, j)).  Quad records would carry both pivots in the
+// FP64 forward sweep too, but at 48 bytes per step its prefetch depth (63
+// memory operations in flight) no longer covers the load latency.
 // ---------------------------------------------------------------------------
+constexpr int kFwdN = 2;  // FP32 forward quad records
+constexpr int kBwdN = 3;
+
 template <typename T, int W>
 __global__ void __launch_bounds__(64 * (W + 2))
-    k_skew_fwd(SkewArgs a, const Rec4T<T> *__restrict__ F, T *__restrict__ yq, T *__restrict__ hand,
-               const KScal *ks) {
+    k_skew_fwd(SkewArgs a, const void *__restrict__ recs, const T *__restrict__ bq, T *__restrict__ yq,
+               T *__restrict__ hand, const KScal *ks) {
     using R = Rec4T<T>;
     extern __shared__ unsigned char skew_lds[];
     T *lds = reinterpret_cast<T *>(skew_lds);
@@ -500,77 +510,92 @@ __global__ void __launch_bounds__(64 * (W + 2))
     const int lane = threadIdx.x & 63, wg = rb * W + w;
     const unsigned S = g.S, j0 = wg * 64;
     const unsigned vo = (g.base + j0 + lane) * (unsigned)sizeof(R), step = S * (unsigned)sizeof(R);
-    // lane 0's south pivot: Dinv of (step T-1, row j0-1), static; lane q < kL
-    // of the batch of group G holds step G*kL + q
+    // lane q < kL of the batch of group G holds step G*kL + q
     const int ql = min(lane, kL - 1);
-    const T *dp = &F[g.base + (size_t)(ql - 1 + kSkewD0) * S + j0 - 1].b;
-    const size_t hstep = (size_t)kL * S * 4;  // one batch of steps, in T
 
     constexpr int M = ring_batches<T>(), PD = M * kL;
     static_assert(kLook % M == 0, "whole rings per unrolled block");
-    // FP32 (the inner solves of mixed precision) works on premultiplied
-    // coefficients with fused multiply-adds: no pivots travel with y.  The
-    // results go out in quads (16 or 32 bytes per four steps).
     constexpr bool kFused = sizeof(T) == 4;
+    constexpr int N = kFwdN, NR = PD / 4;
     const unsigned vq = (g.base + 4 * (j0 + lane)) * (unsigned)sizeof(T), qbytes = a.rbytes / 4;
-    T q[4] = {T(0), T(0), T(0), T(0)};
+    const unsigned vc = N * vq, cstride = N * step;
+    const R *F = static_cast<const R *>(recs);
+    // FP64: lane 0's south pivot, Dinv of (step T-1, row j0-1)
+    const T *dp = &F[g.base + (size_t)(ql - 1 + kSkewD0) * S + j0 - 1].b;
+    const size_t hstep = (size_t)kL * S * 4;  // one batch of steps, in T
     Hand<T> hd;
     hd.start(lds + w * HS, lds + (w + 1) * HS, HS, ql, lane >= 64 - kL, 1 + lane - (64 - kL));
     T db[kLook];  // lane-0 pivots of the next kLook groups (FP64)
 #pragma unroll
     for (int h = 0; h < kLook; ++h) db[h] = kFused ? T(0) : dp[h * hstep];
     dp += kLook * hstep;
-    // The prologue stores +0 into the wave's own pre-start padding (its value
-    // already), so that it issues what a step issues and the loop's memory
-    // waits count the same outstanding operations on every iteration.
-    R ring[PD];
+    R ring[kFused ? 1 : PD];           // FP64 records
+    Vec<T, 4 * N> cr[kFused ? NR : 1];  // FP32 coefficient quads
+    Vec<T, 4> br[kFused ? NR : 1];      // FP32 right-hand-side quads
+    T q[4] = {T(0), T(0), T(0), T(0)};
+    // The prologue stores quads of +0 into the wave's own pre-start padding
+    // (its value already), so that it issues what the loop issues and the
+    // loop's memory waits count the same outstanding operations throughout.
     {
-        const Rsrc rf = rsrc_at(F, a.rbytes, 0, step);
         const Rsrc rq = rsrc_q(yq, qbytes, -PD, step);
 #pragma unroll
         for (int k = 0; k < PD; ++k) {
-            if (k % 4 == 3) bstore4(rq, q, vq, (unsigned)(k / 4) * step);
-            ring[k] = bload(rf, vo, k * step, (R *)nullptr);
+            const int mq = k / 4;
+            if (k % 4 == 3) bstore4(rq, q, vq, (unsigned)mq * step);
             __builtin_amdgcn_sched_barrier(0);
+            if constexpr (kFused) {
+                if (k % 4 == 3) {
+                    bload_quad<T, N>(cr[mq], rsrc_q(recs, N * qbytes, 0, cstride), vc, (unsigned)mq * cstride);
+                    bload_quad<T, 1>(br[mq], rsrc_q(bq, qbytes, 0, step), vq, (unsigned)mq * step);
+                }
+            } else {
+                ring[k] = bload(rsrc_at(F, a.rbytes, 0, step), vo, k * step, (R *)nullptr);
+            }
         }
     }
     T xprev = 0, dprev = 1, pub = 0;
     int broken = 0;
     int G0 = 0;
     do {
-        // this iteration's steps G0*kL + i, i < kLook*kL: stores at i*step, refills PD steps on
-        const Rsrc rf = rsrc_at(F, a.rbytes, G0 * kL + PD, step);
-        const Rsrc rq = rsrc_q(yq, qbytes, G0 * kL, step);
+        // this iteration's steps G0*kL + i, i < kLook*kL: results in quads
+        // from G0*kL on, refills PD steps on
+        const Rsrc rq = rsrc_q(yq, qbytes, G0 * kL, step), rf = rsrc_at(F, a.rbytes, G0 * kL + PD, step);
+        const Rsrc rc = rsrc_q(recs, N * qbytes, G0 * kL + PD, cstride), rbn = rsrc_q(bq, qbytes, G0 * kL + PD, step);
 #pragma unroll
         for (int h = 0; h < kLook; ++h) {
             const int G = G0 + h;
             T yc = hd.take(G, h, broken), dc = db[h];
 #pragma unroll
             for (int k = 0; k < kL; ++k) {
-                const int r = (h % M) * kL + k;
-                const unsigned si = (unsigned)(h * kL + k) * step;
-                const R c = ring[r];
+                const int i = h * kL + k, mq = i / 4, wq = i % 4;
+                const T yn = shl1z(yc);
+                const T xs = dpp_shr1(xprev, yc);
+                yc = yn;
                 T x;
-                if constexpr (kFused) {  // (b, A_rs Dinv_s, A_rw Dinv_w)
-                    const T yn = shl1z(yc);
-                    const T xs = dpp_shr1(xprev, yc);
-                    yc = yn;
-                    x = __builtin_fmaf(-c.c, xprev, __builtin_fmaf(-c.b, xs, c.a));
-                } else {  // (b, Dinv, A_rs, A_rw)
-                    const T yn = shl1z(yc), dn = shl1z(dc);
-                    const T xs = dpp_shr1(xprev, yc);
+                if constexpr (kFused) {  // (A_rs Dinv_s, A_rw Dinv_w) and b
+                    const T *c = &cr[mq % NR].v[N * wq];
+                    x = __builtin_fmaf(-c[1], xprev, __builtin_fmaf(-c[0], xs, br[mq % NR].v[wq]));
+                } else {  // (b, Dinv, A_rs, A_rw): the reference's order of operations
+                    const R c = ring[i % PD];
+                    const T dn = shl1z(dc);
                     const T ds = dpp_shr1(dprev, dc);
-                    yc = yn;
                     dc = dn;
                     x = (c.a - c.c * xs * ds) - c.d * xprev * dprev;
                     dprev = keep(c.b);
                 }
-                q[k % 4] = x;
-                if (k % 4 == 3) bstore4(rq, q, vq, (unsigned)((h * kL + k) / 4) * step);
+                q[wq] = x;
+                if (wq == 3) bstore4(rq, q, vq, (unsigned)mq * step);
                 if (k > 0) pub = dpp_shl1(pub, xprev);  // the previous step's value (its register is free now)
                 xprev = x;
                 __builtin_amdgcn_sched_barrier(0);
-                ring[r] = bload(rf, vo, si, (R *)nullptr);
+                if constexpr (kFused) {
+                    if (wq == 3) {
+                        bload_quad<T, N>(cr[mq % NR], rc, vc, (unsigned)mq * cstride);
+                        bload_quad<T, 1>(br[mq % NR], rbn, vq, (unsigned)mq * step);
+                    }
+                } else {
+                    ring[i % PD] = bload(rf, vo, (unsigned)i * step, (R *)nullptr);
+                }
             }
             pub = dpp_shl1(pub, xprev);
             hd.finish(G, h, pub);
@@ -584,7 +609,7 @@ __global__ void __launch_bounds__(64 * (W + 2))
 
 template <typename T, int W>
 __global__ void __launch_bounds__(64 * (W + 2))
-    k_skew_bwd(SkewArgs a, const void *__restrict__ Brec, const T *__restrict__ yq, T *__restrict__ zp,
+    k_skew_bwd(SkewArgs a, const T *__restrict__ kb, const T *__restrict__ yq, T *__restrict__ zp,
                T *__restrict__ hand, const KScal *ks) {
     using R = Rec4T<T>;
     extern __shared__ unsigned char skew_lds[];
@@ -610,7 +635,7 @@ __global__ void __launch_bounds__(64 * (W + 2))
     const int lane = threadIdx.x & 63, wg = rb * W + w;
     const unsigned S = g.S, j0 = wg * 64;
     const int Ttop = g.NG * kL - 1;  // step u handles anti-diagonal Ttop - u
-    const unsigned vo = (g.base + j0 + lane) * (unsigned)sizeof(R), step = S * (unsigned)sizeof(R);
+    const unsigned step = S * (unsigned)sizeof(R);  // bytes per quad of values
     constexpr int NI = kLook * kL;  // steps per loop iteration
     // lane 63-q of the batch of group G holds step u = G*kL + q (index u)
     const int qh = min(63 - lane, kL - 1);
@@ -619,40 +644,32 @@ __global__ void __launch_bounds__(64 * (W + 2))
     static_assert(kLook % M == 0, "whole rings per unrolled block");
     Hand<T> hd;
     hd.start(lds + (w + 1) * HS, lds + w * HS, HS, qh, lane < kL, kL - lane);
-    // y comes in quads (a ring of PD/4 quads, one refill per four steps) and
-    // z goes out in quads; Ttop + 1 + kSkewQ0 is a multiple of 4, so step i
-    // of an iteration is word 3 - i%4 of the iteration's quad i/4 (four
-    // diagonals, descending).  The coefficients: FP32 reads the last 12 bytes
-    // of each step's record (Dinv, A_re Dinv, A_rn Dinv), premultiplied for
-    // fused multiply-adds; FP64 reads quad records (Dinv, A_re, A_rn for four
-    // diagonals, 96 bytes) with the quads of y.
+    // Ttop + 1 + kSkewQ0 is a multiple of 4, so step i of an iteration is
+    // word 3 - i%4 of the iteration's quad i/4 (four diagonals, descending)
     constexpr bool kFused = sizeof(T) == 4;
-    constexpr int NQ = NI / 4, NR = PD / 4;  // quads per loop iteration, quads in the rings
+    constexpr int N = kBwdN, NQ = NI / 4, NR = PD / 4;  // quads per loop iteration, quads in the rings
     static_assert(NQ % NR == 0, "whole rings per loop iteration");
     const unsigned vq = (g.base + 4 * (j0 + lane)) * (unsigned)sizeof(T), qbytes = a.rbytes / 4;
-    const unsigned vb = (3 * g.base + 12 * (j0 + lane)) * 8u;
-    const R *B32 = static_cast<const R *>(Brec);
-    QuadT<T> yr[NR];
-    RecQ rr[kFused ? 1 : NR];
-    R ring[kFused ? PD : 1];
+    const unsigned vc = N * vq, cstride = N * step;
+    Vec<T, 4 * N> cr[NR];  // coefficient quads
+    Vec<T, 4> yr[NR];      // y quads
     T q[4] = {T(0), T(0), T(0), T(0)};
-    // prologue: the first PD steps' coefficients and y quads, and quads of
-    // +0 into the padding diagonals above the top (see the forward sweep),
-    // in the loop's order; offsets count down from the top of each range
+    // prologue: the first PD steps' quads, and quads of +0 into the padding
+    // diagonals above the top (see the forward sweep), in the loop's order;
+    // offsets count down from the top of each range
     {
         const int tl = Ttop - (NI - 1);
-        const Rsrc ry = rsrc_q(yq, qbytes, tl, step), rzq = rsrc_q(zp, qbytes, Ttop + 1, step);
+        const Rsrc rc = rsrc_q(kb, N * qbytes, tl, cstride), ry = rsrc_q(yq, qbytes, tl, step);
+        const Rsrc rzq = rsrc_q(zp, qbytes, Ttop + 1, step);
 #pragma unroll
         for (int k = 0; k < PD; ++k) {
             const int mq = k / 4;
             if (k % 4 == 3) bstore4(rzq, q, vq, (unsigned)mq * step);
             __builtin_amdgcn_sched_barrier(0);
-            if constexpr (kFused) {
-                ring[k] = bload_bk3(rsrc_at(B32, a.rbytes, tl, step), vo, (unsigned)(NI - 1 - k) * step);
-            } else if (k % 4 == 3) {
-                rr[mq] = bload_rq(rsrc_bq(Brec, 3 * qbytes, tl, step), vb, (unsigned)(NQ - 1 - mq) * 3 * step);
+            if (k % 4 == 3) {
+                bload_quad<T, N>(cr[mq], rc, vc, (unsigned)(NQ - 1 - mq) * cstride);
+                bload_quad<T, 1>(yr[mq], ry, vq, (unsigned)(NQ - 1 - mq) * step);
             }
-            if (k % 4 == 3) yr[mq] = bload_q(ry, vq, (unsigned)(NQ - 1 - mq) * step, (T *)nullptr);
         }
     }
     T zprev = 0, pub = 0;
@@ -661,7 +678,8 @@ __global__ void __launch_bounds__(64 * (W + 2))
     do {
         // this iteration's steps u = G0*kL + i: z of diagonal Ttop - u, refills of Ttop - u - PD
         const int tl = Ttop - G0 * kL - (NI - 1);  // lowest diagonal written this iteration
-        const Rsrc ry = rsrc_q(yq, qbytes, tl - PD, step), rzq = rsrc_q(zp, qbytes, tl, step);
+        const Rsrc rc = rsrc_q(kb, N * qbytes, tl - PD, cstride), ry = rsrc_q(yq, qbytes, tl - PD, step);
+        const Rsrc rzq = rsrc_q(zp, qbytes, tl, step);
 #pragma unroll
         for (int h = 0; h < kLook; ++h) {
             const int G = G0 + h;
@@ -669,29 +687,25 @@ __global__ void __launch_bounds__(64 * (W + 2))
 #pragma unroll
             for (int k = 0; k < kL; ++k) {
                 const int i = h * kL + k, mq = i / 4, wq = 3 - i % 4, rq = mq % NR;
+                const T *c = &cr[rq].v[N * wq];
                 const T zn_next = shr1z(zc);
                 const T zn = dpp_shl1(zprev, zc);
                 zc = zn_next;
                 const T y = yr[rq].v[wq];
                 T z;
-                if constexpr (kFused) {  // (-, Dinv, A_re Dinv, A_rn Dinv)
-                    const R c = ring[i % PD];
-                    z = __builtin_fmaf(-c.d, zn, __builtin_fmaf(-c.c, zprev, y * c.b));
-                } else {  // the reference's order of operations
-                    const double di = rr[rq].v[3 * wq], ae = rr[rq].v[3 * wq + 1], an = rr[rq].v[3 * wq + 2];
-                    z = ((y * di) - ae * zprev * di) - an * zn * di;
-                }
+                if constexpr (kFused)
+                    z = __builtin_fmaf(-c[2], zn, __builtin_fmaf(-c[1], zprev, y * c[0]));
+                else  // the reference's order of operations
+                    z = ((y * c[0]) - c[1] * zprev * c[0]) - c[2] * zn * c[0];
                 q[wq] = z;
                 if (wq == 0) bstore4(rzq, q, vq, (unsigned)(NQ - 1 - mq) * step);
                 if (k > 0) pub = dpp_shr1(pub, zprev);  // the previous step's value (its register is free now)
                 zprev = z;
                 __builtin_amdgcn_sched_barrier(0);
-                if constexpr (kFused) {
-                    ring[i % PD] = bload_bk3(rsrc_at(B32, a.rbytes, tl - PD, step), vo, (unsigned)(NI - 1 - i) * step);
-                } else if (wq == 0) {
-                    rr[rq] = bload_rq(rsrc_bq(Brec, 3 * qbytes, tl - PD, step), vb, (unsigned)(NQ - 1 - mq) * 3 * step);
+                if (wq == 0) {
+                    bload_quad<T, N>(cr[rq], rc, vc, (unsigned)(NQ - 1 - mq) * cstride);
+                    bload_quad<T, 1>(yr[rq], ry, vq, (unsigned)(NQ - 1 - mq) * step);
                 }
-                if (wq == 0) yr[rq] = bload_q(ry, vq, (unsigned)(NQ - 1 - mq) * step, (T *)nullptr);
             }
             pub = dpp_shr1(pub, zprev);
             hd.finish(G, h, pub);
@@ -704,16 +718,12 @@ __global__ void __launch_bounds__(64 * (W + 2))
 // ---------------------------------------------------------------------------
 // All-CU helpers: padding, packing, gather / scatter, FP32 copies
 // ---------------------------------------------------------------------------
-__global__ void k_skew_init(size_t ns, Rec4 *F, Rec4 *Bk, Rec4 *U, double *zp, double *yq, double *bq) {
+__global__ void k_skew_init(size_t ns, Rec4 *F, Rec4 *Bk, Rec4 *U) {
     const size_t s = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= ns) return;
     F[s] = Rec4{0.0, 1.0, 0.0, 0.0};
     Bk[s] = Rec4{0.0, 1.0, 0.0, 0.0};
     U[s] = Rec4{1.0, 0.0, 0.0, 0.0};
-    zp[s] = 0.0;
-    yq[s] = 0.0;
-#pragma unroll
-    for (int q = 0; q < 3; ++q) bq[3 * s + q] = 0.0;
 }
 
 template <typename T>
@@ -760,13 +770,14 @@ struct TileLds {
     T v[kTile][kTile + 1];
 };
 
-// x (natural) -> F.a; blocks beyond the tiles reset the sweeps' global
-// hand-over arrays.
+// x (natural) -> the forward sweep's right-hand side: FP64 into F.a, FP32
+// into quads (bq); padding slots keep their +0.  Blocks beyond the tiles
+// reset the sweeps' global hand-over arrays.
 template <typename T>
 __global__ void __launch_bounds__(256)
     k_skew_gather(int ntiles, const SkewTile *__restrict__ tiles, const SkewGrid *__restrict__ grids,
-                  const T *__restrict__ x, Rec4T<T> *__restrict__ F, T *__restrict__ hand, int nhand, int HS,
-                  const KScal *ks) {
+                  const T *__restrict__ x, Rec4T<T> *__restrict__ F, T *__restrict__ bq, T *__restrict__ hand,
+                  int nhand, int HS, const KScal *ks) {
     if (ks && ks->done) return;
     if ((int)blockIdx.x >= ntiles) {
         const int p = ((int)blockIdx.x - ntiles) * 256 + threadIdx.x;
@@ -790,8 +801,13 @@ __global__ void __launch_bounds__(256)
 #pragma unroll
     for (int it = 0; it < kTile / 2; ++it) {  // diagonal T = i0 + j0 + d, row j0 + lane
         const int d = wv + 4 * it, j = tl.j0 + lane, il = d - lane;
-        if (il >= 0 && il < kTile && tl.i0 + il < g.nx && j < g.ny)
-            F[g.base + (size_t)(tl.i0 + tl.j0 + d + kSkewD0) * g.S + j].a = t.v[lane][il];
+        const int Td = tl.i0 + tl.j0 + d, Tq = Td + kSkewQ0;
+        if (il >= 0 && il < kTile && tl.i0 + il < g.nx && j < g.ny) {
+            if constexpr (sizeof(T) == 8)
+                F[g.base + (size_t)(Td + kSkewD0) * g.S + j].a = t.v[lane][il];
+            else
+                bq[g.base + (size_t)(Tq & ~3) * g.S + 4 * (size_t)j + (Tq & 3)] = t.v[lane][il];
+        }
     }
 }
 
@@ -827,25 +843,6 @@ __global__ void __launch_bounds__(256)
     }
 }
 
-// FP32 records for the inner solves of mixed precision: the coefficients
-// premultiplied by their pivots in FP64 and rounded once,
-//   F32 = (b, A_rs Dinv_s, A_rw Dinv_w, 0),  Bk32 = (y, Dinv_r, A_re Dinv_r, A_rn Dinv_r),
-// the south / west pivots being those of slots s - S - 1 / s - S (grid
-// blockIdx.y; its first diagonal is padding).
-__global__ void k_skew_to32(const SkewGrid *__restrict__ grids, const Rec4 *__restrict__ F,
-                            const Rec4 *__restrict__ Bk, Rec4f *__restrict__ F32, Rec4f *__restrict__ Bk32) {
-    const SkewGrid g = grids[blockIdx.y];
-    const size_t nslot = ((size_t)g.NG * kSkewL + kSkewD0 + kSkewTop) * g.S;
-    const size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= nslot) return;
-    const size_t s = g.base + q;
-    const Rec4 u = F[s], v = Bk[s];
-    const double ds = q >= (size_t)g.S + 1 ? F[s - g.S - 1].b : 1.0;
-    const double dw = q >= (size_t)g.S ? F[s - g.S].b : 1.0;
-    F32[s] = Rec4f{0.f, (float)(u.c * ds), (float)(u.d * dw), 0.f};
-    Bk32[s] = Rec4f{0.f, (float)v.b, (float)(v.c * v.b), (float)(v.d * v.b)};
-}
-
 // The coefficients a cell does not have in the backward sweep (east of each
 // row's last cell, north of the top row) become copysign(0, Dinv), so that
 // their products vanish exactly (x - (+0) == x, x == -0 included).  One
@@ -864,28 +861,51 @@ __global__ void k_skew_edges(const SkewGrid *__restrict__ grids, Rec4 *__restric
     }
 }
 
-// FP64 backward quad records: (Dinv, A_re, A_rn) of Bk, twelve doubles per
-// row and quad of diagonals (quad (T + kSkewQ0)/4, word (T + kSkewQ0)%4),
-// for the diagonals T >= -kSkewPD the sweep reads.  One thread per row and
-// quad (four coalesced record loads, one contiguous 96-byte record);
-// blockIdx.y: the grid.
+// The sweeps' quad records from the records F and Bk (after the diagonal
+// recurrence and the edge coefficients): one thread per row and quad of
+// diagonals, for the diagonals T in [-kSkewPD, NG*kSkewL + kSkewPD) the
+// sweeps read (prefetches reach kSkewPD diagonals beyond either end).
+//   forward  FP32 (A_rs Dinv_s, A_rw Dinv_w)   (FP64 reads F itself)
+//   backward FP64 (Dinv, A_re, A_rn),  FP32 (Dinv, A_re Dinv, A_rn Dinv)
+// FP32 premultiplies in FP64 and rounds once.  Dinv_s / Dinv_w are the
+// pivots of slots (T-1, j-1) / (T-1, j); row 0's south slot is the previous
+// diagonal's last slot, a padding row (pivot 1).  blockIdx.y: the grid.
 static_assert(kSkewPD % 4 == 0 && (kSkewQ0 - kSkewPD) % 4 == 0, "quads start at a padding diagonal");
-__global__ void k_skew_bq(const SkewGrid *__restrict__ grids, const Rec4 *__restrict__ Bk, double *__restrict__ bq) {
+template <typename T>
+__global__ void k_skew_quads(const SkewGrid *__restrict__ grids, const Rec4 *__restrict__ F,
+                             const Rec4 *__restrict__ Bk, T *__restrict__ kf, T *__restrict__ kb) {
+    constexpr int NF = kFwdN, NB = kBwdN;
     const SkewGrid g = grids[blockIdx.y];
     const size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= ((size_t)g.NG * kSkewL + kSkewPD) / 4 * g.S) return;
+    if (q >= ((size_t)g.NG * kSkewL + 2 * kSkewPD) / 4 * g.S) return;
     const int qi = (int)(q / g.S) + (kSkewQ0 - kSkewPD) / 4, j = (int)(q % g.S);
-    double c[12];
+    const size_t s0 = g.base + (size_t)(4 * qi - kSkewQ0 + kSkewD0) * g.S + j;  // slot of (T, j), word 0
+    T cf[4 * NF], cb[4 * NB];
 #pragma unroll
     for (int w = 0; w < 4; ++w) {  // diagonal T = 4 qi + w - kSkewQ0
-        const Rec4 r = Bk[g.base + (size_t)(4 * qi + w - kSkewQ0 + kSkewD0) * g.S + j];
-        c[3 * w] = r.b;
-        c[3 * w + 1] = r.c;
-        c[3 * w + 2] = r.d;
+        const size_t s = s0 + (size_t)w * g.S;
+        const Rec4 u = F[s], v = Bk[s];
+        const double ds = F[s - g.S - 1].b, dw = F[s - g.S].b;
+        if constexpr (sizeof(T) == 8) {
+            (void)u, (void)ds, (void)dw;
+            cb[3 * w] = v.b;
+            cb[3 * w + 1] = v.c;
+            cb[3 * w + 2] = v.d;
+        } else {
+            cf[2 * w] = (float)(u.c * ds);
+            cf[2 * w + 1] = (float)(u.d * dw);
+            cb[3 * w] = (float)v.b;
+            cb[3 * w + 1] = (float)(v.c * v.b);
+            cb[3 * w + 2] = (float)(v.d * v.b);
+        }
     }
-    double2 *o = reinterpret_cast<double2 *>(bq + 3 * (g.base + (size_t)qi * 4 * g.S + 4 * (size_t)j));
+    const size_t o = g.base + (size_t)qi * 4 * g.S + 4 * (size_t)j;  // the quad's first value
+    if constexpr (sizeof(T) == 4) {
 #pragma unroll
-    for (int t = 0; t < 6; ++t) o[t] = double2{c[2 * t], c[2 * t + 1]};
+        for (int t = 0; t < 4 * NF; ++t) kf[NF * o + t] = cf[t];
+    }
+#pragma unroll
+    for (int t = 0; t < 4 * NB; ++t) kb[NB * o + t] = cb[t];
 }
 
 __global__ void k_skew_dinv_natural(int n, const int *__restrict__ slot_of, const Rec4 *__restrict__ F,
@@ -1004,17 +1024,43 @@ static int skew_alloc(fvh_eqn *e) {
     FVH_TRY(s.F.alloc(L.NS));
     FVH_TRY(s.Bk.alloc(L.NS));
     FVH_TRY(s.U.alloc(L.NS));
-    FVH_TRY(s.zp.alloc(L.NS));  // quads
-    FVH_TRY(s.yq.alloc(L.NS));
-    FVH_TRY(s.bq.alloc(3 * L.NS));
-    FVH_TRY(s.hand.alloc((size_t)L.nblk[0] * 2 * L.HS));  // W = 1 has the most blocks
-    FVH_TRY(s.uhand.alloc((size_t)L.nblk[0] * L.HS));
+    FVH_TRY(s.uhand.alloc((size_t)L.nblk[0] * L.HS));  // W = 1 has the most blocks
     if (!s.h_fault) {
         HIP_TRY(hipHostMalloc((void **)&s.h_fault, sizeof(int), hipHostMallocDefault));
         *s.h_fault = 0;
     }
-    hipLaunchKernelGGL(k_skew_init, grid_for(L.NS), dim3(256), 0, stream(), L.NS, s.F.p, s.Bk.p, s.U.p, s.zp.p,
-                       s.yq.p, s.bq.p);
+    hipLaunchKernelGGL(k_skew_init, grid_for(L.NS), dim3(256), 0, stream(), L.NS, s.F.p, s.Bk.p, s.U.p);
+    HIP_TRY(hipGetLastError());
+    return FVH_OK;
+}
+
+// A precision's sweep buffers, all +0 at first (the padding slots of bq keep
+// it: the gather writes only cells).
+template <typename T>
+static int sweep_alloc(SkewSweep<T> &w, const SkewLayout &L) {
+    if (w.zp.n == L.NS) return FVH_OK;
+    const bool q32 = sizeof(T) == 4;  // FP64's forward sweep reads F
+    FVH_TRY(w.kf.alloc(q32 ? kFwdN * L.NS : 0));
+    FVH_TRY(w.kb.alloc(kBwdN * L.NS));
+    FVH_TRY(w.bq.alloc(q32 ? L.NS : 0));
+    FVH_TRY(w.yq.alloc(L.NS));
+    FVH_TRY(w.zp.alloc(L.NS));
+    FVH_TRY(w.hand.alloc((size_t)L.nblk[0] * 2 * L.HS));  // W = 1 has the most blocks
+    for (DBuf<T> *b : {&w.kf, &w.kb, &w.bq, &w.yq, &w.zp})
+        hipLaunchKernelGGL(k_skew_fill<T>, grid_for(b->n), dim3(256), 0, stream(), b->n, b->p, T(0));
+    HIP_TRY(hipGetLastError());
+    return FVH_OK;
+}
+
+// The quad records of one precision from the current F and Bk.
+template <typename T>
+static int sweep_records(fvh_eqn *e, SkewSweep<T> &w) {
+    const SkewLayout &L = *e->m->skew;
+    FVH_TRY(sweep_alloc(w, L));
+    size_t most = 0;
+    for (const SkewGrid &g : L.h_grids) most = std::max(most, ((size_t)g.NG * kSkewL + 2 * kSkewPD) / 4 * g.S);
+    hipLaunchKernelGGL((k_skew_quads<T>), dim3((unsigned)((most + 255) / 256), (unsigned)L.h_grids.size()), dim3(256),
+                       0, stream(), L.grids.p, e->sk.F.p, e->sk.Bk.p, w.kf.p, w.kb.p);
     HIP_TRY(hipGetLastError());
     return FVH_OK;
 }
@@ -1067,33 +1113,30 @@ int skew_update(fvh_eqn *e, const double *val) {
     hipLaunchKernelGGL(k_skew_edges, dim3((unsigned)((edge + 255) / 256), (unsigned)L.h_grids.size()), dim3(256), 0,
                        stream(), L.grids.p, e->sk.Bk.p);
     HIP_TRY(hipGetLastError());
-    size_t most = 0;
-    for (const SkewGrid &g : L.h_grids) most = std::max(most, ((size_t)g.NG * kSkewL + kSkewPD) / 4 * g.S);
-    hipLaunchKernelGGL(k_skew_bq, dim3((unsigned)((most + 255) / 256), (unsigned)L.h_grids.size()), dim3(256), 0,
-                       stream(), L.grids.p, e->sk.Bk.p, e->sk.bq.p);
-    HIP_TRY(hipGetLastError());
+    FVH_TRY(sweep_records(e, e->sk.s64));
     if (!m->skew->W && !forced_waves()) FVH_TRY(skew_tune(e));
     return FVH_OK;
 }
 
 template <typename T, int W>
-static int launch_sweeps_w(fvh_eqn *e, Rec4T<T> *F, const void *Brec, T *yq, T *zp, T *hand, const KScal *ks) {
+static int launch_sweeps_w(fvh_eqn *e, SkewSweep<T> &w, const KScal *ks) {
     const SkewLayout &L = *e->m->skew;
     const SkewArgs a = skew_args<Rec4T<T>>(e, W);
     const size_t lds = (size_t)(W + 1) * L.HS * sizeof(T);
     FVH_TRY(allow_lds(k_skew_fwd<T, W>, lds));
     FVH_TRY(allow_lds(k_skew_bwd<T, W>, lds));
-    hipLaunchKernelGGL((k_skew_fwd<T, W>), dim3(a.nblk), dim3(64 * (W + 2)), lds, stream(), a, F, yq, hand, ks);
+    const void *recs = sizeof(T) == 8 ? static_cast<const void *>(e->sk.F.p) : w.kf.p;
+    hipLaunchKernelGGL((k_skew_fwd<T, W>), dim3(a.nblk), dim3(64 * (W + 2)), lds, stream(), a, recs, w.bq.p, w.yq.p,
+                       w.hand.p, ks);
     HIP_TRY(hipGetLastError());
-    hipLaunchKernelGGL((k_skew_bwd<T, W>), dim3(a.nblk), dim3(64 * (W + 2)), lds, stream(), a, Brec, (const T *)yq, zp,
-                       hand, ks);
+    hipLaunchKernelGGL((k_skew_bwd<T, W>), dim3(a.nblk), dim3(64 * (W + 2)), lds, stream(), a, w.kb.p, w.yq.p, w.zp.p,
+                       w.hand.p, ks);
     HIP_TRY(hipGetLastError());
     return FVH_OK;
 }
 
 template <typename T>
-static int skew_apply_t(fvh_eqn *e, const T *x, T *y, const KScal *ks, Rec4T<T> *F, const void *Brec, T *yq, T *zp,
-                        T *hand) {
+static int skew_apply_t(fvh_eqn *e, const T *x, T *y, const KScal *ks, SkewSweep<T> &w) {
     fvh_mesh *m = e->m;
     const SkewLayout &L = *m->skew;
     const int nhand = 2 * L.nblk[0] * L.HS;
@@ -1102,14 +1145,15 @@ static int skew_apply_t(fvh_eqn *e, const T *x, T *y, const KScal *ks, Rec4T<T> 
     const KScal *ks_all = m->cases ? nullptr : ks;
     const int ntiles = (int)L.tiles.n;
     hipLaunchKernelGGL((k_skew_gather<T>), dim3(ntiles + (nhand + 255) / 256), dim3(256), 0, stream(), ntiles,
-                       L.tiles.p, L.grids.p, x, F, hand, nhand, L.HS, ks_all);
+                       L.tiles.p, L.grids.p, x, sizeof(T) == 8 ? reinterpret_cast<Rec4T<T> *>(e->sk.F.p) : nullptr,
+                       w.bq.p, w.hand.p, nhand, L.HS, ks_all);
     HIP_TRY(hipGetLastError());
     switch (skew_waves(L, sizeof(T) == 4)) {
-        case 1: FVH_TRY((launch_sweeps_w<T, 1>(e, F, Brec, yq, zp, hand, ks))); break;
-        case 2: FVH_TRY((launch_sweeps_w<T, 2>(e, F, Brec, yq, zp, hand, ks))); break;
-        default: FVH_TRY((launch_sweeps_w<T, 3>(e, F, Brec, yq, zp, hand, ks))); break;
+        case 1: FVH_TRY((launch_sweeps_w<T, 1>(e, w, ks))); break;
+        case 2: FVH_TRY((launch_sweeps_w<T, 2>(e, w, ks))); break;
+        default: FVH_TRY((launch_sweeps_w<T, 3>(e, w, ks))); break;
     }
-    hipLaunchKernelGGL((k_skew_scatter_q<T>), dim3(ntiles), dim3(256), 0, stream(), L.tiles.p, L.grids.p, zp, y,
+    hipLaunchKernelGGL((k_skew_scatter_q<T>), dim3(ntiles), dim3(256), 0, stream(), L.tiles.p, L.grids.p, w.zp.p, y,
                        ks_all);
     HIP_TRY(hipGetLastError());
     return FVH_OK;
@@ -1119,7 +1163,7 @@ static int skew_apply_t(fvh_eqn *e, const T *x, T *y, const KScal *ks, Rec4T<T> 
 // width on the freshly packed records (the sweeps only rewrite their own
 // scratch fields); the width never changes a result.
 template <typename T>
-static int skew_tune_t(fvh_eqn *e, Rec4T<T> *F, const void *Brec, T *yq, T *zp, T *hand, int &W_out) {
+static int skew_tune_t(fvh_eqn *e, SkewSweep<T> &w, int &W_out) {
     SkewLayout &L = *e->m->skew;
     const int n = (int)e->m->n;
     DBuf<T> x, y;
@@ -1132,10 +1176,10 @@ static int skew_tune_t(fvh_eqn *e, Rec4T<T> *F, const void *Brec, T *yq, T *zp, 
     float best = -1.f;
     int bestW = 1;
     for (int W = 1; W <= lds_waves_max(L); ++W) {
-        W_out = W;
-        FVH_TRY(skew_apply_t<T>(e, x.p, y.p, nullptr, F, Brec, yq, zp, hand));  // warm
+        W_out = W;  // read by skew_waves
+        FVH_TRY(skew_apply_t<T>(e, x.p, y.p, nullptr, w));  // warm
         HIP_TRY(hipEventRecord(a, stream()));
-        for (int rep = 0; rep < 3; ++rep) FVH_TRY(skew_apply_t<T>(e, x.p, y.p, nullptr, F, Brec, yq, zp, hand));
+        for (int rep = 0; rep < 3; ++rep) FVH_TRY(skew_apply_t<T>(e, x.p, y.p, nullptr, w));
         HIP_TRY(hipEventRecord(b, stream()));
         HIP_TRY(hipEventSynchronize(b));
         float t = 0.f;
@@ -1153,44 +1197,26 @@ static int skew_tune_t(fvh_eqn *e, Rec4T<T> *F, const void *Brec, T *yq, T *zp, 
 }
 
 static int skew_tune(fvh_eqn *e) {
-    SkewStore &s = e->sk;
-    return skew_tune_t<double>(e, s.F.p, s.bq.p, s.yq.p, s.zp.p, s.hand.p, e->m->skew->W);
+    return skew_tune_t<double>(e, e->sk.s64, e->m->skew->W);
 }
 
 int skew_apply(fvh_eqn *e, const double *x, double *y, const KScal *ks) {
     SkewStore &s = e->sk;
-    if (!s.F.p) return set_error(FVH_ERR_ARG, "DILU records not prepared");
-    return skew_apply_t<double>(e, x, y, ks, s.F.p, s.bq.p, s.yq.p, s.zp.p, s.hand.p);
+    if (!s.s64.kf.p) return set_error(FVH_ERR_ARG, "DILU records not prepared");
+    return skew_apply_t<double>(e, x, y, ks, s.s64);
 }
 
 int skew_to32(fvh_eqn *e) {
-    SkewLayout &L = *e->m->skew;
     SkewStore &s = e->sk;
-    if (s.F32.n != L.NS) {
-        FVH_TRY(s.F32.alloc(L.NS));
-        FVH_TRY(s.Bk32.alloc(L.NS));
-        FVH_TRY(s.zp32.alloc(L.NS));
-        FVH_TRY(s.yq32.alloc(L.NS));
-        hipLaunchKernelGGL(k_skew_fill<float>, grid_for(L.NS), dim3(256), 0, stream(), L.NS, s.yq32.p, 0.f);
-        FVH_TRY(s.hand32.alloc((size_t)L.nblk[0] * 2 * L.HS));
-        hipLaunchKernelGGL(k_skew_fill<float>, grid_for(L.NS), dim3(256), 0, stream(), L.NS, s.zp32.p, 0.f);
-        HIP_TRY(hipGetLastError());
-    }
-    size_t most = 0;
-    for (const SkewGrid &g : L.h_grids)
-        most = std::max(most, ((size_t)g.NG * kSkewL + kSkewD0 + kSkewTop) * g.S);
-    hipLaunchKernelGGL(k_skew_to32, dim3((unsigned)((most + 255) / 256), (unsigned)L.h_grids.size()), dim3(256), 0,
-                       stream(), L.grids.p, s.F.p, s.Bk.p, s.F32.p, s.Bk32.p);
-    HIP_TRY(hipGetLastError());
-    if (!L.W32 && !forced_waves())
-        FVH_TRY(skew_tune_t<float>(e, s.F32.p, s.Bk32.p, s.yq32.p, s.zp32.p, s.hand32.p, e->m->skew->W32));
+    FVH_TRY(sweep_records(e, s.s32));
+    if (!e->m->skew->W32 && !forced_waves()) FVH_TRY(skew_tune_t<float>(e, s.s32, e->m->skew->W32));
     return FVH_OK;
 }
 
 int skew_apply32(fvh_eqn *e, const float *x, float *y, const KScal *ks) {
     SkewStore &s = e->sk;
-    if (!s.F32.p) return set_error(FVH_ERR_ARG, "FP32 sweep records not prepared");
-    return skew_apply_t<float>(e, x, y, ks, s.F32.p, s.Bk32.p, s.yq32.p, s.zp32.p, s.hand32.p);
+    if (!s.s32.kf.p) return set_error(FVH_ERR_ARG, "FP32 sweep records not prepared");
+    return skew_apply_t<float>(e, x, y, ks, s.s32);
 }
 
 int skew_dinv_natural(fvh_eqn *e, double *out) {
