@@ -219,31 +219,20 @@ __device__ inline void bload_quad(Vec<T, 4 * N> &c, Rsrc r, unsigned vo, unsigne
     for (int t = 0; t < 4 * N * (int)sizeof(T) / 16; ++t) bload_piece(c, t, r, vo, so);
 }
 
-// Whole-wave lane shifts for the hand-over batches: the next batch position
-// moves to the edge lane (the vacated lane reads 0, never used).
+// Whole-wave lane shift (the FP64 forward sweep's lane-0 pivots): the next
+// value moves to lane 0 (the vacated lane reads 0, never used).
 __device__ inline double shl1z(double v) {
     const int lo = __builtin_amdgcn_mov_dpp(__double2loint(v), 0x130, 0xF, 0xF, true);
     const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), 0x130, 0xF, 0xF, true);
     return __hiloint2double(hi, lo);
-}
-__device__ inline double shr1z(double v) {
-    const int lo = __builtin_amdgcn_mov_dpp(__double2loint(v), 0x138, 0xF, 0xF, true);
-    const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), 0x138, 0xF, 0xF, true);
-    return __hiloint2double(hi, lo);
-}
-__device__ inline float shl1z(float v) {
-    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x130, 0xF, 0xF, true));
-}
-__device__ inline float shr1z(float v) {
-    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x138, 0xF, 0xF, true));
 }
 
 // Resets the block's LDS hand-over arrays ([W+1][HS]; array c lies below
 // compute wave c): index 0 holds the padding value of step -1, the others
 // the sentinel, except array `open`, which no wave or courier writes (the
 // source of the grid's first wave): it holds the padding value throughout.
-// Words [HS-64, HS) of each array are a dummy area beyond every batch index:
-// the lanes that write no batch word write there, so that the hand-over
+// Words [HS - kDummy, HS) of each array are a dummy area beyond every batch
+// index: the lanes that write no batch write there, so that the hand-over
 // store needs no lane mask.
 template <typename T>
 __device__ inline void lds_reset(T *lds, int count, int HS, T first, int open) {
@@ -323,57 +312,91 @@ __device__ void courier_out(const T *lout, T *gout, int NG, int *fault) {
 }
 
 // ---------------------------------------------------------------------------
-// Per-group hand-over, shared by the three sweeps.  A compute wave reads the
-// batch of group G+kLook at the end of group G (kLook groups ahead of use),
-// tests the batch of group G+1 for sentinels at the start of group G (the
-// compare's latency then hides behind a group of steps; the branch to the
-// out-of-line wait is rarely taken), and writes its own batch at the end of
-// each group with every lane storing (lanes without a batch word store into
-// the dummy area), so no lane mask or branch sits in the step sequence.
-// The prologue issues the same LDS pattern as a group end, which keeps the
-// compiler's LDS waits at the true distance.
+// Per-group hand-over, shared by the three sweeps.  A wave's edge lane (the
+// top lane forward, the bottom lane backward) keeps its kL results of a group
+// and writes them as one batch at the group's end; every other lane writes
+// its own kL words of the dummy area, so the store needs no lane mask.  The
+// wave on the other side checks a batch with one word per lane (lanes q < kL,
+// read kLook groups ahead, tested a group ahead, so the compare's latency
+// hides behind a group of steps and the branch to the out-of-line wait is
+// rarely taken), then reads the checked batch into registers a group ahead
+// of use, the same kL words in every lane: step k of the group takes word k
+// as its edge lane's neighbour value, the `old` operand of the step's DPP
+// shift, and no lane shift moves the batch along.
 // ---------------------------------------------------------------------------
 // Batch reads reach index (NG + kLook) * kL - 1, courier windows
 // NG * kL + kWin - kL; the arrays are sized (HS, in skew_setup) to keep
-// both below the dummy area [HS-64, HS).
+// both below the dummy area [HS - kDummy, HS): kL words per lane at a
+// stride of kL + 1 words (odd, so a store's lanes hit distinct banks).
 constexpr int kLook = 4;
+constexpr int kDummy = 64 * (kL + 1);
+static_assert(kLook % 2 == 0, "input batches alternate between two register sets with the group's parity");
 
 template <typename T>
 struct Hand {
-    const T *rd;       // this lane's read word of group 0 (lin + ql)
-    T *wr;             // this lane's write word of group 0 (batch word or dummy)
-    int wstep;         // words the write word advances per group (kL or 0)
-    T yb[kLook];       // batches of the next kLook groups
-    unsigned long long bad;  // lanes of the next group's batch still holding a sentinel
+    const T *rd;       // this lane's check word of group 0 (lin + q)
+    const T *lin;      // the input array
+    T *wr;             // this lane's first write word of group 0 (batch word 1, or its dummy words)
+    int wstep;         // words the write words advance per group (kL or 0)
+    T yb[kLook];       // check words of the next kLook groups
+    T in[2][kL];       // the batches of the current and the next group, by G % 2
+    unsigned long long bad;  // lanes of the next group's check words still holding a sentinel
 
-    __device__ inline void start(const T *lin, T *lout, int HS, int q, bool writer, int widx) {
+    __device__ inline void start(const T *lin_, T *lout, int HS, int q, bool writer) {
         const int lane = threadIdx.x & 63;
+        lin = lin_;
         rd = lin + q;
-        wr = writer ? lout + widx : lout + HS - 64 + lane;
+        T *dummy = lout + HS - kDummy + (kL + 1) * lane;
+        wr = writer ? lout + 1 : dummy;
         wstep = writer ? kL : 0;
-        T *dummy = lout + HS - 64 + lane;
 #pragma unroll
-        for (int h = 0; h < kLook; ++h) {
-            lstore(dummy, T(0));
-            yb[h] = lload(rd + h * kL);
-        }
+        for (int h = 0; h < kLook; ++h) yb[h] = lload(rd + h * kL);
         bad = __builtin_amdgcn_ballot_w64(is_sentinel(yb[0]));
+        if (!bad) read(0, 0);
+        wait_lds();
+        // the LDS pattern of a group's end (kL words written, one check
+        // word read), into the dummy words, so that the loop's LDS waits
+        // count the same operations on entry as on every later iteration
+#pragma unroll
+        for (int k = 0; k < kL; ++k) dummy[k] = T(0);
+        yb[kLook - 1] = lload(rd + (kLook - 1) * kL);
     }
-    // the batch of group G (slot h = G % kLook), waited for if needed
-    __device__ inline T take(int G, int h, int &broken) {
-        T v = yb[h];
+    // all LDS operations done (ends the paths that re-read a batch, so that
+    // the paths agree on what is still in flight)
+    __device__ static inline void wait_lds() { __builtin_amdgcn_s_waitcnt(0xC07F); }
+    // the whole batch of group G, in every lane (after its check)
+    __device__ inline void read(int G, int slot) {
+        __atomic_signal_fence(__ATOMIC_SEQ_CST);  // not above the check
+        const T *p = lin + G * kL;
+#pragma unroll
+        for (int k = 0; k < kL; ++k) in[slot][k] = p[k];
+    }
+    // makes the batch of group G (slot h % 2, h = G % kLook) ready, waiting
+    // for it if its check failed, and reads the next group's batch when its
+    // check passes
+    __device__ inline void take(int G, int h, int &broken) {
         if (bad) {
-            v = wait_batch(rd + G * kL, v, broken);
-            if (__builtin_amdgcn_ballot_w64(is_sentinel(v))) {  // lost: the fault is recorded after the walk
-                broken = 1;
-                v = is_sentinel(v) ? T(0) : v;
-            }
+            const T v = wait_batch(rd + G * kL, yb[h], broken);
+            if (__builtin_amdgcn_ballot_w64(is_sentinel(v))) broken = 1;  // lost: the fault is recorded after the walk
+            read(G, h % 2);
+            wait_lds();
         }
         bad = __builtin_amdgcn_ballot_w64(is_sentinel(yb[(h + 1) % kLook]));
-        return v;
+        if (!bad) read(G + 1, (h + 1) % 2);
     }
-    __device__ inline void finish(int G, int h, T pub) {
-        lstore(wr, pub);
+    // DPP shift by one lane whose edge lane takes the neighbour value of
+    // step k of group G (h = G % kLook); the result replaces the spent
+    // input, so the shift writes the input's own registers
+    template <bool kUp>
+    __device__ inline T shift_in(T v, int h, int k) {
+        T &x = in[h % 2][k];
+        x = kUp ? dpp_shr1(v, x) : dpp_shl1(v, x);
+        return x;
+    }
+    // publishes the group's results (the edge lane's are the batch)
+    __device__ inline void finish(int G, int h, const T (&res)[kL]) {
+#pragma unroll
+        for (int k = 0; k < kL; ++k) wr[k] = res[k];
         wr += wstep;
         yb[h] = lload(rd + (G + kLook) * kL);
     }
@@ -410,7 +433,7 @@ __global__ void __launch_bounds__(64 * (W + 2))
     constexpr int M = ring_batches<double>(), PD = M * kL;
     static_assert(kLook % M == 0, "whole rings per unrolled block");
     Hand<double> hd;
-    hd.start(lds + w * HS, lds + (w + 1) * HS, HS, min(lane, kL - 1), lane >= 64 - kL, 1 + lane - (64 - kL));
+    hd.start(lds + w * HS, lds + (w + 1) * HS, HS, min(lane, kL - 1), lane == 63);
     // the prologue issues what a step issues (two stores of the pivot 1.0
     // that the pre-start padding holds already, one record load), so that the
     // loop's memory waits count the same outstanding operations every step
@@ -426,7 +449,7 @@ __global__ void __launch_bounds__(64 * (W + 2))
             __builtin_amdgcn_sched_barrier(0);  // issue in consumption order
         }
     }
-    double dprev = 1.0, pub = 1.0;
+    double dprev = 1.0, res[kL];
     int broken = 0;
     int G0 = 0;
     do {
@@ -436,26 +459,22 @@ __global__ void __launch_bounds__(64 * (W + 2))
 #pragma unroll
         for (int h = 0; h < kLook; ++h) {
             const int G = G0 + h;
-            double cur = hd.take(G, h, broken);
+            hd.take(G, h, broken);
 #pragma unroll
             for (int k = 0; k < kL; ++k) {
                 const int r = (h % M) * kL + k;
                 const unsigned si = (unsigned)(h * kL + k) * step;
                 const Rec4 u = ring[r];
-                const double nxt = shl1z(cur);
-                const double ds = dpp_shr1(dprev, cur);
-                cur = nxt;
+                const double ds = hd.template shift_in<true>(dprev, h, k);
                 const double d = (u.a - u.b / ds) - u.c / dprev;
                 const double di = 1.0 / d;
                 bstore(rf, di, vo + 8, si);
                 bstore(rbk, di, vo + 8, si);
-                if (k > 0) pub = dpp_shl1(pub, dprev);  // the previous step's D (its register is free now)
-                dprev = d;
+                dprev = res[k] = d;
                 __builtin_amdgcn_sched_barrier(0);  // the slot is consumed: refill it below
                 ring[r] = bload(ru, vo, si, (Rec4 *)nullptr);
             }
-            pub = dpp_shl1(pub, dprev);
-            hd.finish(G, h, pub);
+            hd.finish(G, h, res);
         }
         G0 += kLook;
     } while (G0 < g.NG);  // NG >= 1: no guard between the prologue and the loop
@@ -524,7 +543,7 @@ __global__ void __launch_bounds__(64 * (W + 2))
     const T *dp = &F[g.base + (size_t)(ql - 1 + kSkewD0) * S + j0 - 1].b;
     const size_t hstep = (size_t)kL * S * 4;  // one batch of steps, in T
     Hand<T> hd;
-    hd.start(lds + w * HS, lds + (w + 1) * HS, HS, ql, lane >= 64 - kL, 1 + lane - (64 - kL));
+    hd.start(lds + w * HS, lds + (w + 1) * HS, HS, ql, lane == 63);
     T db[kLook];  // lane-0 pivots of the next kLook groups (FP64)
 #pragma unroll
     for (int h = 0; h < kLook; ++h) db[h] = kFused ? T(0) : dp[h * hstep];
@@ -553,7 +572,7 @@ __global__ void __launch_bounds__(64 * (W + 2))
             }
         }
     }
-    T xprev = 0, dprev = 1, pub = 0;
+    T xprev = 0, dprev = 1, res[kL];
     int broken = 0;
     int G0 = 0;
     do {
@@ -564,13 +583,12 @@ __global__ void __launch_bounds__(64 * (W + 2))
 #pragma unroll
         for (int h = 0; h < kLook; ++h) {
             const int G = G0 + h;
-            T yc = hd.take(G, h, broken), dc = db[h];
+            hd.take(G, h, broken);
+            T dc = db[h];
 #pragma unroll
             for (int k = 0; k < kL; ++k) {
                 const int i = h * kL + k, mq = i / 4, wq = i % 4;
-                const T yn = shl1z(yc);
-                const T xs = dpp_shr1(xprev, yc);
-                yc = yn;
+                const T xs = hd.template shift_in<true>(xprev, h, k);
                 T x;
                 if constexpr (kFused) {  // (A_rs Dinv_s, A_rw Dinv_w) and b
                     const T *c = &cr[mq % NR].v[N * wq];
@@ -585,8 +603,7 @@ __global__ void __launch_bounds__(64 * (W + 2))
                 }
                 q[wq] = x;
                 if (wq == 3) bstore4(rq, q, vq, (unsigned)mq * step);
-                if (k > 0) pub = dpp_shl1(pub, xprev);  // the previous step's value (its register is free now)
-                xprev = x;
+                xprev = res[k] = x;
                 __builtin_amdgcn_sched_barrier(0);
                 if constexpr (kFused) {
                     if (wq == 3) {
@@ -597,8 +614,7 @@ __global__ void __launch_bounds__(64 * (W + 2))
                     ring[i % PD] = bload(rf, vo, (unsigned)i * step, (R *)nullptr);
                 }
             }
-            pub = dpp_shl1(pub, xprev);
-            hd.finish(G, h, pub);
+            hd.finish(G, h, res);
             if constexpr (!kFused) db[h] = *dp;
             dp += hstep;
         }
@@ -643,7 +659,7 @@ __global__ void __launch_bounds__(64 * (W + 2))
     constexpr int M = ring_batches<T>(), PD = M * kL;
     static_assert(kLook % M == 0, "whole rings per unrolled block");
     Hand<T> hd;
-    hd.start(lds + (w + 1) * HS, lds + w * HS, HS, qh, lane < kL, kL - lane);
+    hd.start(lds + (w + 1) * HS, lds + w * HS, HS, qh, lane == 0);
     // Ttop + 1 + kSkewQ0 is a multiple of 4, so step i of an iteration is
     // word 3 - i%4 of the iteration's quad i/4 (four diagonals, descending)
     constexpr bool kFused = sizeof(T) == 4;
@@ -672,7 +688,7 @@ __global__ void __launch_bounds__(64 * (W + 2))
             }
         }
     }
-    T zprev = 0, pub = 0;
+    T zprev = 0, res[kL];
     int broken = 0;
     int G0 = 0;
     do {
@@ -683,14 +699,12 @@ __global__ void __launch_bounds__(64 * (W + 2))
 #pragma unroll
         for (int h = 0; h < kLook; ++h) {
             const int G = G0 + h;
-            T zc = hd.take(G, h, broken);
+            hd.take(G, h, broken);
 #pragma unroll
             for (int k = 0; k < kL; ++k) {
                 const int i = h * kL + k, mq = i / 4, wq = 3 - i % 4, rq = mq % NR;
                 const T *c = &cr[rq].v[N * wq];
-                const T zn_next = shr1z(zc);
-                const T zn = dpp_shl1(zprev, zc);
-                zc = zn_next;
+                const T zn = hd.template shift_in<false>(zprev, h, k);
                 const T y = yr[rq].v[wq];
                 T z;
                 if constexpr (kFused)
@@ -699,16 +713,14 @@ __global__ void __launch_bounds__(64 * (W + 2))
                     z = ((y * c[0]) - c[1] * zprev * c[0]) - c[2] * zn * c[0];
                 q[wq] = z;
                 if (wq == 0) bstore4(rzq, q, vq, (unsigned)(NQ - 1 - mq) * step);
-                if (k > 0) pub = dpp_shr1(pub, zprev);  // the previous step's value (its register is free now)
-                zprev = z;
+                zprev = res[k] = z;
                 __builtin_amdgcn_sched_barrier(0);
                 if (wq == 0) {
                     bload_quad<T, N>(cr[rq], rc, vc, (unsigned)(NQ - 1 - mq) * cstride);
                     bload_quad<T, 1>(yr[rq], ry, vq, (unsigned)(NQ - 1 - mq) * step);
                 }
             }
-            pub = dpp_shr1(pub, zprev);
-            hd.finish(G, h, pub);
+            hd.finish(G, h, res);
         }
         G0 += kLook;
     } while (G0 < g.NG);
@@ -942,7 +954,7 @@ int skew_setup(fvh_mesh *m, const std::vector<SkewRect> &rects) {
     for (const SkewRect &r : rects) maxNG = std::max(maxNG, ng_of(r));
     // hand-over arrays: indices up to NG*kL, read a window (kWin) or a ring of
     // batches (at most 4) past the last one
-    const int HS = (maxNG * kL + kWin + kLook * kL + 64 + 63) / 64 * 64;
+    const int HS = (maxNG * kL + kWin + kLook * kL + kDummy + 63) / 64 * 64;
     if ((size_t)2 * HS * sizeof(double) > (size_t)max_lds_bytes()) return FVH_OK;  // too long even for W = 1
     auto L = std::make_unique<SkewLayout>();
     L->HS = HS;
