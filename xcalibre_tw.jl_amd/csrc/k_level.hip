@@ -118,12 +118,15 @@ __device__ inline T lev_eval(const LevRec<T> &r, T in, const T (&nb)[kLevDeps]) 
         return d;
     } else if (MODE == 1) {  // y_i = x_i - sum (A_ij y_j) Dinv_j
         T s = in;
+        if constexpr (sizeof(T) == 4) {  // a = A_ij Dinv_j; fused, the highest column first
 #pragma unroll
-        for (int q = 0; q < kLevDeps; ++q)
-            if (q < r.m) {
-                if constexpr (sizeof(T) == 4) s = __builtin_fmaf(-r.a[q], nb[q], s);  // a = A_ij Dinv_j
-                else s = s - r.a[q] * nb[q] * r.d[q];
-            }
+            for (int q = kLevDeps - 1; q >= 0; --q)
+                if (q < r.m) s = __builtin_fmaf(-r.a[q], nb[q], s);
+        } else {
+#pragma unroll
+            for (int q = 0; q < kLevDeps; ++q)
+                if (q < r.m) s = s - r.a[q] * nb[q] * r.d[q];
+        }
         return s;
     } else {  // z_i = y_i Dinv_i - sum (A_ij z_j) Dinv_i
         const T di = r.d[0];

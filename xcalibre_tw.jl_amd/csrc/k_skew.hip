@@ -352,7 +352,7 @@ struct Hand {
 #pragma unroll
         for (int h = 0; h < kLook; ++h) yb[h] = lload(rd + h * kL);
         bad = __builtin_amdgcn_ballot_w64(is_sentinel(yb[0]));
-        if (!bad) read(0, 0);
+        read(0, 0);
         wait_lds();
         // the LDS pattern of a group's end (kL words written, one check
         // word read), into the dummy words, so that the loop's LDS waits
@@ -372,8 +372,9 @@ struct Hand {
         for (int k = 0; k < kL; ++k) in[slot][k] = p[k];
     }
     // makes the batch of group G (slot h % 2, h = G % kLook) ready, waiting
-    // for it if its check failed, and reads the next group's batch when its
-    // check passes
+    // for it and reading it again if its check failed, and reads the next
+    // group's batch (read unconditionally: a batch that fails its check is
+    // read again after the wait)
     __device__ inline void take(int G, int h, int &broken) {
         if (bad) {
             const T v = wait_batch(rd + G * kL, yb[h], broken);
@@ -382,7 +383,7 @@ struct Hand {
             wait_lds();
         }
         bad = __builtin_amdgcn_ballot_w64(is_sentinel(yb[(h + 1) % kLook]));
-        if (!bad) read(G + 1, (h + 1) % 2);
+        read(G + 1, (h + 1) % 2);
     }
     // DPP shift by one lane whose edge lane takes the neighbour value of
     // step k of group G (h = G % kLook); the result replaces the spent
@@ -590,9 +591,10 @@ __global__ void __launch_bounds__(64 * (W + 2))
                 const int i = h * kL + k, mq = i / 4, wq = i % 4;
                 const T xs = hd.template shift_in<true>(xprev, h, k);
                 T x;
-                if constexpr (kFused) {  // (A_rs Dinv_s, A_rw Dinv_w) and b
+                if constexpr (kFused) {  // (A_rs Dinv_s, A_rw Dinv_w) and b; the west term first, so the
+                                         // south value's lane shift overlaps it
                     const T *c = &cr[mq % NR].v[N * wq];
-                    x = __builtin_fmaf(-c[1], xprev, __builtin_fmaf(-c[0], xs, br[mq % NR].v[wq]));
+                    x = __builtin_fmaf(-c[0], xs, __builtin_fmaf(-c[1], xprev, br[mq % NR].v[wq]));
                 } else {  // (b, Dinv, A_rs, A_rw): the reference's order of operations
                     const R c = ring[i % PD];
                     const T dn = shl1z(dc);

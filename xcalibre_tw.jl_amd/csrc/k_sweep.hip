@@ -39,15 +39,15 @@ __global__ void __launch_bounds__(kSweepBlock)
     if (ks && ks->done) return;
     // forward: y_i = x_i - sum_{j<i} (A_ij y_j) Dinv_j, j ascending
     // (FP32: val holds A_ij Dinv_j below and A_ij Dinv_i above the diagonal,
-    // rounded once, and the terms are fused multiply-adds)
+    // rounded once, and the terms are fused multiply-adds, j descending)
     for (int l = 0; l < nlev; ++l) {
         for (int k = fptr[l] + threadIdx.x; k < fptr[l + 1]; k += kSweepBlock) {
             const int i = frows[k];
             T s = x[i];
-            for (int p = rowptr[i]; p < diag[i]; ++p) {
-                const int j = col[p];
-                if constexpr (sizeof(T) == 4) s = __builtin_fmaf(-val[p], y[j], s);
-                else s = s - val[p] * y[j] * Dinv[j];
+            if constexpr (sizeof(T) == 4) {
+                for (int p = diag[i] - 1; p >= rowptr[i]; --p) s = __builtin_fmaf(-val[p], y[col[p]], s);
+            } else {
+                for (int p = rowptr[i]; p < diag[i]; ++p) s = s - val[p] * y[col[p]] * Dinv[col[p]];
             }
             y[i] = s;
         }
