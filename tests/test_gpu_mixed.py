@@ -90,9 +90,14 @@ def test_fp32_operators(monkeypatch):
     between the schedules that share an arithmetic."""
     rng = np.random.default_rng(11)
     out = {}
-    for sweep in ("skew", "band1", "band2", "band3", "band4", "levels", "rowwise"):
+    for sweep in ("skew", "skew_w2", "skew_w3", "band1", "band2", "band3", "band4", "levels", "rowwise"):
         monkeypatch.delenv("FVH_LEVEL_RECORDS", raising=False)
-        if sweep in ("levels", "rowwise"):
+        monkeypatch.delenv("FVH_SKEW_WAVES", raising=False)
+        if sweep.startswith("skew_w"):  # two / three compute waves per block
+            monkeypatch.delenv("FVH_SWEEP", raising=False)
+            monkeypatch.delenv("FVH_BAND_ROWS", raising=False)
+            monkeypatch.setenv("FVH_SKEW_WAVES", sweep[-1])
+        elif sweep in ("levels", "rowwise"):
             monkeypatch.setenv("FVH_SWEEP", "levels")
             if sweep == "rowwise":
                 monkeypatch.setenv("FVH_LEVEL_RECORDS", "0")
@@ -129,6 +134,8 @@ def test_fp32_operators(monkeypatch):
     # fused multiply-adds in neighbour order; the single-work-group band
     # variants keep the unfused FP64 order in FP32 (checked against FP64 above)
     assert np.array_equal(out["skew"], out["levels"])
+    assert np.array_equal(out["skew_w2"], out["levels"])
+    assert np.array_equal(out["skew_w3"], out["levels"])
     assert np.array_equal(out["rowwise"], out["levels"])
     assert np.array_equal(out["band1"], out["band2"])
 
