@@ -5,10 +5,11 @@
 // and, at step T, cell (T - j, j) of anti-diagonal T.  Its west value is its
 // own result of step T-1 (a register); its south value is lane l-1's result
 // of step T-1 (DPP wave_shr:1).  Lane 0's south value is the top lane of the
-// wave below: every wave collects its top lane's results over kSkewL steps
-// (a DPP shift per step) and writes them as one batch to an LDS hand-over
-// array; the wave above reads one batch per kSkewL steps, a batch ahead of
-// use.  Between blocks (on other CUs), two courier waves per block relay the
+// wave below: every wave keeps its results of a group of kSkewL steps, and
+// its top lane writes them as one batch to an LDS hand-over array; the wave
+// above reads each batch into registers a group ahead of use, and step k of
+// the group passes word k as the `old` operand of its DPP shift (the value
+// lane 0 keeps).  Between blocks (on other CUs), two courier waves per block relay the
 // batches: one copies the top wave's LDS batches to a device-coherent global
 // array, the other the block below's batches from theirs into its own LDS.
 // Only the couriers issue device-coherent loads and stores, so their latency
@@ -27,8 +28,9 @@
 // to whole blocks, at least one padding row), so one step's records of a wave
 // are contiguous and one step is one uniform buffer offset.  Slots outside the
 // rectangle are padding whose values make every term they enter an exact +0:
-//   F = (b, Dinv, A_rs, A_rw) = (0, 1, 0, 0),  Bk = (y, Dinv, A_re, A_rn) =
-//   (y, 1, 0, 0),  U = (A_rr, A_rs A_sr, A_rw A_wr) = (1, 0, 0);
+//   F = (b, Dinv, A_rs, A_rw) = (0, 1, 0, 0),  Bk = (0, Dinv, A_re, A_rn) =
+//   (0, 1, 0, 0),  U = (A_rr, A_rs A_sr, A_rw A_wr) = (1, 0, 0), and the
+//   sweeps' quad records and right-hand sides built from them;
 // a padding cell's pivot evaluates to 1 and its forward / backward values to
 // +0.  A missing neighbour of a real cell has the coefficient +0 in F and U
 // and copysign(0, Dinv_r) in Bk (set by the diagonal recurrence), so its
@@ -55,10 +57,11 @@
 namespace fvh {
 
 constexpr int kL = kSkewL;
-// The record ring holds M hand-over batches of steps (prefetch distance
-// M*kL): as deep as the vector-memory counter allows for the operations a
-// step issues (FP64: two 16-byte loads and a store; FP32: one load and a
-// store; at most 63 in flight).
+// The record rings hold M hand-over batches of steps (prefetch distance
+// M*kL): as deep as the vector-memory counter allows for what the sweeps
+// issue (FP64 forward: two 16-byte loads per step and two stores per four
+// steps; FP64 backward: ten operations per four steps; FP32: four or five
+// per four steps; at most 63 in flight).
 template <typename T>
 constexpr int ring_batches() {
     return sizeof(T) == 4 ? 4 : 2;
