@@ -126,9 +126,10 @@ __device__ inline void record_fault(int *fault) {
 // batch was not yet written.
 template <typename T>
 __device__ __attribute__((noinline)) T wait_batch(const T *p, T v, int broken) {
-    for (int s = 0; !broken && s < kSpinLimit && __builtin_amdgcn_ballot_w64(is_sentinel(v)); ++s) {
-        __builtin_amdgcn_s_sleep(1);
+    for (int s = 0; !broken && s < kSpinLimit; ++s) {
         v = lload(p);
+        if (!__builtin_amdgcn_ballot_w64(is_sentinel(v))) break;
+        __builtin_amdgcn_s_sleep(1);
     }
     return v;
 }
@@ -319,50 +320,52 @@ __device__ void courier_out(const T *lout, T *gout, int NG, int *fault) {
 // top lane forward, the bottom lane backward) keeps its kL results of a group
 // and writes them as one batch at the group's end; every other lane writes
 // its own kL words of the dummy area, so the store needs no lane mask.  The
-// wave on the other side checks a batch with one word per lane (lanes q < kL,
-// read kLook groups ahead, tested a group ahead, so the compare's latency
-// hides behind a group of steps and the branch to the out-of-line wait is
-// rarely taken), then reads the checked batch into registers a group ahead
-// of use, the same kL words in every lane: step k of the group takes word k
-// as its edge lane's neighbour value, the `old` operand of the step's DPP
-// shift, and no lane shift moves the batch along.
+// wave on the other side reads a group ahead of use both the batch, into
+// registers (the same kL words in every lane: step k of the group takes word
+// k as its edge lane's neighbour value, the `old` operand of the step's DPP
+// shift, so no lane shift moves the batch along), and one check word per
+// lane (lanes q < kL).  At the group's start it tests the check words read a
+// group before (the compare's latency hidden behind a group of steps) and
+// branches to the out-of-line wait, which reads the batch again, only when
+// one was missing.  A wave starts once its source's second batch is there:
+// two groups behind its source, every later batch is complete a group ahead
+// of use, and the waits stay off the step path.
 // ---------------------------------------------------------------------------
-// Batch reads reach index (NG + kLook) * kL - 1, courier windows
+// Batch reads reach index (NG + 1) * kL + kL - 1, courier windows
 // NG * kL + kWin - kL; the arrays are sized (HS, in skew_setup) to keep
 // both below the dummy area [HS - kDummy, HS): kL words per lane at a
 // stride of kL + 1 words (odd, so a store's lanes hit distinct banks).
-constexpr int kLook = 4;
+constexpr int kLook = 4;  // groups per unrolled loop iteration
 constexpr int kDummy = 64 * (kL + 1);
 static_assert(kLook % 2 == 0, "input batches alternate between two register sets with the group's parity");
 
 template <typename T>
 struct Hand {
-    const T *rd;       // this lane's check word of group 0 (lin + q)
-    const T *lin;      // the input array
-    T *wr;             // this lane's first write word of group 0 (batch word 1, or its dummy words)
-    int wstep;         // words the write words advance per group (kL or 0)
-    T yb[kLook];       // check words of the next kLook groups
-    T in[2][kL];       // the batches of the current and the next group, by G % 2
-    unsigned long long bad;  // lanes of the next group's check words still holding a sentinel
+    const T *rd;   // this lane's check word of group 0 (lin + q)
+    const T *lin;  // the input array
+    T *wr;         // this lane's first write word of group 0 (batch word 1, or its dummy words)
+    int wstep;     // words the write words advance per group (kL or 0)
+    T chk;         // this lane's check word of the next group
+    T in[2][kL];   // the batches of the current and the next group, by G % 2
 
-    __device__ inline void start(const T *lin_, T *lout, int HS, int q, bool writer) {
+    __device__ inline void start(const T *lin_, T *lout, int HS, int q, bool writer, int NG) {
         const int lane = threadIdx.x & 63;
         lin = lin_;
         rd = lin + q;
         T *dummy = lout + HS - kDummy + (kL + 1) * lane;
         wr = writer ? lout + 1 : dummy;
         wstep = writer ? kL : 0;
-#pragma unroll
-        for (int h = 0; h < kLook; ++h) yb[h] = lload(rd + h * kL);
-        bad = __builtin_amdgcn_ballot_w64(is_sentinel(yb[0]));
+        // two groups behind the source (bounded: a batch that never comes is
+        // found missing, and recorded, by the checks of the walk)
+        (void)wait_batch(rd + min(1, NG - 1) * kL, T(0), 0);
+        chk = lload(rd);
         read(0, 0);
         wait_lds();
-        // the LDS pattern of a group's end (kL words written, one check
-        // word read), into the dummy words, so that the loop's LDS waits
-        // count the same operations on entry as on every later iteration
+        // the LDS pattern of a group's end (kL words written) into the dummy
+        // words, so that the loop's LDS waits count the same operations on
+        // entry as on every later iteration
 #pragma unroll
         for (int k = 0; k < kL; ++k) dummy[k] = T(0);
-        yb[kLook - 1] = lload(rd + (kLook - 1) * kL);
     }
     // all LDS operations done (ends the paths that re-read a batch, so that
     // the paths agree on what is still in flight)
@@ -376,16 +379,15 @@ struct Hand {
     }
     // makes the batch of group G (slot h % 2, h = G % kLook) ready, waiting
     // for it and reading it again if its check failed, and reads the next
-    // group's batch (read unconditionally: a batch that fails its check is
-    // read again after the wait)
+    // group's check words and batch
     __device__ inline void take(int G, int h, int &broken) {
-        if (bad) {
-            const T v = wait_batch(rd + G * kL, yb[h], broken);
+        if (__builtin_amdgcn_ballot_w64(is_sentinel(chk))) {
+            const T v = wait_batch(rd + G * kL, chk, broken);
             if (__builtin_amdgcn_ballot_w64(is_sentinel(v))) broken = 1;  // lost: the fault is recorded after the walk
             read(G, h % 2);
             wait_lds();
         }
-        bad = __builtin_amdgcn_ballot_w64(is_sentinel(yb[(h + 1) % kLook]));
+        chk = lload(rd + (G + 1) * kL);
         read(G + 1, (h + 1) % 2);
     }
     // DPP shift by one lane whose edge lane takes the neighbour value of
@@ -399,10 +401,10 @@ struct Hand {
     }
     // publishes the group's results (the edge lane's are the batch)
     __device__ inline void finish(int G, int h, const T (&res)[kL]) {
+        (void)G, (void)h;
 #pragma unroll
         for (int k = 0; k < kL; ++k) wr[k] = res[k];
         wr += wstep;
-        yb[h] = lload(rd + (G + kLook) * kL);
     }
 };
 
@@ -437,7 +439,7 @@ __global__ void __launch_bounds__(64 * (W + 2))
     constexpr int M = ring_batches<double>(), PD = M * kL;
     static_assert(kLook % M == 0, "whole rings per unrolled block");
     Hand<double> hd;
-    hd.start(lds + w * HS, lds + (w + 1) * HS, HS, min(lane, kL - 1), lane == 63);
+    hd.start(lds + w * HS, lds + (w + 1) * HS, HS, min(lane, kL - 1), lane == 63, g.NG);
     // the prologue issues what a step issues (two stores of the pivot 1.0
     // that the pre-start padding holds already, one record load), so that the
     // loop's memory waits count the same outstanding operations every step
@@ -547,7 +549,7 @@ __global__ void __launch_bounds__(64 * (W + 2))
     const T *dp = &F[g.base + (size_t)(ql - 1 + kSkewD0) * S + j0 - 1].b;
     const size_t hstep = (size_t)kL * S * 4;  // one batch of steps, in T
     Hand<T> hd;
-    hd.start(lds + w * HS, lds + (w + 1) * HS, HS, ql, lane == 63);
+    hd.start(lds + w * HS, lds + (w + 1) * HS, HS, ql, lane == 63, g.NG);
     T db[kLook];  // lane-0 pivots of the next kLook groups (FP64)
 #pragma unroll
     for (int h = 0; h < kLook; ++h) db[h] = kFused ? T(0) : dp[h * hstep];
@@ -664,7 +666,7 @@ __global__ void __launch_bounds__(64 * (W + 2))
     constexpr int M = ring_batches<T>(), PD = M * kL;
     static_assert(kLook % M == 0, "whole rings per unrolled block");
     Hand<T> hd;
-    hd.start(lds + (w + 1) * HS, lds + w * HS, HS, qh, lane == 0);
+    hd.start(lds + (w + 1) * HS, lds + w * HS, HS, qh, lane == 0, g.NG);
     // Ttop + 1 + kSkewQ0 is a multiple of 4, so step i of an iteration is
     // word 3 - i%4 of the iteration's quad i/4 (four diagonals, descending)
     constexpr bool kFused = sizeof(T) == 4;
