@@ -223,14 +223,6 @@ __device__ inline void bload_quad(Vec<T, 4 * N> &c, Rsrc r, unsigned vo, unsigne
     for (int t = 0; t < 4 * N * (int)sizeof(T) / 16; ++t) bload_piece(c, t, r, vo, so);
 }
 
-// Whole-wave lane shift (the FP64 forward sweep's lane-0 pivots): the next
-// value moves to lane 0 (the vacated lane reads 0, never used).
-__device__ inline double shl1z(double v) {
-    const int lo = __builtin_amdgcn_mov_dpp(__double2loint(v), 0x130, 0xF, 0xF, true);
-    const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), 0x130, 0xF, 0xF, true);
-    return __hiloint2double(hi, lo);
-}
-
 // Resets the block's LDS hand-over arrays ([W+1][HS]; array c lies below
 // compute wave c): index 0 holds the padding value of step -1, the others
 // the sentinel, except array `open`, which no wave or courier writes (the
@@ -347,12 +339,13 @@ struct Hand {
     int wstep;     // words the write words advance per group (kL or 0)
     T chk;         // this lane's check word of the next group
     T in[2][kL];   // the batches of the current and the next group, by G % 2
+    T *dummy;      // this lane's dummy words
 
     __device__ inline void start(const T *lin_, T *lout, int HS, int q, bool writer, int NG) {
         const int lane = threadIdx.x & 63;
         lin = lin_;
         rd = lin + q;
-        T *dummy = lout + HS - kDummy + (kL + 1) * lane;
+        dummy = lout + HS - kDummy + (kL + 1) * lane;
         wr = writer ? lout + 1 : dummy;
         wstep = writer ? kL : 0;
         // two groups behind the source (bounded: a batch that never comes is
@@ -361,9 +354,12 @@ struct Hand {
         chk = lload(rd);
         read(0, 0);
         wait_lds();
-        // the LDS pattern of a group's end (kL words written) into the dummy
-        // words, so that the loop's LDS waits count the same operations on
-        // entry as on every later iteration
+        mirror();
+    }
+    // the LDS pattern of a group's end (kL words written) into the dummy
+    // words, so that the loop's LDS waits count the same operations on entry
+    // as on every later iteration
+    __device__ inline void mirror() {
 #pragma unroll
         for (int k = 0; k < kL; ++k) dummy[k] = T(0);
     }
@@ -545,8 +541,20 @@ __global__ void __launch_bounds__(64 * (W + 2))
     const unsigned vq = (g.base + 4 * (j0 + lane)) * (unsigned)sizeof(T), qbytes = a.rbytes / 4;
     const unsigned vc = N * vq, cstride = N * step;
     const R *F = static_cast<const R *>(recs);
-    // FP64: lane 0's south pivot, Dinv of (step T-1, row j0-1)
+    // FP64: lane 0's south pivot, Dinv of (step T-1, row j0-1), loaded by
+    // lanes q < kL kLook groups ahead (lane q: step q of the group), then
+    // passed through the wave's LDS scratch words into registers of every
+    // lane a group ahead of use (the `old` operand of the south pivot's DPP
+    // shift, as the hand-over batch is for the south value)
     const T *dp = &F[g.base + (size_t)(ql - 1 + kSkewD0) * S + j0 - 1].b;
+    T *pscr = lds + (W + 1) * HS + 64 * w;
+    T pin[2][kL];
+    auto pivots = [&](int slot, T v) {
+        pscr[lane] = v;
+        __atomic_signal_fence(__ATOMIC_SEQ_CST);
+#pragma unroll
+        for (int k = 0; k < kL; ++k) pin[slot][k] = pscr[k];
+    };
     const size_t hstep = (size_t)kL * S * 4;  // one batch of steps, in T
     Hand<T> hd;
     hd.start(lds + w * HS, lds + (w + 1) * HS, HS, ql, lane == 63, g.NG);
@@ -554,6 +562,10 @@ __global__ void __launch_bounds__(64 * (W + 2))
 #pragma unroll
     for (int h = 0; h < kLook; ++h) db[h] = kFused ? T(0) : dp[h * hstep];
     dp += kLook * hstep;
+    if constexpr (!kFused) {
+        pivots(0, db[0]);
+        hd.mirror();
+    }
     R ring[kFused ? 1 : PD];           // FP64 records
     Vec<T, 4 * N> cr[kFused ? NR : 1];  // FP32 coefficient quads
     Vec<T, 4> br[kFused ? NR : 1];      // FP32 right-hand-side quads
@@ -590,7 +602,7 @@ __global__ void __launch_bounds__(64 * (W + 2))
         for (int h = 0; h < kLook; ++h) {
             const int G = G0 + h;
             hd.take(G, h, broken);
-            T dc = db[h];
+            if constexpr (!kFused) pivots((h + 1) % 2, db[(h + 1) % kLook]);
 #pragma unroll
             for (int k = 0; k < kL; ++k) {
                 const int i = h * kL + k, mq = i / 4, wq = i % 4;
@@ -602,9 +614,8 @@ __global__ void __launch_bounds__(64 * (W + 2))
                     x = __builtin_fmaf(-c[0], xs, __builtin_fmaf(-c[1], xprev, br[mq % NR].v[wq]));
                 } else {  // (b, Dinv, A_rs, A_rw): the reference's order of operations
                     const R c = ring[i % PD];
-                    const T dn = shl1z(dc);
-                    const T ds = dpp_shr1(dprev, dc);
-                    dc = dn;
+                    T &ds = pin[h % 2][k];  // the shift writes the spent pivot's registers
+                    ds = dpp_shr1(dprev, ds);
                     x = (c.a - c.c * xs * ds) - c.d * xprev * dprev;
                     dprev = keep(c.b);
                 }
@@ -962,7 +973,7 @@ int skew_setup(fvh_mesh *m, const std::vector<SkewRect> &rects) {
     // hand-over arrays: indices up to NG*kL, read a window (kWin) or a ring of
     // batches (at most 4) past the last one
     const int HS = (maxNG * kL + kWin + kLook * kL + kDummy + 63) / 64 * 64;
-    if ((size_t)2 * HS * sizeof(double) > (size_t)max_lds_bytes()) return FVH_OK;  // too long even for W = 1
+    if (((size_t)2 * HS + 64) * sizeof(double) > (size_t)max_lds_bytes()) return FVH_OK;  // too long even for W = 1
     auto L = std::make_unique<SkewLayout>();
     L->HS = HS;
     size_t base = 0;
@@ -1023,7 +1034,7 @@ static int forced_waves() {
 }
 static int lds_waves_max(const SkewLayout &L) {
     int W = kSkewMaxW;
-    while (W > 1 && (size_t)(W + 1) * L.HS * sizeof(double) > (size_t)max_lds_bytes()) --W;
+    while (W > 1 && ((size_t)(W + 1) * L.HS + 64 * W) * sizeof(double) > (size_t)max_lds_bytes()) --W;
     return W;
 }
 static int skew_waves(const SkewLayout &L, bool fp32 = false) {
@@ -1141,11 +1152,11 @@ template <typename T, int W>
 static int launch_sweeps_w(fvh_eqn *e, SkewSweep<T> &w, const KScal *ks) {
     const SkewLayout &L = *e->m->skew;
     const SkewArgs a = skew_args<Rec4T<T>>(e, W);
-    const size_t lds = (size_t)(W + 1) * L.HS * sizeof(T);
-    FVH_TRY(allow_lds(k_skew_fwd<T, W>, lds));
+    const size_t lds = (size_t)(W + 1) * L.HS * sizeof(T), lds_f = lds + (size_t)64 * W * sizeof(T);
+    FVH_TRY(allow_lds(k_skew_fwd<T, W>, lds_f));
     FVH_TRY(allow_lds(k_skew_bwd<T, W>, lds));
     const void *recs = sizeof(T) == 8 ? static_cast<const void *>(e->sk.F.p) : w.kf.p;
-    hipLaunchKernelGGL((k_skew_fwd<T, W>), dim3(a.nblk), dim3(64 * (W + 2)), lds, stream(), a, recs, w.bq.p, w.yq.p,
+    hipLaunchKernelGGL((k_skew_fwd<T, W>), dim3(a.nblk), dim3(64 * (W + 2)), lds_f, stream(), a, recs, w.bq.p, w.yq.p,
                        w.hand.p, ks);
     HIP_TRY(hipGetLastError());
     hipLaunchKernelGGL((k_skew_bwd<T, W>), dim3(a.nblk), dim3(64 * (W + 2)), lds, stream(), a, w.kb.p, w.yq.p, w.zp.p,
