@@ -12,7 +12,7 @@ import pytest
 import fvmhip as F
 from fvmhip import _lib as L
 from oracle import ref
-from tests.cases import bfs_model, cavity_model, config, flatplate_model, rel_l2, same_bits
+from tests.cases import bfs_model, cavity_model, channel_model, config, flatplate_model, rel_l2, same_bits
 
 pytestmark = pytest.mark.gpu
 
@@ -94,6 +94,24 @@ def test_bfs_small_bitwise():
     for a, b in zip(g.residuals(), r.residuals()):
         assert same_bits(a, b)
     for name in ("Ux", "Uy", "p"):
+        assert same_bits(g.get(name), r.get(name)), name
+
+
+@pytest.mark.parametrize("nx,ny", [(3, 4), (5, 4), (6, 4), (9, 64), (9, 65)])
+def test_small_grids_bitwise(nx, ny, monkeypatch):
+    """Grids at the pipelined sweeps' edges: one group of 8 anti-diagonals
+    (3 x 4, 5 x 4), the first with two (6 x 4), a full wave of rows and the
+    first with a second wave (9 x 64, 9 x 65)."""
+    for env in ("FVH_SWEEP", "FVH_BAND_ROWS", "FVH_SKEW_WAVES", "FVH_LEVEL_RECORDS"):
+        monkeypatch.delenv(env, raising=False)
+    mesh = F.channel(nx, ny)
+    assert mesh.device().sweep_info() == (nx + ny - 1, nx)
+    model = channel_model(mesh)
+    g, r = _sims(model, config(5), conv_break=1e-12)
+    assert g.run(5) == r.run(5)
+    for a, b in zip(g.residuals(), r.residuals()):
+        assert same_bits(a, b)
+    for name in ("ux_dinv", "p_dinv", "Ux", "Uy", "p"):
         assert same_bits(g.get(name), r.get(name)), name
 
 
