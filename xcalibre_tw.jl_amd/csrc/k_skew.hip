@@ -586,17 +586,17 @@ __global__ void __launch_bounds__(64 * (W + 2))
                     bload_quad<T, 1>(br[mq], rsrc_q(bq, qbytes, 0, step), vq, (unsigned)mq * step);
                 }
             } else {
-                ring[k] = bload(rsrc_at(F, a.rbytes, 0, step), vo, k * step, (R *)nullptr);
+                ring[(k + PD - 1) % PD] = bload(rsrc_at(F, a.rbytes, -1, step), vo, k * step, (R *)nullptr);
             }
         }
     }
-    T xprev = 0, dprev = 1, res[kL];
+    T xprev = 0, res[kL];
     int broken = 0;
     int G0 = 0;
     do {
         // this iteration's steps G0*kL + i, i < kLook*kL: results in quads
         // from G0*kL on, refills PD steps on
-        const Rsrc rq = rsrc_q(yq, qbytes, G0 * kL, step), rf = rsrc_at(F, a.rbytes, G0 * kL + PD, step);
+        const Rsrc rq = rsrc_q(yq, qbytes, G0 * kL, step), rf = rsrc_at(F, a.rbytes, G0 * kL + PD - 1, step);
         const Rsrc rc = rsrc_q(recs, N * qbytes, G0 * kL + PD, cstride), rbn = rsrc_q(bq, qbytes, G0 * kL + PD, step);
 #pragma unroll
         for (int h = 0; h < kLook; ++h) {
@@ -614,10 +614,10 @@ __global__ void __launch_bounds__(64 * (W + 2))
                     x = __builtin_fmaf(-c[0], xs, __builtin_fmaf(-c[1], xprev, br[mq % NR].v[wq]));
                 } else {  // (b, Dinv, A_rs, A_rw): the reference's order of operations
                     const R c = ring[i % PD];
+                    const T dprev = ring[(i + PD - 1) % PD].b;  // the previous step's Dinv, still in its slot
                     T &ds = pin[h % 2][k];  // the shift writes the spent pivot's registers
                     ds = dpp_shr1(dprev, ds);
                     x = (c.a - c.c * xs * ds) - c.d * xprev * dprev;
-                    dprev = keep(c.b);
                 }
                 q[wq] = x;
                 if (wq == 3) bstore4(rq, q, vq, (unsigned)mq * step);
@@ -629,7 +629,7 @@ __global__ void __launch_bounds__(64 * (W + 2))
                         bload_quad<T, 1>(br[mq % NR], rbn, vq, (unsigned)mq * step);
                     }
                 } else {
-                    ring[i % PD] = bload(rf, vo, (unsigned)i * step, (R *)nullptr);
+                    ring[(i + PD - 1) % PD] = bload(rf, vo, (unsigned)i * step, (R *)nullptr);
                 }
             }
             hd.finish(G, h, res);
