@@ -405,7 +405,9 @@ struct Hand {
 };
 
 // ---------------------------------------------------------------------------
-// Diagonal recurrence (update_dilu_diagonal!): Dinv into F.b and Bk.b
+// Diagonal recurrence (update_dilu_diagonal!): D into F.b (the quad-record
+// pass, all CUs, turns it into Dinv = 1/D afterwards, off the recurrence's
+// serial path)
 // ---------------------------------------------------------------------------
 template <int W>
 __global__ void __launch_bounds__(64 * (W + 2))
@@ -436,17 +438,15 @@ __global__ void __launch_bounds__(64 * (W + 2))
     static_assert(kLook % M == 0, "whole rings per unrolled block");
     Hand<double> hd;
     hd.start(lds + w * HS, lds + (w + 1) * HS, HS, min(lane, kL - 1), lane == 63, g.NG);
-    // the prologue issues what a step issues (two stores of the pivot 1.0
-    // that the pre-start padding holds already, one record load), so that the
+    // the prologue issues what a step issues (a store of the pivot 1.0 that
+    // the pre-start padding holds already, one record load), so that the
     // loop's memory waits count the same outstanding operations every step
     Rec4 ring[PD];
     {
-        const Rsrc ru = rsrc_at(U, a.rbytes, 0, step), rf = rsrc_at(F, a.rbytes, -PD, step),
-                   rbk = rsrc_at(Bk, a.rbytes, -PD, step);
+        const Rsrc ru = rsrc_at(U, a.rbytes, 0, step), rf = rsrc_at(F, a.rbytes, -PD, step);
 #pragma unroll
         for (int k = 0; k < PD; ++k) {
             bstore(rf, 1.0, vo + 8, k * step);
-            bstore(rbk, 1.0, vo + 8, k * step);
             ring[k] = bload(ru, vo, k * step, (Rec4 *)nullptr);
             __builtin_amdgcn_sched_barrier(0);  // issue in consumption order
         }
@@ -456,8 +456,7 @@ __global__ void __launch_bounds__(64 * (W + 2))
     int G0 = 0;
     do {
         // this iteration's steps G0*kL + i, i < kLook*kL: stores at i*step, refills PD steps on
-        const Rsrc ru = rsrc_at(U, a.rbytes, G0 * kL + PD, step), rf = rsrc_at(F, a.rbytes, G0 * kL, step),
-                   rbk = rsrc_at(Bk, a.rbytes, G0 * kL, step);
+        const Rsrc ru = rsrc_at(U, a.rbytes, G0 * kL + PD, step), rf = rsrc_at(F, a.rbytes, G0 * kL, step);
 #pragma unroll
         for (int h = 0; h < kLook; ++h) {
             const int G = G0 + h;
@@ -469,9 +468,7 @@ __global__ void __launch_bounds__(64 * (W + 2))
                 const Rec4 u = ring[r];
                 const double ds = hd.template shift_in<true>(dprev, h, k);
                 const double d = (u.a - u.b / ds) - u.c / dprev;
-                const double di = 1.0 / d;
-                bstore(rf, di, vo + 8, si);
-                bstore(rbk, di, vo + 8, si);
+                bstore(rf, d, vo + 8, si);
                 dprev = res[k] = d;
                 __builtin_amdgcn_sched_barrier(0);  // the slot is consumed: refill it below
                 ring[r] = bload(ru, vo, si, (Rec4 *)nullptr);
@@ -875,19 +872,20 @@ __global__ void __launch_bounds__(256)
 
 // The coefficients a cell does not have in the backward sweep (east of each
 // row's last cell, north of the top row) become copysign(0, Dinv), so that
-// their products vanish exactly (x - (+0) == x, x == -0 included).  One
-// thread per edge cell, blockIdx.y: the grid.
-__global__ void k_skew_edges(const SkewGrid *__restrict__ grids, Rec4 *__restrict__ Bk) {
+// their products vanish exactly (x - (+0) == x, x == -0 included); Dinv's
+// sign is D's (F.b, written by the diagonal recurrence).  One thread per
+// edge cell, blockIdx.y: the grid.
+__global__ void k_skew_edges(const SkewGrid *__restrict__ grids, const Rec4 *__restrict__ F, Rec4 *__restrict__ Bk) {
     const SkewGrid g = grids[blockIdx.y];
     const int q = (int)(blockIdx.x * blockDim.x + threadIdx.x);
     auto slot = [&](int T, int j) { return g.base + (size_t)(T + kSkewD0) * g.S + (size_t)j; };
     if (q < g.ny) {  // row q's east-edge cell, step q + nx - 1
-        Rec4 &r = Bk[slot(q + g.nx - 1, q)];
-        r.c = copysign(0.0, r.b);
+        const size_t s = slot(q + g.nx - 1, q);
+        Bk[s].c = copysign(0.0, F[s].b);
     } else if (q < g.ny + g.nx) {  // top-row cell of column q - ny
         const int j = g.ny - 1;
-        Rec4 &r = Bk[slot(j + q - g.ny, j)];
-        r.d = copysign(0.0, r.b);
+        const size_t s = slot(j + q - g.ny, j);
+        Bk[s].d = copysign(0.0, F[s].b);
     }
 }
 
@@ -902,8 +900,8 @@ __global__ void k_skew_edges(const SkewGrid *__restrict__ grids, Rec4 *__restric
 // diagonal's last slot, a padding row (pivot 1).  blockIdx.y: the grid.
 static_assert(kSkewPD % 4 == 0 && (kSkewQ0 - kSkewPD) % 4 == 0, "quads start at a padding diagonal");
 template <typename T>
-__global__ void k_skew_quads(const SkewGrid *__restrict__ grids, const Rec4 *__restrict__ F,
-                             const Rec4 *__restrict__ Bk, T *__restrict__ kf, T *__restrict__ kb) {
+__global__ void k_skew_quads(const SkewGrid *__restrict__ grids, Rec4 *__restrict__ F, Rec4 *__restrict__ Bk,
+                             T *__restrict__ kf, T *__restrict__ kb) {
     constexpr int NF = kFwdN, NB = kBwdN;
     const SkewGrid g = grids[blockIdx.y];
     const size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -914,14 +912,17 @@ __global__ void k_skew_quads(const SkewGrid *__restrict__ grids, const Rec4 *__r
 #pragma unroll
     for (int w = 0; w < 4; ++w) {  // diagonal T = 4 qi + w - kSkewQ0
         const size_t s = s0 + (size_t)w * g.S;
-        const Rec4 u = F[s], v = Bk[s];
-        const double ds = F[s - g.S - 1].b, dw = F[s - g.S].b;
-        if constexpr (sizeof(T) == 8) {
-            (void)u, (void)ds, (void)dw;
-            cb[3 * w] = v.b;
+        if constexpr (sizeof(T) == 8) {  // F.b holds D: Dinv into F.b, Bk.b and the quad record
+            const double di = 1.0 / F[s].b;
+            F[s].b = di;
+            Bk[s].b = di;
+            const Rec4 v = Bk[s];
+            cb[3 * w] = di;
             cb[3 * w + 1] = v.c;
             cb[3 * w + 2] = v.d;
         } else {
+            const Rec4 u = F[s], v = Bk[s];
+            const double ds = F[s - g.S - 1].b, dw = F[s - g.S].b;
             cf[2 * w] = (float)(u.c * ds);
             cf[2 * w + 1] = (float)(u.d * dw);
             cb[3 * w] = (float)v.b;
@@ -1141,7 +1142,7 @@ int skew_update(fvh_eqn *e, const double *val) {
     int edge = 0;
     for (const SkewGrid &g : L.h_grids) edge = std::max(edge, g.nx + g.ny);
     hipLaunchKernelGGL(k_skew_edges, dim3((unsigned)((edge + 255) / 256), (unsigned)L.h_grids.size()), dim3(256), 0,
-                       stream(), L.grids.p, e->sk.Bk.p);
+                       stream(), L.grids.p, e->sk.F.p, e->sk.Bk.p);
     HIP_TRY(hipGetLastError());
     FVH_TRY(sweep_records(e, e->sk.s64));
     if (!m->skew->W && !forced_waves()) FVH_TRY(skew_tune(e));
