@@ -162,16 +162,9 @@ __device__ inline Rec4 bload(Rsrc r, unsigned vo, unsigned so, Rec4 *) {
     const auto v = __builtin_bit_cast(double2, __builtin_amdgcn_raw_buffer_load_b128(r, vo + 16, so, 0));
     return Rec4{u.x, u.y, v.x, v.y};
 }
-__device__ inline Rec4f bload(Rsrc r, unsigned vo, unsigned so, Rec4f *) {
-    const auto u = __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(r, vo, so, 0));
-    return Rec4f{u.x, u.y, u.z, u.w};
-}
 typedef unsigned u32x2 __attribute__((ext_vector_type(2)));
 __device__ inline void bstore(Rsrc r, double v, unsigned vo, unsigned so) {
     __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), r, vo, so, 0);
-}
-__device__ inline void bstore(Rsrc r, float v, unsigned vo, unsigned so) {
-    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, vo, so, 0);
 }
 
 // Quads: the values of four consecutive diagonals of one row, stored
