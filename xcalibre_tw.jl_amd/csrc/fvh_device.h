@@ -255,6 +255,7 @@ namespace fvh {
 struct KScal {
     double rho, alpha, omega, beta, rnorm, eps, gamma, pnorm2, rnorm0, ts, tt, sigma;
     double hbis, zeta;
+    double gprev;  // CG: gamma at the start of the iteration (read by the fused beta stage)
     int done, iter, status, k;
 };
 

@@ -564,6 +564,73 @@ static int bicgstab(const Sys<T> &S, const fvh_solver_cfg &cfg, T *xu, fvh_stats
 }
 
 // ---------------------------------------------------------------------------
+// CG's two scalar updates fused into the vector updates that use them.  Every
+// block reduces the partial sums itself (the canonical order of k_final, so
+// the same bits), computes the scalar and maps its elements with it; block 0
+// alone stores the Krylov scalars.  What the blocks read from ks is not
+// written in the same launch (gamma and pnorm2 in the alpha stage; gprev and
+// eps in the beta stage), and what decides a block's early exit is computed
+// from those, so every block takes the same path.  The `done` flag is read
+// once per block, before the block-wide reduction.
+// ---------------------------------------------------------------------------
+__device__ inline bool block_done(const KScal *ks) {
+    __shared__ int d;
+    if (threadIdx.x == 0) d = ks->done;
+    __syncthreads();
+    return d != 0;
+}
+inline dim3 fused_grid(int64_t n) { return dim3((unsigned)std::max<int64_t>(1, std::min<int64_t>((n + kBlock - 1) / kBlock, 1024))); }
+
+// alpha = gamma / pAp (or stop on zero curvature), then x += alpha p, r -= alpha Ap
+template <class Op>
+__global__ void __launch_bounds__(kBlock) k_cg_alpha(int n, const double *part, int nch, KScal *ks, double epsT, Op op) {
+    if (block_done(ks)) return;
+    double pa[1];
+    final_reduce<1>(part, nch, pa);
+    const double pAp = pa[0], gamma = ks->gamma, pn2 = ks->pnorm2;
+    const bool zc = pAp <= epsT * pn2 && fabs(pAp) <= epsT * pn2;
+    const double al = gamma / pAp;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        ks->gprev = gamma;
+        if (zc) {
+            ks->done = 1;
+            ks->status = FVH_KRYLOV_ZERO_CURV;
+        } else {
+            ks->alpha = al;
+        }
+    }
+    if (zc) return;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) op(i, al);
+}
+
+// beta = gamma_new / gamma (or stop: solved, itmax), then p = z + beta p
+template <class Op>
+__global__ void __launch_bounds__(kBlock) k_cg_beta(int n, const double *part, int nch, KScal *ks, int itmax, Op op) {
+    if (block_done(ks)) return;
+    double g[1];
+    final_reduce<1>(part, nch, g);
+    const double gn = g[0], rn = sqrt(gn);
+    const bool solved = (rn <= ks->eps) || (rn + 1.0 <= 1.0);
+    const double beta = gn / ks->gprev;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        ks->rnorm = rn;
+        if (!solved) {
+            ks->pnorm2 = gn + beta * beta * ks->pnorm2;
+            ks->gamma = gn;
+            ks->beta = beta;
+        }
+        ks->iter += 1;
+        const bool tired = ks->iter >= itmax;
+        if (solved || tired) {
+            ks->done = 1;
+            ks->status = solved ? FVH_KRYLOV_SOLVED : FVH_KRYLOV_ITMAX;
+        }
+    }
+    if (solved) return;  // at itmax p is updated once more: nothing reads it
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) op(i, beta);
+}
+
+// ---------------------------------------------------------------------------
 // CG (Krylov.jl cg!) on sigma*A x = sigma*b
 // ---------------------------------------------------------------------------
 template <typename T>
@@ -622,45 +689,21 @@ static int cg(const Sys<T> &S, const fvh_solver_cfg &cfg, T *xu, fvh_stats *st, 
         int nb = std::min(next_batch(enq), itmax - enq);
         for (int it = 0; it < nb; ++it) {
             FVH_TRY(sys_spmv_dot(S, p, Ap, sigma, ks, p, part));
-            FVH_TRY(finalize<1>(n, part, ks, true, [=] __device__(KScal * k, double(&pa)[1]) {
-                double pAp = pa[0];
-                if (pAp <= epsT * k->pnorm2 && fabs(pAp) <= epsT * k->pnorm2) {
-                    k->done = 1;
-                    k->status = FVH_KRYLOV_ZERO_CURV;
-                    return;
-                }
-                k->alpha = k->gamma / pAp;
-            }));
-            FVH_TRY(map(n, ks, [=] __device__(int i) {
-                double al = ks->alpha;
-                xk[i] = D(xk[i]) + al * D(p[i]);
-                r[i] = D(r[i]) + (-al) * D(Ap[i]);
-            }));
+            hipLaunchKernelGGL(k_cg_alpha, fused_grid(n), dim3(kBlock), 0, stream(), (int)n, part, nchunks(n), ks, epsT,
+                               [=] __device__(int i, double al) {
+                                   xk[i] = D(xk[i]) + al * D(p[i]);
+                                   r[i] = D(r[i]) + (-al) * D(Ap[i]);
+                               });
+            HIP_TRY(hipGetLastError());
             FVH_TRY(sys_precond(S, z, r, ks));
             FVH_TRY(mapred<1>(n, ks, part, [=] __device__(int i, double(&a)[1]) {
                 const T zi = sigma * D(z[i]);
                 z[i] = zi;
                 a[0] = a[0] + D(r[i]) * D(zi);
             }));
-            FVH_TRY(finalize<1>(n, part, ks, true, [=] __device__(KScal * k, double(&g)[1]) {
-                double gn = g[0];
-                double rn = sqrt(gn);
-                k->rnorm = rn;
-                bool solved = (rn <= k->eps) || (rn + 1.0 <= 1.0);
-                if (!solved) {
-                    double beta = gn / k->gamma;
-                    k->pnorm2 = gn + beta * beta * k->pnorm2;
-                    k->gamma = gn;
-                    k->beta = beta;
-                }
-                k->iter += 1;
-                bool tired = k->iter >= itmax;
-                if (solved || tired) {
-                    k->done = 1;
-                    k->status = solved ? FVH_KRYLOV_SOLVED : FVH_KRYLOV_ITMAX;
-                }
-            }));
-            FVH_TRY(map(n, ks, [=] __device__(int i) { p[i] = D(z[i]) + ks->beta * D(p[i]); }));
+            hipLaunchKernelGGL(k_cg_beta, fused_grid(n), dim3(kBlock), 0, stream(), (int)n, part, nchunks(n), ks, itmax,
+                               [=] __device__(int i, double be) { p[i] = D(z[i]) + be * D(p[i]); });
+            HIP_TRY(hipGetLastError());
         }
         enq += nb;
         FVH_TRY(read_ks(e));
