@@ -596,13 +596,16 @@ __global__ void __launch_bounds__(64 * (W + 2))
 #pragma unroll
             for (int k = 0; k < kL; ++k) {
                 const int i = h * kL + k, mq = i / 4, wq = i % 4;
-                const T xs = hd.template shift_in<true>(xprev, h, k);
                 T x;
-                if constexpr (kFused) {  // (A_rs Dinv_s, A_rw Dinv_w) and b; the west term first, so the
-                                         // south value's lane shift overlaps it
+                if constexpr (kFused) {  // (A_rs Dinv_s, A_rw Dinv_w) and b; the west term first: it
+                                         // fills the wait between xprev's FMA and the lane shift
                     const T *c = &cr[mq % NR].v[N * wq];
-                    x = __builtin_fmaf(-c[0], xs, __builtin_fmaf(-c[1], xprev, br[mq % NR].v[wq]));
+                    const T t = __builtin_fmaf(-c[1], xprev, br[mq % NR].v[wq]);
+                    __builtin_amdgcn_sched_barrier(0);
+                    const T xs = hd.template shift_in<true>(xprev, h, k);
+                    x = __builtin_fmaf(-c[0], xs, t);
                 } else {  // (b, Dinv, A_rs, A_rw): the reference's order of operations
+                    const T xs = hd.template shift_in<true>(xprev, h, k);
                     const R c = ring[i % PD];
                     const T dprev = ring[(i + PD - 1) % PD].b;  // the previous step's Dinv, still in its slot
                     T &ds = pin[h % 2][k];  // the shift writes the spent pivot's registers
@@ -712,13 +715,17 @@ __global__ void __launch_bounds__(64 * (W + 2))
             for (int k = 0; k < kL; ++k) {
                 const int i = h * kL + k, mq = i / 4, wq = 3 - i % 4, rq = mq % NR;
                 const T *c = &cr[rq].v[N * wq];
-                const T zn = hd.template shift_in<false>(zprev, h, k);
                 const T y = yr[rq].v[wq];
                 T z;
-                if constexpr (kFused)
-                    z = __builtin_fmaf(-c[2], zn, __builtin_fmaf(-c[1], zprev, y * c[0]));
-                else  // the reference's order of operations
+                if constexpr (kFused) {  // the east term first: it fills the wait before the lane shift
+                    const T t = __builtin_fmaf(-c[1], zprev, y * c[0]);
+                    __builtin_amdgcn_sched_barrier(0);
+                    const T zn = hd.template shift_in<false>(zprev, h, k);
+                    z = __builtin_fmaf(-c[2], zn, t);
+                } else {  // the reference's order of operations
+                    const T zn = hd.template shift_in<false>(zprev, h, k);
                     z = ((y * c[0]) - c[1] * zprev * c[0]) - c[2] * zn * c[0];
+                }
                 q[wq] = z;
                 if (wq == 0) bstore4(rzq, q, vq, (unsigned)(NQ - 1 - mq) * step);
                 zprev = res[k] = z;
