@@ -604,16 +604,13 @@ __global__ void __launch_bounds__(64 * (W + 2))
                     __builtin_amdgcn_sched_barrier(0);
                     const T xs = hd.template shift_in<true>(xprev, h, k);
                     x = __builtin_fmaf(-c[0], xs, t);
-                } else {  // (b, Dinv, A_rs, A_rw): the reference's order of operations; the
-                          // west term before the lane shifts
+                } else {  // (b, Dinv, A_rs, A_rw): the reference's order of operations
+                    const T xs = hd.template shift_in<true>(xprev, h, k);
                     const R c = ring[i % PD];
                     const T dprev = ring[(i + PD - 1) % PD].b;  // the previous step's Dinv, still in its slot
-                    const T wt = c.d * xprev * dprev;
-                    __builtin_amdgcn_sched_barrier(0);
-                    const T xs = hd.template shift_in<true>(xprev, h, k);
                     T &ds = pin[h % 2][k];  // the shift writes the spent pivot's registers
                     ds = dpp_shr1(dprev, ds);
-                    x = (c.a - c.c * xs * ds) - wt;
+                    x = (c.a - c.c * xs * ds) - c.d * xprev * dprev;
                 }
                 q[wq] = x;
                 if (wq == 3) bstore4(rq, q, vq, (unsigned)mq * step);
@@ -725,11 +722,9 @@ __global__ void __launch_bounds__(64 * (W + 2))
                     __builtin_amdgcn_sched_barrier(0);
                     const T zn = hd.template shift_in<false>(zprev, h, k);
                     z = __builtin_fmaf(-c[2], zn, t);
-                } else {  // the reference's order of operations; the east term before the lane shift
-                    const T t = (y * c[0]) - c[1] * zprev * c[0];
-                    __builtin_amdgcn_sched_barrier(0);
+                } else {  // the reference's order of operations
                     const T zn = hd.template shift_in<false>(zprev, h, k);
-                    z = t - c[2] * zn * c[0];
+                    z = ((y * c[0]) - c[1] * zprev * c[0]) - c[2] * zn * c[0];
                 }
                 q[wq] = z;
                 if (wq == 0) bstore4(rzq, q, vq, (unsigned)(NQ - 1 - mq) * step);
