@@ -67,7 +67,7 @@ constexpr int ring_batches() {
     return sizeof(T) == 4 ? 4 : 2;
 }
 static_assert(4 * kL <= kSkewPD, "padding diagonals cover the deepest ring");
-constexpr int kSpinLimit = 1 << 19;  // idle polls (about 30 ms) before a hand-over is declared lost
+constexpr int kSpinLimit = 1 << 19;  // idle polls (tens of ms) before a hand-over is declared lost
 constexpr int kWin = 32;             // hand-over words a courier moves per poll (4 batches)
 
 template <typename T>
@@ -296,7 +296,7 @@ __device__ void courier_out(const T *lout, T *gout, int NG, int *fault) {
             idle = 0;
         } else {
             ++idle;
-            __builtin_amdgcn_s_sleep(1);
+            __builtin_amdgcn_s_sleep(2);  // about 128 cycles (1 and 4 measured slower)
         }
     }
     if (go < NG) record_fault(fault);
