@@ -318,6 +318,14 @@ int fvh_precond_update(fvh_eqn *e);
 int fvh_precond_apply(fvh_eqn *e, double *y, const double *x);
 /* Host copy of the DILU reciprocal diagonal (ncells). */
 int fvh_precond_dinv(fvh_eqn *e, double *dinv_host);
+/* The pipelined DILU sweeps bound every wait between work-groups; a wait
+ * that runs out leaves a pending fault, which the next call that ran the
+ * sweeps (fvh_precond_update / _apply / _apply_f32, fvh_solve, fvh_time_op)
+ * returns as FVH_ERR_HIP, once.  This call reads the pending flag and, with
+ * raise != 0, sets it, so a caller can exercise its error path.  No
+ * counterpart in the reference (whose sweeps are sequential);
+ * FVH_ERR_UNSUPPORTED when the preconditioner has no pipelined sweeps. */
+int fvh_precond_fault(fvh_eqn *e, int32_t raise, int32_t *pending);
 
 /* run!(eqn, setup): Krylov solve of A x = b warm-started from x (device,
  * overwritten)  [src/Solve/Solve_1_api.jl:32-46; Krylov.jl 0.9.5 solve!] */

@@ -37,11 +37,26 @@ def lib():
         L.ref_sim_get.argtypes = [vp, C.c_char_p, P_D, C.POINTER(C.c_int64)]
         L.ref_sim_stats.argtypes = [vp, C.c_char_p, vp]
         L.ref_det_dot.restype = C.c_double
+        for fn, nargs in (("ref_log", 1), ("ref_y_plus_laminar", 2), ("ref_nut_wall", 4), ("ref_pow", 2)):
+            getattr(L, fn).restype = C.c_double
+            getattr(L, fn).argtypes = [C.c_double] * nargs
         L.ref_det_dot.argtypes = [C.c_int64, P_D, P_D]
         L.ref_solve_csr.argtypes = [vp, P_D, P_D, vp, P_D, vp, P_D]
         L.ref_precond_apply_csr.argtypes = [vp, P_D, C.c_int32, P_D, P_D]
         L.ref_spmv_csr.argtypes = [vp, P_D, P_D, P_D]
         L.ref_grad.argtypes = [vp, C.c_int32, vp, C.c_int32, C.c_int32, P_D, P_D, P_D, P_D, P_D]
+        L.ref_interpolate.argtypes = [vp, P_D, P_D]
+        L.ref_interpolate_vector.argtypes = [vp, P_D, P_D, P_D, P_D]
+        L.ref_correct_boundaries.argtypes = [vp, vp, C.c_int32, C.c_int32, P_D, P_D]
+        L.ref_green_gauss.argtypes = [vp, P_D, P_D, P_D, P_D]
+        L.ref_div.argtypes = [vp, P_D, P_D, P_D, P_D]
+        L.ref_flux.argtypes = [vp, P_D, P_D, P_D, P_D]
+        L.ref_assemble.argtypes = [vp, vp, C.c_int32, vp, C.c_int32, P_D, C.c_double, vp, C.c_int32, C.c_int32,
+                                   P_D, C.c_double, C.c_int32, C.c_double, C.c_int64, P_D, P_D]
+        L.ref_bound.argtypes = [vp, P_D, C.c_double]
+        L.ref_strain_production.argtypes = [vp, C.c_int32, vp, C.c_int32, P_D, P_D, P_D, P_D]
+        L.ref_omega_constrain.argtypes = [vp, vp, C.c_int32, P_D, C.c_double, P_D, P_D]
+        L.ref_simple.argtypes = [vp, vp, P_D, P_D, P_D, P_D, P_D, P_D, P_D, P_D, P_D, P_D, C.POINTER(C.c_int32)]
         _lib = L
     return _lib
 
@@ -150,3 +165,113 @@ def grad(mesh, scheme, bcs_arr, nbc, comp, phi):
     _check(lib().ref_grad(C.byref(d), int(scheme), C.cast(bcs_arr, C.c_void_p), int(nbc), int(comp), _d(phi), _d(pf),
                           *(_d(a) for a in g)))
     return pf, g
+
+
+def _arr(a, n=None):
+    a = np.ascontiguousarray(a, np.float64)
+    return a if n is None else a.reshape(n)
+
+
+def interpolate(mesh, phi):
+    d = mesh.desc()
+    out = np.zeros(mesh.nfaces)
+    _check(lib().ref_interpolate(C.byref(d), _d(_arr(phi)), _d(out)))
+    return out
+
+
+def interpolate_vector(mesh, ux, uy, fx=None, fy=None):
+    """Writes x and y only: fx / fy start from the given arrays (zeros by default)."""
+    d = mesh.desc()
+    fx = np.zeros(mesh.nfaces) if fx is None else np.array(fx, np.float64)
+    fy = np.zeros(mesh.nfaces) if fy is None else np.array(fy, np.float64)
+    ux, uy = _arr(ux), _arr(uy)
+    _check(lib().ref_interpolate_vector(C.byref(d), _d(ux), _d(uy), _d(fx), _d(fy)))
+    return fx, fy
+
+
+def correct_boundaries(mesh, bcs_arr, nbc, comp, phi, phif):
+    d = mesh.desc()
+    out = np.array(phif, np.float64)
+    _check(lib().ref_correct_boundaries(C.byref(d), C.cast(bcs_arr, C.c_void_p), int(nbc), int(comp),
+                                        _d(_arr(phi)), _d(out)))
+    return out
+
+
+def green_gauss(mesh, phif):
+    d = mesh.desc()
+    g = [np.zeros(mesh.ncells) for _ in range(3)]
+    _check(lib().ref_green_gauss(C.byref(d), _d(_arr(phif)), *(_d(a) for a in g)))
+    return g
+
+
+def div(mesh, fx, fy, fz):
+    d = mesh.desc()
+    out = np.zeros(mesh.ncells)
+    fx, fy, fz = _arr(fx), _arr(fy), _arr(fz)
+    _check(lib().ref_div(C.byref(d), _d(fx), _d(fy), _d(fz), _d(out)))
+    return out
+
+
+def flux(mesh, fx, fy, fz):
+    d = mesh.desc()
+    out = np.zeros(mesh.nfaces)
+    fx, fy, fz = _arr(fx), _arr(fy), _arr(fz)
+    _check(lib().ref_flux(C.byref(d), _d(fx), _d(fy), _d(fz), _d(out)))
+    return out
+
+
+def assemble(mesh, terms, nterms, srcs, nsrc, prev, dt, bcs_arr, nbc, comp, relax_field=None, alpha=0.0,
+             pref=None, refcell=0):
+    """A (CSR order) and b of one equation: discretise!, BCs, then optional
+    implicit relaxation and setReference!.  `terms` / `srcs` are fvh_term /
+    fvh_source arrays whose pointers are HOST arrays."""
+    d = mesh.desc()
+    nnz = mesh.ncells + 2 * (mesh.nfaces - mesh.boundary_faces.size)
+    vals = np.zeros(nnz)
+    b = np.zeros(mesh.ncells)
+    prev = None if prev is None else _arr(prev)
+    rf = None if relax_field is None else _arr(relax_field)
+    _check(lib().ref_assemble(C.byref(d), C.cast(terms, C.c_void_p), int(nterms),
+                              None if srcs is None else C.cast(srcs, C.c_void_p), int(nsrc), _d(prev), float(dt),
+                              C.cast(bcs_arr, C.c_void_p), int(nbc), int(comp), _d(rf), float(alpha),
+                              0 if pref is None else 1, 0.0 if pref is None else float(pref), int(refcell), _d(vals),
+                              _d(b)))
+    return vals, b
+
+
+def simple(mesh, case, fields):
+    """ref_simple: the whole run on host copies of `fields` (dict of cell
+    arrays Ux Uy Uz p [k omega nut]); returns (fields, (R_ux, R_uy, R_p), done)."""
+    d = mesh.desc()
+    f = {k: np.array(v, np.float64) for k, v in fields.items()}
+    it = case.c.iterations
+    R = [np.ones(it) for _ in range(3)]
+    done = C.c_int32()
+    g = lambda k: _d(f[k]) if k in f else None  # noqa: E731
+    _check(lib().ref_simple(C.byref(d), C.byref(case.c), g("Ux"), g("Uy"), g("Uz"), g("p"), g("k"), g("omega"),
+                            g("nut"), *(_d(r) for r in R), C.byref(done)))
+    return f, tuple(R), done.value
+
+
+def bound(mesh, field, bnd):
+    d = mesh.desc()
+    f = np.array(field, np.float64)
+    _check(lib().ref_bound(C.byref(d), _d(f), float(bnd)))
+    return f
+
+
+def strain_production(mesh, gradient, bcs_arr, nbc, ux, uy, uz):
+    d = mesh.desc()
+    out = np.zeros(mesh.ncells)
+    ux, uy, uz = _arr(ux), _arr(uy), _arr(uz)
+    _check(lib().ref_strain_production(C.byref(d), int(gradient), C.cast(bcs_arr, C.c_void_p), int(nbc), _d(ux),
+                                       _d(uy), _d(uz), _d(out)))
+    return out
+
+
+def omega_constrain(mesh, bcs_arr, nbc, k, nu, vals, b):
+    d = mesh.desc()
+    v, bb = np.array(vals, np.float64), np.array(b, np.float64)
+    _check(lib().ref_omega_constrain(C.byref(d), C.cast(bcs_arr, C.c_void_p), int(nbc), _d(_arr(k)), float(nu), _d(v),
+                                     _d(bb)))
+    return v, bb

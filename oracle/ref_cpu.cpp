@@ -1048,12 +1048,10 @@ static void bound(const Mesh &m, Vec &field, double bnd) {  // bound!: sequentia
     }
 }
 
-static void turbulence(Sim &S) {
-    const Mesh &m = S.m;
-    const double nu = S.cs.nu;
-    const double betaP = 0.09, alpha1 = 0.52, beta1 = 0.072, sigk = 0.5, sigw = 0.5;  // get_coeffs
-    // magnitude2!(Pk, S, scale_factor=2): S = 0.5*(gradU + gradU^T), M[j,k] = d U_k / d x_j
-    for (int64_t c = 0; c < m.n; ++c) {
+// magnitude2!(Pk, S, scale_factor=2): S = 0.5*(gradU + gradU^T), M[j,k] = d U_k / d x_j
+// (RANSModels_functions.jl:5-12,46-60)
+static void strain_magnitude2(Sim &S) {
+    for (int64_t c = 0; c < S.m.n; ++c) {
         double M[3][3] = {{S.gUxx[c], S.gUxy[c], S.gUxz[c]},
                           {S.gUyx[c], S.gUyy[c], S.gUyz[c]},
                           {S.gUzx[c], S.gUzy[c], S.gUzz[c]}};
@@ -1065,6 +1063,33 @@ static void turbulence(Sim &S) {
             }
         S.Pk[c] = sum * 2.0;
     }
+}
+
+// constrain_equation! of every OmegaWallFunction face (RANSModels_kOmega.jl:242-277):
+// b += A_cc * omega_wall; A_cc *= 2, once per wall face (a corner cell twice).
+static void omega_constrain(const Mesh &m, const std::vector<fvh_bc> &bw, const Vec &k, double nu, Equation &pe) {
+    for (const fvh_bc &bc : bw) {
+        if (bc.kind != FVH_BC_OMEGAWALL) continue;
+        WallConst w = wall_const(bc);
+        const Boundary &B = m.boundaries[bc.patch];
+        for (size_t i = 0; i < B.cellsID.size(); ++i) {
+            int64_t c = B.cellsID[i], f = B.facesID[i];
+            double y = m.fdelta[f];
+            double wvis = 6.0 * nu / (w.beta1 * (y * y));
+            double wlog = std::sqrt(k[c]) / (w.cmu25 * w.kappa * y);
+            double yplus = w.cmu25 * y * std::sqrt(k[c]) / nu;
+            double wc = yplus > w.ylam ? wlog : wvis;
+            pe.b[c] += pe.A.at(c, c) * wc;
+            pe.A.at(c, c) += pe.A.at(c, c);
+        }
+    }
+}
+
+static void turbulence(Sim &S) {
+    const Mesh &m = S.m;
+    const double nu = S.cs.nu;
+    const double betaP = 0.09, alpha1 = 0.52, beta1 = 0.072, sigk = 0.5, sigw = 0.5;  // get_coeffs
+    strain_magnitude2(S);
     for (int64_t c = 0; c < m.n; ++c) S.Pw[c] = alpha1 * S.Pk[c];
     for (int64_t c = 0; c < m.n; ++c) S.Dwf[c] = beta1 * S.omega[c];
     for (int64_t f = 0; f < m.nf; ++f) S.nueffw[f] = nu + sigw * S.nutf[f];
@@ -1096,21 +1121,7 @@ static void turbulence(Sim &S) {
     discretise(m, S.mw, S.pe, &S.prev, S.cs.dt);
     apply_bcs(m, S.mw, S.pe, S.bw, 0);
     implicit_relaxation(S.pe, S.prev, S.cs.vomega.relax);
-    for (const fvh_bc &bc : S.bw) {  // constrain_equation!
-        if (bc.kind != FVH_BC_OMEGAWALL) continue;
-        WallConst w = wall_const(bc);
-        const Boundary &B = m.boundaries[bc.patch];
-        for (size_t i = 0; i < B.cellsID.size(); ++i) {
-            int64_t c = B.cellsID[i], f = B.facesID[i];
-            double y = m.fdelta[f];
-            double wvis = 6.0 * nu / (w.beta1 * (y * y));
-            double wlog = std::sqrt(S.k[c]) / (w.cmu25 * w.kappa * y);
-            double yplus = w.cmu25 * y * std::sqrt(S.k[c]) / nu;
-            double wc = yplus > w.ylam ? wlog : wvis;
-            S.pe.b[c] += S.pe.A.at(c, c) * wc;
-            S.pe.A.at(c, c) += S.pe.A.at(c, c);
-        }
-    }
+    omega_constrain(m, S.bw, S.k, nu, S.pe);
     precond_update(S.Pw_);
     S.stats["omega"] = run(S.pe, S.Pw_, S.cs.vomega, S.omega);
     for (const fvh_bc &bc : S.bw) {  // constrain_boundary! -> set_cell_value!
@@ -1416,6 +1427,12 @@ int ref_sim_get(ref_sim *h, const char *name, double *out, int64_t *len) {
 
 // ---- operator-level entry points used by the unit tests ----
 
+// The wall-function elementary functions, shared with the GPU build.
+double ref_log(double x) { return fvh_log(x); }
+double ref_y_plus_laminar(double E, double kappa) { return fvh_y_plus_laminar(E, kappa); }
+double ref_nut_wall(double nu, double yplus, double kappa, double E) { return fvh_nut_wall(nu, yplus, kappa, E); }
+double ref_pow(double x, double y) { return std::pow(x, y); }
+
 double ref_det_dot(int64_t n, const double *x, const double *y) {
     return det_sum(n, [&](int64_t i) { return x[i] * y[i]; });
 }
@@ -1511,6 +1528,189 @@ int ref_grad(const fvh_mesh_desc *d, int32_t scheme, const fvh_bc *bcs, int32_t 
     std::copy(gx.begin(), gx.end(), dx);
     std::copy(gy.begin(), gy.end(), dy);
     std::copy(gz.begin(), gz.end(), dz);
+    return FVH_OK;
+}
+
+// CSC matrix on the mesh pattern from values in CSR order.
+static void csc_from_csr(const Mesh &m, CSC &A, const double *csr_vals) {
+    A = make_pattern(m);
+    std::vector<std::vector<std::pair<int64_t, int64_t>>> rows(m.n);
+    for (int64_t j = 0; j < m.n; ++j)
+        for (int64_t p = A.colptr[j]; p < A.colptr[j + 1]; ++p) rows[A.rowval[p]].push_back({j, p});
+    int64_t q = 0;
+    for (int64_t i = 0; i < m.n; ++i) {
+        std::sort(rows[i].begin(), rows[i].end());
+        for (auto &e : rows[i]) A.nzval[e.second] = csr_vals[q++];
+    }
+}
+
+// bound!(field, bound): the sequential in-place pass, field updated in place.
+int ref_bound(const fvh_mesh_desc *d, double *field, double bnd) {
+    Mesh m;
+    load_mesh(d, m);
+    Vec f(field, field + m.n);
+    bound(m, f, bnd);
+    std::copy(f.begin(), f.end(), field);
+    return FVH_OK;
+}
+
+// grad!(gradU, Uf, U, U.BCs) (scheme `gradient`) then magnitude2!(Pk, S, 2):
+// the strain-rate production of the k equation before the nut factor.
+int ref_strain_production(const fvh_mesh_desc *d, int32_t gradient, const fvh_bc *bcs, int32_t nbc, const double *Ux,
+                          const double *Uy, const double *Uz, double *Pk) {
+    Sim S;
+    load_mesh(d, S.m);
+    const int64_t n = S.m.n, nf = S.m.nf;
+    S.cs.sU.gradient = gradient;
+    S.bU.assign(bcs, bcs + nbc);
+    for (auto &bc : S.bU) bc.patch -= (int32_t)d->index_base;
+    S.Ux.assign(Ux, Ux + n);
+    S.Uy.assign(Uy, Uy + n);
+    S.Uz.assign(Uz, Uz + n);
+    for (Vec *v : {&S.Ufx, &S.Ufy, &S.Ufz}) v->assign(nf, 0.0);
+    for (Vec *v : {&S.gUxx, &S.gUyx, &S.gUzx, &S.gUxy, &S.gUyy, &S.gUzy, &S.gUxz, &S.gUyz, &S.gUzz, &S.Pk})
+        v->assign(n, 0.0);
+    grad_vector(S);
+    strain_magnitude2(S);
+    std::copy(S.Pk.begin(), S.Pk.end(), Pk);
+    return FVH_OK;
+}
+
+// constrain_equation! of the OmegaWallFunction BCs on a matrix given in CSR
+// order (values and b updated in place), with the k field and viscosity nu.
+int ref_omega_constrain(const fvh_mesh_desc *d, const fvh_bc *bcs, int32_t nbc, const double *k, double nu,
+                        double *csr_vals, double *b) {
+    Mesh m;
+    load_mesh(d, m);
+    std::vector<fvh_bc> bw(bcs, bcs + nbc);
+    for (auto &bc : bw) bc.patch -= (int32_t)d->index_base;
+    Equation eq;
+    csc_from_csr(m, eq.A, csr_vals);
+    eq.b.assign(b, b + m.n);
+    omega_constrain(m, bw, Vec(k, k + m.n), nu, eq);
+    Vec vals;
+    csr_values(eq.A, vals);
+    std::copy(vals.begin(), vals.end(), csr_vals);
+    std::copy(eq.b.begin(), eq.b.end(), b);
+    return FVH_OK;
+}
+
+// ---- per-function entry points (each mirrors one fvmhip.h Calculate /
+// Equation call; host arrays, zero-based patches after index_base) ----
+
+int ref_interpolate(const fvh_mesh_desc *d, const double *phi, double *phif) {
+    Mesh m;
+    load_mesh(d, m);
+    Vec ph(phi, phi + m.n), pf(m.nf, 0.0);
+    interpolate(m, ph, pf);
+    std::copy(pf.begin(), pf.end(), phif);
+    return FVH_OK;
+}
+
+int ref_interpolate_vector(const fvh_mesh_desc *d, const double *ux, const double *uy, double *fx, double *fy) {
+    Mesh m;
+    load_mesh(d, m);
+    Vec x(ux, ux + m.n), y(uy, uy + m.n), a(fx, fx + m.nf), b(fy, fy + m.nf);
+    interpolate_vec(m, x, y, a, b);
+    std::copy(a.begin(), a.end(), fx);
+    std::copy(b.begin(), b.end(), fy);
+    return FVH_OK;
+}
+
+// phif is updated in place (faces of the named patches only).
+int ref_correct_boundaries(const fvh_mesh_desc *d, const fvh_bc *bcs, int32_t nbc, int32_t comp, const double *phi,
+                           double *phif) {
+    Mesh m;
+    load_mesh(d, m);
+    std::vector<fvh_bc> b(bcs, bcs + nbc);
+    for (auto &bc : b) bc.patch -= (int32_t)d->index_base;
+    Vec ph(phi, phi + m.n), pf(phif, phif + m.nf);
+    correct_boundaries(m, b, comp, ph, pf);
+    std::copy(pf.begin(), pf.end(), phif);
+    return FVH_OK;
+}
+
+int ref_green_gauss(const fvh_mesh_desc *d, const double *phif, double *dx, double *dy, double *dz) {
+    Mesh m;
+    load_mesh(d, m);
+    Vec pf(phif, phif + m.nf), gx(m.n), gy(m.n), gz(m.n);
+    green_gauss(m, pf, gx, gy, gz);
+    std::copy(gx.begin(), gx.end(), dx);
+    std::copy(gy.begin(), gy.end(), dy);
+    std::copy(gz.begin(), gz.end(), dz);
+    return FVH_OK;
+}
+
+int ref_div(const fvh_mesh_desc *d, const double *fx, const double *fy, const double *fz, double *out) {
+    Mesh m;
+    load_mesh(d, m);
+    Vec a(fx, fx + m.nf), b(fy, fy + m.nf), c(fz, fz + m.nf), o(m.n);
+    div(m, a, b, c, o);
+    std::copy(o.begin(), o.end(), out);
+    return FVH_OK;
+}
+
+int ref_flux(const fvh_mesh_desc *d, const double *fx, const double *fy, const double *fz, double *mdot) {
+    Mesh m;
+    load_mesh(d, m);
+    Vec a(fx, fx + m.nf), b(fy, fy + m.nf), c(fz, fz + m.nf), o(m.nf);
+    flux(m, a, b, c, o);
+    std::copy(o.begin(), o.end(), mdot);
+    return FVH_OK;
+}
+
+// One equation assembled as a loop body assembles it: discretise! (terms in
+// model order, sources, Time{Euler} from prev with dt), then
+// apply_boundary_conditions! (BC-tuple order, component comp), then, when
+// alpha > 0, implicit_relaxation! against relax_field, and, when has_ref,
+// setReference! at refcell.  Term fluxes and phi are host arrays (faces for
+// Laplacian / Divergence, cells for Si; NULL = the constant 1).  Outputs A's
+// values in CSR order and b.
+int ref_assemble(const fvh_mesh_desc *d, const fvh_term *terms, int32_t nterms, const fvh_source *srcs, int32_t nsrc,
+                 const double *prev, double dt, const fvh_bc *bcs, int32_t nbc, int32_t comp,
+                 const double *relax_field, double alpha, int32_t has_ref, double pref, int64_t refcell,
+                 double *csr_out, double *b_out) {
+    Mesh m;
+    load_mesh(d, m);
+    std::vector<Vec> store;
+    store.reserve(2 * (size_t)nterms + (size_t)nsrc + 1);
+    Model md;
+    for (int k = 0; k < nterms; ++k) {
+        const fvh_term &t = terms[k];
+        const int64_t len = t.op == FVH_OP_SI ? m.n : m.nf;
+        const Vec *fl = nullptr, *ph = nullptr;
+        if (t.flux) {
+            store.emplace_back(t.flux, t.flux + len);
+            fl = &store.back();
+        }
+        if (t.phi) {
+            store.emplace_back(t.phi, t.phi + m.n);
+            ph = &store.back();
+        }
+        md.terms.push_back(Term{t.op, t.scheme, t.sign, fl, ph});
+    }
+    for (int k = 0; k < nsrc; ++k) {
+        store.emplace_back(srcs[k].field, srcs[k].field + m.n);
+        md.sources.push_back(Source{srcs[k].sign, &store.back()});
+    }
+    Equation eq;
+    eq.A = make_pattern(m);
+    eq.b.assign(m.n, 0.0);
+    eq.R.assign(m.n, 0.0);
+    eq.Fx.assign(m.n, 0.0);
+    Vec pv;
+    if (prev) pv.assign(prev, prev + m.n);
+    discretise(m, md, eq, prev ? &pv : nullptr, dt);
+    std::vector<fvh_bc> b(bcs, bcs + nbc);
+    for (auto &bc : b) bc.patch -= (int32_t)d->index_base;
+    int rc = apply_bcs(m, md, eq, b, comp);
+    if (rc) return rc;
+    if (alpha > 0.0 && relax_field) implicit_relaxation(eq, Vec(relax_field, relax_field + m.n), alpha);
+    set_reference(eq, has_ref != 0, pref, refcell);
+    Vec vals;
+    csr_values(eq.A, vals);
+    std::copy(vals.begin(), vals.end(), csr_out);
+    std::copy(eq.b.begin(), eq.b.end(), b_out);
     return FVH_OK;
 }
 

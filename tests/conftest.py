@@ -48,6 +48,25 @@ def trig40_path(tmp_path_factory):
     return _unzip(tmp_path_factory, "trig40.unv")
 
 
+REFERENCE_MESHES = ("quad40", "trig40", "quad100", "trig100", "backwardFacingStep_10mm", "backwardFacingStep_5mm")
+
+
+@pytest.fixture(scope="session")
+def ref_mesh(tmp_path_factory):
+    """build_mesh of one of the reference's unv_sample_meshes (fixture data),
+    scaled by 0.001 as the reference's case scripts do; cached per name."""
+    cache = {}
+
+    def get(name):
+        if name not in cache:
+            from fvmhip import build_mesh
+
+            cache[name] = build_mesh(_unzip(tmp_path_factory, name + ".unv"), scale=0.001)
+        return cache[name]
+
+    return get
+
+
 @pytest.fixture(scope="session")
 def quad40(quad40_path):
     from fvmhip import build_mesh

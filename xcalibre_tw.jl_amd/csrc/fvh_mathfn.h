@@ -2,12 +2,11 @@
 //
 // The same source is compiled for the host and for gfx950 with
 // -ffp-contract=off; it uses only IEEE +, -, *, / and comparisons, so the
-// device result is bit-identical to the host one.  Natural logarithm by the
-// classical decomposition x = m * 2^e, m in [sqrt(1/2), sqrt(2)), and
-//     log(m) = 2 atanh(s) = 2 (s + s^3/3 + s^5/5 + ...),  s = (m - 1)/(m + 1),
-// |s| <= 0.1716, summed to s^25 (truncation < 1e-19 relative), with ln 2 split
-// into a head exactly representable times e and a tail.  Accuracy is about
-// 1 ulp; it is a deterministic stand-in for the platform log, not a copy of any.
+// device result is bit-identical to the host one.  The natural logarithm is
+// evaluated in double-double arithmetic and rounded once, which makes it the
+// correctly rounded value outside extremely rare hard cases; it is written
+// from the atanh series and Dekker's error-free products, not copied from
+// any library.
 #pragma once
 
 #include <cstdint>
@@ -28,6 +27,59 @@ FVH_HD inline double fvh_jmax(double x, double y) {
     return take_y ? y : x;
 }
 
+// Double-double helpers (Dekker / Knuth error-free transformations; exact
+// only without contraction to FMA, which the build forbids).
+struct fvh_dd {
+    double hi, lo;
+};
+FVH_HD inline fvh_dd fvh_two_sum(double a, double b) {
+    const double s = a + b;
+    const double bb = s - a;
+    return {s, (a - (s - bb)) + (b - bb)};
+}
+FVH_HD inline fvh_dd fvh_fast_two_sum(double a, double b) {  // |a| >= |b|
+    const double s = a + b;
+    return {s, b - (s - a)};
+}
+FVH_HD inline fvh_dd fvh_two_prod(double a, double b) {
+    const double p = a * b;
+    const double ca = 134217729.0 * a, cb = 134217729.0 * b;  // 2^27 + 1: Veltkamp split
+    const double ah = ca - (ca - a), al = a - ah;
+    const double bh = cb - (cb - b), bl = b - bh;
+    return {p, ((ah * bh - p) + ah * bl + al * bh) + al * bl};
+}
+FVH_HD inline fvh_dd fvh_dd_add(fvh_dd a, fvh_dd b) {
+    fvh_dd s = fvh_two_sum(a.hi, b.hi);
+    const double t = s.lo + (a.lo + b.lo);
+    return fvh_fast_two_sum(s.hi, t);
+}
+FVH_HD inline fvh_dd fvh_dd_mul(fvh_dd a, fvh_dd b) {
+    fvh_dd p = fvh_two_prod(a.hi, b.hi);
+    const double t = p.lo + (a.hi * b.lo + a.lo * b.hi);
+    return fvh_fast_two_sum(p.hi, t);
+}
+FVH_HD inline fvh_dd fvh_dd_div(fvh_dd a, fvh_dd b) {
+    const double q1 = a.hi / b.hi;
+    fvh_dd r = fvh_dd_add(a, fvh_dd_mul({-q1, 0.0}, b));
+    const double q2 = r.hi / b.hi;
+    r = fvh_dd_add(r, fvh_dd_mul({-q2, 0.0}, b));
+    const double q3 = r.hi / b.hi;
+    return fvh_dd_add(fvh_fast_two_sum(q1, q2), {q3, 0.0});
+}
+// 1 / k as a double-double: hi the rounded quotient, lo the rounded rest.
+FVH_HD inline fvh_dd fvh_dd_recip(double k) {
+    const double hi = 1.0 / k;
+    const fvh_dd p = fvh_two_prod(hi, k);  // hi * k = p.hi + p.lo exactly
+    return {hi, ((1.0 - p.hi) - p.lo) / k};
+}
+
+// Natural logarithm.  x = 2^e m with m in [sqrt(1/2), sqrt(2)), and
+//     log(m) = 2 atanh(s) = 2 (s + s^3/3 + ... + s^25/25),  s = (m - 1)/(m + 1),
+// |s| <= 0.1716 (the omitted terms are below 2^-60 relative), evaluated in
+// double-double arithmetic (about 100 significant bits), and e ln 2 added in
+// double-double before the one final rounding.  The result is the correctly
+// rounded logarithm except where the exact value lies within about 2^-45 ulp
+// of a rounding boundary.
 FVH_HD inline double fvh_log(double x) {
     if (x != x) return x;
     if (x < 0.0) return __builtin_nan("");
@@ -45,30 +97,26 @@ FVH_HD inline double fvh_log(double x) {
     double m;
     __builtin_memcpy(&m, &mb, sizeof m);  // m in [1, 2)
     if (m > 1.4142135623730951) {
-        m = m * 0.5;
+        m = m * 0.5;  // exact
         ex += 1;
     }
     e += ex;
-    const double s = (m - 1.0) / (m + 1.0);
-    const double s2 = s * s;
-    // Horner form of 1 + s2/3 + s2^2/5 + ... + s2^12/25
-    double p = 1.0 / 25.0;
-    p = p * s2 + 1.0 / 23.0;
-    p = p * s2 + 1.0 / 21.0;
-    p = p * s2 + 1.0 / 19.0;
-    p = p * s2 + 1.0 / 17.0;
-    p = p * s2 + 1.0 / 15.0;
-    p = p * s2 + 1.0 / 13.0;
-    p = p * s2 + 1.0 / 11.0;
-    p = p * s2 + 1.0 / 9.0;
-    p = p * s2 + 1.0 / 7.0;
-    p = p * s2 + 1.0 / 5.0;
-    p = p * s2 + 1.0 / 3.0;
-    const double lm = 2.0 * s + 2.0 * s * (s2 * p);
-    const double ln2_hi = 6.93147180369123816490e-01;  // 0x3FE62E42FEE00000, 11 trailing zero bits
-    const double ln2_lo = 1.90821492927058770002e-10;  // ln 2 - ln2_hi
+    // s = (m - 1) / (m + 1): m - 1 is exact (Sterbenz), m + 1 exact as a double-double
+    const fvh_dd s = fvh_dd_div({m - 1.0, 0.0}, fvh_two_sum(m, 1.0));
+    const fvh_dd s2 = fvh_dd_mul(s, s);
+    fvh_dd p = fvh_dd_recip(25.0);
+    for (int k = 23; k >= 3; k -= 2) p = fvh_dd_add(fvh_dd_mul(p, s2), fvh_dd_recip((double)k));
+    p = fvh_dd_add(fvh_dd_mul(p, s2), {1.0, 0.0});
+    fvh_dd lm = fvh_dd_mul(s, p);
+    lm = {2.0 * lm.hi, 2.0 * lm.lo};  // exact
+    // ln 2 = L1 + L2 + L3: L1 has 11 trailing zero bits, so e * L1 is exact for |e| < 2048
+    const double L1 = 6.93147180369123816490e-01;  // 0x3FE62E42FEE00000
+    const double L2 = 1.90821492927058770002e-10;  // 0x3DEA39EF35793C76
+    const double L3 = 1.1612227229362532e-26;      // ln 2 - L1 - L2, rounded
     const double de = (double)e;
-    return (de * ln2_hi + (lm + de * ln2_lo));
+    const fvh_dd eln2 = fvh_dd_add({de * L1, 0.0}, fvh_dd_add(fvh_two_prod(de, L2), {de * L3, 0.0}));
+    const fvh_dd r = fvh_dd_add(eln2, lm);
+    return r.hi + r.lo;
 }
 
 // y_plus_laminar(E, kappa)  [src/RANSModels/RANSModels_kOmega.jl:200-203]

@@ -1261,13 +1261,30 @@ int skew_dinv_natural(fvh_eqn *e, double *out) {
 }
 
 // A hand-over wait of a sweep ran out since the last check (reported once).
+// The sweeps of an equation that borrows another's preconditioner record
+// their faults in the owner's flag, so both are read.
 int skew_check(fvh_eqn *e) {
-    int *f = e->sk.h_fault;
-    if (f && __atomic_load_n(f, __ATOMIC_ACQUIRE)) {
-        *f = 0;
-        return set_error(FVH_ERR_HIP, "band sweep: a hand-over between waves did not arrive in time");
+    fvh_eqn *owners[2] = {e, e->pshare};
+    for (fvh_eqn *q : owners) {
+        int *f = q ? q->sk.h_fault : nullptr;
+        if (f && __atomic_load_n(f, __ATOMIC_ACQUIRE)) {
+            __atomic_store_n(f, 0, __ATOMIC_RELEASE);
+            return set_error(FVH_ERR_HIP, "band sweep: a hand-over between waves did not arrive in time");
+        }
     }
     return FVH_OK;
 }
 
 }  // namespace fvh
+
+// Reads, and with raise != 0 sets, the pending hand-over fault of the
+// equation's pipelined sweeps (pinned host memory; no device work).
+extern "C" int fvh_precond_fault(fvh_eqn *e, int32_t raise, int32_t *pending) {
+    if (!e) return fvh::set_error(FVH_ERR_ARG, "null equation");
+    fvh_eqn *q = e->pshare ? e->pshare : e;
+    int *f = q->sk.h_fault;
+    if (!f) return fvh::set_error(FVH_ERR_UNSUPPORTED, "the equation's preconditioner has no pipelined sweeps");
+    if (raise) __atomic_store_n(f, 1, __ATOMIC_RELEASE);
+    if (pending) *pending = __atomic_load_n(f, __ATOMIC_ACQUIRE);
+    return FVH_OK;
+}

@@ -1047,14 +1047,14 @@ extern "C" int fvh_precond_update(fvh_eqn *e) {
     if (!e) return set_error(FVH_ERR_ARG, "null equation");
     FVH_TRY(precond_update(e));
     HIP_TRY(hipStreamSynchronize(stream()));
-    return FVH_OK;
+    return skew_check(e);
 }
 
 extern "C" int fvh_precond_apply(fvh_eqn *e, double *y, const double *x) {
     if (!e || !x || !y) return set_error(FVH_ERR_ARG, "null argument");
     FVH_TRY(precond_apply(e, y, x, nullptr, 1.0));
     HIP_TRY(hipStreamSynchronize(stream()));
-    return FVH_OK;
+    return skew_check(e);
 }
 
 int fvh::precond_dinv_natural(fvh_eqn *e, const double **out) {
@@ -1100,7 +1100,7 @@ extern "C" int fvh_precond_apply_f32(fvh_eqn *e, float *y, const float *x) {
     FVH_TRY(prepare32(e));
     FVH_TRY(precond_apply32(e, y, x, nullptr));
     HIP_TRY(hipStreamSynchronize(stream()));
-    return FVH_OK;
+    return skew_check(e);
     FVH_GUARD_END
 }
 
@@ -1157,6 +1157,6 @@ extern "C" int fvh_time_op(fvh_eqn *e, int32_t op, int32_t reps, const double *x
     (void)hipEventDestroy(a);
     (void)hipEventDestroy(b);
     *ms = (double)t / reps;
-    return FVH_OK;
+    return skew_check(e);
     FVH_GUARD_END
 }

@@ -121,6 +121,7 @@ SIGNATURES = {
     "fvh_precond_update": (C.c_int, [_VP]),
     "fvh_precond_apply": (C.c_int, [_VP, _VP, _VP]),
     "fvh_precond_dinv": (C.c_int, [_VP, P_D]),
+    "fvh_precond_fault": (C.c_int, [_VP, C.c_int32, P_I32]),
     "fvh_solve": (C.c_int, [_VP, C.POINTER(SolverCfg), _VP, C.POINTER(Stats)]),
     "fvh_spmv_f32": (C.c_int, [_VP, _VP, _VP]),
     "fvh_mesh_set_cases": (C.c_int, [_VP, C.c_int32, P_I64]),
