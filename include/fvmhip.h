@@ -307,6 +307,14 @@ int fvh_spmv(fvh_eqn *e, const double *x, double *y);
 /* residual!(R, eqn, phi, it): sqrt(mean((Ax-b)^2))/norm(b) -> *res (host)
  * [src/Solvers/Solvers_0_functions.jl:13-63] */
 int fvh_residual(fvh_eqn *e, const double *phi, double *res);
+/* fvh_residual without any allocation: no counterpart in the reference; for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  The chunk partials
+ * and the result live in a file-scope device array of 128 KB that each GPU
+ * holds until the process ends; at most 16,777,216 cells (FVH_ERR_SIZE
+ * beyond).  Same bits as fvh_residual.  Calls that overlap on one GPU share
+ * that array and race. */
+int fvh_residual_noalloc(fvh_eqn *e, const double *phi, double *res);
 
 /* set_preconditioner / update_preconditioner!  [src/Solve/Preconditioners/
  * preconditioners_2_functions.jl:4-72].  `matrix_of` names the equation whose

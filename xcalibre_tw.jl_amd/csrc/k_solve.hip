@@ -1018,6 +1018,63 @@ extern "C" int fvh_residual(fvh_eqn *e, const double *phi, double *res) {
     return residual(e, phi, res);
 }
 
+// ---------------------------------------------------------------------------
+// residual! for callers that cannot allocate: the chunk partials and the
+// result live in `scratch`, a file-scope device array of which every GPU
+// holds one copy for the life of the process.  Its device address is taken
+// once per device (hipGetSymbolAddress, with that device current) and only
+// that address reaches kernels and copies: in host code `scratch` names a
+// host-side stand-in, not device memory.
+// ---------------------------------------------------------------------------
+namespace fvh {
+constexpr int kNoallocMaxChunks = 8192;                       // n <= 8192 * 2048 = 16,777,216 cells
+constexpr int kNoallocResult = 2 * kNoallocMaxChunks;         // result word after both partial arrays
+constexpr int kNoallocWords = kNoallocResult + 32;            // 128 KB + one 256-byte line
+static __device__ __align__(256) double scratch[kNoallocWords];
+
+constexpr int kNoallocMaxDevices = 64;
+static double *g_scratch_ptr[kNoallocMaxDevices];
+
+// Device address of `scratch` on the current device, looked up on the
+// device's first call and kept.
+static int noalloc_scratch(double **out) {
+    int dev = 0;
+    HIP_TRY(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kNoallocMaxDevices)
+        return set_error(FVH_ERR_SIZE, "device %d beyond the %d devices of the no-allocation entry points", dev,
+                         kNoallocMaxDevices);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        HIP_TRY(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) return set_error(FVH_ERR_HIP, "hipGetSymbolAddress(scratch) returned a null address");
+        g_scratch_ptr[dev] = (double *)p;
+    }
+    *out = g_scratch_ptr[dev];
+    return FVH_OK;
+}
+}  // namespace fvh
+
+extern "C" int fvh_residual_noalloc(fvh_eqn *e, const double *phi, double *res) {
+    if (!e || !phi || !res) return set_error(FVH_ERR_ARG, "null argument");
+    const fvh_mesh *m = e->m;
+    const int nch = nchunks(m->n);
+    if (nch > kNoallocMaxChunks)
+        return set_error(FVH_ERR_SIZE, "fvh_residual_noalloc: %lld cells beyond the maximum of %d",
+                         (long long)m->n, kNoallocMaxChunks * kChunk);
+    double *s = nullptr;
+    FVH_TRY(noalloc_scratch(&s));
+    // Every partial this call reads is written by this call (one per chunk,
+    // per sum) before the final reduction, so nothing needs clearing.
+    hipLaunchKernelGGL(k_residual_part, dim3(nch), dim3(kWide), 0, stream(), (int)m->n, m->d.rowptr, m->d.col, e->nz.p,
+                       phi, e->b.p, s, nch);
+    HIP_TRY(hipGetLastError());
+    hipLaunchKernelGGL(k_residual_final, dim3(1), dim3(kBlock), 0, stream(), s, nch, (int)m->n, s + kNoallocResult);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(res, s + kNoallocResult, sizeof(double), hipMemcpyDeviceToHost, stream()));
+    HIP_TRY(hipStreamSynchronize(stream()));
+    return FVH_OK;
+}
+
 extern "C" int fvh_precond_set(fvh_eqn *e, int32_t kind, fvh_eqn *matrix_of) {
     FVH_GUARD_BEGIN
     if (!e) return set_error(FVH_ERR_ARG, "null equation");

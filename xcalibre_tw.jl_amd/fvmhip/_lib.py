@@ -117,6 +117,7 @@ SIGNATURES = {
     "fvh_set_reference": (C.c_int, [_VP, C.c_double, C.c_int64]),
     "fvh_spmv": (C.c_int, [_VP, _VP, _VP]),
     "fvh_residual": (C.c_int, [_VP, _VP, P_D]),
+    "fvh_residual_noalloc": (C.c_int, [_VP, _VP, P_D]),
     "fvh_precond_set": (C.c_int, [_VP, C.c_int32, _VP]),
     "fvh_precond_update": (C.c_int, [_VP]),
     "fvh_precond_apply": (C.c_int, [_VP, _VP, _VP]),
