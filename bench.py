@@ -105,11 +105,115 @@ def run_gpu(args, precision):
         b = RL.dilu_apply(n, nnz, 4, 4, 4, 4, 4)
         t = timed(4, 20)
         ops["dilu_apply_fp32"] = dict(ms=t, bytes=b, GBps=RL.gbps(b, t))
+    # loop phases on the simulation's own fields (HIP events, library stream)
+    nif = mesh.nfaces - mesh.boundary_faces.size
+    phases = {}
+    for op, key, reps, nbytes in (
+            (0, "assembly_ux", 50, RL.assembly(nif, nnz, n, k_f=4, k_c=3)),
+            (1, "assembly_p", 50, RL.assembly(nif, nnz, n, k_f=3, k_c=2)),
+            (2, "green_gauss_p", 100, RL.green_gauss(n, nif))):
+        L.check(L.lib().fvh_sim_time_op(sim.h, op, reps, C.byref(ms)))
+        phases[key] = dict(ms=ms.value, bytes=nbytes, GBps=RL.gbps(nbytes, ms.value),
+                           frac=RL.gbps(nbytes, ms.value) / RL.HBM_PEAK_GBS)
     res["ops"] = ops
+    res["phases"] = phases
     res["n"] = n
     res["nnz"] = nnz
     del sim
     return res
+
+
+def run_spmv_large(args):
+    """SpMV on a working set larger than the 256 MB Infinity Cache: a
+    4000 x 1000 channel (4M cells, nnz 20M; CSR 336 MB with x and y), with
+    synthetic diagonally dominant values, timed over back-to-back launches."""
+    import fvmhip as F
+    from fvmhip import _lib as L
+    from fvmhip import roofline as RL
+
+    nx, ny = args.spmv_large
+    t0 = time.time()
+    mesh = F.channel(nx, ny)
+    dm = mesh.device()
+    log(f"[bench] spmv-large: {mesh.ncells} cells, mesh {time.time() - t0:.1f}s")
+    h = C.c_void_p()
+    L.check(L.lib().fvh_eqn_create(dm.h, C.byref(h)))
+    nz, b = C.c_void_p(), C.c_void_p()
+    L.check(L.lib().fvh_eqn_arrays(h, C.byref(nz), C.byref(b)))
+    rp, ci = dm.pattern()
+    n, nnz = mesh.ncells, int(ci.size)
+    rows = np.repeat(np.arange(n), np.diff(rp))
+    vals = np.where(rows == ci, 4.5, -0.5 - 0.1 * np.random.default_rng(1).random(ci.size))
+    L.check(L.lib().fvh_memcpy(nz, vals.ctypes.data_as(C.c_void_p), vals.nbytes))
+    del rows, vals
+    x = L.DeviceArray.from_host(np.random.default_rng(0).standard_normal(n))
+    y = L.DeviceArray(n)
+    ms = C.c_double()
+    out = dict(cells=n, nnz=nnz, workload=f"{nx}x{ny} channel")
+    for op, key, nbytes in ((0, "csr_fp64", RL.spmv(n, nnz)), (6, "band_fp64", RL.spmv_band(n))):
+        L.check(L.lib().fvh_time_op(h, op, 50, x.vp, y.vp, C.byref(ms)))
+        out[key] = dict(ms=ms.value, bytes=nbytes, GBps=RL.gbps(nbytes, ms.value),
+                        frac=RL.gbps(nbytes, ms.value) / RL.HBM_PEAK_GBS)
+        log(f"[bench] spmv-large {key}: {ms.value * 1e3:.1f} us, {RL.gbps(nbytes, ms.value):.0f} GB/s "
+            f"({nbytes / 1e6:.0f} MB)")
+    L.lib().fvh_eqn_destroy(h)
+    return out
+
+
+def host_cpu():
+    """Model name of the host CPU and the cores this process may use."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except AttributeError:
+        avail = os.cpu_count()
+    return model, avail
+
+
+def run_to_break(args):
+    """C2 run to the 1e-6 convergence break (Solvers_1_SIMPLE.jl:200-215) in
+    both forms, capped at --max-iters: iterations to the break, time to
+    solution, and the mixed form's per-field relative L2 error against the
+    FP64 form."""
+    import fvmhip as F
+    from fvmhip import _lib as L
+    from tests.cases import rel_l2
+
+    out, fields = {}, {}
+    for prec, name in ((L.FP64, "fp64"), (L.MIXED, "mixed")):
+        mesh, model, cfg = make_case(args.nx, args.ny, args.max_iters, prec)
+        sim = F.Simulation(model, cfg, None, L.ALG_SIMPLE, conv_break=1e-6)
+        L.check(L.lib().fvh_synchronize())
+        t1 = time.perf_counter()
+        done, chunk = 0, 200
+        while done < args.max_iters:
+            k = sim.run(min(chunk, args.max_iters - done))
+            done += k
+            R = sim.residuals()
+            last = [float(R[q][done - 1]) for q in range(3)]
+            log(f"[bench] to-break {name}: {done} iterations, {time.perf_counter() - t1:.1f}s, residuals {last}")
+            if k < chunk or max(last) <= 1e-6:
+                break
+        L.check(L.lib().fvh_synchronize())
+        dt = time.perf_counter() - t1
+        R = sim.residuals()
+        last = [float(R[q][done - 1]) for q in range(3)]
+        out[name] = dict(iterations=done, seconds=dt, converged=bool(max(last) <= 1e-6), residuals_last=last,
+                         iters_per_s=done / dt, cell_updates_per_s=mesh.ncells * done / dt)
+        fields[name] = {f: sim.get(f) for f in ("Ux", "Uy", "p")}
+        del sim
+    out["time_to_solution_mixed_vs_fp64"] = out["fp64"]["seconds"] / out["mixed"]["seconds"]
+    out["rel_l2_mixed_vs_fp64"] = {f: rel_l2(fields["mixed"][f], fields["fp64"][f]) for f in ("Ux", "Uy", "p")}
+    out["cells"] = args.nx * args.ny
+    return out
 
 
 def run_batch(args):
@@ -295,6 +399,12 @@ def main():
     ap.add_argument("--workload", default="c2", choices=["c2", "c3", "c4"],
                     help="c2: the headline BFS (default); c3: Delaunay transport solve; c4: k-omega flat plate")
     ap.add_argument("--c3-points", type=int, default=1_000_000)
+    ap.add_argument("--spmv-large", type=int, nargs=2, default=[4000, 1000], metavar=("NX", "NY"),
+                    help="grid of the SpMV leg whose working set exceeds the Infinity Cache")
+    ap.add_argument("--no-spmv-large", action="store_true")
+    ap.add_argument("--to-break", action="store_true",
+                    help="run C2 to the 1e-6 break in both forms (minutes) and print that record instead")
+    ap.add_argument("--max-iters", type=int, default=6000)
     ap.add_argument("--profile-ops", action="store_true",
                     help="run each operator once on the C2 case (copy, SpMV, DILU apply; FP64 and FP32), for counters")
     args = ap.parse_args()
@@ -316,6 +426,9 @@ def main():
         res = run_c3(args) if args.workload == "c3" else run_c4(args)
         print(json.dumps({"workload": args.workload, **res}))
         return
+    if args.to_break:
+        print(json.dumps({"workload": "c2-to-break", **run_to_break(args)}))
+        return
     forms = {"fp64": [L.FP64], "mixed": [L.MIXED], "both": [L.FP64, L.MIXED]}[args.precision]
     out = {}
     for p in forms:
@@ -328,9 +441,12 @@ def main():
     cpu = None
     if not args.no_cpu:
         c = run_cpu(args)
+        model, avail = host_cpu()
         cpu = dict(value=c["cell_updates_per_s"], unit="cell-updates/s", cores=1, kind="port",
+                   host_cpu=model, host_cores_available=avail,
                    sample=f"{c['iters']} SIMPLE iterations of the same {args.nx}x{args.ny} case, oracle FP64 "
-                          f"restatement, {c['seconds']:.1f}s (set-up {c['setup_s']:.1f}s excluded)")
+                          f"restatement, {c['seconds']:.1f}s (set-up {c['setup_s']:.1f}s excluded), "
+                          f"1 of {avail} cores of a {model}")
         log(f"[bench] cpu: {c['iters_per_s']:.4f} it/s")
     # dominant kernel of the step: the DILU apply (banded sweep), called by every Krylov iteration
     dk = "dilu_apply_fp64" if hk == "fp64" else "dilu_apply_fp32"
@@ -364,6 +480,9 @@ def main():
         "simple_iters_per_s": head["iters_per_s"],
         "roofline": {"bound": "hbm", "achieved": d["GBps"], "peak": RL.HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": d["GBps"] / RL.HBM_PEAK_GBS, "traffic": traffic,
+                     "traffic_source": ("rocprofv3 FETCH_SIZE / WRITE_SIZE passes of bench.py --profile-ops, "
+                                        "committed in profiles/c2_operators_hbm_traffic.json (not measured in "
+                                        "this run)") if traffic is not None else None,
                      "kernel": "DILU apply: k_skew_gather + k_skew_fwd + k_skew_bwd + k_skew_scatter "
                                "(pipelined anti-diagonal sweeps)",
                      "bytes_per_launch": d["bytes"], "ms_per_launch": d["ms"]},
@@ -375,9 +494,12 @@ def main():
                        "bytes_per_launch": head["ops"]["spmv_band_fp64"]["bytes"],
                        "ms_per_launch": head["ops"]["spmv_band_fp64"]["ms"]}
                       if "spmv_band_fp64" in head["ops"] else None),
+        "phases": {k: {q: head["phases"][k][q] for q in ("GBps", "frac", "ms", "bytes")} for k in head["phases"]},
         "cpu_baseline": cpu,
         "forms": out,
     }
+    if not args.no_spmv_large:
+        line["spmv_large"] = run_spmv_large(args)
     if "fp64" in out and "mixed" in out:
         line["mixed_vs_fp64"] = out["mixed"]["iters_per_s"] / out["fp64"]["iters_per_s"]
     if not args.no_batch:

@@ -61,7 +61,9 @@ enum { FVH_BC_DIRICHLET = 0, FVH_BC_NEUMANN = 1, FVH_BC_KWALL = 2,
  *                          src/Solve/Preconditioners/preconditioners_0_types.jl:6-12] */
 enum { FVH_BICGSTAB = 0, FVH_CG = 1, FVH_GMRES = 2 };
 enum { FVH_PRECOND_NONE = 0, FVH_PRECOND_DILU = 1, FVH_PRECOND_ILU0 = 2,
-       FVH_PRECOND_JACOBI = 3, FVH_PRECOND_NORMDIAG = 4 };
+       FVH_PRECOND_JACOBI = 3, FVH_PRECOND_NORMDIAG = 4,
+       FVH_PRECOND_LDL = 5 };  /* factor of the matrix at the first update, kept
+                                  (opLDL, preconditioners_0_types.jl:36-42) */
 
 /* Arithmetic of the Krylov inner solve */
 enum { FVH_FP64 = 0,          /* plain FP64, bit-identical to the oracle             */
@@ -169,6 +171,9 @@ typedef struct fvh_case {
     fvh_bcset U, p, k, omega, nut;
     fvh_schemes sU, sp, sk, somega;
     fvh_solver_cfg vU, vp, vk, vomega;
+    int32_t nonorth_correctors;  /* the reference's disabled `correct` block
+                                    (Solvers_1_SIMPLE.jl:165-180): 0 = off   */
+    int32_t pad_;
 } fvh_case;
 
 /* Per-solve Krylov statistics */
@@ -220,6 +225,12 @@ int fvh_hmesh_read_unv(const char *path, double scale, fvh_hmesh **out);
 /* Fills a zero-based description whose arrays live as long as the hmesh. */
 int fvh_hmesh_desc(const fvh_hmesh *m, fvh_mesh_desc *out);
 int fvh_hmesh_boundary_name(const fvh_hmesh *m, int64_t i, char *buf, int64_t len);
+/* The mesh's nodes (scaled, xyz interleaved) and each cell's nodes in
+ * element order, zero based (cell c: cell_nodes[cell_nodes_ptr[c] ..
+ * cell_nodes_ptr[c+1]-1]); valid while the hmesh lives
+ * [mesh.nodes, mesh.cell_nodes: src/Mesh/Mesh_0_types.jl, UNV2_3_builder.jl:165-175] */
+int fvh_hmesh_nodes(const fvh_hmesh *m, int64_t *npoints, const double **xyz, const int64_t **cell_nodes_ptr,
+                    const int64_t **cell_nodes);
 void fvh_hmesh_free(fvh_hmesh *m);
 
 /* ------------------------------------------------------------------ */
@@ -279,6 +290,66 @@ int fvh_div(const fvh_mesh *m, const double *psif_x, const double *psif_y,
 /* flux!(phif, psif)  [src/Solvers/Solvers_0_functions.jl:65-73] */
 int fvh_flux(const fvh_mesh *m, const double *psif_x, const double *psif_y,
              const double *psif_z, double *phif);
+/* ------------------------------------------------------------------ */
+/* Output and post-processing                                          */
+/* ------------------------------------------------------------------ */
+/* write_vtk(name, mesh, args...): legacy ASCII VTK of cell data in the
+ * reference's layout, numbers printed as Julia prints Float64.  Host
+ * arrays: xyz [3*npoints], cell nodes zero based; field k is a scalar
+ * (ncomp 1: data[k][ncells]) or a vector (ncomp 3: x, y, z blocks of
+ * ncells)  [src/VTK/VTK_writer.jl:22-116] */
+int fvh_write_vtk(const char *path, int64_t npoints, const double *xyz, int64_t ncells,
+                  const int64_t *cell_nodes_ptr, const int64_t *cell_nodes, int32_t nfields,
+                  const char *const *names, const int32_t *ncomp, const double *const *data);
+/* The number format of fvh_write_vtk: v as Julia's print(::Float64) writes
+ * it, into buf (at least 32 bytes). */
+int fvh_format_julia(double v, char *buf, int64_t len);
+/* Faces of boundary patch `patch` (zero based), in patch order. */
+int fvh_patch_size(const fvh_mesh *m, int32_t patch, int64_t *nfaces);
+/* surface_flux(snflux, facesID, cellsID, phi): phi[cell] area normal per
+ * patch face; device arrays  [src/Calculate/Calculate_5_surface_normal_gradient.jl:3-18] */
+int fvh_surface_flux(const fvh_mesh *m, int32_t patch, const double *phi, double *fx, double *fy, double *fz);
+/* surface_normal_gradient(snGrad, facesID, cellsID, U, Uw): the part of
+ * U[cell] - Uw parallel to the face over delta; uw: host double[3]
+ * [src/Calculate/Calculate_5_surface_normal_gradient.jl:20-36] */
+int fvh_surface_normal_gradient(const fvh_mesh *m, int32_t patch, const double *ux, const double *uy,
+                                const double *uz, const double *uw, double *gx, double *gy, double *gz);
+/* pressure_force(patch, p, rho) -> force[3] (host), summed over the patch
+ * faces in order  [src/Postprocess/Postprocess_forces.jl:5-27] */
+int fvh_pressure_force(const fvh_mesh *m, int32_t patch, const double *p, double rho, double *force);
+/* viscous_force(patch, U, rho, nu, nut) -> force[3] (host); uw = the
+ * patch's velocity BC value (host double[3]) or NULL when no velocity BC
+ * names the patch (the force is then zero); nut: device cell array or NULL
+ * (zero)  [src/Postprocess/Postprocess_forces.jl:29-60] */
+int fvh_viscous_force(const fvh_mesh *m, int32_t patch, const double *ux, const double *uy, const double *uz,
+                      const double *uw, double rho, double nu, const double *nut, double *force);
+/* wall_shear_stress(patch, model): tau = snGrad (nu + nut[cell]) per patch
+ * face (device arrays) and the face centres (device [3*nfaces], may be
+ * NULL)  [src/Postprocess/Postprocess_forces.jl:62-96] */
+int fvh_wall_shear_stress(const fvh_mesh *m, int32_t patch, const double *ux, const double *uy, const double *uz,
+                          const double *uw, double nu, const double *nut, double *tx, double *ty, double *tz,
+                          double *pos);
+
+/* bound!(field, bound): in place, with the reference's in-order semantics
+ * (a negative cell takes the mean of max(neighbour, eps) over its
+ * neighbours, seeing those bounded earlier in the same pass)
+ * [src/RANSModels/RANSModels_functions.jl:62-88].  field: device array of
+ * ncells.  *chained (may be NULL) = cells that needed the ordered pass. */
+int fvh_bound(const fvh_mesh *m, double *field, double bound, int32_t *chained);
+
+/* Non-orthogonal correction operators.  The reference keeps them behind a
+ * switch that is off (`correct = false`, Solvers_1_SIMPLE.jl:165-180) and
+ * defines their interpolation weight for the Linear scheme only; these
+ * entries use that weight whatever the gradient scheme.
+ * interpolate!(gradf, grad, phi): internal faces only, device arrays
+ * [src/Calculate/Calculate_2_interpolation.jl:152-176,
+ *  Calculate_3_orthogonality_correction.jl:81-93] */
+int fvh_interpolate_gradient(const fvh_mesh *m, const double *gx, const double *gy, const double *gz,
+                             const double *phi, double *gfx, double *gfy, double *gfz);
+/* nonorthogonal_flux!(phif, gradf): phif = gradf . (n - e) area on internal
+ * faces (boundary faces untouched)  [Calculate_3_orthogonality_correction.jl:66-79] */
+int fvh_nonorthogonal_flux(const fvh_mesh *m, const double *gfx, const double *gfy, const double *gfz,
+                           double *phif);
 
 /* ------------------------------------------------------------------ */
 /* Equation: A (CSR values on the mesh pattern), b, R, Fx on the device  */
@@ -302,6 +373,10 @@ int fvh_apply_bcs(fvh_eqn *e, const fvh_term *terms, int32_t nterms,
 int fvh_implicit_relax(fvh_eqn *e, const double *phi, double alpha);
 /* setReference!(eqn, pref, cellID)  [src/Solve/Solve_1_api.jl:62-69]; cell is 0-based */
 int fvh_set_reference(fvh_eqn *e, double pref, int64_t cell);
+/* correct!(eqn, term, corr_flux): b[owner] += -sign J_f corr_f and
+ * b[neighbour] -= the same, over every face in id order (device arrays J,
+ * corr of nfaces)  [src/Calculate/Calculate_3_orthogonality_correction.jl:6-24] */
+int fvh_nonorthogonal_correct(fvh_eqn *e, int32_t sign, const double *J, const double *corr);
 /* mul!(y, A, x)  [SparseArrays CSC mul!, used at Solvers_0_functions.jl:18] */
 int fvh_spmv(fvh_eqn *e, const double *x, double *y);
 /* residual!(R, eqn, phi, it): sqrt(mean((Ax-b)^2))/norm(b) -> *res (host)
@@ -389,6 +464,13 @@ int fvh_sim_stats(fvh_sim *s, const char *field, fvh_stats *out);
 int fvh_sim_timing(fvh_sim *s, double *ms_assembly, double *ms_solve, double *ms_other);
 /* The equation handle of "ux", "uy" or "p" (owned by the sim). */
 int fvh_sim_equation(fvh_sim *s, const char *name, fvh_eqn **out);
+/* Measurement helper (no counterpart in the reference): average device time
+ * of one launch of a loop phase on the simulation's current fields, by HIP
+ * events on the library stream.  op 0 = the Ux momentum assembly as the loop
+ * runs it (discretise!, sources, BCs, relaxation), 1 = the pressure
+ * equation's assembly, 2 = the Green-Gauss kernel on p's face values.  The
+ * phase's outputs are recomputed in place from the current fields. */
+int fvh_sim_time_op(fvh_sim *s, int32_t op, int32_t reps, double *ms_per_launch);
 
 /* Measurement helper (no counterpart in the reference): average device time
  * of one launch of an operator over `reps` back-to-back launches, by HIP

@@ -51,6 +51,14 @@ def lib():
         L.ref_green_gauss.argtypes = [vp, P_D, P_D, P_D, P_D]
         L.ref_div.argtypes = [vp, P_D, P_D, P_D, P_D]
         L.ref_flux.argtypes = [vp, P_D, P_D, P_D, P_D]
+        L.ref_interpolate_gradient.argtypes = [vp, P_D, P_D, P_D, P_D, P_D, P_D, P_D]
+        L.ref_surface_flux.argtypes = [vp, C.c_int32, P_D, P_D, P_D, P_D]
+        L.ref_surface_normal_gradient.argtypes = [vp, C.c_int32, P_D, P_D, P_D, P_D, P_D, P_D, P_D]
+        L.ref_pressure_force.argtypes = [vp, C.c_int32, P_D, C.c_double, P_D]
+        L.ref_viscous_force.argtypes = [vp, C.c_int32, P_D, P_D, P_D, P_D, C.c_double, C.c_double, P_D, P_D]
+        L.ref_wall_shear_stress.argtypes = [vp, C.c_int32, P_D, P_D, P_D, P_D, C.c_double, P_D, P_D, P_D, P_D, P_D]
+        L.ref_nonorthogonal_flux.argtypes = [vp, P_D, P_D, P_D, P_D]
+        L.ref_nonorthogonal_correct.argtypes = [vp, C.c_int32, P_D, P_D, P_D]
         L.ref_assemble.argtypes = [vp, vp, C.c_int32, vp, C.c_int32, P_D, C.c_double, vp, C.c_int32, C.c_int32,
                                    P_D, C.c_double, C.c_int32, C.c_double, C.c_int64, P_D, P_D]
         L.ref_bound.argtypes = [vp, P_D, C.c_double]
@@ -217,6 +225,86 @@ def flux(mesh, fx, fy, fz):
     out = np.zeros(mesh.nfaces)
     fx, fy, fz = _arr(fx), _arr(fy), _arr(fz)
     _check(lib().ref_flux(C.byref(d), _d(fx), _d(fy), _d(fz), _d(out)))
+    return out
+
+
+def interpolate_gradient(mesh, gx, gy, gz, phi, fx=None, fy=None, fz=None):
+    """interpolate!(gradf, grad, phi): internal faces; the face arrays start
+    from the given values (zeros by default) on boundary faces."""
+    d = mesh.desc()
+    out = [np.zeros(mesh.nfaces) if a is None else np.array(a, np.float64) for a in (fx, fy, fz)]
+    gx, gy, gz, phi = _arr(gx), _arr(gy), _arr(gz), _arr(phi)
+    _check(lib().ref_interpolate_gradient(C.byref(d), _d(gx), _d(gy), _d(gz), _d(phi), *(_d(a) for a in out)))
+    return tuple(out)
+
+
+def _patch(mesh, patch):
+    b = mesh.boundary_index(patch) if isinstance(patch, str) else int(patch)
+    s, e = mesh.boundary_range[b]
+    return b, int(e - s + 1)
+
+
+def surface_flux(mesh, patch, phi):
+    b, nb = _patch(mesh, patch)
+    d = mesh.desc()
+    out = [np.zeros(max(nb, 1)) for _ in range(3)]
+    _check(lib().ref_surface_flux(C.byref(d), b, _d(_arr(phi)), *(_d(o) for o in out)))
+    return tuple(o[:nb] for o in out)
+
+
+def surface_normal_gradient(mesh, patch, ux, uy, uz, uw):
+    b, nb = _patch(mesh, patch)
+    d = mesh.desc()
+    out = [np.zeros(max(nb, 1)) for _ in range(3)]
+    _check(lib().ref_surface_normal_gradient(C.byref(d), b, _d(_arr(ux)), _d(_arr(uy)), _d(_arr(uz)),
+                                             _d(_arr(uw)), *(_d(o) for o in out)))
+    return tuple(o[:nb] for o in out)
+
+
+def pressure_force(mesh, patch, p, rho):
+    b, _ = _patch(mesh, patch)
+    d = mesh.desc()
+    F = np.zeros(3)
+    _check(lib().ref_pressure_force(C.byref(d), b, _d(_arr(p)), float(rho), _d(F)))
+    return F
+
+
+def viscous_force(mesh, patch, ux, uy, uz, uw, rho, nu, nut=None):
+    b, _ = _patch(mesh, patch)
+    d = mesh.desc()
+    F = np.zeros(3)
+    _check(lib().ref_viscous_force(C.byref(d), b, _d(_arr(ux)), _d(_arr(uy)), _d(_arr(uz)),
+                                   None if uw is None else _d(_arr(uw)), float(rho), float(nu),
+                                   None if nut is None else _d(_arr(nut)), _d(F)))
+    return F
+
+
+def wall_shear_stress(mesh, patch, ux, uy, uz, uw, nu, nut=None):
+    b, nb = _patch(mesh, patch)
+    d = mesh.desc()
+    out = [np.zeros(max(nb, 1)) for _ in range(3)]
+    pos = np.zeros(3 * max(nb, 1))
+    _check(lib().ref_wall_shear_stress(C.byref(d), b, _d(_arr(ux)), _d(_arr(uy)), _d(_arr(uz)), _d(_arr(uw)),
+                                       float(nu), None if nut is None else _d(_arr(nut)), *(_d(o) for o in out),
+                                       _d(pos)))
+    return tuple(o[:nb] for o in out), pos[:3 * nb].reshape(nb, 3)
+
+
+def nonorthogonal_flux(mesh, fx, fy, fz, phif):
+    """nonorthogonal_flux!: phif (copied) with its internal faces overwritten."""
+    d = mesh.desc()
+    out = np.array(phif, np.float64)
+    fx, fy, fz = _arr(fx), _arr(fy), _arr(fz)
+    _check(lib().ref_nonorthogonal_flux(C.byref(d), _d(fx), _d(fy), _d(fz), _d(out)))
+    return out
+
+
+def nonorthogonal_correct(mesh, sign, J, corr, b):
+    """correct!(eqn, term, corr_flux) on a copy of b."""
+    d = mesh.desc()
+    out = np.array(b, np.float64)
+    J, corr = _arr(J), _arr(corr)
+    _check(lib().ref_nonorthogonal_correct(C.byref(d), int(sign), _d(J), _d(corr), _d(out)))
     return out
 
 

@@ -15,10 +15,16 @@
 //   * relaxation, pressure reference .......................... Solve_1_api.jl:48-69
 //   * column-scatter SpMV (SparseArrays mul!) ................. Solvers_0_functions.jl:18
 //   * DILU with Di / Ri / J lists, column-oriented sweeps ..... preconditioners_1_DILU.jl:5-167
+//   * LDL of the set-up matrix, never updated ................. preconditioners_0_types.jl:36-42,
+//                                                               preconditioners_2_functions.jl:52-57
 //   * Jacobi / NormDiagonal ................................... preconditioners_2_functions.jl:15-50
 //   * interpolation, boundary correction, Green-Gauss, div .... Calculate_2_interpolation.jl,
 //                                                               Calculate_1_green_gauss.jl, Calculate_0_divergence.jl
 //   * SIMPLE and PISO loop bodies ............................. Solvers_1_SIMPLE.jl:69-231, Solvers_2_PISO.jl
+//   * surface flux / normal gradient, forces, wall shear ...... Calculate_5_surface_normal_gradient.jl,
+//                                                               Postprocess/Postprocess_forces.jl:5-96
+//   * non-orthogonal correction (the loops' `correct` block) .. Calculate_3_orthogonality_correction.jl,
+//                                                               Calculate_2_interpolation.jl:152-176
 //   * k-omega transport, wall functions, bound! ............... RANSModels_kOmega.jl, RANSModels_functions.jl
 //
 // Arithmetic that lives in packages absent from the reference tree is
@@ -26,6 +32,9 @@
 //   * Krylov.jl 0.9.5 BiCGSTAB / CG / GMRES (left preconditioning, warm start
 //     via the Delta-x correction, stopping at atol + rtol*||r0||);
 //   * ILUZero.jl 0.2.0 -> textbook ILU(0), Saad "Iterative Methods" Alg. 10.4;
+//   * LDLFactorizations.jl 0.10.1 -> Davis' up-looking LDL^T (ACM TOMS 849),
+//     on A's upper triangle in A's own order (the package orders with AMD.jl,
+//     absent here);
 //   * BLAS dot / nrm2 and Statistics.mean -> one fixed blocked summation
 //     order (chunks of 2048 = 256 lanes x 8, pairwise tree of 256), which the
 //     GPU library reproduces;  Julia's own arithmetic never contracts to FMA,
@@ -386,6 +395,11 @@ struct Precond {
     IVec rp, ci, dg;
     Vec lu;
     std::vector<IVec> upd_pairs;  // per lower entry: (slot_ij, slot_kj) pairs
+    // LDL: the set-up matrix's factor (L by columns, strictly lower), kept
+    IVec Lp, Li;
+    Vec Lx, Dl;
+    bool ldl_ready = false;
+    int ldl_status = FVH_OK;
 };
 
 static void dilu_setup(Precond &P) {
@@ -501,6 +515,78 @@ static void ilu0_apply(const Precond &P, Vec &x, const Vec &b) {
     }
 }
 
+// LDL (preconditioners_0_types.jl:36-42): opLDL(A) of the set-up matrix,
+// never updated (preconditioners_2_functions.jl:52-57).  Restated from
+// LDLFactorizations 0.10.1's published algorithm (Davis' up-looking LDL^T,
+// ldl_symbolic / ldl_numeric, then ldl_lsolve / ldl_dsolve / ldl_ltsolve),
+// on A's upper triangle, in A's own order (LDLFactorizations orders with
+// AMD.jl, which is absent here: ordering and rounding are parity unpinned).
+static int ldl_setup(Precond &P) {
+    const CSC &A = *P.A;
+    const int64_t n = A.n;
+    IVec parent(n, -1), flag(n), lnz(n, 0);
+    for (int64_t k = 0; k < n; ++k) {  // ldl_symbolic: column k of triu(A)
+        flag[k] = k;
+        for (int64_t p = A.colptr[k]; p < A.colptr[k + 1]; ++p) {
+            int64_t i = A.rowval[p];
+            if (i >= k) continue;
+            for (; flag[i] != k; i = parent[i]) {
+                if (parent[i] == -1) parent[i] = k;
+                lnz[i]++;
+                flag[i] = k;
+            }
+        }
+    }
+    P.Lp.assign(n + 1, 0);
+    for (int64_t k = 0; k < n; ++k) P.Lp[k + 1] = P.Lp[k] + lnz[k];
+    P.Li.assign(P.Lp[n], 0);
+    P.Lx.assign(P.Lp[n], 0.0);
+    P.Dl.assign(n, 0.0);
+    Vec y(n, 0.0);
+    IVec pattern(n), fill(n, 0);
+    for (int64_t k = 0; k < n; ++k) {  // ldl_numeric
+        int64_t top = n;
+        flag[k] = k;
+        for (int64_t p = A.colptr[k]; p < A.colptr[k + 1]; ++p) {
+            int64_t i = A.rowval[p];
+            if (i > k) continue;
+            y[i] += A.nzval[p];
+            int64_t len = 0;
+            for (; flag[i] != k; i = parent[i]) {
+                pattern[len++] = i;
+                flag[i] = k;
+            }
+            while (len > 0) pattern[--top] = pattern[--len];
+        }
+        double d = y[k];
+        y[k] = 0.0;
+        for (; top < n; ++top) {
+            int64_t i = pattern[top];
+            double yi = y[i];
+            y[i] = 0.0;
+            int64_t p2 = P.Lp[i] + fill[i];
+            for (int64_t p = P.Lp[i]; p < p2; ++p) y[P.Li[p]] -= P.Lx[p] * yi;
+            double l_ki = yi / P.Dl[i];
+            d -= l_ki * yi;
+            P.Li[p2] = k;
+            P.Lx[p2] = l_ki;
+            fill[i]++;
+        }
+        if (d == 0.0 || !std::isfinite(d)) return fail(FVH_ERR_ARG, "LDL: pivot %lld is %g", (long long)k, d);
+        P.Dl[k] = d;
+    }
+    return FVH_OK;
+}
+static void ldl_apply(const Precond &P, Vec &x, const Vec &b) {
+    const int64_t n = (int64_t)b.size();
+    x = b;
+    for (int64_t j = 0; j < n; ++j)  // ldl_lsolve
+        for (int64_t p = P.Lp[j]; p < P.Lp[j + 1]; ++p) x[P.Li[p]] -= P.Lx[p] * x[j];
+    for (int64_t j = 0; j < n; ++j) x[j] /= P.Dl[j];  // ldl_dsolve
+    for (int64_t j = n - 1; j >= 0; --j)  // ldl_ltsolve
+        for (int64_t p = P.Lp[j]; p < P.Lp[j + 1]; ++p) x[j] -= P.Lx[p] * x[P.Li[p]];
+}
+
 static void precond_setup(Precond &P, int kind, CSC *A) {
     P.kind = kind;
     P.A = A;
@@ -510,6 +596,13 @@ static void precond_setup(Precond &P, int kind, CSC *A) {
 }
 static void precond_update(Precond &P) {
     const CSC &A = *P.A;
+    if (P.kind == FVH_PRECOND_LDL) {
+        if (!P.ldl_ready) {
+            P.ldl_status = ldl_setup(P);
+            P.ldl_ready = true;
+        }
+        return;
+    }
     if (P.kind == FVH_PRECOND_DILU) dilu_update(P);
     else if (P.kind == FVH_PRECOND_ILU0) ilu0_update(P);
     else if (P.kind == FVH_PRECOND_JACOBI) {
@@ -540,7 +633,8 @@ static void precond_update(Precond &P) {
     }
 }
 static void precond_apply(const Precond &P, Vec &y, const Vec &x) {
-    if (P.kind == FVH_PRECOND_DILU) dilu_apply(P, y, x);
+    if (P.kind == FVH_PRECOND_LDL) ldl_apply(P, y, x);
+    else if (P.kind == FVH_PRECOND_DILU) dilu_apply(P, y, x);
     else if (P.kind == FVH_PRECOND_ILU0) ilu0_apply(P, y, x);
     else if (P.kind == FVH_PRECOND_JACOBI || P.kind == FVH_PRECOND_NORMDIAG)
         for (size_t i = 0; i < x.size(); ++i) y[i] = P.storage[i] * x[i];
@@ -553,8 +647,13 @@ static void precond_apply(const Precond &P, Vec &y, const Vec &x) {
 struct Op {
     const CSC *A;
     const Precond *P;
+    bool negate_prec = false;  // -M^-1 (CG on -A with a kept LDL factor of A's set-up matrix)
     void mul(const Vec &x, Vec &y) const { spmv(*A, x, y); }
-    void prec(const Vec &x, Vec &y) const { precond_apply(*P, y, x); }
+    void prec(const Vec &x, Vec &y) const {
+        precond_apply(*P, y, x);
+        if (negate_prec)
+            for (double &v : y) v = -v;
+    }
 };
 
 static fvh_stats bicgstab(const Op &op, const Vec &b, Vec &x, int itmax, double atol, double rtol) {
@@ -811,6 +910,16 @@ static fvh_stats run(Equation &eq, Precond &P, const fvh_solver_cfg &cfg, Vec &x
         Op op{&ng->An, &ng->Pn};
         return cg(op, bn, x, cfg.itmax, cfg.atol, cfg.rtol);
     }
+    if (cfg.solver == FVH_CG && P.kind == FVH_PRECOND_LDL && eq.A.nzval[eq.A.slot(0, 0)] < 0.0) {
+        // CG on -A: the LDL factor is of A's set-up matrix and stays as it
+        // is; -A's preconditioner is its negation
+        CSC An = eq.A;
+        for (double &v : An.nzval) v = -v;
+        Vec bn(eq.b.size());
+        for (size_t i = 0; i < bn.size(); ++i) bn[i] = -eq.b[i];
+        Op op{&An, &P, true};
+        return cg(op, bn, x, cfg.itmax, cfg.atol, cfg.rtol);
+    }
     Op op{&eq.A, &P};
     if (cfg.solver == FVH_BICGSTAB) return bicgstab(op, eq.b, x, cfg.itmax, cfg.atol, cfg.rtol);
     if (cfg.solver == FVH_CG) return cg(op, eq.b, x, cfg.itmax, cfg.atol, cfg.rtol);
@@ -904,6 +1013,49 @@ static void grad_scalar(const Mesh &m, int scheme, const std::vector<fvh_bc> &bc
         green_gauss(m, phif, dx, dy, dz);
     }
 }
+// ---- non-orthogonal correction (Calculate_3_orthogonality_correction.jl) ----
+// weight(::Type{Linear}, cells, faces, fi) (:81-93)
+static double nonorth_weight(const Mesh &m, int64_t f) {
+    V3 c1 = m.ccentre[m.own1[f]], c2 = m.ccentre[m.own2[f]], centre = m.fcentre[f];
+    V3 c1_f = sub(centre, c1), c1_c2 = sub(c2, c1);
+    double q = vdot(c1_f, c1_c2) / vdot(c1_c2, c1_c2);
+    V3 d12 = sub(c1, c2);
+    V3 f_prime{c1.x - q * d12.x, c1.y - q * d12.y, c1.z - q * d12.z};
+    return vnorm(sub(c2, f_prime)) / vnorm(sub(c2, c1));
+}
+// interpolate!(gradf, grad, phi) (Calculate_2_interpolation.jl:152-176), internal faces
+static void interpolate_gradient(const Mesh &m, const Vec &gx, const Vec &gy, const Vec &gz, const Vec &phi, Vec &fx,
+                                 Vec &fy, Vec &fz) {
+    for (int64_t f = m.nbf; f < m.nf; ++f) {
+        int64_t c1 = m.own1[f], c2 = m.own2[f];
+        double w = nonorth_weight(m, f);
+        double omw = 1 - w;
+        V3 ave{w * gx[c1] + omw * gx[c2], w * gy[c1] + omw * gy[c2], w * gz[c1] + omw * gz[c2]};
+        V3 e = m.fe[f];
+        double s = (phi[c2] - phi[c1]) / m.fdelta[f] - vdot(ave, e);
+        fx[f] = ave.x + s * e.x;
+        fy[f] = ave.y + s * e.y;
+        fz[f] = ave.z + s * e.z;
+    }
+}
+// nonorthogonal_flux!(phif, gradf) (:66-79), internal faces
+static void nonorthogonal_flux(const Mesh &m, const Vec &fx, const Vec &fy, const Vec &fz, Vec &phif) {
+    for (int64_t f = m.nbf; f < m.nf; ++f) {
+        V3 n = m.fnormal[f], e = m.fe[f];
+        double a = m.farea[f];
+        V3 T{(n.x - e.x) * a, (n.y - e.y) * a, (n.z - e.z) * a};
+        phif[f] = vdot(V3{fx[f], fy[f], fz[f]}, T);
+    }
+}
+// correct!(eqn, term, corr_flux) (:6-24): every face in id order
+static void nonorthogonal_correct(const Mesh &m, int sign, const Vec &J, const Vec &corr, Vec &b) {
+    for (int64_t f = 0; f < m.nf; ++f) {
+        double correction = (double)(-sign) * J[f] * corr[f];
+        b[m.own1[f]] += correction;
+        b[m.own2[f]] += correction * (-1.0);
+    }
+}
+
 static void div(const Mesh &m, const Vec &fx, const Vec &fy, const Vec &fz, Vec &out) {
     for (int64_t c = 0; c < m.n; ++c) {
         double vol = m.volume[c];
@@ -1204,10 +1356,12 @@ static int sim_init(Sim &S) {
     if (int rc = apply_bcs(m, S.mux, S.ux, S.bU, 0)) return rc;
     precond_setup(S.Pux, c.vU.precond, &S.ux.A);
     precond_update(S.Pux);
+    if (S.Pux.ldl_status != FVH_OK) return S.Pux.ldl_status;
     discretise(m, S.mp, S.pe, &zero, c.dt);
     if (int rc = apply_bcs(m, S.mp, S.pe, S.bp, 0)) return rc;
     precond_setup(S.Pp, c.vp.precond, &S.pe.A);
     precond_update(S.Pp);
+    if (S.Pp.ldl_status != FVH_OK) return S.Pp.ldl_status;
     if (c.model == FVH_KOMEGA) {
         S.mk.terms = {{FVH_OP_TIME, c.sk.time, 1, nullptr, &S.k},
                       {FVH_OP_DIVERGENCE, c.sk.divergence, 1, &S.mdotf, &S.k},
@@ -1223,10 +1377,12 @@ static int sim_init(Sim &S) {
         if (int rc = apply_bcs(m, S.mk, S.pe, S.bk, 0)) return rc;
         precond_setup(S.Pk_, c.vk.precond, &S.pe.A);
         precond_update(S.Pk_);
+        if (S.Pk_.ldl_status != FVH_OK) return S.Pk_.ldl_status;
         discretise(m, S.mw, S.pe, &zero, c.dt);
         if (int rc = apply_bcs(m, S.mw, S.pe, S.bw, 0)) return rc;
         precond_setup(S.Pw_, c.vomega.precond, &S.pe.A);
         precond_update(S.Pw_);
+        if (S.Pw_.ldl_status != FVH_OK) return S.Pw_.ldl_status;
     }
     S.Rux.assign(S.iterations, 1.0);
     S.Ruy.assign(S.iterations, 1.0);
@@ -1278,6 +1434,18 @@ static void pressure_corrector(Sim &S, int iter) {
     explicit_relaxation(S.p, S.prev, S.cs.vp.relax);
     S.Rp[iter] = residual(S.pe, S.p);
     grad_scalar(m, S.cs.sp.gradient, S.bp, 0, S.p, S.pf, S.gpx, S.gpy, S.gpz);
+    // the `correct` block (Solvers_1_SIMPLE.jl:165-180, Solvers_2_PISO.jl:170-182)
+    for (int i = 0; i < S.cs.nonorth_correctors; ++i) {
+        discretise(m, S.mp, S.pe, &S.prev, S.cs.dt);
+        apply_bcs(m, S.mp, S.pe, S.bp, 0);
+        set_reference(S.pe, S.cs.has_pref != 0, S.cs.pref, 0);
+        Vec gfx(m.nf, 0.0), gfy(m.nf, 0.0), gfz(m.nf, 0.0);
+        interpolate_gradient(m, S.gpx, S.gpy, S.gpz, S.p, gfx, gfy, gfz);
+        nonorthogonal_flux(m, gfx, gfy, gfz, S.pf);
+        nonorthogonal_correct(m, 1, S.rDf, S.pf, S.pe.b);
+        S.stats["p"] = run(S.pe, S.Pp, S.cs.vp, S.p);
+        grad_scalar(m, S.cs.sp.gradient, S.bp, 0, S.p, S.pf, S.gpx, S.gpy, S.gpz);
+    }
     for (int64_t c = 0; c < m.n; ++c) {  // correct_velocity!
         double r = S.rD[c];
         S.Ux[c] = S.Hx[c] - S.gpx[c] * r;
@@ -1490,6 +1658,7 @@ int ref_precond_apply_csr(const fvh_mesh_desc *d, const double *csr_vals, int32_
     Precond P;
     precond_setup(P, kind, &eq.A);
     precond_update(P);
+    if (P.ldl_status != FVH_OK) return P.ldl_status;
     Vec xv(x, x + m.n), yv(m.n);
     precond_apply(P, yv, xv);
     std::copy(yv.begin(), yv.end(), y);
@@ -1638,6 +1807,170 @@ int ref_green_gauss(const fvh_mesh_desc *d, const double *phif, double *dx, doub
     std::copy(gx.begin(), gx.end(), dx);
     std::copy(gy.begin(), gy.end(), dy);
     std::copy(gz.begin(), gz.end(), dz);
+    return FVH_OK;
+}
+
+int ref_interpolate_gradient(const fvh_mesh_desc *d, const double *gx, const double *gy, const double *gz,
+                             const double *phi, double *fx, double *fy, double *fz) {
+    Mesh m;
+    load_mesh(d, m);
+    Vec a(gx, gx + m.n), b(gy, gy + m.n), c(gz, gz + m.n), ph(phi, phi + m.n);
+    Vec ox(fx, fx + m.nf), oy(fy, fy + m.nf), oz(fz, fz + m.nf);
+    interpolate_gradient(m, a, b, c, ph, ox, oy, oz);
+    std::copy(ox.begin(), ox.end(), fx);
+    std::copy(oy.begin(), oy.end(), fy);
+    std::copy(oz.begin(), oz.end(), fz);
+    return FVH_OK;
+}
+
+int ref_nonorthogonal_flux(const fvh_mesh_desc *d, const double *fx, const double *fy, const double *fz,
+                           double *phif) {
+    Mesh m;
+    load_mesh(d, m);
+    Vec a(fx, fx + m.nf), b(fy, fy + m.nf), c(fz, fz + m.nf), o(phif, phif + m.nf);
+    nonorthogonal_flux(m, a, b, c, o);
+    std::copy(o.begin(), o.end(), phif);
+    return FVH_OK;
+}
+
+int ref_nonorthogonal_correct(const fvh_mesh_desc *d, int32_t sign, const double *J, const double *corr, double *b) {
+    Mesh m;
+    load_mesh(d, m);
+    Vec j(J, J + m.nf), c(corr, corr + m.nf), bb(b, b + m.n);
+    nonorthogonal_correct(m, sign, j, c, bb);
+    std::copy(bb.begin(), bb.end(), b);
+    return FVH_OK;
+}
+
+// ---- post-processing (Postprocess_forces.jl, Calculate_5_surface_normal_gradient.jl) ----
+// surface_flux (:3-18): Sf = area*normal; flux = phi[cID]*Sf
+static void surface_flux(const Mesh &m, int patch, const Vec &phi, Vec &x, Vec &y, Vec &z) {
+    const Boundary &B = m.boundaries[patch];
+    const size_t nb = B.facesID.size();
+    x.assign(nb, 0.0);
+    y.assign(nb, 0.0);
+    z.assign(nb, 0.0);
+    for (size_t i = 0; i < nb; ++i) {
+        int64_t c = B.cellsID[i], f = B.facesID[i];
+        double a = m.farea[f];
+        V3 n = m.fnormal[f];
+        V3 Sf{a * n.x, a * n.y, a * n.z};
+        x[i] = phi[c] * Sf.x;
+        y[i] = phi[c] * Sf.y;
+        z[i] = phi[c] * Sf.z;
+    }
+}
+// surface_normal_gradient (:20-36): Up = Udiff - (Udiff . n) n; grad = Up / delta
+static void surface_normal_gradient(const Mesh &m, int patch, const Vec &ux, const Vec &uy, const Vec &uz,
+                                    const double *uw, Vec &x, Vec &y, Vec &z) {
+    const Boundary &B = m.boundaries[patch];
+    const size_t nb = B.facesID.size();
+    x.assign(nb, 0.0);
+    y.assign(nb, 0.0);
+    z.assign(nb, 0.0);
+    for (size_t i = 0; i < nb; ++i) {
+        int64_t c = B.cellsID[i], f = B.facesID[i];
+        V3 n = m.fnormal[f];
+        V3 ud{ux[c] - uw[0], uy[c] - uw[1], uz[c] - uw[2]};
+        double dn = vdot(ud, n);
+        V3 up{ud.x - dn * n.x, ud.y - dn * n.y, ud.z - dn * n.z};
+        double delta = m.fdelta[f];
+        x[i] = up.x / delta;
+        y[i] = up.y / delta;
+        z[i] = up.z / delta;
+    }
+}
+
+int ref_surface_flux(const fvh_mesh_desc *d, int32_t patch, const double *phi, double *fx, double *fy, double *fz) {
+    Mesh m;
+    load_mesh(d, m);
+    if (patch < 0 || patch >= (int)m.boundaries.size()) return fail(FVH_ERR_ARG, "patch %d", patch);
+    Vec ph(phi, phi + m.n), x, y, z;
+    surface_flux(m, patch, ph, x, y, z);
+    std::copy(x.begin(), x.end(), fx);
+    std::copy(y.begin(), y.end(), fy);
+    std::copy(z.begin(), z.end(), fz);
+    return FVH_OK;
+}
+
+int ref_surface_normal_gradient(const fvh_mesh_desc *d, int32_t patch, const double *ux, const double *uy,
+                                const double *uz, const double *uw, double *gx, double *gy, double *gz) {
+    Mesh m;
+    load_mesh(d, m);
+    if (patch < 0 || patch >= (int)m.boundaries.size()) return fail(FVH_ERR_ARG, "patch %d", patch);
+    Vec a(ux, ux + m.n), b(uy, uy + m.n), c(uz, uz + m.n), x, y, z;
+    surface_normal_gradient(m, patch, a, b, c, uw, x, y, z);
+    std::copy(x.begin(), x.end(), gx);
+    std::copy(y.begin(), y.end(), gy);
+    std::copy(z.begin(), z.end(), gz);
+    return FVH_OK;
+}
+
+// pressure_force (Postprocess_forces.jl:5-27): sums in patch face order, then rho .* sums
+int ref_pressure_force(const fvh_mesh_desc *d, int32_t patch, const double *p, double rho, double *F) {
+    Mesh m;
+    load_mesh(d, m);
+    if (patch < 0 || patch >= (int)m.boundaries.size()) return fail(FVH_ERR_ARG, "patch %d", patch);
+    Vec ph(p, p + m.n), x, y, z;
+    surface_flux(m, patch, ph, x, y, z);
+    double sx = 0.0, sy = 0.0, sz = 0.0;
+    for (size_t i = 0; i < x.size(); ++i) {
+        sx += x[i];
+        sy += y[i];
+        sz += z[i];
+    }
+    F[0] = rho * sx;
+    F[1] = rho * sy;
+    F[2] = rho * sz;
+    return FVH_OK;
+}
+
+// viscous_force (:29-60): snGrad * area * (nu + nut[cID]); zero gradient when
+// no velocity BC names the patch (uw == NULL); nut NULL = 0
+int ref_viscous_force(const fvh_mesh_desc *d, int32_t patch, const double *ux, const double *uy, const double *uz,
+                      const double *uw, double rho, double nu, const double *nut, double *F) {
+    Mesh m;
+    load_mesh(d, m);
+    if (patch < 0 || patch >= (int)m.boundaries.size()) return fail(FVH_ERR_ARG, "patch %d", patch);
+    const Boundary &B = m.boundaries[patch];
+    Vec a(ux, ux + m.n), b(uy, uy + m.n), c(uz, uz + m.n), x(B.facesID.size(), 0.0), y(x), z(x);
+    if (uw) surface_normal_gradient(m, patch, a, b, c, uw, x, y, z);
+    double sx = 0.0, sy = 0.0, sz = 0.0;
+    for (size_t i = 0; i < x.size(); ++i) {
+        int64_t cID = B.cellsID[i], fID = B.facesID[i];
+        double area = m.farea[fID], nue = nu + (nut ? nut[cID] : 0.0);
+        sx += x[i] * area * nue;
+        sy += y[i] * area * nue;
+        sz += z[i] * area * nue;
+    }
+    F[0] = rho * sx;
+    F[1] = rho * sy;
+    F[2] = rho * sz;
+    return FVH_OK;
+}
+
+// wall_shear_stress (:62-96): tauw = snGrad * (nu + nut[cID]), with face centres
+int ref_wall_shear_stress(const fvh_mesh_desc *d, int32_t patch, const double *ux, const double *uy,
+                          const double *uz, const double *uw, double nu, const double *nut, double *tx, double *ty,
+                          double *tz, double *pos) {
+    Mesh m;
+    load_mesh(d, m);
+    if (patch < 0 || patch >= (int)m.boundaries.size()) return fail(FVH_ERR_ARG, "patch %d", patch);
+    const Boundary &B = m.boundaries[patch];
+    Vec a(ux, ux + m.n), b(uy, uy + m.n), c(uz, uz + m.n), x, y, z;
+    surface_normal_gradient(m, patch, a, b, c, uw, x, y, z);
+    for (size_t i = 0; i < x.size(); ++i) {
+        int64_t cID = B.cellsID[i], fID = B.facesID[i];
+        double nueff = nu + (nut ? nut[cID] : 0.0);
+        tx[i] = x[i] * nueff;
+        ty[i] = y[i] * nueff;
+        tz[i] = z[i] * nueff;
+        if (pos) {
+            pos[3 * i] = m.fcentre[fID].x;
+            pos[3 * i + 1] = m.fcentre[fID].y;
+            pos[3 * i + 2] = m.fcentre[fID].z;
+        }
+    }
     return FVH_OK;
 }
 

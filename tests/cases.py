@@ -48,6 +48,21 @@ def bfs_model(mesh, U=0.5, nu=1e-5):
     return model
 
 
+def bfs_case_model(mesh, U=0.5, nu=1e-3):
+    """CASE_UNV_BFS.jl on the reference's backwardFacingStep meshes (patches
+    inlet / outlet / wall / top): uniform inlet, zero-gradient outlet with
+    p = 0, no-slip wall and top."""
+    model = F.RANS(F.Laminar, mesh=mesh, viscosity=F.ConstantScalar(nu))
+    v, z = [U, 0.0, 0.0], [0.0, 0.0, 0.0]
+    F.assign(model, "U", F.Dirichlet("inlet", v), F.Neumann("outlet", 0.0), F.Dirichlet("wall", z),
+             F.Dirichlet("top", z))
+    F.assign(model, "p", F.Neumann("inlet", 0.0), F.Dirichlet("outlet", 0.0), F.Neumann("wall", 0.0),
+             F.Neumann("top", 0.0))
+    F.initialise(model.U, v)
+    F.initialise(model.p, 0.0)
+    return model
+
+
 def channel_model(mesh, nu=0.01, U=1.0):
     """Plane channel: uniform inlet, zero-gradient outlet with p = 0, no-slip walls
     (patches inlet/outlet/bottom/top as the generated channel)."""

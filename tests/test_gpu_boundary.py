@@ -14,7 +14,7 @@ import fvmhip as F
 from fvmhip import _lib as L
 from fvmhip.api import _bcs
 from oracle import ref
-from tests.cases import bfs_model, cavity_model, config, rel_l2, same_bits
+from tests.cases import bfs_case_model, bfs_model, cavity_model, config, rel_l2, same_bits
 from tests.conftest import REFERENCE_MESHES
 
 pytestmark = pytest.mark.gpu
@@ -154,7 +154,7 @@ def test_equation_entry_points_bitwise(ref_mesh, name, divergence):
     spec = [(L.OP_TIME, F.Euler, 1, None, None),
             (L.OP_DIVERGENCE, divergence, 1, mdotf, None),
             (L.OP_LAPLACIAN, F.Linear, -1, nueff, None),
-            (L.OP_SI, F.Linear, 1, si, None)]
+            (L.OP_SI, F.Linear, 1, si, ux)]
     T = _Terms(spec, n)
     srcs_d, srcs_h = (L.Source * 1)(), (L.Source * 1)()
     d_src = _dev(src)
@@ -259,7 +259,7 @@ def test_fvh_simple_whole_run_bitwise(ref_mesh, name, kind):
         model, pref = cavity_model(mesh), 0.0
         cfg = config(40)
     else:
-        model, pref = bfs_model(mesh), None
+        model, pref = bfs_case_model(mesh), None
         cfg = config(40, U_solver=F.GmresSolver, p_solver=F.GmresSolver, relax_U=0.8, relax_p=0.2)
     case = F.CaseDesc(model, cfg, pref)
     fields = F.api._fields(model)
@@ -277,3 +277,189 @@ def test_fvh_simple_whole_run_bitwise(ref_mesh, name, kind):
         assert same_bits(a, b)
     for k in ("Ux", "Uy", "p"):
         assert same_bits(f[k], fr[k]), k
+
+
+@pytest.mark.parametrize("bnd", [np.finfo(float).eps, 1e-3])
+def test_bound_entry_point_against_the_sequential_oracle(bnd):
+    """bound! with chains of adjacent negative cells (a whole grid row, a
+    diagonal staircase, a random third of the cells, NaN and -0.0 among
+    them), through fvh_bound, bit for bit against the oracle's in-order
+    pass; values between 0 and the bound move too when the bound exceeds
+    eps."""
+    mesh = F.cavity(64)
+    dm = mesh.device()
+    n = mesh.ncells
+    rng = np.random.default_rng(3)
+    base = rng.uniform(0.5, 2.0, n)
+    fields = []
+    f = base.copy()
+    f[64 * 10:64 * 11] = -rng.uniform(0.1, 1.0, 64)        # one grid row: a chain of length 64
+    fields.append(f)
+    f = base.copy()
+    for k in range(40):                                     # a staircase (west and south neighbours)
+        f[64 * k + k] = -1.0
+        f[64 * k + k + 1] = -0.5
+    fields.append(f)
+    f = np.where(rng.random(n) < 0.33, -rng.uniform(0.0, 1.0, n), base)
+    f[5], f[6], f[70] = -0.0, np.nan, 1e-300               # -0.0, a NaN, a positive value below any bound
+    f[rng.random(n) < 0.05] = 0.5 * bnd
+    fields.append(f)
+    for t, f in enumerate(fields):
+        d = _dev(f)
+        chained = C.c_int32(-1)
+        L.check(LIB().fvh_bound(dm.h, d.vp, bnd, C.byref(chained)), "fvh_bound")
+        want = ref.bound(mesh, f, bnd)
+        got = d.host()
+        print(f"field {t}: {chained.value} cells through the ordered pass", flush=True)
+        assert chained.value > 0
+        assert same_bits(got, want), (t, int(np.sum(~((got == want) | (np.isnan(got) & np.isnan(want))))))
+    # no negative cell: nothing is chained, values below the bound are lifted
+    d = _dev(base)
+    chained = C.c_int32(-1)
+    L.check(LIB().fvh_bound(dm.h, d.vp, bnd, C.byref(chained)), "fvh_bound")
+    assert chained.value == 0 and same_bits(d.host(), ref.bound(mesh, base, bnd))
+
+
+@pytest.mark.parametrize("name", REFERENCE_MESHES)
+def test_nonorthogonal_entry_points_bitwise(ref_mesh, name):
+    """interpolate!(gradf, grad, phi), nonorthogonal_flux! and correct!
+    through their entry points, bit for bit against the oracle."""
+    mesh = ref_mesh(name)
+    dm = mesh.device()
+    n, nf = mesh.ncells, mesh.nfaces
+    ux, uy, phi = _fields(mesh, 21)
+    gz = 0.1 * ux * uy
+    start = np.linspace(-3.0, 3.0, nf)
+    d_g = [_dev(a) for a in (ux, uy, gz)]
+    d_phi = _dev(phi)
+    d_f = [_dev(start) for _ in range(3)]
+    L.check(LIB().fvh_interpolate_gradient(dm.h, *(a.vp for a in d_g), d_phi.vp, *(a.vp for a in d_f)),
+            "fvh_interpolate_gradient")
+    want = ref.interpolate_gradient(mesh, ux, uy, gz, phi, start, start, start)
+    for a, b in zip(d_f, want):
+        assert same_bits(a.host(), b), "fvh_interpolate_gradient"
+    d_pf = _dev(start)
+    L.check(LIB().fvh_nonorthogonal_flux(dm.h, *(a.vp for a in d_f), d_pf.vp), "fvh_nonorthogonal_flux")
+    corr = ref.nonorthogonal_flux(mesh, *want, start)
+    assert same_bits(d_pf.host(), corr), "fvh_nonorthogonal_flux"
+    # correct! on an assembled equation's b
+    J = 0.5 + np.abs(np.cos(np.arange(nf) * 0.37))
+    eq = C.c_void_p()
+    L.check(LIB().fvh_eqn_create(dm.h, C.byref(eq)), "fvh_eqn_create")
+    try:
+        T = _Terms([(L.OP_LAPLACIAN, F.Linear, 1, J, None)], n)
+        src = np.sin(np.arange(n) * 0.05)
+        srcs_d = (L.Source * 1)()
+        d_src = _dev(src)
+        srcs_d[0].sign, srcs_d[0].field = 1, d_src.ptr
+        L.check(LIB().fvh_discretise(eq, T.dev, T.n, srcs_d, 1, None, 0.0), "fvh_discretise")
+        _, b0 = _eq_arrays(eq, dm, n)
+        d_J = _dev(J)
+        L.check(LIB().fvh_nonorthogonal_correct(eq, 1, d_J.vp, d_pf.vp), "fvh_nonorthogonal_correct")
+        _, b1 = _eq_arrays(eq, dm, n)
+        assert same_bits(b1, ref.nonorthogonal_correct(mesh, 1, J, corr, b0)), "fvh_nonorthogonal_correct"
+    finally:
+        LIB().fvh_eqn_destroy(eq)
+
+
+@pytest.mark.parametrize("mesh_name,algorithm", [("trig40", L.ALG_SIMPLE), ("quad40", L.ALG_SIMPLE),
+                                                 ("trig40", L.ALG_PISO)])
+def test_loop_with_nonorthogonal_correctors_bitwise(ref_mesh, mesh_name, algorithm):
+    """The loops' `correct` block switched on (one corrector): a cavity on
+    the reference's triangle mesh, bit for bit against the oracle."""
+    mesh = ref_mesh(mesh_name)
+    model = cavity_model(mesh)
+    it = 6
+    kw = {} if algorithm == L.ALG_SIMPLE else dict(time=F.Euler, dt=0.01)
+    cfg = config(it + 1, **kw)
+    g = F.Simulation(model, cfg, 0.0, algorithm, 1e-7, nonorth_correctors=1)
+    r = ref.RefSimulation(model, F.CaseDesc(model, cfg, 0.0, algorithm, 1e-7, nonorth_correctors=1))
+    assert g.run(it) == r.run(it) == it
+    for nm in ("Ux", "Uy", "p"):
+        assert same_bits(g.get(nm), r.get(nm)), f"{nm} rel {rel_l2(g.get(nm), r.get(nm)):.3e}"
+    for a, b in zip(g.residuals(), r.residuals()):
+        assert same_bits(a[:it], b[:it])
+    # the correctors change the answer on triangles (not on quads, to rounding)
+    g0 = F.Simulation(model, cfg, 0.0, algorithm, 1e-7)
+    g0.run(it)
+    d = rel_l2(g.get("p"), g0.get("p"))
+    print(f"{mesh_name}: correctors move p by {d:.3e} (rel L2)")
+    if mesh_name == "trig40":
+        assert d > 1e-8
+
+
+@pytest.mark.parametrize("name", REFERENCE_MESHES)
+def test_postprocess_entry_points_bitwise(ref_mesh, name):
+    """surface_flux, surface_normal_gradient, pressure_force, viscous_force
+    and wall_shear_stress on every patch, bit for bit against the oracle."""
+    mesh = ref_mesh(name)
+    dm = mesh.device()
+    n = mesh.ncells
+    ux, uy, phi = _fields(mesh, 31)
+    uz = 0.05 * ux
+    nut = 1e-3 * (1.0 + np.abs(uy))
+    d_u = [_dev(a) for a in (ux, uy, uz)]
+    d_p, d_nut = _dev(phi), _dev(nut)
+    uw = np.array([0.25, -0.125, 0.0])
+    for b, patch in enumerate(mesh.boundary_names):
+        nb = C.c_int64()
+        L.check(LIB().fvh_patch_size(dm.h, b, C.byref(nb)))
+        k = nb.value
+        out = [_dev(np.zeros(max(k, 1))) for _ in range(3)]
+        L.check(LIB().fvh_surface_flux(dm.h, b, d_p.vp, *(o.vp for o in out)), "fvh_surface_flux")
+        for o, w in zip(out, ref.surface_flux(mesh, b, phi)):
+            assert same_bits(o.host()[:k], w), ("surface_flux", patch)
+        L.check(LIB().fvh_surface_normal_gradient(dm.h, b, *(a.vp for a in d_u), L.dptr(uw), *(o.vp for o in out)),
+                "fvh_surface_normal_gradient")
+        for o, w in zip(out, ref.surface_normal_gradient(mesh, b, ux, uy, uz, uw)):
+            assert same_bits(o.host()[:k], w), ("surface_normal_gradient", patch)
+        Fp = np.zeros(3)
+        L.check(LIB().fvh_pressure_force(dm.h, b, d_p.vp, 1.2, L.dptr(Fp)), "fvh_pressure_force")
+        assert same_bits(Fp, ref.pressure_force(mesh, b, phi, 1.2)), ("pressure_force", patch)
+        for with_nut in (False, True):
+            for w in (uw, None):
+                Fv = np.zeros(3)
+                L.check(LIB().fvh_viscous_force(dm.h, b, *(a.vp for a in d_u), None if w is None else L.dptr(w), 1.2,
+                                                1.5e-5, d_nut.vp if with_nut else None, L.dptr(Fv)), "fvh_viscous_force")
+                want = ref.viscous_force(mesh, b, ux, uy, uz, w, 1.2, 1.5e-5, nut if with_nut else None)
+                assert same_bits(Fv, want), ("viscous_force", patch, with_nut, w is None)
+        pos = _dev(np.zeros(3 * max(k, 1)))
+        L.check(LIB().fvh_wall_shear_stress(dm.h, b, *(a.vp for a in d_u), L.dptr(uw), 1.5e-5, d_nut.vp,
+                                            *(o.vp for o in out), pos.vp), "fvh_wall_shear_stress")
+        (tx, ty, tz), wp = ref.wall_shear_stress(mesh, b, ux, uy, uz, uw, 1.5e-5, nut)
+        for o, w in zip(out, (tx, ty, tz)):
+            assert same_bits(o.host()[:k], w), ("wall_shear_stress", patch)
+        assert same_bits(pos.host()[:3 * k], wp.reshape(-1)), ("wall_shear_stress positions", patch)
+
+
+def test_simple_writes_vtk_as_write_interval_asks(ref_mesh, tmp_path, monkeypatch):
+    """simple! with write_interval = 4 on the cavity: iteration_000004.vtk,
+    _000008 ... and the file at convergence, holding the fields the model
+    was updated with (Solvers_1_SIMPLE.jl:200-228)."""
+    import os
+    mesh = ref_mesh("quad40")
+    model = cavity_model(mesh)
+    cfg = config(10)
+    cfg.runtime["write_interval"] = 4
+    monkeypatch.chdir(tmp_path)
+    R = F.simple(model, cfg, pref=0.0)
+    files = sorted(os.listdir(tmp_path))
+    assert files == ["iteration_000004.vtk", "iteration_000008.vtk"], files
+    with open(tmp_path / "iteration_000008.vtk") as f:
+        text = f.read()
+    assert text.startswith("# vtk DataFile Version 3.0\njCFD simulation data\n")
+    F.model2vtk(model, str(tmp_path / "again"))
+    with open(tmp_path / "again.vtk") as f:
+        assert f.read() != text  # fields moved on after iteration 8 (10 iterations ran)
+    assert len(R[0]) == 10
+    # a run that converges inside an interval writes at the break
+    model2 = cavity_model(mesh)
+    cfg2 = config(1000)
+    cfg2.runtime["write_interval"] = 100
+    for fn in files:
+        os.remove(tmp_path / fn)
+    os.remove(tmp_path / "again.vtk")
+    sim_its = F.Simulation(cavity_model(mesh), config(1000), 0.0).run(1000)
+    F.simple(model2, cfg2, pref=0.0)
+    want = [f"iteration_{k:06d}.vtk" for k in range(100, sim_its, 100)] + [f"iteration_{sim_its:06d}.vtk"]
+    assert sorted(os.listdir(tmp_path)) == want

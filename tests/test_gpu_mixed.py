@@ -14,7 +14,7 @@ import pytest
 import fvmhip as F
 from fvmhip import _lib as L
 from oracle import ref
-from tests.cases import bfs_model, cavity_model, config, rel_l2
+from tests.cases import flatplate_model, bfs_model, cavity_model, config, rel_l2
 
 pytestmark = pytest.mark.gpu
 
@@ -225,5 +225,71 @@ def test_bfs_mixed_converges_to_oracle():
     assert ng < iters and nr < iters
     errs = {f: rel_l2(g.get(f), r.get(f)) for f in ("Ux", "Uy", "p")}
     print(f"BFS 120x40 mixed: {ng} iterations vs FP64 oracle {nr}; rel L2 {errs}")
+    # At a 1e-6 break the two runs stop at different points of the same
+    # convergence path, so their distance is the break's, not the
+    # arithmetic's: it is of the order of the FP64 run's own distance from
+    # the converged solution (the test below holds 1e-6 there).
+    r_long = ref.RefSimulation(model, F.CaseDesc(model, config(1501, schemes=dict(divergence=F.Upwind)), None,
+                                                 L.ALG_SIMPLE, 1e-14))
+    assert r_long.run(1500) == 1500
     for f, err in errs.items():
-        assert err <= 1e-5, f"{f}: {err:.3e}"
+        own = rel_l2(r.get(f), r_long.get(f))
+        print(f"  {f}: mixed vs FP64 {err:.2e}; FP64 at the 1e-6 break vs converged {own:.2e}")
+        assert err <= 1e-5 and err <= 3.0 * own + 1e-9, f"{f}: {err:.3e} (FP64's own distance {own:.3e})"
+
+
+def test_bfs_mixed_fields_converged():
+    """The same step run 1500 iterations in both forms, past the point where
+    the residuals stop falling (p plateaus near 1e-8): the mixed form's Ux,
+    Uy and p within 1e-6 relative L2 of the FP64 oracle's."""
+    mesh = F.backward_facing_step(nx=120, ny=40)
+    model = bfs_model(mesh, nu=1e-3)
+    iters = 1500
+    cfg = config(iters + 1, schemes=dict(divergence=F.Upwind), precision=L.MIXED)
+    cfg64 = config(iters + 1, schemes=dict(divergence=F.Upwind))
+    g = F.Simulation(model, cfg, None, L.ALG_SIMPLE, 1e-14)
+    r = ref.RefSimulation(model, F.CaseDesc(model, cfg64, None, L.ALG_SIMPLE, 1e-14))
+    ng, nr = g.run(iters), r.run(iters)
+    assert ng == nr == iters
+    errs = {f: rel_l2(g.get(f), r.get(f)) for f in ("Ux", "Uy", "p")}
+    Rg, Rr = g.residuals(), r.residuals()
+    print(f"BFS 120x40 mixed, {iters} iterations: rel L2 {errs}; residuals mixed "
+          f"{[float(Rg[q][iters - 1]) for q in range(3)]}, FP64 {[float(Rr[q][iters - 1]) for q in range(3)]}")
+    for f, err in errs.items():
+        assert err <= 1e-6, f"{f}: {err:.3e}"
+
+
+def test_komega_flatplate_mixed_fields():
+    """k-omega flat plate (60 x 30, wall grading 1.1) with U and p in mixed
+    precision and k / omega (and the wall treatment) in FP64, against the
+    FP64 oracle.  This case does not settle (its fields keep changing by
+    O(1) between iteration 1000 and 3000, in FP64 too), so the two forms are
+    compared with the U and p solves converged tightly (rtol 1e-9): what
+    differs is then the arithmetic alone, not where two Krylov runs stopped
+    within the default rtol of 1e-3 (at that rtol the fields differ by up to
+    2e-4 relative after 10 iterations, the size of an O(1) update times the
+    solve tolerance)."""
+    mesh = F.flat_plate(nx=60, ny=30, first=1e-3, ratio=1.1)
+    model = flatplate_model(mesh)
+    names = ("Ux", "Uy", "p", "k", "omega")
+    kw = dict(schemes=dict(divergence=F.Upwind), turbulent=True, p_solver=F.CgSolver, itmax=1000)
+
+    def tight(cfg):
+        for f in ("U", "p"):
+            cfg.solvers[f]["rtol"] = 1e-9
+            cfg.solvers[f]["atol"] = 0.0
+        return cfg
+
+    g = F.Simulation(model, tight(config(201, precision=L.MIXED, **kw)), None, L.ALG_SIMPLE, 1e-14)
+    r = ref.RefSimulation(model, F.CaseDesc(model, tight(config(201, **kw)), None, L.ALG_SIMPLE, 1e-14))
+    done, errs = 0, {}
+    for upto in (10, 50, 200):
+        g.run(upto - done)
+        r.run(upto - done)
+        done = upto
+        errs[upto] = {f: rel_l2(g.get(f), r.get(f)) for f in names}
+        print(f"k-omega 60x30 mixed vs FP64 oracle after {upto} iterations: "
+              f"{ {f: f'{e:.2e}' for f, e in errs[upto].items()} }", flush=True)
+    for upto in (10, 50):
+        for f in names:
+            assert errs[upto][f] <= 1e-6, f"{f}: {errs[upto][f]:.3e} after {upto} iterations"

@@ -52,3 +52,61 @@ def test_upwind_transport_obeys_the_maximum_principle():
     eq.solve(rtol=1e-12, itmax=2000)
     x = eq.values()
     assert x.min() >= -1e-10 and x.max() <= 1.0 + 1e-10
+
+
+# ---------------------------------------------------------------------------
+# C3 at its real size: the Delaunay triangulation of 1M seeded points (about
+# 2M cells and 4,700 levels per sweep)
+# ---------------------------------------------------------------------------
+@pytest.fixture(scope="module")
+def delaunay_full():
+    return F.delaunay_unit_square(1_000_000, seed=0)
+
+
+@pytest.mark.parametrize("walk", ["syncfree", "levels", "levels-ring256"])
+def test_delaunay_full_size_dilu_bitwise(delaunay_full, walk, monkeypatch):
+    """SpMV, the DILU update and the DILU apply (FP64) on random values,
+    bit for bit against the oracle, on the sync-free walk (the default), on
+    the one-work-group level walk, and on the level walk with its LDS ring
+    cut to 256 results (so most neighbours come through its global-memory
+    and after-barrier paths)."""
+    import ctypes as C
+
+    mesh = delaunay_full
+    monkeypatch.setenv("FVH_LEVEL_SYNCFREE", "1" if walk == "syncfree" else "0")
+    if walk == "levels-ring256":
+        monkeypatch.setenv("FVH_LEVEL_RING", "256")
+    else:
+        monkeypatch.delenv("FVH_LEVEL_RING", raising=False)
+    mesh._dev = None  # the schedule (and its ring) is built with the device mesh
+    dm = mesh.device()
+    nlev, band = dm.sweep_info()
+    assert band == 0 and nlev > 1000
+    h = C.c_void_p()
+    L.check(L.lib().fvh_eqn_create(dm.h, C.byref(h)))
+    try:
+        nz = C.c_void_p()
+        L.check(L.lib().fvh_eqn_arrays(h, C.byref(nz), None))
+        rp, ci = dm.pattern()
+        rng = np.random.default_rng(17)
+        rows = np.repeat(np.arange(mesh.ncells), np.diff(rp))
+        vals = np.where(rows == ci, 4.5, 0.0) - rng.uniform(0.1, 1.0, ci.size) * (rows != ci)
+        L.check(L.lib().fvh_memcpy(nz, vals.ctypes.data_as(C.c_void_p), vals.nbytes))
+        x = rng.standard_normal(mesh.ncells)
+        xd, yd = L.DeviceArray.from_host(x), L.DeviceArray(mesh.ncells)
+        L.check(L.lib().fvh_spmv(h, xd.vp, yd.vp))
+        assert same_bits(yd.host(), ref.spmv_csr(mesh, vals, x))
+        L.check(L.lib().fvh_precond_set(h, L.PRECOND_DILU, None))
+        L.check(L.lib().fvh_precond_update(h), "fvh_precond_update")
+        dinv = np.zeros(mesh.ncells)
+        L.check(L.lib().fvh_precond_dinv(h, L.dptr(dinv)))
+        L.check(L.lib().fvh_precond_apply(h, yd.vp, xd.vp), "fvh_precond_apply")
+        want = ref.precond_apply_csr(mesh, vals, L.PRECOND_DILU, x)
+        got = yd.host()
+        print(f"{walk}: {mesh.ncells} cells, {nlev} levels; apply rel diff "
+              f"{np.linalg.norm(got - want) / np.linalg.norm(want):.2e}", flush=True)
+        assert np.all(np.isfinite(dinv)) and np.all(dinv > 0)
+        assert same_bits(got, want)
+    finally:
+        L.lib().fvh_eqn_destroy(h)
+        mesh._dev = None

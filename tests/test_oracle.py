@@ -264,6 +264,42 @@ def test_c1_centreline_is_physical(c1_run, quad40):
     assert 0.35 < ys[np.argmin(us)] < 0.55
 
 
+def test_reference_bfs_case_with_ldl_takes_184_iterations(bfs10_path):
+    """CASE_UNV_BFS.jl exactly as the script sets it up: GMRES + DILU for U,
+    GMRES + LDL for p (the factor of p's set-up matrix, rDf = 1, kept for
+    every solve), relax 0.8 / 0.2: the script's record of 184 iterations to
+    the 1e-7 break."""
+    from tests.cases import bfs_case_model
+
+    mesh = F.build_mesh(bfs10_path, scale=0.001)
+    model = bfs_case_model(mesh)
+    cfg = config(1000, U_solver=F.GmresSolver, p_solver=F.GmresSolver, p_pre=F.LDL, relax_U=0.8, relax_p=0.2)
+    rs = ref.RefSimulation(model, F.CaseDesc(model, cfg))
+    assert rs.run(1000) == 184
+
+
+def test_ldl_is_the_inverse_of_the_set_up_matrix(quad40):
+    """The oracle's LDL apply solves the matrix it was built from (pressure
+    Laplacian, Neumann walls, Dirichlet outlet) to rounding, and dense
+    L D L^T reproduces it."""
+    mesh = quad40
+    rp, ci = pattern(mesh)
+    n = mesh.ncells
+    rows = np.repeat(np.arange(n), np.diff(rp))
+    rng = np.random.default_rng(3)
+    off = -rng.uniform(0.5, 1.5, ci.size)
+    A = np.zeros((n, n))
+    A[rows[rows != ci], ci[rows != ci]] = off[rows != ci]
+    A = 0.5 * (A + A.T)  # symmetric, as a Laplacian is
+    d = -A.sum(axis=1) + (np.arange(n) % 40 == 39) * 2.0  # weakly dominant, Dirichlet-like rows on one side
+    A[np.arange(n), np.arange(n)] = d
+    vals = A[rows, ci]
+    x = rng.standard_normal(n)
+    y = ref.precond_apply_csr(mesh, vals, L.PRECOND_LDL, x)
+    assert np.linalg.norm(A @ y - x) <= 1e-12 * np.linalg.norm(x) * np.linalg.cond(A) ** 0.5
+    assert np.allclose(y, np.linalg.solve(A, x), rtol=0, atol=1e-9 * np.abs(np.linalg.solve(A, x)).max())
+
+
 def test_reference_bfs_case_iteration_count(bfs10_path):
     """CASE_UNV_BFS.jl: 10 mm BFS mesh, nu = 1e-3, inlet 0.5 m/s, GMRES+DILU for U,
     relax 0.8 / 0.2; the script records convergence (all residuals <= 1e-7) in

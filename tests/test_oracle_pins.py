@@ -290,6 +290,118 @@ def test_interpolation_weights_and_boundary_correction(quad40):
 
 
 # ---------------------------------------------------------------------------
+# Non-orthogonal correction (Calculate_3_orthogonality_correction.jl,
+# Calculate_2_interpolation.jl:152-176)
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("name", ["quad40", "trig40", "backwardFacingStep_10mm"])
+def test_gradient_interpolation_is_exact_for_a_linear_field(ref_mesh, name):
+    """phi = a x + b y with its exact gradient in every cell: the weighted
+    average is that gradient and the correction along e vanishes, since
+    (phi2 - phi1)/delta = g . e for e = (c2 - c1)/delta; boundary faces are
+    left as they were."""
+    mesh = ref_mesh(name)
+    a, b = 2.5, -1.25
+    cc = mesh.cell_centre
+    phi = a * cc[:, 0] + b * cc[:, 1]
+    n = mesh.ncells
+    sentinel = np.full(mesh.nfaces, 7.0)
+    fx, fy, fz = ref.interpolate_gradient(mesh, np.full(n, a), np.full(n, b), np.zeros(n), phi, sentinel, sentinel,
+                                          sentinel)
+    inner = mesh.face_owners[:, 0] != mesh.face_owners[:, 1]
+    scale = np.abs(cc).max() / mesh.face_delta[inner].min()
+    tol = 64 * EPS * scale * max(abs(a), abs(b))
+    assert np.abs(fx[inner] - a).max() <= tol and np.abs(fy[inner] - b).max() <= tol
+    assert np.abs(fz[inner]).max() <= tol
+    assert np.all(fx[~inner] == 7.0) and np.all(fy[~inner] == 7.0) and np.all(fz[~inner] == 7.0)
+
+
+def test_gradient_interpolation_weight_is_the_projection(ref_mesh):
+    """Two cell gradients g1, g2 and phi = 0: the face value is
+    w g1 + (1 - w) g2 minus its component along e, with w the distance from
+    c2 to the face centre's projection on the centre line over |c2 - c1|
+    (1/2 on a uniform grid)."""
+    mesh = ref_mesh("trig40")
+    n = mesh.ncells
+    rng = np.random.default_rng(4)
+    gx, gy = rng.standard_normal(n), rng.standard_normal(n)
+    fx, fy, _ = ref.interpolate_gradient(mesh, gx, gy, np.zeros(n), np.zeros(n))
+    o1, o2 = mesh.face_owners[:, 0], mesh.face_owners[:, 1]
+    inner = np.nonzero(o1 != o2)[0]
+    c1, c2, cf = mesh.cell_centre[o1[inner]], mesh.cell_centre[o2[inner]], mesh.face_centre[inner]
+    d = c2 - c1
+    t = np.einsum("ij,ij->i", cf - c1, d) / np.einsum("ij,ij->i", d, d)
+    w = 1.0 - t  # |c2 - f'| / |c2 - c1| with f' = c1 + t d, 0 <= t <= 1 here
+    assert np.all((t > 0) & (t < 1))
+    e = mesh.face_e[inner]
+    ax = w * gx[o1[inner]] + (1 - w) * gx[o2[inner]]
+    ay = w * gy[o1[inner]] + (1 - w) * gy[o2[inner]]
+    s = ax * e[:, 0] + ay * e[:, 1]
+    assert np.allclose(fx[inner], ax - s * e[:, 0], atol=1e-12)
+    assert np.allclose(fy[inner], ay - s * e[:, 1], atol=1e-12)
+    q = ref_mesh("quad40")
+    g = np.ones(q.ncells)
+    qx, _, _ = ref.interpolate_gradient(q, np.where(np.arange(q.ncells) % 2 == 0, 1.0, 3.0), g * 0, g * 0, g * 0)
+    qi = q.face_owners[:, 0] != q.face_owners[:, 1]
+    vertical = np.abs(q.face_normal[:, 0]) < 0.5
+    # faces between x-neighbours alternate 1 and 3 -> mean 2, then the x
+    # component is removed along e = x; faces between y-neighbours keep 2 or
+    # the cell value pair mean
+    assert np.allclose(qx[qi & vertical], 2.0, atol=1e-12)
+
+
+def test_nonorthogonal_flux_vanishes_on_orthogonal_faces(ref_mesh):
+    """(n - e) A: zero on a uniform quad grid (n = e); on triangles it is
+    the face gradient times the tangential offset, here checked in closed
+    form for a constant face gradient; boundary faces are not written."""
+    q = ref_mesh("quad40")
+    nf = q.nfaces
+    out = ref.nonorthogonal_flux(q, np.full(nf, 1.5), np.full(nf, -2.0), np.zeros(nf), np.full(nf, 4.0))
+    inner = q.face_owners[:, 0] != q.face_owners[:, 1]
+    # quad40's node coordinates carry rounding (scale 0.001), so n - e is
+    # zero to the last bits only: |g| A 1e-12 at most
+    assert np.abs(out[inner]).max() <= 2.5 * q.face_area.max() * 1e-12 and np.all(out[~inner] == 4.0)
+    t = ref_mesh("trig40")
+    nf = t.nfaces
+    out = ref.nonorthogonal_flux(t, np.full(nf, 1.5), np.full(nf, -2.0), np.zeros(nf), np.full(nf, 4.0))
+    inner = t.face_owners[:, 0] != t.face_owners[:, 1]
+    T = (t.face_normal - t.face_e) * t.face_area[:, None]
+    want = 1.5 * T[:, 0] - 2.0 * T[:, 1]
+    assert np.allclose(out[inner], want[inner], atol=1e-15)
+    assert np.abs(out[inner]).max() > 1e-6  # the triangles are not orthogonal
+    assert np.all(out[~inner] == 4.0)
+
+
+def test_correct_moves_flux_from_owner_to_neighbour(ref_mesh):
+    """correct!: each internal face adds c = -sign J corr to its owner and
+    -c to its neighbour, so the sum over cells is unchanged (to rounding);
+    one face alone changes exactly its two cells; a boundary face adds +c
+    then -c to its cell."""
+    mesh = ref_mesh("trig40")
+    nf, n = mesh.nfaces, mesh.ncells
+    rng = np.random.default_rng(2)
+    J, corr, b = rng.uniform(0.5, 2.0, nf), rng.standard_normal(nf), rng.standard_normal(n)
+    out = ref.nonorthogonal_correct(mesh, 1, J, corr, b)
+    assert abs(out.sum() - b.sum()) < 1e-12 * np.abs(b).sum()
+    f = mesh.nbfaces + 17
+    one = np.zeros(nf)
+    one[f] = 0.5
+    o1, o2 = mesh.face_owners[f]
+    out = ref.nonorthogonal_correct(mesh, -1, J, one, np.zeros(n))
+    want = np.zeros(n)
+    want[o1], want[o2] = 0.5 * J[f], -0.5 * J[f]
+    assert np.array_equal(out, want)
+    fb = 3
+    one = np.zeros(nf)
+    one[fb] = 1e-3
+    c = mesh.face_owners[fb, 0]
+    b0 = np.zeros(n)
+    b0[c] = 1.0
+    out = ref.nonorthogonal_correct(mesh, 1, J, one, b0)
+    k = -J[fb] * 1e-3
+    assert out[c] == (1.0 + k) + (-k) and np.all(np.delete(out, c) == 0.0)
+
+
+# ---------------------------------------------------------------------------
 # k-omega (RANSModels_functions.jl:5-88, RANSModels_kOmega.jl:114-277)
 # ---------------------------------------------------------------------------
 def test_production_is_twice_the_strain_rate_norm_for_a_linear_field(quad40):

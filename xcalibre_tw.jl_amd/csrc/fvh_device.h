@@ -88,10 +88,14 @@ struct DevMesh {
     const int *rowptr, *col, *diag, *tslot;         // CSR pattern of A
     const int *cf_ptr, *cf_face, *cf_nbr, *cf_slot;  // cell -> internal faces (faces_range order)
     const double *cf_sign, *cf_w, *cf_omw;           // nsign, divergence weights of that side
+    int nif;                                         // internal faces (faces nbf .. nf-1)
+    const int2 *fi_slot;                             // internal face: slots of A[o, n] and A[n, o] (~slot: nsign -1)
+    const double2 *fi_w;                             // internal face: divergence weights of the o and n sides
     const int *own1, *own2;                          // face owners (boundary: own1 == own2)
     const double *fw, *farea, *fdelta;               // face weight, area, delta
     const double *Sfx, *Sfy, *Sfz;                   // area * normal
     const double *nx, *ny, *nz;                      // unit normals
+    const double *ex, *ey, *ez;                      // unit vectors along the centre line (face e)
     const double *vol;                               // cell volumes
     const double *ccx, *ccy, *ccz, *fcx, *fcy, *fcz;  // centres
     const int *cb_ptr, *cb_face;                     // cell -> boundary faces, ascending face id
@@ -211,6 +215,12 @@ struct Schedule {
     // the level barrier; only next to levels wider than the ring).  f2b[k]:
     // the forward position of the row at backward position k.
     DBuf<int> lcode, f2b;
+    DBuf<int> lpos;  // per entry: the neighbour's position in the walk's order (the sync-free walk)
+    // The sync-free walk's waves, forward and backward: (first position,
+    // count <= 64) per wave, each wave inside one level, waves in level order
+    DBuf<int> swf, swb;
+    int nswf = 0, nswb = 0;
+    int ring = kLevRing;  // results the level walk's LDS ring holds (power of two <= kLevRing)
     int band_nx = 0;  // >0: A is a banded grid (deps only r-1, r-nx)
     DBuf<unsigned char> bmask;  // banded grid: bit k of row r = entry r + (-nx, -1, 0, 1, nx)[k] present
     int band_R = 0;   // grid rows per lane of the sweeps (0: not tuned yet)
@@ -226,7 +236,12 @@ struct fvh_mesh {
     int nboundaries = 0;
     // owned device storage
     fvh::DBuf<int> rowptr, col, diag, tslot, cf_ptr, cf_face, cf_nbr, cf_slot, own1, own2, cb_ptr, cb_face,
-        bf_face, bf_cell, bf_patch;
+        bf_face, bf_cell, bf_patch, fi_slot;
+    fvh::DBuf<double> fi_w, ex, ey, ez;
+    bool cf_ascending = true;  // every cell's internal faces in ascending face id
+    // per-slot diagonal terms handed from the face pass of the assembly to
+    // its cell pass ([slot][face term]); sized by the largest term count seen
+    fvh::DBuf<double> asm_terms;
     fvh::DBuf<double> cf_sign, cf_w, cf_omw, fw, farea, fdelta, Sfx, Sfy, Sfz, nx, ny, nz, vol, ccx, ccy, ccz, fcx,
         fcy, fcz;
     // host copies used by set-up code

@@ -22,6 +22,8 @@ struct AsmArgs {
     int mode;
     int nterms;
     KTerm t[kMaxTerms];
+    int nft;              // face terms (Laplacian, Divergence), set by launch_assemble
+    KTerm ft[kMaxTerms];  // ... in model order
     int nsrc;
     double ssign[kMaxSrc];
     const double *sfield[kMaxSrc];
@@ -125,8 +127,35 @@ struct alignas(16) LevBlob32 {
 struct LevelStore {
     DBuf<LevBlob64> upd, fwd, bwd;
     DBuf<LevBlob32> fwd32, bwd32;
-    DBuf<double> a, b;
-    DBuf<float> a32, b32;
+    DBuf<double> a, b, c;
+    DBuf<float> a32, b32, c32;
+    bool sf = false;         // the records hold walk positions (sync-free walk)
+    int *h_fault = nullptr;  // pinned: a wait of the sync-free walk ran out
+    LevelStore() = default;
+    LevelStore(const LevelStore &) = delete;
+    LevelStore &operator=(const LevelStore &) = delete;
+    ~LevelStore() {
+        if (h_fault) (void)hipHostFree(h_fault);
+    }
+};
+}  // namespace fvh
+
+namespace fvh {
+// LDL preconditioner: the factor of the set-up matrix in its ordering (L by
+// rows for the forward solve, by columns for the backward one), its pivots,
+// the solve's work vectors and a pinned fault word.
+struct LdlStore {
+    bool ready = false;
+    int nnzL = 0;
+    DBuf<int> perm, fptr, fidx, bptr, bidx;
+    DBuf<double> fval, bval, D, rhs, y, z;
+    int *h_fault = nullptr;
+    LdlStore() = default;
+    LdlStore(const LdlStore &) = delete;
+    LdlStore &operator=(const LdlStore &) = delete;
+    ~LdlStore() {
+        if (h_fault) (void)hipHostFree(h_fault);
+    }
 };
 }  // namespace fvh
 
@@ -142,6 +171,7 @@ struct fvh_eqn {
     fvh::BandStore bd;  // DILU on banded grids (packed order)
     fvh::SkewStore sk;  // DILU on banded grids (pipelined sweeps)
     fvh::LevelStore lv; // DILU on unstructured meshes (level-scheduled records)
+    fvh::LdlStore ldl;  // LDL: factor of the set-up matrix
     fvh::KrylovWork kw;
     fvh::DBuf<double> red;  // residual reductions
 };
@@ -160,6 +190,11 @@ int solve(fvh_eqn *e, const fvh_solver_cfg &cfg, double *x, fvh_stats *st);
 int residual(fvh_eqn *e, const double *phi, double *res_host);
 int residual_async(fvh_eqn *e, const double *phi, double *res_dev);
 int precond_dinv_natural(fvh_eqn *e, const double **out);
+// LDL preconditioner (k_ldl.hip)
+int ldl_setup(fvh_eqn *e, const double *val_dev);
+template <typename T>
+int ldl_apply(fvh_eqn *e, const T *x, T *out, const KScal *ks);
+int ldl_check(fvh_eqn *e);
 
 // shared by the single-case and the per-case (batched) solvers
 int ensure_work(fvh_eqn *e, int basis);

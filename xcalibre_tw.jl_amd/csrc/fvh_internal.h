@@ -20,6 +20,8 @@ struct HMesh {
     std::vector<double> face_centre, face_normal, face_e, face_area, face_delta, face_weight;
     std::vector<int64_t> boundary_range, boundary_faces, boundary_cells;
     std::vector<std::string> names;
+    std::vector<double> node_xyz;                         // [3 * npoints]
+    std::vector<int64_t> cell_nodes_ptr, cell_nodes;      // zero-based node ids per cell
 };
 
 // Records a formatted message for fvh_last_error and returns `code`.

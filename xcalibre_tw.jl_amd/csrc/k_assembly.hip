@@ -1,14 +1,16 @@
 // Discretise: face-loop assembly of the model terms into A (CSR values) and
 // b, with boundary conditions, implicit relaxation and the pressure
-// reference fused into the same pass.
+// reference fused into the cell pass.
 //
-// One thread owns one cell and walks its internal faces in faces_range
-// order (Discretise_2_generated_distretisation.jl:55-73).  Each (cell, face)
-// side writes the single off-diagonal slot A[cell, neighbour] through the
-// precomputed face->CSR slot map, so the scatter is deterministic and needs
-// no atomics; the diagonal and b are accumulated in registers in exactly the
-// reference order:
-//   faces (terms in model order) -> scheme_source! terms -> sources
+// Two passes, no atomics.  The face pass (one thread per internal face,
+// face arrays read coalesced) evaluates every face term for the owner and
+// the neighbour side (Discretise_1_schemes.jl:40-92), writes the two
+// off-diagonal slots through the precomputed face->CSR slot map, and leaves
+// each side's diagonal terms in a per-slot array.  The cell pass (one
+// thread per cell) sums those terms in the reference's order
+// (Discretise_2_generated_distretisation.jl:55-73):
+//   faces in faces_range order (terms in model order)
+//   -> scheme_source! terms -> sources
 //   -> boundary conditions (BC tuple order, then patch face order, then terms)
 //   -> implicit relaxation -> setReference!.
 #include "fvh_eqn.h"
@@ -52,37 +54,80 @@ __device__ inline void apply_bc_entries(const DevMesh &m, const AsmArgs &a, int 
     }
 }
 
-__global__ void k_assemble(DevMesh m, AsmArgs a, double *__restrict__ nz, double *__restrict__ bv) {
+// Face pass: one thread per internal face, reading the face arrays
+// coalesced.  It evaluates every face term for both sides of the face
+// (owner o with its nsign, neighbour n with its own), writes the two
+// off-diagonal slots A[o, n] and A[n, o] (each the sum of its terms from 0
+// in model order, as the reference accumulates them into the zeroed
+// nzval), and hands the per-side diagonal terms to the cell pass through
+// td[slot][term], indexed by the off-diagonal slot of that side.
+template <int NFT>
+__global__ void __launch_bounds__(256) k_asm_faces(DevMesh m, AsmArgs a, double *__restrict__ nz,
+                                                   double *__restrict__ td) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= m.nif) return;
+    const int f = m.nbf + k;
+    const int2 sv = m.fi_slot[k];
+    const int so = sv.x >= 0 ? sv.x : ~sv.x, sn = sv.y >= 0 ? sv.y : ~sv.y;
+    const double nso = sv.x >= 0 ? 1.0 : -1.0, nsn = sv.y >= 0 ? 1.0 : -1.0;
+    double offo = 0.0, offn = 0.0;
+    double dto[NFT > 0 ? NFT : 1], dtn[NFT > 0 ? NFT : 1];
+#pragma unroll
+    for (int q = 0; q < NFT; ++q) {
+        const KTerm &t = a.ft[q];
+        const double flux = t.flux[f];
+        if (t.op == FVH_OP_LAPLACIAN) {
+            // the same coefficient on both sides (no nsign)
+            const double ap = (t.sign * (-flux * m.farea[f])) / m.fdelta[f];
+            dto[q] = ap;
+            dtn[q] = ap;
+            offo += -ap;
+            offn += -ap;
+        } else {
+            const double apo = t.sign * (flux * nso), apn = t.sign * (flux * nsn);
+            if (t.scheme == FVH_LINEAR) {
+                const double2 w = m.fi_w[k];
+                dto[q] = apo * (1.0 - w.x);
+                dtn[q] = apn * (1.0 - w.y);
+                offo += apo * w.x;
+                offn += apn * w.y;
+            } else {
+                dto[q] = jmax0(apo);
+                dtn[q] = jmax0(apn);
+                offo += -jmax0(-apo);
+                offn += -jmax0(-apn);
+            }
+        }
+    }
+    nz[so] = offo;
+    nz[sn] = offn;
+#pragma unroll
+    for (int q = 0; q < NFT; ++q) {
+        td[(size_t)so * NFT + q] = dto[q];
+        td[(size_t)sn * NFT + q] = dtn[q];
+    }
+}
+
+// Cell pass: one thread per cell.  NFT >= 0: full assembly, the diagonal
+// summed from 0 over the cell's internal faces in faces_range order, terms
+// in model order within a face (the face pass's per-side terms), then the
+// cell terms and sources; NFT < 0: the diagonal and b as they stand.  Then
+// boundary conditions, implicit relaxation and the reference cell.
+template <int NFT>
+__global__ void __launch_bounds__(256) k_asm_cells(DevMesh m, AsmArgs a, const double *__restrict__ td,
+                                                   double *__restrict__ nz, double *__restrict__ bv) {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= m.n) return;
     const int dslot = m.diag[c];
     double diag, b;
-    if (a.mode == ASM_FULL) {
+    if (NFT >= 0) {
         diag = 0.0;
-        for (int i = m.cf_ptr[c]; i < m.cf_ptr[c + 1]; ++i) {
-            const int f = m.cf_face[i];
-            const double ns = m.cf_sign[i];
-            double off = 0.0;
+        if (NFT > 0) {
+            for (int i = m.cf_ptr[c]; i < m.cf_ptr[c + 1]; ++i) {
+                const double *p = td + (size_t)m.cf_slot[i] * (NFT > 0 ? NFT : 1);
 #pragma unroll
-            for (int ti = 0; ti < kMaxTerms; ++ti) {
-                if (ti >= a.nterms) break;
-                const KTerm &t = a.t[ti];
-                if (t.op == FVH_OP_LAPLACIAN) {
-                    double ap = (t.sign * (-t.flux[f] * m.farea[f])) / m.fdelta[f];
-                    diag += ap;
-                    off += -ap;
-                } else if (t.op == FVH_OP_DIVERGENCE) {
-                    double ap = t.sign * (t.flux[f] * ns);
-                    if (t.scheme == FVH_LINEAR) {
-                        diag += ap * m.cf_omw[i];
-                        off += ap * m.cf_w[i];
-                    } else {
-                        diag += jmax0(ap);
-                        off += -jmax0(-ap);
-                    }
-                }
+                for (int q = 0; q < (NFT > 0 ? NFT : 0); ++q) diag += p[q];
             }
-            nz[m.cf_slot[i]] = off;
         }
         b = 0.0;
         const double volume = m.vol[c];
@@ -167,10 +212,45 @@ int check_bc_terms(const AsmArgs &a, const fvh_bc *bcs, int nbc) {
     return FVH_OK;
 }
 
-int launch_assemble(fvh_eqn *e, const AsmArgs &a) {
-    hipLaunchKernelGGL(k_assemble, grid1d(e->m->n), dim3(256), 0, stream(), e->m->d, a, e->nz.p, e->b.p);
+template <int NFT>
+static int launch_full(fvh_eqn *e, const AsmArgs &a) {
+    fvh_mesh *m = e->m;
+    double *td = nullptr;
+    if (NFT > 0) {
+        const size_t need = (size_t)m->nnz * NFT;
+        if (m->asm_terms.n < need) FVH_TRY(m->asm_terms.alloc(need));
+        td = m->asm_terms.p;
+    }
+    if (m->d.nif > 0) {
+        hipLaunchKernelGGL((k_asm_faces<NFT>), grid1d(m->d.nif), dim3(256), 0, stream(), m->d, a, e->nz.p, td);
+        HIP_TRY(hipGetLastError());
+    }
+    hipLaunchKernelGGL((k_asm_cells<NFT>), grid1d(m->n), dim3(256), 0, stream(), m->d, a, td, e->nz.p, e->b.p);
     HIP_TRY(hipGetLastError());
     return FVH_OK;
+}
+
+// Full assembly: the face pass then the cell pass (the face terms compacted
+// into a.ft in model order); the partial modes run the cell pass alone.
+int launch_assemble(fvh_eqn *e, const AsmArgs &a0) {
+    if (a0.mode != ASM_FULL) {
+        hipLaunchKernelGGL((k_asm_cells<-1>), grid1d(e->m->n), dim3(256), 0, stream(), e->m->d, a0, nullptr, e->nz.p,
+                           e->b.p);
+        HIP_TRY(hipGetLastError());
+        return FVH_OK;
+    }
+    AsmArgs a = a0;
+    a.nft = 0;
+    for (int i = 0; i < a.nterms; ++i)
+        if (a.t[i].op == FVH_OP_LAPLACIAN || a.t[i].op == FVH_OP_DIVERGENCE) a.ft[a.nft++] = a.t[i];
+    switch (a.nft) {
+        case 0: return launch_full<0>(e, a);
+        case 1: return launch_full<1>(e, a);
+        case 2: return launch_full<2>(e, a);
+        case 3: return launch_full<3>(e, a);
+        case 4: return launch_full<4>(e, a);
+        default: return set_error(FVH_ERR_SIZE, "at most %d face terms", kMaxTerms);
+    }
 }
 
 }  // namespace fvh
@@ -269,6 +349,15 @@ extern "C" int fvh_set_reference(fvh_eqn *e, double pref, int64_t cell) {
     a.pref = pref;
     a.refcell = (int)cell;
     FVH_TRY(launch_assemble(e, a));
+    HIP_TRY(hipStreamSynchronize(stream()));
+    return FVH_OK;
+}
+
+// correct!(eqn, term, corr_flux) [Calculate_3_orthogonality_correction.jl:6-24]
+extern "C" int fvh_nonorthogonal_correct(fvh_eqn *e, int32_t sign, const double *J, const double *corr) {
+    if (!e || !J || !corr) return set_error(FVH_ERR_ARG, "null argument");
+    if (sign != 1 && sign != -1) return set_error(FVH_ERR_ARG, "sign must be +1 or -1");
+    FVH_TRY(launch_nonorthogonal_correct(e->m, sign, J, corr, e->b.p));
     HIP_TRY(hipStreamSynchronize(stream()));
     return FVH_OK;
 }

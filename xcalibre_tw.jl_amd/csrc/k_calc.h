@@ -31,6 +31,10 @@ int launch_correct_interpolation(const fvh_mesh *m, const double *dx, const doub
 int launch_grad(const fvh_mesh *m, int scheme, const BcVals &bc, const double *phi, double *phif, double *dx, double *dy,
                 double *dz);
 int launch_div(const fvh_mesh *m, const double *fx, const double *fy, const double *fz, double *out);
+int launch_interpolate_gradient(const fvh_mesh *m, const double *gx, const double *gy, const double *gz,
+                                const double *phi, double *fx, double *fy, double *fz);
+int launch_nonorthogonal_flux(const fvh_mesh *m, const double *fx, const double *fy, const double *fz, double *phif);
+int launch_nonorthogonal_correct(const fvh_mesh *m, int sign, const double *J, const double *corr, double *b);
 int launch_flux(const fvh_mesh *m, const double *fx, const double *fy, const double *fz, double *out);
 
 inline dim3 grid1d(int64_t n, int block = 256) {

@@ -226,6 +226,104 @@ int launch_flux(const fvh_mesh *m, const double *fx, const double *fy, const dou
     return FVH_OK;
 }
 
+// ---------------------------------------------------------------------------
+// Non-orthogonal correction (Calculate_3_orthogonality_correction.jl,
+// Calculate_2_interpolation.jl:152-176).  The reference defines the
+// interpolation weight for these operators only for the Linear scheme
+// (weight(::Type{Linear}, ...), :81-93): the face centre projected on the
+// line between the cell centres.
+// ---------------------------------------------------------------------------
+__device__ inline double nonorth_weight(const DevMesh &m, int f, int c1, int c2) {
+    const double c1x = m.ccx[c1], c1y = m.ccy[c1], c1z = m.ccz[c1];
+    const double c2x = m.ccx[c2], c2y = m.ccy[c2], c2z = m.ccz[c2];
+    const double ax = m.fcx[f] - c1x, ay = m.fcy[f] - c1y, az = m.fcz[f] - c1z;  // c1_f
+    const double bx = c2x - c1x, by = c2y - c1y, bz = c2z - c1z;              // c1_c2
+    const double q = ((ax * bx + ay * by) + az * bz) / ((bx * bx + by * by) + bz * bz);
+    // f' = c1 - q (c1 - c2)
+    const double px = c1x - q * (c1x - c2x), py = c1y - q * (c1y - c2y), pz = c1z - q * (c1z - c2z);
+    const double dx = c2x - px, dy = c2y - py, dz = c2z - pz;
+    const double ex = c2x - c1x, ey = c2y - c1y, ez = c2z - c1z;
+    return sqrt((dx * dx + dy * dy) + dz * dz) / sqrt((ex * ex + ey * ey) + ez * ez);
+}
+
+// interpolate!(gradf, grad, phi): internal faces, weighted average of the
+// two cell gradients corrected along e by the face-normal difference.
+__global__ void k_interpolate_gradient(DevMesh m, const double *__restrict__ gx, const double *__restrict__ gy,
+                                       const double *__restrict__ gz, const double *__restrict__ phi,
+                                       double *__restrict__ fx, double *__restrict__ fy, double *__restrict__ fz) {
+    const int f = m.nbf + blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= m.nf) return;
+    const int c1 = m.own1[f], c2 = m.own2[f];
+    const double w = nonorth_weight(m, f, c1, c2);
+    const double omw = 1.0 - w;
+    const double ax = w * gx[c1] + omw * gx[c2];
+    const double ay = w * gy[c1] + omw * gy[c2];
+    const double az = w * gz[c1] + omw * gz[c2];
+    const double ex = m.ex[f], ey = m.ey[f], ez = m.ez[f];
+    const double s = (phi[c2] - phi[c1]) / m.fdelta[f] - ((ax * ex + ay * ey) + az * ez);
+    fx[f] = ax + s * ex;
+    fy[f] = ay + s * ey;
+    fz[f] = az + s * ez;
+}
+
+// nonorthogonal_flux!(phif, gradf): internal faces, gradf . (normal - e) area.
+__global__ void k_nonorthogonal_flux(DevMesh m, const double *__restrict__ fx, const double *__restrict__ fy,
+                                     const double *__restrict__ fz, double *__restrict__ phif) {
+    const int f = m.nbf + blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= m.nf) return;
+    const double a = m.farea[f];
+    const double tx = (m.nx[f] - m.ex[f]) * a, ty = (m.ny[f] - m.ey[f]) * a, tz = (m.nz[f] - m.ez[f]) * a;
+    phif[f] = (fx[f] * tx + fy[f] * ty) + fz[f] * tz;
+}
+
+// correct!(eqn, term, corr_flux): the reference walks every face in id
+// order, adding c = -sign J_f corr_f to its first owner's b and -c to its
+// second (a boundary face gives its one cell +c then -c).  Per cell that is
+// its boundary faces in ascending id, then its internal faces in ascending
+// id (faces_range order), each boundary face first +c then -c.
+__global__ void k_nonorthogonal_correct(DevMesh m, double msign, const double *__restrict__ J,
+                                        const double *__restrict__ corr, double *__restrict__ b) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= m.n) return;
+    double v = b[c];
+    for (int i = m.cb_ptr[c]; i < m.cb_ptr[c + 1]; ++i) {
+        const int f = m.cb_face[i];
+        const double k = msign * J[f] * corr[f];
+        v += k;
+        v += k * (-1.0);
+    }
+    for (int i = m.cf_ptr[c]; i < m.cf_ptr[c + 1]; ++i) {
+        const int f = m.cf_face[i];
+        const double k = msign * J[f] * corr[f];
+        v += (m.own1[f] == c) ? k : k * (-1.0);
+    }
+    b[c] = v;
+}
+
+int launch_interpolate_gradient(const fvh_mesh *m, const double *gx, const double *gy, const double *gz,
+                                const double *phi, double *fx, double *fy, double *fz) {
+    const int64_t ni = m->nf - m->nbf;
+    if (ni <= 0) return FVH_OK;
+    hipLaunchKernelGGL(k_interpolate_gradient, grid_of(ni), dim3(256), 0, stream(), m->d, gx, gy, gz, phi, fx, fy, fz);
+    HIP_TRY(hipGetLastError());
+    return FVH_OK;
+}
+int launch_nonorthogonal_flux(const fvh_mesh *m, const double *fx, const double *fy, const double *fz, double *phif) {
+    const int64_t ni = m->nf - m->nbf;
+    if (ni <= 0) return FVH_OK;
+    hipLaunchKernelGGL(k_nonorthogonal_flux, grid_of(ni), dim3(256), 0, stream(), m->d, fx, fy, fz, phif);
+    HIP_TRY(hipGetLastError());
+    return FVH_OK;
+}
+int launch_nonorthogonal_correct(const fvh_mesh *m, int sign, const double *J, const double *corr, double *b) {
+    if (!m->cf_ascending)
+        return set_error(FVH_ERR_UNSUPPORTED, "correct!: a cell's internal faces are not in ascending face order");
+    hipLaunchKernelGGL(k_nonorthogonal_correct, grid_of(m->n), dim3(256), 0, stream(), m->d, (double)(-sign), J, corr,
+                       b);
+    HIP_TRY(hipGetLastError());
+    return FVH_OK;
+}
+
 // grad!(grad, phif, phi, BCs) for a scalar field
 int launch_grad(const fvh_mesh *m, int scheme, const BcVals &bc, const double *phi, double *phif, double *dx, double *dy,
                 double *dz) {
@@ -295,5 +393,17 @@ extern "C" int fvh_div(const fvh_mesh *m, const double *fx, const double *fy, co
 extern "C" int fvh_flux(const fvh_mesh *m, const double *fx, const double *fy, const double *fz, double *out) {
     if (!m || !fx || !fy || !fz || !out) return set_error(FVH_ERR_ARG, "null argument");
     FVH_TRY(launch_flux(m, fx, fy, fz, out));
+    SYNC_RETURN();
+}
+extern "C" int fvh_interpolate_gradient(const fvh_mesh *m, const double *gx, const double *gy, const double *gz,
+                                        const double *phi, double *fx, double *fy, double *fz) {
+    if (!m || !gx || !gy || !gz || !phi || !fx || !fy || !fz) return set_error(FVH_ERR_ARG, "null argument");
+    FVH_TRY(launch_interpolate_gradient(m, gx, gy, gz, phi, fx, fy, fz));
+    SYNC_RETURN();
+}
+extern "C" int fvh_nonorthogonal_flux(const fvh_mesh *m, const double *fx, const double *fy, const double *fz,
+                                      double *phif) {
+    if (!m || !fx || !fy || !fz || !phif) return set_error(FVH_ERR_ARG, "null argument");
+    FVH_TRY(launch_nonorthogonal_flux(m, fx, fy, fz, phif));
     SYNC_RETURN();
 }

@@ -1,0 +1,187 @@
+// LDL preconditioner on the device: y = P^T L^-T D^-1 L^-1 P x with the
+// factor of the set-up matrix (ldl_host.cpp), frozen after the first
+// update as the reference freezes opLDL (preconditioners_0_types.jl:36-42,
+// preconditioners_2_functions.jl:52-57).
+//
+// The two triangular solves are sync-free: one wave per row of L (forward,
+// rows in factor order) or of L^T (backward, rows in reverse order), the
+// lanes splitting the row's entries.  A row's inputs are earlier rows'
+// results, which their waves publish with device-coherent stores into an
+// array filled with a signalling-NaN sentinel before the solve; a lane polls
+// a value until it is no longer the sentinel.  Every wave waits only for
+// rows of waves dispatched before it, so the solve always drains; each poll
+// is bounded, and a wait that runs out sets a fault word (pinned host
+// memory) that the caller reports.  Per row: lane l sums its entries l,
+// l + 64, ... in order from 0, then the lanes are added by a fixed tree.
+#include <algorithm>
+#include <vector>
+
+#include "fvh_eqn.h"
+#include "ldl_host.h"
+
+namespace fvh {
+
+namespace {
+
+constexpr unsigned long long kLdlSent = 0x7FF4DEAD5EA11A57ull;
+constexpr int kLdlSpin = 1 << 20;  // polls (a few 100 ms) before a wait is declared lost
+constexpr int kLdlWaves = 4;       // waves (rows) per work-group
+
+__device__ inline double ldl_load(const double *p) {
+    return __builtin_bit_cast(double, __hip_atomic_load(reinterpret_cast<const unsigned long long *>(p),
+                                                        __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+}
+__device__ inline void ldl_store(double *p, double v) {
+    __hip_atomic_store(reinterpret_cast<unsigned long long *>(p), __builtin_bit_cast(unsigned long long, v),
+                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline bool ldl_sent(double v) { return __builtin_bit_cast(unsigned long long, v) == kLdlSent; }
+
+// Value of row j once published (bounded wait).
+__device__ inline double ldl_wait(const double *y, int j, int *fault) {
+    double v = ldl_load(y + j);
+    for (int s = 0; ldl_sent(v); ++s) {
+        if (s >= kLdlSpin) {
+            __hip_atomic_store(fault, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+            return 0.0;
+        }
+        __builtin_amdgcn_s_sleep(1);
+        v = ldl_load(y + j);
+    }
+    return v;
+}
+
+__device__ inline double wave_sum(double a) {
+    for (int off = 32; off > 0; off >>= 1) a = a + __shfl_down(a, off, 64);
+    return a;
+}
+
+}  // namespace
+
+// rhs[k] = x[perm[k]] (in FP64); both result arrays to the sentinel
+template <typename T>
+__global__ void k_ldl_prep(int n, const int *__restrict__ perm, const T *__restrict__ x, double *__restrict__ rhs,
+                           double *__restrict__ y, double *__restrict__ z, const KScal *ks) {
+    if (ks && ks->done) return;
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    rhs[k] = (double)x[perm[k]];
+    y[k] = __builtin_bit_cast(double, kLdlSent);
+    z[k] = __builtin_bit_cast(double, kLdlSent);
+}
+
+// BWD = false: y_r = rhs_r - sum_j L_rj y_j (rows of L in order);
+// BWD = true:  z_r = y_r / D_r - sum_k L_kr z_k (rows of L^T, last first).
+template <bool BWD>
+__global__ void __launch_bounds__(64 * kLdlWaves)
+    k_ldl_sweep(int n, const int *__restrict__ ptr, const int *__restrict__ idx, const double *__restrict__ val,
+                const double *__restrict__ rhs, const double *__restrict__ D, const double *src, double *dst,
+                int *fault, const KScal *ks) {
+    if (ks && ks->done) return;
+    const int w = blockIdx.x * kLdlWaves + (int)(threadIdx.x >> 6);
+    if (w >= n) return;
+    const int lane = threadIdx.x & 63;
+    const int r = BWD ? n - 1 - w : w;
+    double acc = 0.0;
+    for (int p = ptr[r] + lane; p < ptr[r + 1]; p += 64) acc = acc + val[p] * ldl_wait(src, idx[p], fault);
+    const double s = wave_sum(acc);
+    if (lane == 0) {
+        const double base = BWD ? ldl_wait(rhs, r, fault) / D[r] : rhs[r];
+        ldl_store(dst + r, base - s);
+    }
+}
+
+template <typename T>
+__global__ void k_ldl_out(int n, const int *__restrict__ perm, const double *__restrict__ z, T *__restrict__ out,
+                          const KScal *ks) {
+    if (ks && ks->done) return;
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    out[perm[k]] = (T)z[k];
+}
+
+// Factor A (values of the equation the preconditioner reads) on the host,
+// once, and upload L by rows (forward) and by columns (backward).
+int ldl_setup(fvh_eqn *e, const double *val_dev) {
+    fvh_mesh *m = e->m;
+    LdlStore &S = e->ldl;
+    if (S.ready) return FVH_OK;  // frozen after the set-up factorization
+    if (m->cases) return set_error(FVH_ERR_UNSUPPORTED, "LDL preconditioner: not with independent cases");
+    const int n = (int)m->n;
+    std::vector<double> val(m->nnz), cx(n), cy(n);
+    HIP_TRY(hipMemcpy(val.data(), val_dev, sizeof(double) * m->nnz, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(cx.data(), m->ccx.p, sizeof(double) * n, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(cy.data(), m->ccy.p, sizeof(double) * n, hipMemcpyDeviceToHost));
+    std::vector<int> perm;
+    ldl_order(n, m->h_rowptr.data(), m->h_col.data(), cx.data(), cy.data(), perm);
+    LdlHost F;
+    FVH_TRY(ldl_factor(n, m->h_rowptr.data(), m->h_col.data(), val.data(), perm, F));
+    // rows of L (strictly lower, columns ascending) from its columns
+    const int nnzL = F.Lp[n];
+    std::vector<int> rp(n + 1, 0), ri(nnzL);
+    std::vector<double> rv(nnzL);
+    for (int p = 0; p < nnzL; ++p) rp[F.Li[p] + 1]++;
+    for (int i = 0; i < n; ++i) rp[i + 1] += rp[i];
+    {
+        std::vector<int> pos(rp.begin(), rp.end() - 1);
+        for (int j = 0; j < n; ++j)
+            for (int p = F.Lp[j]; p < F.Lp[j + 1]; ++p) {
+                const int q = pos[F.Li[p]]++;
+                ri[q] = j;
+                rv[q] = F.Lx[p];
+            }
+    }
+    FVH_TRY(S.perm.upload(F.perm));
+    FVH_TRY(S.fptr.upload(rp));
+    FVH_TRY(S.fidx.upload(ri));
+    FVH_TRY(S.fval.upload(rv));
+    FVH_TRY(S.bptr.upload(F.Lp));
+    FVH_TRY(S.bidx.upload(F.Li));
+    FVH_TRY(S.bval.upload(F.Lx));
+    FVH_TRY(S.D.upload(F.D));
+    FVH_TRY(S.rhs.alloc(n));
+    FVH_TRY(S.y.alloc(n));
+    FVH_TRY(S.z.alloc(n));
+    if (!S.h_fault) {
+        HIP_TRY(hipHostMalloc((void **)&S.h_fault, sizeof(int), hipHostMallocCoherent | hipHostMallocMapped));
+        *S.h_fault = 0;
+    }
+    S.nnzL = nnzL;
+    S.ready = true;
+    return FVH_OK;
+}
+
+template <typename T>
+int ldl_apply(fvh_eqn *e, const T *x, T *out, const KScal *ks) {
+    LdlStore &S = e->ldl;
+    if (!S.ready) return set_error(FVH_ERR_ARG, "LDL preconditioner applied before its set-up factorization");
+    const int n = (int)e->m->n;
+    int *fault = nullptr;
+    HIP_TRY(hipHostGetDevicePointer((void **)&fault, S.h_fault, 0));
+    hipLaunchKernelGGL((k_ldl_prep<T>), grid1d(n), dim3(256), 0, stream(), n, S.perm.p, x, S.rhs.p, S.y.p, S.z.p, ks);
+    HIP_TRY(hipGetLastError());
+    const dim3 g((unsigned)((n + kLdlWaves - 1) / kLdlWaves));
+    hipLaunchKernelGGL((k_ldl_sweep<false>), g, dim3(64 * kLdlWaves), 0, stream(), n, S.fptr.p, S.fidx.p, S.fval.p,
+                       S.rhs.p, S.D.p, S.y.p, S.y.p, fault, ks);
+    HIP_TRY(hipGetLastError());
+    hipLaunchKernelGGL((k_ldl_sweep<true>), g, dim3(64 * kLdlWaves), 0, stream(), n, S.bptr.p, S.bidx.p, S.bval.p,
+                       S.y.p, S.D.p, S.z.p, S.z.p, fault, ks);
+    HIP_TRY(hipGetLastError());
+    hipLaunchKernelGGL((k_ldl_out<T>), grid1d(n), dim3(256), 0, stream(), n, S.perm.p, S.z.p, out, ks);
+    HIP_TRY(hipGetLastError());
+    return FVH_OK;
+}
+template int ldl_apply<double>(fvh_eqn *, const double *, double *, const KScal *);
+template int ldl_apply<float>(fvh_eqn *, const float *, float *, const KScal *);
+
+// A wait of the last solves ran out (reported once).
+int ldl_check(fvh_eqn *e) {
+    int *f = e->ldl.h_fault;
+    if (f && __atomic_load_n(f, __ATOMIC_ACQUIRE)) {
+        __atomic_store_n(f, 0, __ATOMIC_RELEASE);
+        return set_error(FVH_ERR_HIP, "LDL solve: a row's input did not arrive in time");
+    }
+    return FVH_OK;
+}
+
+}  // namespace fvh

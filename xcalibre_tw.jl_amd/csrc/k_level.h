@@ -16,5 +16,7 @@ int lev_dilu_apply(fvh_eqn *e, const double *x, double *y, const KScal *ks);
 // FP32 copies of the sweep records (after each update) and the FP32 sweeps
 int lev_to32(fvh_eqn *e);
 int lev_dilu_apply32(fvh_eqn *e, const float *x, float *y, const KScal *ks);
+// a wait of the sync-free walk ran out since the last check (reported once)
+int lev_check(fvh_eqn *e);
 
 }  // namespace fvh

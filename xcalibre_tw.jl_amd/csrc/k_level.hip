@@ -158,7 +158,7 @@ struct LevSet {  // one level's first-pass row of this thread
 template <typename T, int MODE>
 __global__ void __launch_bounds__(kLevBlock)
     k_lev_walk(int nlev, const int *__restrict__ lptr, const LevRec<T> *__restrict__ R, const T *__restrict__ in,
-               T *out, T *dinv, const KScal *ks) {
+               T *out, T *dinv, const KScal *ks, int ringmask) {
     if (ks && ks->done) return;
     extern __shared__ unsigned char lev_lds[];
     T *ring = reinterpret_cast<T *>(lev_lds);  // [kLevRing] results by position
@@ -177,7 +177,7 @@ __global__ void __launch_bounds__(kLevBlock)
         S.v = (MODE != 0 && S.r.m >= 0) ? in[lptr[l] + t] : T(0);
     };
     auto finish = [&](const LevRec<T> &r, T v, int pos) {
-        ring[pos & (kLevRing - 1)] = v;
+        ring[pos & ringmask] = v;
         out[pos] = v;
         if (MODE == 0) dinv[r.i] = T(1) / v;
     };
@@ -243,14 +243,120 @@ __global__ void k_lev_perm(int n, const int *__restrict__ idx, const T *__restri
 }
 
 // ---------------------------------------------------------------------------
+// Sync-free walk (the default): one lane per row at its walk position, all
+// CUs at once.  A row's neighbours are rows of earlier levels, whose lanes
+// publish their results with device-coherent stores into an array filled
+// with a signalling-NaN sentinel before the sweep; a lane polls its
+// neighbours' words until none is the sentinel, then evaluates the row (the
+// same lev_eval as the level walk, so the results are bit-identical) and
+// publishes it.  Each wave holds rows of one level only (a table of
+// (first position, count) per wave, waves in level order), so no lane ever
+// waits for a lane of its own wave, whose store the compiler may place
+// after the wave's polling loop has ended for every lane; a wave waits only
+// for waves of lower numbers, which the in-order dispatch of work-groups
+// has started, so the sweep always drains.  The sleep between polls doubles
+// while nothing arrives (up to about 2 us), which keeps waves far ahead of
+// the front from flooding the memory system; each wait is bounded and one
+// that runs out sets the sweep fault word.
+// ---------------------------------------------------------------------------
+template <typename T>
+struct SfWord;
+template <>
+struct SfWord<double> {
+    using U = unsigned long long;
+    static constexpr U kSent = 0x7FF4DEAD5EA11A57ull;
+};
+template <>
+struct SfWord<float> {
+    using U = unsigned int;
+    static constexpr U kSent = 0x7FA11A57u;
+};
+template <typename T>
+__device__ inline T sf_load(const T *p) {
+    using U = typename SfWord<T>::U;
+    return __builtin_bit_cast(T, __hip_atomic_load(reinterpret_cast<const U *>(p), __ATOMIC_RELAXED,
+                                                   __HIP_MEMORY_SCOPE_AGENT));
+}
+template <typename T>
+__device__ inline void sf_store(T *p, T v) {
+    using U = typename SfWord<T>::U;
+    __hip_atomic_store(reinterpret_cast<U *>(p), __builtin_bit_cast(U, v), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <typename T>
+__device__ inline bool sf_sent(T v) {
+    return __builtin_bit_cast(typename SfWord<T>::U, v) == SfWord<T>::kSent;
+}
+template <typename T>
+__host__ __device__ inline T sf_sentinel() {
+    return __builtin_bit_cast(T, SfWord<T>::kSent);
+}
+constexpr int kSfSpin = 1 << 19;  // polls (with back-off: about a second) before a wait is declared lost
+
+template <typename T, int MODE>
+__global__ void __launch_bounds__(256)
+    k_lev_sf(int nw, const int2 *__restrict__ waves, const LevRec<T> *__restrict__ R, const T *__restrict__ in,
+             T *out, T *dinv, int *fault, const KScal *ks) {
+    if (ks && ks->done) return;
+    const int w = blockIdx.x * 4 + (int)(threadIdx.x >> 6);
+    if (w >= nw) return;
+    const int2 seg = waves[w];
+    const int lane = threadIdx.x & 63;
+    if (lane >= seg.y) return;
+    const int k = seg.x + lane;
+    const LevRec<T> r = R[k];
+    const T v0 = MODE != 0 ? in[k] : T(0);
+    T nb[kLevDeps];
+#pragma unroll
+    for (int q = 0; q < kLevDeps; ++q) nb[q] = T(0);
+    unsigned need = r.m > 0 ? (1u << r.m) - 1u : 0u;
+    int nap = 1;
+    for (int s = 0; need != 0u; ++s) {
+#pragma unroll
+        for (int q = 0; q < kLevDeps; ++q)
+            if (need & (1u << q)) {
+                const T v = sf_load(out + r.j[q]);
+                if (!sf_sent(v)) {
+                    nb[q] = v;
+                    need &= ~(1u << q);
+                }
+            }
+        if (need == 0u) break;
+        if (s >= kSfSpin) {
+            __hip_atomic_store(fault, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+            break;
+        }
+        for (int z = 0; z < nap; ++z) __builtin_amdgcn_s_sleep(2);
+        nap = nap < 32 ? 2 * nap : 32;
+    }
+    const T res = lev_eval<T, MODE>(r, v0, nb);
+    sf_store(out + k, res);
+    if (MODE == 0) dinv[r.i] = T(1) / res;
+}
+
+// x (row order) into walk order, with the two result arrays of the sweeps
+// set to the sentinel
+template <typename T>
+__global__ void k_lev_sf_prep(int n, const int *__restrict__ rows, const T *__restrict__ x, T *__restrict__ wa,
+                              T *__restrict__ wb, T *__restrict__ wc, const KScal *ks) {
+    if (ks && ks->done) return;
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    if (x) wa[k] = x[rows[k]];
+    wb[k] = sf_sentinel<T>();
+    if (wc) wc[k] = sf_sentinel<T>();
+}
+
+// ---------------------------------------------------------------------------
 // Host side
 // ---------------------------------------------------------------------------
 static dim3 lev_grid(int n) { return dim3((unsigned)((n + 255) / 256)); }
+static dim3 sf_grid(int nwaves) { return dim3((unsigned)((nwaves + 3) / 4)); }  // four waves per work-group
 
 // The walk's LDS ring of results (FP64: 128 KiB) needs the larger dynamic
 // LDS allowance, set once per instantiation.
 template <typename T, int MODE>
-static int launch_walk(int nlev, const int *lptr, const LevRec<T> *R, const T *in, T *out, T *dinv, const KScal *ks) {
+static int launch_walk(int nlev, const int *lptr, const LevRec<T> *R, const T *in, T *out, T *dinv, const KScal *ks,
+                       int ring) {
     constexpr size_t lds = kLevRing * sizeof(T);
     static bool allowed = false;
     if (!allowed) {
@@ -258,8 +364,35 @@ static int launch_walk(int nlev, const int *lptr, const LevRec<T> *R, const T *i
                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
         allowed = true;
     }
-    hipLaunchKernelGGL((k_lev_walk<T, MODE>), dim3(1), dim3(kLevBlock), lds, stream(), nlev, lptr, R, in, out, dinv, ks);
+    hipLaunchKernelGGL((k_lev_walk<T, MODE>), dim3(1), dim3(kLevBlock), lds, stream(), nlev, lptr, R, in, out, dinv, ks,
+                       ring - 1);
     HIP_TRY(hipGetLastError());
+    return FVH_OK;
+}
+
+// The sync-free walk unless FVH_LEVEL_SYNCFREE=0 (the one-work-group level
+// walk); both give the same bits.
+static bool lev_syncfree() {
+    const char *env = std::getenv("FVH_LEVEL_SYNCFREE");
+    return !(env && env[0] == '0' && env[1] == 0);
+}
+
+static int lev_fault_ptr(fvh_eqn *e, int **dev) {
+    LevelStore &L = e->lv;
+    if (!L.h_fault) {
+        HIP_TRY(hipHostMalloc((void **)&L.h_fault, sizeof(int), hipHostMallocCoherent | hipHostMallocMapped));
+        *L.h_fault = 0;
+    }
+    HIP_TRY(hipHostGetDevicePointer((void **)dev, L.h_fault, 0));
+    return FVH_OK;
+}
+
+int lev_check(fvh_eqn *e) {
+    int *f = e->lv.h_fault;
+    if (f && __atomic_load_n(f, __ATOMIC_ACQUIRE)) {
+        __atomic_store_n(f, 0, __ATOMIC_RELEASE);
+        return set_error(FVH_ERR_HIP, "level sweep: a row's neighbour did not arrive in time");
+    }
     return FVH_OK;
 }
 
@@ -284,17 +417,34 @@ int lev_dilu_update(fvh_eqn *e, const double *val) {
     auto *U = reinterpret_cast<LevRec<double> *>(L.upd.p);
     auto *Fw = reinterpret_cast<LevRec<double> *>(L.fwd.p);
     auto *Bw = reinterpret_cast<LevRec<double> *>(L.bwd.p);
+    const bool sf = lev_syncfree();
+    if (sf && L.c.n != (size_t)n) FVH_TRY(L.c.alloc(n));
+    const int *codes = sf ? s.lpos.p : s.lcode.p;  // walk positions, or the level walk's ring codes
     hipLaunchKernelGGL(k_lev_pack, lev_grid(n), dim3(256), 0, stream(), n, s.frows.p, m->d.rowptr, m->d.col, m->d.diag,
-                       m->d.tslot, s.lcode.p, val, (const double *)nullptr, 0, U);
+                       m->d.tslot, codes, val, (const double *)nullptr, 0, U);
     HIP_TRY(hipGetLastError());
-    FVH_TRY((launch_walk<double, 0>(s.nlev, s.fptr.p, U, (const double *)nullptr, L.b.p, e->Dinv.p,
-                                    (const KScal *)nullptr)));
+    if (sf) {
+        int *fault = nullptr;
+        FVH_TRY(lev_fault_ptr(e, &fault));
+        hipLaunchKernelGGL((k_lev_sf_prep<double>), lev_grid(n), dim3(256), 0, stream(), n, s.frows.p,
+                           (const double *)nullptr, (double *)nullptr, L.b.p, (double *)nullptr,
+                           (const KScal *)nullptr);
+        HIP_TRY(hipGetLastError());
+        hipLaunchKernelGGL((k_lev_sf<double, 0>), sf_grid(s.nswf), dim3(256), 0, stream(), s.nswf,
+                           reinterpret_cast<const int2 *>(s.swf.p), U, (const double *)nullptr, L.b.p, e->Dinv.p,
+                           fault, (const KScal *)nullptr);
+        HIP_TRY(hipGetLastError());
+    } else {
+        FVH_TRY((launch_walk<double, 0>(s.nlev, s.fptr.p, U, (const double *)nullptr, L.b.p, e->Dinv.p,
+                                        (const KScal *)nullptr, s.ring)));
+    }
     hipLaunchKernelGGL(k_lev_pack, lev_grid(n), dim3(256), 0, stream(), n, s.frows.p, m->d.rowptr, m->d.col, m->d.diag,
-                       m->d.tslot, s.lcode.p, val, e->Dinv.p, 1, Fw);
+                       m->d.tslot, codes, val, e->Dinv.p, 1, Fw);
     HIP_TRY(hipGetLastError());
     hipLaunchKernelGGL(k_lev_pack, lev_grid(n), dim3(256), 0, stream(), n, s.brows.p, m->d.rowptr, m->d.col, m->d.diag,
-                       m->d.tslot, s.lcode.p, val, e->Dinv.p, 2, Bw);
+                       m->d.tslot, codes, val, e->Dinv.p, 2, Bw);
     HIP_TRY(hipGetLastError());
+    L.sf = sf;
     return FVH_OK;
 }
 
@@ -302,15 +452,32 @@ int lev_dilu_update(fvh_eqn *e, const double *val) {
 // -> backward walk -> y (row order); wa / wb: two vectors in walk order.
 template <typename T>
 static int lev_apply_t(fvh_eqn *e, const LevRec<T> *Fw, const LevRec<T> *Bw, const T *x, T *y, T *wa, T *wb,
-                       const KScal *ks) {
+                       T *wc, const KScal *ks) {
     const Schedule &s = e->m->sched;
     const int n = (int)e->m->n;
+    if (e->lv.sf) {  // records hold walk positions: the sync-free walk
+        int *fault = nullptr;
+        FVH_TRY(lev_fault_ptr(e, &fault));
+        hipLaunchKernelGGL((k_lev_sf_prep<T>), lev_grid(n), dim3(256), 0, stream(), n, s.frows.p, x, wa, wb, wc, ks);
+        HIP_TRY(hipGetLastError());
+        hipLaunchKernelGGL((k_lev_sf<T, 1>), sf_grid(s.nswf), dim3(256), 0, stream(), s.nswf,
+                           reinterpret_cast<const int2 *>(s.swf.p), Fw, wa, wb, (T *)nullptr, fault, ks);
+        HIP_TRY(hipGetLastError());
+        hipLaunchKernelGGL((k_lev_perm<T, false>), lev_grid(n), dim3(256), 0, stream(), n, s.f2b.p, wb, wa, ks);
+        HIP_TRY(hipGetLastError());
+        hipLaunchKernelGGL((k_lev_sf<T, 2>), sf_grid(s.nswb), dim3(256), 0, stream(), s.nswb,
+                           reinterpret_cast<const int2 *>(s.swb.p), Bw, wa, wc, (T *)nullptr, fault, ks);
+        HIP_TRY(hipGetLastError());
+        hipLaunchKernelGGL((k_lev_perm<T, true>), lev_grid(n), dim3(256), 0, stream(), n, s.brows.p, wc, y, ks);
+        HIP_TRY(hipGetLastError());
+        return FVH_OK;
+    }
     hipLaunchKernelGGL((k_lev_perm<T, false>), lev_grid(n), dim3(256), 0, stream(), n, s.frows.p, x, wa, ks);
     HIP_TRY(hipGetLastError());
-    FVH_TRY((launch_walk<T, 1>(s.nlev, s.fptr.p, Fw, wa, wb, (T *)nullptr, ks)));
+    FVH_TRY((launch_walk<T, 1>(s.nlev, s.fptr.p, Fw, wa, wb, (T *)nullptr, ks, s.ring)));
     hipLaunchKernelGGL((k_lev_perm<T, false>), lev_grid(n), dim3(256), 0, stream(), n, s.f2b.p, wb, wa, ks);
     HIP_TRY(hipGetLastError());
-    FVH_TRY((launch_walk<T, 2>(s.nlev, s.bptr.p, Bw, wa, wb, (T *)nullptr, ks)));
+    FVH_TRY((launch_walk<T, 2>(s.nlev, s.bptr.p, Bw, wa, wb, (T *)nullptr, ks, s.ring)));
     hipLaunchKernelGGL((k_lev_perm<T, true>), lev_grid(n), dim3(256), 0, stream(), n, s.brows.p, wb, y, ks);
     HIP_TRY(hipGetLastError());
     return FVH_OK;
@@ -320,7 +487,7 @@ int lev_dilu_apply(fvh_eqn *e, const double *x, double *y, const KScal *ks) {
     LevelStore &L = e->lv;
     if (!L.fwd.p) return set_error(FVH_ERR_ARG, "DILU records not prepared");
     return lev_apply_t<double>(e, reinterpret_cast<const LevRec<double> *>(L.fwd.p),
-                               reinterpret_cast<const LevRec<double> *>(L.bwd.p), x, y, L.a.p, L.b.p, ks);
+                               reinterpret_cast<const LevRec<double> *>(L.bwd.p), x, y, L.a.p, L.b.p, L.c.p, ks);
 }
 
 int lev_to32(fvh_eqn *e) {
@@ -331,6 +498,7 @@ int lev_to32(fvh_eqn *e) {
         FVH_TRY(L.bwd32.alloc(n));
         FVH_TRY(L.a32.alloc(n));
         FVH_TRY(L.b32.alloc(n));
+        FVH_TRY(L.c32.alloc(n));
     }
     hipLaunchKernelGGL(k_lev_to32, lev_grid(n), dim3(256), 0, stream(), n, 1,
                        reinterpret_cast<const LevRec<double> *>(L.fwd.p), reinterpret_cast<LevRec<float> *>(L.fwd32.p));
@@ -345,7 +513,8 @@ int lev_dilu_apply32(fvh_eqn *e, const float *x, float *y, const KScal *ks) {
     LevelStore &L = e->lv;
     if (!L.fwd32.p) return set_error(FVH_ERR_ARG, "FP32 DILU records not prepared");
     return lev_apply_t<float>(e, reinterpret_cast<const LevRec<float> *>(L.fwd32.p),
-                              reinterpret_cast<const LevRec<float> *>(L.bwd32.p), x, y, L.a32.p, L.b32.p, ks);
+                              reinterpret_cast<const LevRec<float> *>(L.bwd32.p), x, y, L.a32.p, L.b32.p, L.c32.p,
+                              ks);
 }
 
 static_assert(sizeof(LevRec<double>) == kLevRecBytes64 && sizeof(LevRec<float>) == kLevRecBytes32,

@@ -24,6 +24,16 @@
 
 namespace fvh {
 
+// A bounded wait of the sweeps (pipelined DILU or LDL) ran out since the
+// last check: reported once, by the next call that ran them.
+static int fault_check(fvh_eqn *e) {
+    fvh_eqn *q = e->pshare ? e->pshare : e;
+    const int a = skew_check(e);
+    const int b = ldl_check(q);
+    const int c = lev_check(q);
+    return a != FVH_OK ? a : (b != FVH_OK ? b : c);
+}
+
 // Banded grids (single case): during a solve the SpMV reads A by band
 // position (five values per row, no column indices), bit-identical to CSR.
 static bool band_spmv(const fvh_mesh *m) { return !m->cases && m->sched.band_nx > 0; }
@@ -193,6 +203,7 @@ int precond_update(fvh_eqn *eq) {
             if (lev_path(m)) return lev_dilu_update(e, src->nz.p);
             return dilu_update(m, src->nz.p, e->D.p, e->Dinv.p);
         case FVH_PRECOND_ILU0: return ilu0_factor(m, src->nz.p, e->lu.p);
+        case FVH_PRECOND_LDL: return ldl_setup(e, src->nz.p);  // the first update factors; later ones keep it
         default: return set_error(FVH_ERR_UNSUPPORTED, "preconditioner %d", e->pkind);
     }
 }
@@ -220,6 +231,7 @@ int precond_apply(fvh_eqn *eq, double *y, const double *x, const KScal *ks_in, d
             if (lev_path(m) && e->lv.fwd.p) return lev_dilu_apply(e, x, y, ks);
             return dilu_apply(m, src->nz.p, e->Dinv.p, x, y, ks);
         case FVH_PRECOND_ILU0: return ilu0_apply(m, e->lu.p, x, y, ks);
+        case FVH_PRECOND_LDL: return ldl_apply<double>(e, x, y, ks);
         default: return set_error(FVH_ERR_UNSUPPORTED, "preconditioner %d", e->pkind);
     }
 }
@@ -277,6 +289,7 @@ int prepare32(fvh_eqn *e) {
     fvh_eqn *src = P->pmat ? P->pmat : P;
     switch (P->pkind) {
         case FVH_PRECOND_NONE: return FVH_OK;
+        case FVH_PRECOND_LDL: return FVH_OK;  // the factor keeps its own FP32 copy, made at set-up
         case FVH_PRECOND_JACOBI:
         case FVH_PRECOND_NORMDIAG: {
             if (P->p32a.n != (size_t)n) FVH_TRY(P->p32a.alloc(n));
@@ -331,6 +344,7 @@ int precond_apply32(fvh_eqn *eq, float *y, const float *x, const KScal *ks_in) {
             if (lev_path(m) && e->lv.fwd32.p) return lev_dilu_apply32(e, x, y, ks);
             return dilu_apply32(m, e->p32b.p, e->p32a.p, x, y, ks);
         case FVH_PRECOND_ILU0: return ilu0_apply32(m, e->p32b.p, x, y, ks);
+        case FVH_PRECOND_LDL: return ldl_apply<float>(e, x, y, ks);  // the FP64 factor, FP32 vectors
         default: return set_error(FVH_ERR_UNSUPPORTED, "preconditioner %d", e->pkind);
     }
 }
@@ -890,7 +904,7 @@ static int krylov(const Sys<T> &S, const fvh_solver_cfg &cfg, T *x, fvh_stats *s
 int cg_sigma(fvh_eqn *e, double *sigma) {
     *sigma = 1.0;
     const fvh_eqn *P = e->pshare ? e->pshare : e;
-    if (P->pkind == FVH_PRECOND_DILU || P->pkind == FVH_PRECOND_ILU0) {
+    if (P->pkind == FVH_PRECOND_DILU || P->pkind == FVH_PRECOND_ILU0 || P->pkind == FVH_PRECOND_LDL) {
         double a00 = 0.0;
         HIP_TRY(hipMemcpyAsync(&a00, e->nz.p + e->m->h_diag[0], sizeof(double), hipMemcpyDeviceToHost, stream()));
         HIP_TRY(hipStreamSynchronize(stream()));
@@ -998,7 +1012,7 @@ int solve(fvh_eqn *e, const fvh_solver_cfg &cfg, double *x, fvh_stats *st) {
     }
     const int rc = solve_any(e, cfg, x, st);
     e->kw.band_live = false;
-    const int fault = skew_check(e);
+    const int fault = fault_check(e);
     return rc != FVH_OK ? rc : fault;
 }
 
@@ -1094,7 +1108,11 @@ extern "C" int fvh_precond_set(fvh_eqn *e, int32_t kind, fvh_eqn *matrix_of) {
             FVH_TRY(ilu_setup(e->m));
             FVH_TRY(e->lu.alloc(e->m->nnz));
             break;
-        default: return set_error(FVH_ERR_UNSUPPORTED, "preconditioner %d (LDL is not on the device path)", kind);
+        case FVH_PRECOND_LDL:
+            if (e->m->cases) return set_error(FVH_ERR_UNSUPPORTED, "LDL preconditioner: not with independent cases");
+            e->ldl.ready = false;  // factored at the next update
+            break;
+        default: return set_error(FVH_ERR_UNSUPPORTED, "preconditioner %d", kind);
     }
     return FVH_OK;
     FVH_GUARD_END
@@ -1104,14 +1122,14 @@ extern "C" int fvh_precond_update(fvh_eqn *e) {
     if (!e) return set_error(FVH_ERR_ARG, "null equation");
     FVH_TRY(precond_update(e));
     HIP_TRY(hipStreamSynchronize(stream()));
-    return skew_check(e);
+    return fault_check(e);
 }
 
 extern "C" int fvh_precond_apply(fvh_eqn *e, double *y, const double *x) {
     if (!e || !x || !y) return set_error(FVH_ERR_ARG, "null argument");
     FVH_TRY(precond_apply(e, y, x, nullptr, 1.0));
     HIP_TRY(hipStreamSynchronize(stream()));
-    return skew_check(e);
+    return fault_check(e);
 }
 
 int fvh::precond_dinv_natural(fvh_eqn *e, const double **out) {
@@ -1157,7 +1175,7 @@ extern "C" int fvh_precond_apply_f32(fvh_eqn *e, float *y, const float *x) {
     FVH_TRY(prepare32(e));
     FVH_TRY(precond_apply32(e, y, x, nullptr));
     HIP_TRY(hipStreamSynchronize(stream()));
-    return skew_check(e);
+    return fault_check(e);
     FVH_GUARD_END
 }
 
@@ -1214,6 +1232,6 @@ extern "C" int fvh_time_op(fvh_eqn *e, int32_t op, int32_t reps, const double *x
     (void)hipEventDestroy(a);
     (void)hipEventDestroy(b);
     *ms = (double)t / reps;
-    return skew_check(e);
+    return fault_check(e);
     FVH_GUARD_END
 }

@@ -185,16 +185,31 @@ int build_schedule(fvh_mesh *m) {
     // a structurally symmetric pattern: the longest dependency chain is the same both ways
     if (nlevb != nlev) return set_error(FVH_ERR_ARG, "matrix pattern is not structurally symmetric");
     m->sched.nlev = nlev;
+    // the level walk's LDS ring: kLevRing results, or fewer (a power of two)
+    // with FVH_LEVEL_RING, which sends more neighbours through the global
+    // and the after-barrier paths (a test knob; the results are the same)
+    int ring = kLevRing;
+    if (const char *env = std::getenv("FVH_LEVEL_RING")) {
+        const int r = std::atoi(env);
+        if (r >= 64 && r <= kLevRing && (r & (r - 1)) == 0) ring = r;
+    }
+    m->sched.ring = ring;
     {  // where the level walk finds each neighbour's result (positions in the walk's order)
-        std::vector<int> pos(n), posf(n), code(rp[n]), f2b(n);
+        std::vector<int> pos(n), posf(n), code(rp[n]), f2b(n), lpos(rp[n]);
         auto codes = [&](const std::vector<int> &lev, const std::vector<int> &ptr, const std::vector<int> &rows,
                          bool upper) {
             for (int k = 0; k < n; ++k) pos[rows[k]] = k;
             for (int i = 0; i < n; ++i) {
-                const int end = ptr[lev[i] + 1];  // the ring holds positions [end - kLevRing, end) when i's level ends
+                // the ring holds position q when i's level is walked if no later
+                // position up to the end of that level shares its word
+                // (q >= end - ring), and no earlier position of q's own level
+                // does (their lanes write the word at the same time when a level
+                // is wider than the ring)
+                const int end = ptr[lev[i] + 1];
                 for (int p = upper ? dg[i] + 1 : rp[i]; p < (upper ? rp[i + 1] : dg[i]); ++p) {
                     const int j = col[p];
-                    if (pos[j] >= end - kLevRing) code[p] = -(1 + (pos[j] & (kLevRing - 1)));
+                    lpos[p] = pos[j];
+                    if (pos[j] >= end - ring && pos[j] - ptr[lev[j]] < ring) code[p] = -(1 + (pos[j] & (ring - 1)));
                     else if (lev[i] - lev[j] >= 2) code[p] = pos[j];
                     else code[p] = pos[j] | kLevLate;
                 }
@@ -203,10 +218,27 @@ int build_schedule(fvh_mesh *m) {
         codes(lf, m->sched.h_fptr, frows, false);
         posf = pos;
         codes(lb, m->sched.h_bptr, brows, true);
-        for (int i = 0; i < n; ++i) code[dg[i]] = i;
+        for (int i = 0; i < n; ++i) code[dg[i]] = lpos[dg[i]] = i;
         for (int k = 0; k < n; ++k) f2b[k] = posf[brows[k]];
         FVH_TRY(m->sched.lcode.upload(code));
+        FVH_TRY(m->sched.lpos.upload(lpos));
         FVH_TRY(m->sched.f2b.upload(f2b));
+    }
+    {  // the sync-free walk's waves: no wave holds two rows of different levels
+        auto waves = [](const std::vector<int> &ptr, std::vector<int> &tab) {
+            tab.clear();
+            for (size_t l = 0; l + 1 < ptr.size(); ++l)
+                for (int q = ptr[l]; q < ptr[l + 1]; q += 64) {
+                    tab.push_back(q);
+                    tab.push_back(std::min(64, ptr[l + 1] - q));
+                }
+            return (int)(tab.size() / 2);
+        };
+        std::vector<int> tab;
+        m->sched.nswf = waves(m->sched.h_fptr, tab);
+        FVH_TRY(m->sched.swf.upload(tab));
+        m->sched.nswb = waves(m->sched.h_bptr, tab);
+        FVH_TRY(m->sched.swb.upload(tab));
     }
     int deg = 0;  // most lower / upper neighbours of a row
     for (int i = 0; i < n; ++i) deg = std::max(deg, std::max(dg[i] - rp[i], rp[i + 1] - dg[i] - 1));
@@ -353,6 +385,26 @@ extern "C" int fvh_mesh_create(const fvh_mesh_desc *d, fvh_mesh **out) {
             cf_w[i] = w;
             cf_omw[i] = 1.0 - w;
         }
+    // internal faces seen from the face: the slots A[o, n] and A[n, o] and
+    // the two sides' divergence weights, for the face-parallel assembly pass
+    const int nif = nf - nbf;
+    constexpr int kNoSide = INT32_MIN;
+    std::vector<int> fi_slot(2 * (size_t)std::max(nif, 1), kNoSide);
+    std::vector<double> fi_w(2 * (size_t)std::max(nif, 1), 0.0);
+    for (int c = 0; c < n; ++c)
+        for (int i = cf_ptr[c]; i < cf_ptr[c + 1]; ++i) {
+            const int f = cf_face[i];
+            if (f < nbf) return set_error(FVH_ERR_MESH, "cell %d lists boundary face %d among its internal faces", c, f);
+            const int side = (own1[f] == c) ? 0 : (own2[f] == c ? 1 : -1);
+            if (side < 0) return set_error(FVH_ERR_MESH, "cell %d lists face %d of which it is no owner", c, f);
+            const size_t k = 2 * (size_t)(f - nbf) + side;
+            if (fi_slot[k] != kNoSide) return set_error(FVH_ERR_MESH, "face %d listed twice by cell %d", f, c);
+            // the side's nsign rides in the slot's sign bit: ~slot when -1
+            fi_slot[k] = cf_sign[i] > 0.0 ? cf_slot[i] : ~cf_slot[i];
+            fi_w[k] = cf_w[i];
+        }
+    for (int k = 0; k < 2 * nif; ++k)
+        if (fi_slot[k] == kNoSide) return set_error(FVH_ERR_MESH, "internal face %d is missing from a cell's face list", nbf + k / 2);
     // cell -> boundary faces in ascending face id
     std::vector<int> cb_ptr(n + 1, 0), cb_face;
     for (int f = 0; f < nbf; ++f) cb_ptr[own1[f] + 1]++;
@@ -378,6 +430,18 @@ extern "C" int fvh_mesh_create(const fvh_mesh_desc *d, fvh_mesh **out) {
         fcy[f] = d->face_centre[3 * f + 1];
         fcz[f] = d->face_centre[3 * f + 2];
     }
+    std::vector<double> exv(nf), eyv(nf), ezv(nf);
+    for (int f = 0; f < nf; ++f) {
+        exv[f] = d->face_e[3 * f];
+        eyv[f] = d->face_e[3 * f + 1];
+        ezv[f] = d->face_e[3 * f + 2];
+    }
+    for (int c = 0; c < n && m->cf_ascending; ++c)
+        for (int i = cf_ptr[c] + 1; i < cf_ptr[c + 1]; ++i)
+            if (cf_face[i] <= cf_face[i - 1]) {
+                m->cf_ascending = false;
+                break;
+            }
     std::vector<double> vol(n), ccx(n), ccy(n), ccz(n);
     for (int c = 0; c < n; ++c) {
         vol[c] = d->cell_volume[c];
@@ -396,6 +460,8 @@ extern "C" int fvh_mesh_create(const fvh_mesh_desc *d, fvh_mesh **out) {
     FVH_TRY(m->cf_sign.upload(cf_sign));
     FVH_TRY(m->cf_w.upload(cf_w));
     FVH_TRY(m->cf_omw.upload(cf_omw));
+    FVH_TRY(m->fi_slot.upload(fi_slot));
+    FVH_TRY(m->fi_w.upload(fi_w));
     FVH_TRY(m->own1.upload(own1));
     FVH_TRY(m->own2.upload(own2));
     FVH_TRY(m->cb_ptr.upload(cb_ptr));
@@ -412,6 +478,9 @@ extern "C" int fvh_mesh_create(const fvh_mesh_desc *d, fvh_mesh **out) {
     FVH_TRY(m->nx.upload(nxv));
     FVH_TRY(m->ny.upload(nyv));
     FVH_TRY(m->nz.upload(nzv));
+    FVH_TRY(m->ex.upload(exv));
+    FVH_TRY(m->ey.upload(eyv));
+    FVH_TRY(m->ez.upload(ezv));
     FVH_TRY(m->vol.upload(vol));
     FVH_TRY(m->ccx.upload(ccx));
     FVH_TRY(m->ccy.upload(ccy));
@@ -438,8 +507,10 @@ extern "C" int fvh_mesh_create(const fvh_mesh_desc *d, fvh_mesh **out) {
     v.rowptr = m->rowptr.p; v.col = m->col.p; v.diag = m->diag.p; v.tslot = m->tslot.p;
     v.cf_ptr = m->cf_ptr.p; v.cf_face = m->cf_face.p; v.cf_nbr = m->cf_nbr.p; v.cf_slot = m->cf_slot.p;
     v.cf_sign = m->cf_sign.p; v.cf_w = m->cf_w.p; v.cf_omw = m->cf_omw.p;
+    v.nif = nif; v.fi_slot = (const int2 *)m->fi_slot.p; v.fi_w = (const double2 *)m->fi_w.p;
     v.own1 = m->own1.p; v.own2 = m->own2.p; v.fw = m->fw.p; v.farea = m->farea.p; v.fdelta = m->fdelta.p;
     v.Sfx = m->Sfx.p; v.Sfy = m->Sfy.p; v.Sfz = m->Sfz.p; v.nx = m->nx.p; v.ny = m->ny.p; v.nz = m->nz.p;
+    v.ex = m->ex.p; v.ey = m->ey.p; v.ez = m->ez.p;
     v.vol = m->vol.p; v.ccx = m->ccx.p; v.ccy = m->ccy.p; v.ccz = m->ccz.p; v.fcx = m->fcx.p; v.fcy = m->fcy.p; v.fcz = m->fcz.p;
     v.cb_ptr = m->cb_ptr.p; v.cb_face = m->cb_face.p;
     v.bf_face = m->bf_face.p; v.bf_cell = m->bf_cell.p; v.bf_patch = m->bf_patch.p;

@@ -276,6 +276,18 @@ static int build_hmesh(std::vector<V3> &pts, std::vector<Element> &elements,
     }
     out.boundary_faces = bfaces_id;
     out.boundary_cells = bcells_id;
+    // nodes (scaled) and each cell's nodes in element order, for output
+    out.node_xyz.resize(3 * (size_t)npts);
+    for (int64_t i = 0; i < npts; ++i) {
+        out.node_xyz[3 * i] = pts[i].x;
+        out.node_xyz[3 * i + 1] = pts[i].y;
+        out.node_xyz[3 * i + 2] = pts[i].z;
+    }
+    out.cell_nodes_ptr.assign(1, 0);
+    for (int64_t e = first; e < nel; ++e) {
+        for (int64_t v : elements[e].verts) out.cell_nodes.push_back(v - 1);
+        out.cell_nodes_ptr.push_back((int64_t)out.cell_nodes.size());
+    }
     return FVH_OK;
 }
 
@@ -446,6 +458,16 @@ extern "C" int fvh_hmesh_boundary_name(const fvh_hmesh *h, int64_t i, char *buf,
     if (!h || !buf || len <= 0 || i < 0 || i >= (int64_t)h->m.names.size())
         return set_error(FVH_ERR_ARG, "fvh_hmesh_boundary_name: bad arguments");
     std::snprintf(buf, (size_t)len, "%s", h->m.names[i].c_str());
+    return FVH_OK;
+}
+
+extern "C" int fvh_hmesh_nodes(const fvh_hmesh *h, int64_t *npoints, const double **xyz, const int64_t **cell_nodes_ptr,
+                               const int64_t **cell_nodes) {
+    if (!h || !npoints || !xyz || !cell_nodes_ptr || !cell_nodes) return set_error(FVH_ERR_ARG, "fvh_hmesh_nodes: null");
+    *npoints = (int64_t)(h->m.node_xyz.size() / 3);
+    *xyz = h->m.node_xyz.data();
+    *cell_nodes_ptr = h->m.cell_nodes_ptr.data();
+    *cell_nodes = h->m.cell_nodes.data();
     return FVH_OK;
 }
 

@@ -47,6 +47,15 @@ static int wall_plan(fvh_mesh *m, const std::vector<fvh_bc> &all, int kind, Wall
     return FVH_OK;
 }
 
+// Buffers of bound!: the bounded values, then per cell a pending flag, the
+// list of cells the chain pass resolves, its length and a failure flag.
+struct BoundWork {
+    DBuf<double> tmp;
+    DBuf<int> idx;
+    int last_chained = 0;  // cells the chain pass resolved in the last call
+};
+int bound_field(const fvh_mesh *m, double *field, double bnd, BoundWork &w);
+
 struct Sim {
     fvh_mesh *m = nullptr;
     fvh_case cs{};
@@ -56,7 +65,9 @@ struct Sim {
     // fields
     DBuf<double> Ux, Uy, Uz, p, k, omega, nut;
     DBuf<double> mdotf, nueff, rDf, pf, Ufx, Ufy, Ufz, nutf, nueffk, nueffw, nutf_tmp;
-    DBuf<double> divHv, rD, Hx, Hy, Hz, prev, gpx, gpy, gpz, Dkf, Dwf, Pk, Pw, bnd_tmp;
+    DBuf<double> divHv, rD, Hx, Hy, Hz, prev, gpx, gpy, gpz, Dkf, Dwf, Pk, Pw;
+    BoundWork bwork;
+    DBuf<double> gfx, gfy, gfz;  // face gradient of p (non-orthogonal correction)
     DBuf<double> gU[9];  // xx yx zx xy yy zy xz yz zz
     DBuf<double> res_dev;
     DBuf<int> flag;
@@ -74,6 +85,11 @@ struct Sim {
     DBuf<double> nu_face;           // viscosity of the case owning each face
     DBuf<unsigned char> refflag;    // cells taking setReference!
     std::vector<double> Rc;         // residual histories [K][iterations][3]
+    // a case that reached the break keeps being computed with the others,
+    // but its fields, residual history and statistics stay as they were at
+    // its break (the reference's loop would have stopped there)
+    std::vector<int> case_break;    // iteration of each case's break, -1 = running
+    DBuf<double> snapUx, snapUy, snapP;
     std::map<std::string, std::vector<fvh_stats>> cstats;
 };
 
@@ -243,12 +259,19 @@ __global__ void k_nut_wall(DevMesh m, WallArgs w, const int *bptr, const int *bf
     }
 }
 
-// bound!(field, eps): the reference updates in place, in cell order, so a
-// cell's neighbour average sees neighbours bounded earlier in the pass.  The
-// parallel pass is exact unless a negative cell has a negative lower
-// neighbour; such cells raise `flag` and the ordered pass is used instead.
+// bound!(field, bound) (RANSModels_functions.jl:62-88) updates in place, in
+// cell order, so a cell's neighbour average sees neighbours bounded earlier
+// in the same pass.  Only a cell with its sign bit set reads its neighbours
+// (a non-negative cell becomes max(phi, bound)), and bounding leaves a
+// neighbour's value unchanged when it is non-negative and at least `bound`
+// (`moves` is true otherwise, NaN included).  The parallel pass is therefore
+// exact for every cell except a negative one with a lower neighbour that
+// moves; it lists those cells, which the chain pass then resolves in
+// dependency order.
+__device__ inline bool bound_moves(double v, double bnd) { return signbit(v) || !(v >= bnd); }
+
 __global__ void k_bound_parallel(DevMesh m, const double *__restrict__ f, double *__restrict__ out, double bnd,
-                                 int *flag) {
+                                 int *__restrict__ pend, int *__restrict__ list, int *__restrict__ count) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= m.n) return;
     const double epsv = 2.220446049250313e-16;
@@ -259,52 +282,111 @@ __global__ void k_bound_parallel(DevMesh m, const double *__restrict__ f, double
     for (int q = m.cf_ptr[i]; q < m.cf_ptr[i + 1]; ++q) {
         int j = m.cf_nbr[q];
         double v = f[j];
-        if (neg && j < i && signbit(v)) conflict = true;
+        if (neg && j < i && bound_moves(v, bnd)) conflict = true;
         sum += fvh_jmax(v, epsv);
         cnt += 1;
     }
     double avg = sum / (double)cnt;
     double sb = neg ? avg : copysign(0.0, avg);
     out[i] = fvh_jmax(fvh_jmax(f[i], sb), bnd);
-    if (conflict) flag[0] = 1;
+    pend[i] = conflict ? 1 : 0;
+    if (conflict) list[atomicAdd(count, 1)] = i;
 }
 
-__global__ void k_bound_ordered(DevMesh m, double *f, double bnd) {
-    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+// The listed cells, by one work-group in rounds: a cell is evaluated once
+// none of its lower neighbours is still pending, reading the new value of
+// every lower neighbour that moves and the old value of the others, as the
+// sequential pass does.  The lowest pending cell is always ready, so every
+// round resolves at least one cell: after count rounds nothing is left, and
+// a further round (which cannot happen) sets *fail and ends the loop.
+__global__ void __launch_bounds__(1024) k_bound_chains(DevMesh m, const double *__restrict__ f, double *out,
+                                                       double bnd, int *pend, const int *__restrict__ list,
+                                                       int count, int *fail) {
+    __shared__ int left;
     const double epsv = 2.220446049250313e-16;
-    for (int i = 0; i < m.n; ++i) {
-        double sum = 0.0;
-        int cnt = 0;
-        for (int q = m.cf_ptr[i]; q < m.cf_ptr[i + 1]; ++q) {
-            sum += fvh_jmax(f[m.cf_nbr[q]], epsv);
-            cnt += 1;
+    for (int round = 0;; ++round) {
+        if (round > count) {
+            if (threadIdx.x == 0) *fail = 1;
+            break;
         }
-        double avg = sum / (double)cnt;
-        double sb = signbit(f[i]) ? avg : copysign(0.0, avg);
-        f[i] = fvh_jmax(fvh_jmax(f[i], sb), bnd);
+        if (threadIdx.x == 0) left = 0;
+        __syncthreads();
+        for (int q = threadIdx.x; q < count; q += blockDim.x) {
+            const int i = list[q];
+            if (!__hip_atomic_load(&pend[i], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP)) continue;
+            bool ready = true;
+            for (int r = m.cf_ptr[i]; r < m.cf_ptr[i + 1]; ++r) {
+                const int j = m.cf_nbr[r];
+                if (j < i && __hip_atomic_load(&pend[j], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP)) ready = false;
+            }
+            if (!ready) {
+                left = 1;
+                continue;
+            }
+            double sum = 0.0;
+            int cnt = 0;
+            for (int r = m.cf_ptr[i]; r < m.cf_ptr[i + 1]; ++r) {
+                const int j = m.cf_nbr[r];
+                const double v = (j < i && bound_moves(f[j], bnd))
+                                     ? __hip_atomic_load(&out[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)
+                                     : f[j];
+                sum += fvh_jmax(v, epsv);
+                cnt += 1;
+            }
+            const double avg = sum / (double)cnt;
+            __hip_atomic_store(&out[i], fvh_jmax(fvh_jmax(f[i], avg), bnd), __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_WORKGROUP);
+            __hip_atomic_store(&pend[i], 0, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+        }
+        __syncthreads();
+        const int more = left;
+        __syncthreads();
+        if (!more) break;
     }
 }
 
-static int bound(Sim &S, double *field) {
-    fvh_mesh *m = S.m;
-    HIP_TRY(hipMemsetAsync(S.flag.p, 0, sizeof(int), stream()));
-    hipLaunchKernelGGL(k_bound_parallel, grid1d(m->n), dim3(256), 0, stream(), m->d, field, S.bnd_tmp.p,
-                       std::numeric_limits<double>::epsilon(), S.flag.p);
+// bound!(field, bnd) on the device, in place; w holds n doubles and 2n + 2 ints.
+int bound_field(const fvh_mesh *m, double *field, double bnd, BoundWork &w) {
+    const int n = (int)m->n;
+    if (w.tmp.n < (size_t)n) FVH_TRY(w.tmp.alloc(n));
+    if (w.idx.n < 2 * (size_t)n + 2) FVH_TRY(w.idx.alloc(2 * (size_t)n + 2));
+    int *pend = w.idx.p, *list = w.idx.p + n, *count = w.idx.p + 2 * (size_t)n, *fail = count + 1;
+    HIP_TRY(hipMemsetAsync(count, 0, 2 * sizeof(int), stream()));
+    hipLaunchKernelGGL(k_bound_parallel, grid1d(n), dim3(256), 0, stream(), m->d, field, w.tmp.p, bnd, pend, list,
+                       count);
     HIP_TRY(hipGetLastError());
     int h = 0;
-    HIP_TRY(hipMemcpyAsync(&h, S.flag.p, sizeof(int), hipMemcpyDeviceToHost, stream()));
+    HIP_TRY(hipMemcpyAsync(&h, count, sizeof(int), hipMemcpyDeviceToHost, stream()));
     HIP_TRY(hipStreamSynchronize(stream()));
-    if (h == 0) return copy_dev(field, S.bnd_tmp.p, m->n);
-    hipLaunchKernelGGL(k_bound_ordered, dim3(1), dim3(64), 0, stream(), m->d, field, std::numeric_limits<double>::epsilon());
-    HIP_TRY(hipGetLastError());
-    return FVH_OK;
+    if (h < 0 || h > n) return set_error(FVH_ERR_HIP, "bound!: %d conflicting cells of %d", h, n);
+    if (h > 0) {
+        hipLaunchKernelGGL(k_bound_chains, dim3(1), dim3(1024), 0, stream(), m->d, field, w.tmp.p, bnd, pend, list, h,
+                           fail);
+        HIP_TRY(hipGetLastError());
+        int bad = 0;
+        HIP_TRY(hipMemcpyAsync(&bad, fail, sizeof(int), hipMemcpyDeviceToHost, stream()));
+        HIP_TRY(hipStreamSynchronize(stream()));
+        if (bad) return set_error(FVH_ERR_HIP, "bound!: the ordered pass did not resolve its %d cells", h);
+    }
+    w.last_chained = h;
+    return copy_dev(field, w.tmp.p, n);
+}
+
+static int bound(Sim &S, double *field) {
+    return bound_field(S.m, field, std::numeric_limits<double>::epsilon(), S.bwork);
 }
 
 static int run_solve(Sim &S, fvh_eqn *e, const fvh_solver_cfg &cfg, double *x, const char *name) {
     fvh_stats st{};
     FVH_TRY(solve(e, cfg, x, &st));
     S.stats[name] = st;
-    if (S.cases) S.cstats[name] = e->kw.case_stats;
+    if (S.cases) {
+        auto &cs = S.cstats[name];
+        const auto &now = e->kw.case_stats;
+        if (cs.size() != now.size()) cs = now;
+        for (size_t k = 0; k < now.size() && k < S.case_break.size(); ++k)
+            if (S.case_break[k] < 0) cs[k] = now[k];
+    }
     return FVH_OK;
 }
 
@@ -457,6 +539,17 @@ static int pressure_corrector(Sim &S) {
     }
     FVH_TRY(residual_slot(S, S.pe.get(), S.p.p, 2));
     FVH_TRY(launch_grad(m, S.cs.sp.gradient, S.vp, S.p.p, S.pf.p, S.gpx.p, S.gpy.p, S.gpz.p));
+    // the loops' `correct` block (Solvers_1_SIMPLE.jl:165-180, Solvers_2_PISO.jl:170-182),
+    // off unless nonorth_correctors > 0; pf carries the correction flux
+    for (int i = 0; i < S.cs.nonorth_correctors; ++i) {
+        std::vector<KTerm> t = {term(FVH_OP_LAPLACIAN, S.cs.sp.laplacian, 1, S.rDf.p, S.p.p)};
+        FVH_TRY(assemble(S, S.pe.get(), t, {{1.0, S.divHv.p}}, S.bp, 0, S.planP, false, 1.0, S.cs.has_pref != 0));
+        FVH_TRY(launch_interpolate_gradient(m, S.gpx.p, S.gpy.p, S.gpz.p, S.p.p, S.gfx.p, S.gfy.p, S.gfz.p));
+        FVH_TRY(launch_nonorthogonal_flux(m, S.gfx.p, S.gfy.p, S.gfz.p, S.pf.p));
+        FVH_TRY(launch_nonorthogonal_correct(m, 1, S.rDf.p, S.pf.p, S.pe->b.p));
+        FVH_TRY(run_solve(S, S.pe.get(), S.cs.vp, S.p.p, "p"));
+        FVH_TRY(launch_grad(m, S.cs.sp.gradient, S.vp, S.p.p, S.pf.p, S.gpx.p, S.gpy.p, S.gpz.p));
+    }
     {
         double *ux = S.Ux.p, *uy = S.Uy.p;
         const double *hx = S.Hx.p, *hy = S.Hy.p, *gx = S.gpx.p, *gy = S.gpy.p, *rD = S.rD.p;
@@ -505,10 +598,20 @@ static int iterate_once(Sim &S) {
         HIP_TRY(hipMemcpyAsync(r.data(), S.res_dev.p, r.size() * sizeof(double), hipMemcpyDeviceToHost, stream()));
         HIP_TRY(hipStreamSynchronize(stream()));
         bool all = !piso;
+        const std::vector<int> &off = m->cases->h_off;
         for (int k = 0; k < S.K; ++k) {
+            if (S.case_break[k] >= 0) continue;  // frozen at its break
             for (int q = 0; q < 3; ++q) S.Rc[((size_t)k * S.iterations + iter) * 3 + q] = r[3 * k + q];
-            all = all && r[3 * k] <= cb && r[3 * k + 1] <= cb && r[3 * k + 2] <= cb;
+            const bool brk = !piso && r[3 * k] <= cb && r[3 * k + 1] <= cb && r[3 * k + 2] <= cb;
+            if (brk) {
+                S.case_break[k] = iter;
+                const size_t o = (size_t)off[k], len = (size_t)(off[k + 1] - off[k]);
+                HIP_TRY(hipMemcpyAsync(S.snapUx.p + o, S.Ux.p + o, len * sizeof(double), hipMemcpyDeviceToDevice, stream()));
+                HIP_TRY(hipMemcpyAsync(S.snapUy.p + o, S.Uy.p + o, len * sizeof(double), hipMemcpyDeviceToDevice, stream()));
+                HIP_TRY(hipMemcpyAsync(S.snapP.p + o, S.p.p + o, len * sizeof(double), hipMemcpyDeviceToDevice, stream()));
+            }
         }
+        for (int k = 0; k < S.K; ++k) all = all && S.case_break[k] >= 0;
         S.Rux[iter] = r[0];
         S.Ruy[iter] = r[1];
         S.Rp[iter] = r[2];
@@ -545,7 +648,7 @@ static int sim_init(Sim &S, const double *Ux, const double *Uy, const double *Uz
     FVH_TRY(cell(S.omega, omega, 0.0));
     FVH_TRY(cell(S.nut, nut, 0.0));
     for (DBuf<double> *b : {&S.divHv, &S.rD, &S.Hx, &S.Hy, &S.Hz, &S.prev, &S.gpx, &S.gpy, &S.gpz, &S.Dkf, &S.Dwf, &S.Pk,
-                            &S.Pw, &S.bnd_tmp})
+                            &S.Pw})
         FVH_TRY(cell(*b, nullptr, 0.0));
     for (auto &g : S.gU) FVH_TRY(cell(g, nullptr, 0.0));
     for (DBuf<double> *b : {&S.mdotf, &S.nueff, &S.pf, &S.Ufx, &S.Ufy, &S.Ufz, &S.nutf, &S.nueffk, &S.nueffw, &S.nutf_tmp})
@@ -553,6 +656,12 @@ static int sim_init(Sim &S, const double *Ux, const double *Uy, const double *Uz
     FVH_TRY(face(S.rDf, 1.0));
     FVH_TRY(S.res_dev.alloc(S.cases ? 3 * S.K + 1 : 4));
     FVH_TRY(S.flag.alloc(1));
+    if (S.cs.nonorth_correctors < 0 || S.cs.nonorth_correctors > 8)
+        return set_error(FVH_ERR_ARG, "nonorth_correctors must be 0..8 (got %d)", S.cs.nonorth_correctors);
+    if (S.cs.nonorth_correctors > 0) {
+        if (S.cases) return set_error(FVH_ERR_UNSUPPORTED, "independent cases: no non-orthogonal correctors");
+        for (auto *b : {&S.gfx, &S.gfy, &S.gfz}) FVH_TRY(face(*b, 0.0));
+    }
     // equations and preconditioners (uy shares ux's preconditioner, which
     // reads ux's matrix; k and omega assemble into p's equation)
     fvh_eqn *raw = nullptr;
@@ -604,10 +713,30 @@ static int sim_init(Sim &S, const double *Ux, const double *Uy, const double *Uz
         if (S.cs.sU.divergence != FVH_LINEAR && S.cs.sU.divergence != FVH_UPWIND)
             return set_error(FVH_ERR_UNSUPPORTED, "Divergence scheme %d", S.cs.sU.divergence);
     }
+    // LDL: the reference factors each equation's set-up matrix once
+    // (set_preconditioner, preconditioners_2_functions.jl:4-13, before the
+    // loop: rDf = 1, and no face flux or viscosity yet, which leaves the
+    // momentum and turbulence matrices zero) and never updates the factor.
+    if (S.cs.vU.precond == FVH_PRECOND_LDL ||
+        (S.cs.model == FVH_KOMEGA && (S.cs.vk.precond == FVH_PRECOND_LDL || S.cs.vomega.precond == FVH_PRECOND_LDL)))
+        return set_error(FVH_ERR_UNSUPPORTED,
+                         "LDL: the momentum and turbulence equations' set-up matrices are zero (no flux and no "
+                         "viscosity before the loop), so their factorization has a zero pivot");
+    if (S.cs.vp.precond == FVH_PRECOND_LDL) {
+        std::vector<KTerm> t = {term(FVH_OP_LAPLACIAN, S.cs.sp.laplacian, 1, S.rDf.p, S.p.p)};
+        FVH_TRY(assemble(S, S.pe.get(), t, {{1.0, S.divHv.p}}, S.bp, 0, S.planP, false, 1.0, false));
+        FVH_TRY(precond_update(S.pe.get()));
+    }
     S.Rux.assign(S.iterations, 1.0);
     S.Ruy.assign(S.iterations, 1.0);
     S.Rp.assign(S.iterations, 1.0);
-    if (S.cases) S.Rc.assign((size_t)S.K * S.iterations * 3, 1.0);
+    if (S.cases) {
+        S.Rc.assign((size_t)S.K * S.iterations * 3, 1.0);
+        S.case_break.assign(S.K, -1);
+        FVH_TRY(S.snapUx.alloc(m->n));
+        FVH_TRY(S.snapUy.alloc(m->n));
+        FVH_TRY(S.snapP.alloc(m->n));
+    }
     // loop prologue (Solvers_1_SIMPLE.jl:114-119)
     FVH_TRY(velocity_faces(S, S.Ux.p, S.Uy.p, S.Uz.p));
     FVH_TRY(launch_flux(m, S.Ufx.p, S.Ufy.p, S.Ufz.p, S.mdotf.p));
@@ -761,6 +890,61 @@ extern "C" int fvh_sim_timing(fvh_sim *h, double *a, double *s, double *o) {
     return FVH_OK;
 }
 
+// Average device time of one launch of a loop phase on the simulation's
+// current fields, by HIP events on the library stream.  op 0: the Ux
+// momentum assembly as the loop runs it (terms, source, BCs, relaxation);
+// op 1: the pressure equation's assembly; op 2: the Green-Gauss kernel on
+// p's face values.  The phase's outputs are recomputed in place.
+extern "C" int fvh_sim_time_op(fvh_sim *h, int32_t op, int32_t reps, double *ms) {
+    FVH_GUARD_BEGIN
+    if (!h || !ms || reps <= 0) return set_error(FVH_ERR_ARG, "bad argument");
+    if (op < 0 || op > 2) return set_error(FVH_ERR_UNSUPPORTED, "fvh_sim_time_op: op %d", op);
+    Sim &S = h->S;
+    fvh_mesh *m = S.m;
+    auto launch = [&]() -> int {
+        if (op == 0) {
+            std::vector<KTerm> t = {term(FVH_OP_TIME, S.cs.sU.time, 1, nullptr, S.Ux.p),
+                                    term(FVH_OP_DIVERGENCE, S.cs.sU.divergence, 1, S.mdotf.p, S.Ux.p),
+                                    term(FVH_OP_LAPLACIAN, S.cs.sU.laplacian, -1, S.nueff.p, S.Ux.p)};
+            return assemble(S, S.ux.get(), t, {{-1.0, S.gpx.p}}, S.bU, 0, S.planU, S.cs.algorithm != FVH_PISO,
+                            S.cs.vU.relax, false);
+        }
+        if (op == 1) {
+            std::vector<KTerm> t = {term(FVH_OP_LAPLACIAN, S.cs.sp.laplacian, 1, S.rDf.p, S.p.p)};
+            return assemble(S, S.pe.get(), t, {{1.0, S.divHv.p}}, S.bp, 0, S.planP, false, 1.0, S.cs.has_pref != 0);
+        }
+        return launch_green_gauss(m, S.pf.p, S.gpx.p, S.gpy.p, S.gpz.p);
+    };
+    FVH_TRY(launch());  // warm
+    hipEvent_t a, b;
+    HIP_TRY(hipEventCreate(&a));
+    HIP_TRY(hipEventCreate(&b));
+    int rc = FVH_OK;
+    if (hipEventRecord(a, stream()) != hipSuccess) rc = set_error(FVH_ERR_HIP, "hipEventRecord");
+    for (int r = 0; r < reps && rc == FVH_OK; ++r) rc = launch();
+    if (rc == FVH_OK && hipEventRecord(b, stream()) != hipSuccess) rc = set_error(FVH_ERR_HIP, "hipEventRecord");
+    if (rc == FVH_OK && hipEventSynchronize(b) != hipSuccess) rc = set_error(FVH_ERR_HIP, "hipEventSynchronize");
+    float t = 0.f;
+    if (rc == FVH_OK && hipEventElapsedTime(&t, a, b) != hipSuccess) rc = set_error(FVH_ERR_HIP, "hipEventElapsedTime");
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    if (rc != FVH_OK) return rc;
+    *ms = (double)t / reps;
+    return FVH_OK;
+    FVH_GUARD_END
+}
+
+extern "C" int fvh_bound(const fvh_mesh *m, double *field, double bnd, int32_t *chained) {
+    FVH_GUARD_BEGIN
+    if (!m || !field) return set_error(FVH_ERR_ARG, "null argument");
+    BoundWork w;
+    FVH_TRY(bound_field(m, field, bnd, w));
+    HIP_TRY(hipStreamSynchronize(stream()));
+    if (chained) *chained = w.last_chained;
+    return FVH_OK;
+    FVH_GUARD_END
+}
+
 extern "C" int fvh_sim_equation(fvh_sim *h, const char *name, fvh_eqn **out) {
     if (!h || !name || !out) return set_error(FVH_ERR_ARG, "null argument");
     std::string nm(name);
@@ -806,6 +990,15 @@ extern "C" int fvh_sim_get(fvh_sim *h, const char *name, double *out, int64_t *l
     if (out) {
         HIP_TRY(hipStreamSynchronize(stream()));
         HIP_TRY(hipMemcpy(out, src, sizeof(double) * cnt, hipMemcpyDeviceToHost));
+        // independent cases: a case that reached its break reports its fields there
+        const double *snap = nm == "Ux" ? S.snapUx.p : nm == "Uy" ? S.snapUy.p : nm == "p" ? S.snapP.p : nullptr;
+        if (S.cases && snap) {
+            const std::vector<int> &off = S.m->cases->h_off;
+            for (int k = 0; k < S.K; ++k)
+                if (S.case_break[k] >= 0)
+                    HIP_TRY(hipMemcpy(out + off[k], snap + off[k], sizeof(double) * (off[k + 1] - off[k]),
+                                      hipMemcpyDeviceToHost));
+        }
     }
     return FVH_OK;
     FVH_GUARD_END

@@ -12,6 +12,7 @@ from .api import (  # noqa: F401
     DILU, ILU0, LDL, BicgstabSolver, CgSolver, Configuration, ConstantScalar, Dirichlet, Euler, GmresSolver, Jacobi,
     KOmega, KWallFunction, Laminar, Linear, Midpoint, Neumann, NormDiagonal, NutWallFunction, OmegaWallFunction,
     Orthogonal, RANS, BatchSimulation, CaseDesc, ScalarEquation, Simulation, Steady, Upwind, assign, initialise,
-    isturbulent, piso, set_runtime, set_schemes, set_solver, simple, taylor_green_flux)
+    isturbulent, model2vtk, piso, pressure_force, set_runtime, set_schemes, set_solver, simple, surface_flux,
+    taylor_green_flux, viscous_force, wall_shear_stress, write_vtk)
 from .mesh import (Mesh2, backward_facing_step, build_mesh, cavity, channel, delaunay_unit_square,  # noqa: F401
                    flat_plate, from_elements, rect_grid, union)

@@ -18,7 +18,7 @@ OP_TIME, OP_LAPLACIAN, OP_DIVERGENCE, OP_SI = range(4)
 STEADY, EULER, LINEAR, UPWIND, ORTHOGONAL, MIDPOINT = range(6)
 BC_DIRICHLET, BC_NEUMANN, BC_KWALL, BC_OMEGAWALL, BC_NUTWALL = range(5)
 BICGSTAB, CG, GMRES = range(3)
-PRECOND_NONE, PRECOND_DILU, PRECOND_ILU0, PRECOND_JACOBI, PRECOND_NORMDIAG = range(5)
+PRECOND_NONE, PRECOND_DILU, PRECOND_ILU0, PRECOND_JACOBI, PRECOND_NORMDIAG, PRECOND_LDL = range(6)
 FP64, MIXED, MIXED_FP16 = range(3)
 LAMINAR, KOMEGA = range(2)
 ALG_SIMPLE, ALG_PISO = range(2)
@@ -72,6 +72,7 @@ class Case(C.Structure):
         ("U", BCSet), ("p", BCSet), ("k", BCSet), ("omega", BCSet), ("nut", BCSet),
         ("sU", Schemes), ("sp", Schemes), ("sk", Schemes), ("somega", Schemes),
         ("vU", SolverCfg), ("vp", SolverCfg), ("vk", SolverCfg), ("vomega", SolverCfg),
+        ("nonorth_correctors", C.c_int32), ("pad_", C.c_int32),
     ]
 
 
@@ -96,6 +97,7 @@ SIGNATURES = {
     "fvh_hmesh_read_unv": (C.c_int, [C.c_char_p, C.c_double, C.POINTER(_VP)]),
     "fvh_hmesh_desc": (C.c_int, [_VP, C.POINTER(MeshDesc)]),
     "fvh_hmesh_boundary_name": (C.c_int, [_VP, C.c_int64, C.c_char_p, C.c_int64]),
+    "fvh_hmesh_nodes": (C.c_int, [_VP, C.POINTER(C.c_int64), C.POINTER(P_D), C.POINTER(P_I64), C.POINTER(P_I64)]),
     "fvh_hmesh_free": (None, [_VP]),
     "fvh_mesh_create": (C.c_int, [C.POINTER(MeshDesc), C.POINTER(_VP)]),
     "fvh_mesh_destroy": (None, [_VP]),
@@ -108,6 +110,19 @@ SIGNATURES = {
     "fvh_grad": (C.c_int, [_VP, C.c_int32, C.POINTER(BC), C.c_int32, C.c_int32, _VP, _VP, _VP, _VP, _VP]),
     "fvh_div": (C.c_int, [_VP, _VP, _VP, _VP, _VP]),
     "fvh_flux": (C.c_int, [_VP, _VP, _VP, _VP, _VP]),
+    "fvh_bound": (C.c_int, [_VP, _VP, C.c_double, P_I32]),
+    "fvh_write_vtk": (C.c_int, [C.c_char_p, C.c_int64, P_D, C.c_int64, P_I64, P_I64, C.c_int32,
+                                C.POINTER(C.c_char_p), P_I32, C.POINTER(P_D)]),
+    "fvh_format_julia": (C.c_int, [C.c_double, C.c_char_p, C.c_int64]),
+    "fvh_patch_size": (C.c_int, [_VP, C.c_int32, C.POINTER(C.c_int64)]),
+    "fvh_surface_flux": (C.c_int, [_VP, C.c_int32, _VP, _VP, _VP, _VP]),
+    "fvh_surface_normal_gradient": (C.c_int, [_VP, C.c_int32, _VP, _VP, _VP, P_D, _VP, _VP, _VP]),
+    "fvh_pressure_force": (C.c_int, [_VP, C.c_int32, _VP, C.c_double, P_D]),
+    "fvh_viscous_force": (C.c_int, [_VP, C.c_int32, _VP, _VP, _VP, P_D, C.c_double, C.c_double, _VP, P_D]),
+    "fvh_wall_shear_stress": (C.c_int, [_VP, C.c_int32, _VP, _VP, _VP, P_D, C.c_double, _VP, _VP, _VP, _VP, _VP]),
+    "fvh_interpolate_gradient": (C.c_int, [_VP, _VP, _VP, _VP, _VP, _VP, _VP, _VP]),
+    "fvh_nonorthogonal_flux": (C.c_int, [_VP, _VP, _VP, _VP, _VP]),
+    "fvh_nonorthogonal_correct": (C.c_int, [_VP, C.c_int32, _VP, _VP]),
     "fvh_eqn_create": (C.c_int, [_VP, C.POINTER(_VP)]),
     "fvh_eqn_destroy": (None, [_VP]),
     "fvh_eqn_arrays": (C.c_int, [_VP, C.POINTER(_VP), C.POINTER(_VP)]),
@@ -140,6 +155,7 @@ SIGNATURES = {
     "fvh_sim_stats": (C.c_int, [_VP, C.c_char_p, C.POINTER(Stats)]),
     "fvh_sim_timing": (C.c_int, [_VP, P_D, P_D, P_D]),
     "fvh_sim_equation": (C.c_int, [_VP, C.c_char_p, C.POINTER(_VP)]),
+    "fvh_sim_time_op": (C.c_int, [_VP, C.c_int32, C.c_int32, P_D]),
     "fvh_time_op": (C.c_int, [_VP, C.c_int32, C.c_int32, _VP, _VP, P_D]),
     "fvh_simple": (C.c_int, [_VP, C.POINTER(Case), P_D, P_D, P_D, P_D, P_D, P_D, P_D, P_D, P_D, P_D, P_I32]),
 }

@@ -56,8 +56,9 @@ class NormDiagonal(PreconditionerType):
 
 
 class LDL(PreconditionerType):
-    """Exact sparse LDL' (LDLFactorizations.jl): a direct solve, not on the device path."""
-    kind = None
+    """LDL' factor of the equation's set-up matrix (opLDL over
+    LDLFactorizations.jl), kept for every later solve."""
+    kind = L.PRECOND_LDL
 
 
 def set_solver(field, *, solver, preconditioner, convergence, relax, itmax=100, atol=None, rtol=1e-3, precision=L.FP64):
@@ -225,9 +226,10 @@ def _solver(v):
 class CaseDesc:
     """fvh_case for (model, config; pref) plus the buffers it points into."""
 
-    def __init__(self, model, config, pref=None, algorithm=L.ALG_SIMPLE, conv_break=1e-7):
+    def __init__(self, model, config, pref=None, algorithm=L.ALG_SIMPLE, conv_break=1e-7, nonorth_correctors=0):
         mesh = model.mesh
         c = L.Case()
+        c.nonorth_correctors = int(nonorth_correctors)
         c.model = L.KOMEGA if isturbulent(model) else L.LAMINAR
         c.algorithm = algorithm
         c.iterations = config.runtime["iterations"]
@@ -281,9 +283,9 @@ class Simulation:
     `run(n)` executes up to n loop iterations (stopping at the reference's
     convergence break); `get(name)` copies a state array to the host."""
 
-    def __init__(self, model, config, pref=None, algorithm=L.ALG_SIMPLE, conv_break=1e-7):
+    def __init__(self, model, config, pref=None, algorithm=L.ALG_SIMPLE, conv_break=1e-7, nonorth_correctors=0):
         self.model, self.config = model, config
-        self.case = CaseDesc(model, config, pref, algorithm, conv_break)
+        self.case = CaseDesc(model, config, pref, algorithm, conv_break, nonorth_correctors)
         self.dm = model.mesh.device()
         f = _fields(model)
         self._f = f
@@ -528,18 +530,174 @@ def FVHError_unsupported(msg):
     return L.FVHError(L.FVH_ERR_UNSUPPORTED, msg)
 
 
+def _run_writing(sim, model, config, prefix, algorithm):
+    """The loop with the reference's output: model2vtk every write_interval
+    iterations (`iteration % write_interval + signbit(write_interval) == 0`)
+    and, for simple!, at convergence when write_interval >= 0
+    (Solvers_1_SIMPLE.jl:200-228, Solvers_2_PISO.jl:228-235)."""
+    wi = int(config.runtime.get("write_interval", -1))
+    if wi == 0:
+        raise ZeroDivisionError("write_interval = 0 (the reference's iteration % 0 throws DivideError)")
+    total = sim.iterations
+    if wi < 0:
+        sim.run(total)
+        sim.write_back()
+        return
+    done = 0
+    while done < total:
+        want = min(wi - done % wi, total - done)
+        k = sim.run(want)
+        done += k
+        sim.write_back()
+        if k < want:  # the loop broke at its convergence test
+            if algorithm == L.ALG_SIMPLE:
+                model2vtk(model, f"{prefix}_{done:06d}")
+            return
+        if done % wi == 0:
+            model2vtk(model, f"{prefix}_{done:06d}")
+
+
 def simple(model, config, pref=None, conv_break=1e-7):
     """simple!(model, config; pref): returns (R_ux, R_uy, R_p); fields are
-    updated in place on the model, as in the reference."""
+    updated in place on the model, as in the reference; VTK files
+    iteration_NNNNNN.vtk as write_interval asks."""
     sim = Simulation(model, config, pref, L.ALG_SIMPLE, conv_break)
-    sim.run(sim.iterations)
-    sim.write_back()
+    _run_writing(sim, model, config, "iteration", L.ALG_SIMPLE)
     return sim.residuals()
 
 
 def piso(model, config, pref=None):
-    """piso!(model, config; pref)."""
+    """piso!(model, config; pref); VTK files timestep_NNNNNN.vtk as
+    write_interval asks."""
     sim = Simulation(model, config, pref, L.ALG_PISO)
-    sim.run(sim.iterations)
-    sim.write_back()
+    _run_writing(sim, model, config, "timestep", L.ALG_PISO)
     return sim.residuals()
+
+
+# ---- output and post-processing (src/VTK/VTK_writer.jl, src/Postprocess/Postprocess_forces.jl) ----
+def _field_of(value, n):
+    """(ncomp, flat array) of a ScalarField / VectorField / ndarray of n or (n, 3) values."""
+    if isinstance(value, VectorField):
+        return 3, np.concatenate([value.x.values, value.y.values, value.z.values]).astype(np.float64)
+    if isinstance(value, ScalarField):
+        return 1, np.ascontiguousarray(value.values, np.float64)
+    raise TypeError('Input data should be a ScalarField or VectorField e.g. ("U", U)')
+
+
+def write_vtk(name, mesh, *args):
+    """write_vtk(name, mesh, ("U", U), ("p", p), ...): `name`.vtk, legacy ASCII
+    cell data in the reference's layout (src/VTK/VTK_writer.jl:22-116)."""
+    if getattr(mesh, "nodes", None) is None:
+        raise ValueError("this mesh carries no nodes (a batched union); write each case with its own mesh")
+    n = mesh.ncells
+    names, comps, datas = [], [], []
+    for label, field in args:
+        k, d = _field_of(field, n)
+        if d.size != k * n:
+            raise ValueError(f"{label}: {d.size} values for {n} cells")
+        names.append(str(label).encode())
+        comps.append(k)
+        datas.append(d)
+    cn = (C.c_char_p * max(1, len(names)))(*names)
+    cc = (C.c_int32 * max(1, len(comps)))(*comps)
+    cd = (C.POINTER(C.c_double) * max(1, len(datas)))(*[L.dptr(d) for d in datas])
+    L.check(L.lib().fvh_write_vtk((str(name) + ".vtk").encode(), mesh.nodes.shape[0], L.dptr(mesh.nodes.reshape(-1)),
+                                  n, L.iptr(mesh.cell_nodes_ptr), L.iptr(mesh.cell_nodes), len(names), cn, cc, cd),
+            "fvh_write_vtk")
+    return str(name) + ".vtk"
+
+
+def model2vtk(model, name):
+    """model2vtk(model, name): U and p, plus k, omega and nut for k-omega
+    (src/VTK/VTK_writer.jl:3-20)."""
+    args = [("U", model.U), ("p", model.p)]
+    if isturbulent(model):
+        t = model.turbulence
+        args += [("k", t["k"]), ("omega", t["omega"]), ("nut", t["nut"])]
+    return write_vtk(name, model.mesh, *args)
+
+
+def _dev_cells(values, n):
+    a = np.ascontiguousarray(values, np.float64)
+    if a.size != n:
+        raise ValueError(f"{a.size} values for {n} cells")
+    return L.DeviceArray.from_host(a)
+
+
+def _wall_velocity(U, patch_id):
+    """Value of the last velocity BC that names the patch, or None."""
+    uw = None
+    for bc in U.BCs:
+        if bc.ID == patch_id:
+            uw = bc.value
+    return uw
+
+
+def surface_flux(patch, phi):
+    """surface_flux over a patch: phi[cell] * (area * normal) per face
+    (src/Calculate/Calculate_5_surface_normal_gradient.jl:3-18); (x, y, z)."""
+    mesh = phi.mesh
+    b = mesh.boundary_index(patch)
+    nb = C.c_int64()
+    L.check(L.lib().fvh_patch_size(mesh.device().h, b, C.byref(nb)))
+    out = [L.DeviceArray(max(nb.value, 1)) for _ in range(3)]
+    d = _dev_cells(phi.values, mesh.ncells)
+    L.check(L.lib().fvh_surface_flux(mesh.device().h, b, d.vp, *(o.vp for o in out)), "fvh_surface_flux")
+    return tuple(o.host()[:nb.value] for o in out)
+
+
+def pressure_force(patch, p, rho):
+    """pressure_force(patch, p, rho) (Postprocess_forces.jl:5-27): rho times
+    the sum over the patch of p[cell] area normal."""
+    mesh = p.mesh
+    b = mesh.boundary_index(patch)
+    d = _dev_cells(p.values, mesh.ncells)
+    F = np.zeros(3)
+    L.check(L.lib().fvh_pressure_force(mesh.device().h, b, d.vp, float(rho), L.dptr(F)), "fvh_pressure_force")
+    return F
+
+
+def viscous_force(patch, U, rho, nu, nut):
+    """viscous_force(patch, U, rho, nu, nut) (Postprocess_forces.jl:29-60):
+    rho times the sum of the wall-parallel normal gradient times area times
+    (nu + nut[cell]); zero when no velocity BC names the patch."""
+    mesh = U.mesh
+    b = mesh.boundary_index(patch)
+    name = mesh.boundary_names[b]
+    d = [_dev_cells(c.values, mesh.ncells) for c in (U.x, U.y, U.z)]
+    uw = _wall_velocity(U, name)
+    uwa = None if uw is None else np.ascontiguousarray(uw, np.float64)
+    nt = None
+    if nut is not None and not isinstance(nut, ConstantScalar):
+        nt = _dev_cells(nut.values if isinstance(nut, ScalarField) else nut, mesh.ncells)
+    nu0 = float(nu.values if isinstance(nu, ConstantScalar) else nu)
+    if isinstance(nut, ConstantScalar):
+        nu0 = nu0 + float(nut.values)
+    F = np.zeros(3)
+    L.check(L.lib().fvh_viscous_force(mesh.device().h, b, *(a.vp for a in d), None if uwa is None else L.dptr(uwa),
+                                      float(rho), nu0, None if nt is None else nt.vp, L.dptr(F)), "fvh_viscous_force")
+    return F
+
+
+def wall_shear_stress(patch, model):
+    """wall_shear_stress(patch, model) (Postprocess_forces.jl:62-96):
+    ((tau_x, tau_y, tau_z), face centres) with tau = snGrad (nu + nut)."""
+    mesh = model.mesh
+    b = mesh.boundary_index(patch)
+    name = mesh.boundary_names[b]
+    uw = _wall_velocity(model.U, name)
+    if uw is None:
+        raise ValueError(f"no velocity boundary condition names patch {name!r}")
+    uwa = np.ascontiguousarray(uw, np.float64)
+    d = [_dev_cells(c.values, mesh.ncells) for c in (model.U.x, model.U.y, model.U.z)]
+    nt = _dev_cells(model.turbulence["nut"].values, mesh.ncells) if isturbulent(model) else None
+    nb = C.c_int64()
+    L.check(L.lib().fvh_patch_size(mesh.device().h, b, C.byref(nb)))
+    out = [L.DeviceArray(max(nb.value, 1)) for _ in range(3)]
+    pos = L.DeviceArray(3 * max(nb.value, 1))
+    nu = float(model.nu.values if isinstance(model.nu, ConstantScalar) else model.nu)
+    L.check(L.lib().fvh_wall_shear_stress(mesh.device().h, b, *(a.vp for a in d), L.dptr(uwa), nu,
+                                          None if nt is None else nt.vp, *(o.vp for o in out), pos.vp),
+            "fvh_wall_shear_stress")
+    k = nb.value
+    return tuple(o.host()[:k] for o in out), pos.host()[:3 * k].reshape(k, 3)

@@ -13,6 +13,9 @@ import numpy as np
 
 from . import _lib as L
 
+P_D_ = L.P_D
+P_I64_ = L.P_I64
+
 
 class Mesh2:
     """Zero-based Mesh2 arrays (src/Mesh/Mesh_0_types.jl:82-93), numpy."""
@@ -45,6 +48,12 @@ class Mesh2:
         self.boundary_range = arr(d.boundary_range, 2 * nb, np.int64).reshape(nb, 2)
         self.boundary_faces = arr(d.boundary_faces, nbf, np.int64)
         self.boundary_cells = arr(d.boundary_cells, nbf, np.int64)
+        npt = C.c_int64()
+        xyz, cnp, cn = P_D_(), P_I64_(), P_I64_()
+        L.check(L.lib().fvh_hmesh_nodes(h, C.byref(npt), C.byref(xyz), C.byref(cnp), C.byref(cn)), "fvh_hmesh_nodes")
+        self.nodes = arr(xyz, 3 * npt.value, np.float64).reshape(-1, 3)  # mesh.nodes coords (scaled)
+        self.cell_nodes_ptr = arr(cnp, n + 1, np.int64)
+        self.cell_nodes = arr(cn, int(self.cell_nodes_ptr[-1]) if n else 0, np.int64)
         buf = C.create_string_buffer(256)
         self.boundary_names = []
         for i in range(nb):
@@ -418,6 +427,10 @@ def union(meshes, families):
     u.boundary_faces = np.concatenate(bfaces).astype(np.int64)
     u.boundary_cells = np.concatenate(bcells).astype(np.int64)
     u.boundary_names = [f"{fam}:{nm}" for fam, nm in order]
+    noff = np.concatenate([[0], np.cumsum([m.nodes.shape[0] for m in meshes])]).astype(np.int64)
+    u.nodes = np.concatenate([m.nodes for m in meshes])
+    u.cell_nodes = np.concatenate([m.cell_nodes + noff[k] for k, m in enumerate(meshes)])
+    u.cell_nodes_ptr = np.concatenate([[0], np.cumsum(np.concatenate([np.diff(m.cell_nodes_ptr) for m in meshes]))])
     u._dev = None
     u._keep = None
     u.case_offsets = coff
