@@ -77,6 +77,7 @@ struct DBuf {
     }
     int download(std::vector<T> &v, size_t count) const {
         v.resize(count);
+        HIP_TRY(hipStreamSynchronize(stream()));  // kernels of the library's (non-blocking) stream first
         if (count) HIP_TRY(hipMemcpy(v.data(), p, count * sizeof(T), hipMemcpyDeviceToHost));
         return FVH_OK;
     }

@@ -109,6 +109,9 @@ int ldl_setup(fvh_eqn *e, const double *val_dev) {
     if (m->cases) return set_error(FVH_ERR_UNSUPPORTED, "LDL preconditioner: not with independent cases");
     const int n = (int)m->n;
     std::vector<double> val(m->nnz), cx(n), cy(n);
+    // the values may still be in assembly on the library's stream (a
+    // non-blocking stream, which the synchronous copies below do not wait for)
+    HIP_TRY(hipStreamSynchronize(stream()));
     HIP_TRY(hipMemcpy(val.data(), val_dev, sizeof(double) * m->nnz, hipMemcpyDeviceToHost));
     HIP_TRY(hipMemcpy(cx.data(), m->ccx.p, sizeof(double) * n, hipMemcpyDeviceToHost));
     HIP_TRY(hipMemcpy(cy.data(), m->ccy.p, sizeof(double) * n, hipMemcpyDeviceToHost));

@@ -290,12 +290,13 @@ template <typename T>
 __host__ __device__ inline T sf_sentinel() {
     return __builtin_bit_cast(T, SfWord<T>::kSent);
 }
+constexpr int kSfNap = 2;       // longest sleep between polls (s_sleep 2 units), unless FVH_SF_NAP says otherwise
 constexpr int kSfSpin = 1 << 19;  // polls (with back-off: about a second) before a wait is declared lost
 
 template <typename T, int MODE>
 __global__ void __launch_bounds__(256)
     k_lev_sf(int nw, const int2 *__restrict__ waves, const LevRec<T> *__restrict__ R, const T *__restrict__ in,
-             T *out, T *dinv, int *fault, const KScal *ks) {
+             T *out, T *dinv, int *fault, int napmax, const KScal *ks) {
     if (ks && ks->done) return;
     const int w = blockIdx.x * 4 + (int)(threadIdx.x >> 6);
     if (w >= nw) return;
@@ -311,6 +312,7 @@ __global__ void __launch_bounds__(256)
     unsigned need = r.m > 0 ? (1u << r.m) - 1u : 0u;
     int nap = 1;
     for (int s = 0; need != 0u; ++s) {
+        const unsigned before = need;
 #pragma unroll
         for (int q = 0; q < kLevDeps; ++q)
             if (need & (1u << q)) {
@@ -325,8 +327,10 @@ __global__ void __launch_bounds__(256)
             __hip_atomic_store(fault, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
             break;
         }
+        // the front is here when a neighbour has just arrived: poll at once
+        if (need != before) nap = 1;
         for (int z = 0; z < nap; ++z) __builtin_amdgcn_s_sleep(2);
-        nap = nap < 32 ? 2 * nap : 32;
+        nap = nap < napmax ? 2 * nap : napmax;
     }
     const T res = lev_eval<T, MODE>(r, v0, nb);
     sf_store(out + k, res);
@@ -351,6 +355,16 @@ __global__ void k_lev_sf_prep(int n, const int *__restrict__ rows, const T *__re
 // ---------------------------------------------------------------------------
 static dim3 lev_grid(int n) { return dim3((unsigned)((n + 255) / 256)); }
 static dim3 sf_grid(int nwaves) { return dim3((unsigned)((nwaves + 3) / 4)); }  // four waves per work-group
+// Longest sleep between two polls of the sync-free walk, in units of about
+// 50 ns (FVH_SF_NAP, a power of two up to 64)
+static int sf_napmax() {
+    int v = kSfNap;
+    if (const char *env = std::getenv("FVH_SF_NAP")) {
+        const int r = std::atoi(env);
+        if (r >= 1 && r <= 64 && (r & (r - 1)) == 0) v = r;
+    }
+    return v;
+}
 
 // The walk's LDS ring of results (FP64: 128 KiB) needs the larger dynamic
 // LDS allowance, set once per instantiation.
@@ -432,7 +446,7 @@ int lev_dilu_update(fvh_eqn *e, const double *val) {
         HIP_TRY(hipGetLastError());
         hipLaunchKernelGGL((k_lev_sf<double, 0>), sf_grid(s.nswf), dim3(256), 0, stream(), s.nswf,
                            reinterpret_cast<const int2 *>(s.swf.p), U, (const double *)nullptr, L.b.p, e->Dinv.p,
-                           fault, (const KScal *)nullptr);
+                           fault, sf_napmax(), (const KScal *)nullptr);
         HIP_TRY(hipGetLastError());
     } else {
         FVH_TRY((launch_walk<double, 0>(s.nlev, s.fptr.p, U, (const double *)nullptr, L.b.p, e->Dinv.p,
@@ -461,12 +475,12 @@ static int lev_apply_t(fvh_eqn *e, const LevRec<T> *Fw, const LevRec<T> *Bw, con
         hipLaunchKernelGGL((k_lev_sf_prep<T>), lev_grid(n), dim3(256), 0, stream(), n, s.frows.p, x, wa, wb, wc, ks);
         HIP_TRY(hipGetLastError());
         hipLaunchKernelGGL((k_lev_sf<T, 1>), sf_grid(s.nswf), dim3(256), 0, stream(), s.nswf,
-                           reinterpret_cast<const int2 *>(s.swf.p), Fw, wa, wb, (T *)nullptr, fault, ks);
+                           reinterpret_cast<const int2 *>(s.swf.p), Fw, wa, wb, (T *)nullptr, fault, sf_napmax(), ks);
         HIP_TRY(hipGetLastError());
         hipLaunchKernelGGL((k_lev_perm<T, false>), lev_grid(n), dim3(256), 0, stream(), n, s.f2b.p, wb, wa, ks);
         HIP_TRY(hipGetLastError());
         hipLaunchKernelGGL((k_lev_sf<T, 2>), sf_grid(s.nswb), dim3(256), 0, stream(), s.nswb,
-                           reinterpret_cast<const int2 *>(s.swb.p), Bw, wa, wc, (T *)nullptr, fault, ks);
+                           reinterpret_cast<const int2 *>(s.swb.p), Bw, wa, wc, (T *)nullptr, fault, sf_napmax(), ks);
         HIP_TRY(hipGetLastError());
         hipLaunchKernelGGL((k_lev_perm<T, true>), lev_grid(n), dim3(256), 0, stream(), n, s.brows.p, wc, y, ks);
         HIP_TRY(hipGetLastError());

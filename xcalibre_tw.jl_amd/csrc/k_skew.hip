@@ -28,7 +28,7 @@
 // to whole blocks, at least one padding row), so one step's records of a wave
 // are contiguous and one step is one uniform buffer offset.  Slots outside the
 // rectangle are padding whose values make every term they enter an exact +0:
-//   F = (b, Dinv, A_rs, A_rw) = (0, 1, 0, 0),  Bk = (0, Dinv, A_re, A_rn) =
+//   F = (b, Dinv, A_nr, A_rw) = (0, 1, 0, 0),  Bk = (0, Dinv, A_re, A_rn) =
 //   (0, 1, 0, 0),  U = (A_rr, A_rs A_sr, A_rw A_wr) = (1, 0, 0), and the
 //   sweeps' quad records and right-hand sides built from them;
 // a padding cell's pivot evaluates to 1 and its forward / backward values to
@@ -481,11 +481,15 @@ __global__ void __launch_bounds__(64 * (W + 2))
 // of mixed precision) works on coefficients premultiplied by their pivots,
 // with fused multiply-adds.
 //
-// FP64 forward: one record per step, F = (b, Dinv, A_rs, A_rw); the west
-// pivot is the wave's own previous Dinv, the south pivot lane l-1's (lane
-// 0's comes with the hand-over: the wave below's top row pivots, read from
-// F kLook groups ahead).  The other sweeps read quad records, N values per
-// diagonal for four diagonals at once (one refill per four steps):
+// FP64 forward: one record per step, F = (b, Dinv, A_nr, A_rw), A_nr the
+// north neighbour's coefficient of this row (A[r+nx, r]).  Each lane forms,
+// right after its own y_r, the term the row above subtracts, (A_nr y_r)
+// Dinv_r, which is that row's (A_rs y_s) Dinv_s with the same operands in
+// the same order; the term moves up one lane (DPP) as the step's only
+// cross-lane value (the wave's top lane's terms are its hand-over batch),
+// and the west pivot is the wave's own previous Dinv.  The other sweeps
+// read quad records, N values per diagonal for four diagonals at once (one
+// refill per four steps):
 //   forward  FP32 (b quads apart)   (A_rs Dinv_s, A_rw Dinv_w)
 //   backward FP64 (Dinv, A_re, A_rn),   FP32 (Dinv, A_re Dinv, A_rn Dinv)
 // with Dinv_s / Dinv_w the pivots of the south / west neighbours (slots
@@ -533,31 +537,8 @@ __global__ void __launch_bounds__(64 * (W + 2))
     const unsigned vq = (g.base + 4 * (j0 + lane)) * (unsigned)sizeof(T), qbytes = a.rbytes / 4;
     const unsigned vc = N * vq, cstride = N * step;
     const R *F = static_cast<const R *>(recs);
-    // FP64: lane 0's south pivot, Dinv of (step T-1, row j0-1), loaded by
-    // lanes q < kL kLook groups ahead (lane q: step q of the group), then
-    // passed through the wave's LDS scratch words into registers of every
-    // lane a group ahead of use (the `old` operand of the south pivot's DPP
-    // shift, as the hand-over batch is for the south value)
-    const T *dp = &F[g.base + (size_t)(ql - 1 + kSkewD0) * S + j0 - 1].b;
-    T *pscr = lds + (W + 1) * HS + 64 * w;
-    T pin[2][kL];
-    auto pivots = [&](int slot, T v) {
-        pscr[lane] = v;
-        __atomic_signal_fence(__ATOMIC_SEQ_CST);
-#pragma unroll
-        for (int k = 0; k < kL; ++k) pin[slot][k] = pscr[k];
-    };
-    const size_t hstep = (size_t)kL * S * 4;  // one batch of steps, in T
     Hand<T> hd;
     hd.start(lds + w * HS, lds + (w + 1) * HS, HS, ql, lane == 63, g.NG);
-    T db[kLook];  // lane-0 pivots of the next kLook groups (FP64)
-#pragma unroll
-    for (int h = 0; h < kLook; ++h) db[h] = kFused ? T(0) : dp[h * hstep];
-    dp += kLook * hstep;
-    if constexpr (!kFused) {
-        pivots(0, db[0]);
-        hd.mirror();
-    }
     R ring[kFused ? 1 : PD];           // FP64 records
     Vec<T, 4 * N> cr[kFused ? NR : 1];  // FP32 coefficient quads
     Vec<T, 4> br[kFused ? NR : 1];      // FP32 right-hand-side quads
@@ -582,7 +563,7 @@ __global__ void __launch_bounds__(64 * (W + 2))
             }
         }
     }
-    T xprev = 0, res[kL];
+    T xprev = 0, mprev = 0, res[kL];
     int broken = 0;
     int G0 = 0;
     do {
@@ -594,7 +575,6 @@ __global__ void __launch_bounds__(64 * (W + 2))
         for (int h = 0; h < kLook; ++h) {
             const int G = G0 + h;
             hd.take(G, h, broken);
-            if constexpr (!kFused) pivots((h + 1) % 2, db[(h + 1) % kLook]);
 #pragma unroll
             for (int k = 0; k < kL; ++k) {
                 const int i = h * kL + k, mq = i / 4, wq = i % 4;
@@ -606,17 +586,18 @@ __global__ void __launch_bounds__(64 * (W + 2))
                     __builtin_amdgcn_sched_barrier(0);
                     const T xs = hd.template shift_in<true>(xprev, h, k);
                     x = __builtin_fmaf(-c[0], xs, t);
-                } else {  // (b, Dinv, A_rs, A_rw): the reference's order of operations
-                    const T xs = hd.template shift_in<true>(xprev, h, k);
+                } else {  // (b, Dinv, A_nr, A_rw): the reference's order of operations, the south
+                          // term (A_rs y_s) Dinv_s formed by the lane below and shifted in
+                    const T ms = hd.template shift_in<true>(mprev, h, k);
                     const R c = ring[i % PD];
                     const T dprev = ring[(i + PD - 1) % PD].b;  // the previous step's Dinv, still in its slot
-                    T &ds = pin[h % 2][k];  // the shift writes the spent pivot's registers
-                    ds = dpp_shr1(dprev, ds);
-                    x = (c.a - c.c * xs * ds) - c.d * xprev * dprev;
+                    x = (c.a - ms) - c.d * xprev * dprev;
+                    mprev = c.c * x * c.b;  // (A_nr y_r) Dinv_r: the row above's south term
                 }
                 q[wq] = x;
                 if (wq == 3) bstore4(rq, q, vq, (unsigned)mq * step);
-                xprev = res[k] = x;
+                xprev = x;
+                res[k] = kFused ? x : mprev;
                 __builtin_amdgcn_sched_barrier(0);
                 if constexpr (kFused) {
                     if (wq == 3) {
@@ -628,8 +609,6 @@ __global__ void __launch_bounds__(64 * (W + 2))
                 }
             }
             hd.finish(G, h, res);
-            if constexpr (!kFused) db[h] = *dp;
-            dp += hstep;
         }
         G0 += kLook;
     } while (G0 < g.NG);
@@ -777,12 +756,13 @@ __global__ void k_skew_pack(int n, const SkewGrid *__restrict__ grids, const int
     const bool S = j > 0, Wn = i > 0, E = i < nx - 1, N = j < g.ny - 1;
     const int d = diag[p];
     const double a_rs = S ? val[rowptr[p]] : 0.0;               // A[r, r-nx]
+    const double a_nr = N ? val[rowptr[p + nx]] : 0.0;          // A[r+nx, r]
     const double a_sr = S ? val[rowptr[p - nx + 1] - 1] : 0.0;  // A[r-nx, r]
     const double a_rw = Wn ? val[d - 1] : 0.0;                  // A[r, r-1]
     const double a_wr = Wn ? val[diag[p - 1] + 1] : 0.0;        // A[r-1, r]
     const int s = slot_of[p];
     U[s] = Rec4{val[d], a_rs * a_sr, a_rw * a_wr, 0.0};
-    F[s].c = a_rs;
+    F[s].c = a_nr;
     F[s].d = a_rw;
     Bk[s].c = E ? val[d + 1] : 0.0;                // A[r, r+1]
     Bk[s].d = N ? val[rowptr[p + 1] - 1] : 0.0;    // A[r, r+nx]
@@ -925,7 +905,7 @@ __global__ void k_skew_quads(const SkewGrid *__restrict__ grids, Rec4 *__restric
         } else {
             const Rec4 u = F[s], v = Bk[s];
             const double ds = F[s - g.S - 1].b, dw = F[s - g.S].b;
-            cf[2 * w] = (float)(u.c * ds);
+            cf[2 * w] = (float)(F[s - g.S - 1].c * ds);  // the south slot's A_nr is A_rs
             cf[2 * w + 1] = (float)(u.d * dw);
             cb[3 * w] = (float)v.b;
             cb[3 * w + 1] = (float)(v.c * v.b);
@@ -1155,7 +1135,7 @@ template <typename T, int W>
 static int launch_sweeps_w(fvh_eqn *e, SkewSweep<T> &w, const KScal *ks) {
     const SkewLayout &L = *e->m->skew;
     const SkewArgs a = skew_args<Rec4T<T>>(e, W);
-    const size_t lds = (size_t)(W + 1) * L.HS * sizeof(T), lds_f = lds + (size_t)64 * W * sizeof(T);
+    const size_t lds = (size_t)(W + 1) * L.HS * sizeof(T), lds_f = lds;
     FVH_TRY(allow_lds(k_skew_fwd<T, W>, lds_f));
     FVH_TRY(allow_lds(k_skew_bwd<T, W>, lds));
     const void *recs = sizeof(T) == 8 ? static_cast<const void *>(e->sk.F.p) : w.kf.p;
