@@ -192,6 +192,7 @@ struct SkewLayout {
     DBuf<SkewGrid> grids;
     DBuf<int> blk_grid[kSkewMaxW];  // block -> grid, per W
     DBuf<int> slot_of;              // natural row -> slot
+    DBuf<int> qoff;                 // natural row -> word of its value in the sweeps' quads
     DBuf<SkewTile> tiles;           // 64 x 64 tiles of the grids (gather / scatter)
 };
 

@@ -984,12 +984,15 @@ int skew_setup(fvh_mesh *m, const std::vector<SkewRect> &rects) {
         L->h_grids.push_back(g);
     }
     L->NS = base;
-    std::vector<int> slot_of(m->n, 0);
+    std::vector<int> slot_of(m->n, 0), qoff(m->n, 0);
     for (const SkewGrid &g : L->h_grids)
         for (int j = 0; j < g.ny; ++j)
-            for (int i = 0; i < g.nx; ++i)
-                slot_of[(size_t)g.o + i + (size_t)j * g.nx] =
-                    (int)(g.base + (size_t)(i + j + kSkewD0) * g.S + j);
+            for (int i = 0; i < g.nx; ++i) {
+                const size_t r = (size_t)g.o + i + (size_t)j * g.nx;
+                const int Tq = i + j + kSkewQ0;
+                slot_of[r] = (int)(g.base + (size_t)(i + j + kSkewD0) * g.S + j);
+                qoff[r] = (int)(g.base + (size_t)(Tq & ~3) * g.S + 4 * (size_t)j + (Tq & 3));
+            }
     std::vector<SkewTile> tiles;
     for (size_t k = 0; k < L->h_grids.size(); ++k)
         for (int j0 = 0; j0 < L->h_grids[k].ny; j0 += kTile)
@@ -997,6 +1000,7 @@ int skew_setup(fvh_mesh *m, const std::vector<SkewRect> &rects) {
     FVH_TRY(L->grids.upload(L->h_grids));
     for (int W = 1; W <= kSkewMaxW; ++W) FVH_TRY(L->blk_grid[W - 1].upload(blk_grid[W - 1]));
     FVH_TRY(L->slot_of.upload(slot_of));
+    FVH_TRY(L->qoff.upload(qoff));
     FVH_TRY(L->tiles.upload(tiles));
     m->skew = std::move(L);
     return FVH_OK;
@@ -1148,8 +1152,9 @@ static int launch_sweeps_w(fvh_eqn *e, SkewSweep<T> &w, const KScal *ks) {
     return FVH_OK;
 }
 
+// gather and the two sweeps: z in quads (w.zp)
 template <typename T>
-static int skew_apply_t(fvh_eqn *e, const T *x, T *y, const KScal *ks, SkewSweep<T> &w) {
+static int skew_sweeps_t(fvh_eqn *e, const T *x, const KScal *ks, SkewSweep<T> &w) {
     fvh_mesh *m = e->m;
     const SkewLayout &L = *m->skew;
     const int nhand = 2 * L.nblk[0] * L.HS;
@@ -1166,8 +1171,15 @@ static int skew_apply_t(fvh_eqn *e, const T *x, T *y, const KScal *ks, SkewSweep
         case 2: FVH_TRY((launch_sweeps_w<T, 2>(e, w, ks))); break;
         default: FVH_TRY((launch_sweeps_w<T, 3>(e, w, ks))); break;
     }
-    hipLaunchKernelGGL((k_skew_scatter_q<T>), dim3(ntiles), dim3(256), 0, stream(), L.tiles.p, L.grids.p, w.zp.p, y,
-                       ks_all);
+    return FVH_OK;
+}
+
+template <typename T>
+static int skew_apply_t(fvh_eqn *e, const T *x, T *y, const KScal *ks, SkewSweep<T> &w) {
+    FVH_TRY(skew_sweeps_t<T>(e, x, ks, w));
+    const SkewLayout &L = *e->m->skew;
+    hipLaunchKernelGGL((k_skew_scatter_q<T>), dim3((unsigned)L.tiles.n), dim3(256), 0, stream(), L.tiles.p, L.grids.p,
+                       w.zp.p, y, e->m->cases ? nullptr : ks);
     HIP_TRY(hipGetLastError());
     return FVH_OK;
 }
@@ -1231,6 +1243,24 @@ int skew_apply32(fvh_eqn *e, const float *x, float *y, const KScal *ks) {
     if (!s.s32.kf.p) return set_error(FVH_ERR_ARG, "FP32 sweep records not prepared");
     return skew_apply_t<float>(e, x, y, ks, s.s32);
 }
+
+int skew_apply_quads(fvh_eqn *e, const double *x, const KScal *ks, const double **zq) {
+    SkewStore &s = e->sk;
+    if (!s.s64.kf.p) return set_error(FVH_ERR_ARG, "DILU records not prepared");
+    FVH_TRY(skew_sweeps_t<double>(e, x, ks, s.s64));
+    *zq = s.s64.zp.p;
+    return FVH_OK;
+}
+
+int skew_apply_quads32(fvh_eqn *e, const float *x, const KScal *ks, const float **zq) {
+    SkewStore &s = e->sk;
+    if (!s.s32.kf.p) return set_error(FVH_ERR_ARG, "FP32 sweep records not prepared");
+    FVH_TRY(skew_sweeps_t<float>(e, x, ks, s.s32));
+    *zq = s.s32.zp.p;
+    return FVH_OK;
+}
+
+const int *skew_quad_map(const fvh_mesh *m) { return m->skew ? m->skew->qoff.p : nullptr; }
 
 int skew_dinv_natural(fvh_eqn *e, double *out) {
     const int n = (int)e->m->n;

@@ -449,6 +449,22 @@ static int sys_precond(const Sys<double> &S, double *y, const double *x, const K
 static int sys_precond(const Sys<float> &S, float *y, const float *x, const KScal *ks) {
     return precond_apply32(S.e, y, x, ks);
 }
+// The map of natural rows into the banded DILU sweeps' result quads when the
+// system's preconditioner is that apply (pipelined band sweeps, one case),
+// else null: CG then reads z from the quads and skips the scatter.
+template <typename T>
+static const int *sys_quad_map(const Sys<T> &S) {
+    const fvh_eqn *P = S.e->pshare ? S.e->pshare : S.e;
+    const fvh_mesh *m = S.e->m;
+    if (P->pkind != FVH_PRECOND_DILU || m->cases || !band_path(m) || !skew_active(m)) return nullptr;
+    return skew_quad_map(m);
+}
+static int sys_precond_quads(const Sys<double> &S, const double *x, const KScal *ks, const double **zq) {
+    return skew_apply_quads(S.e->pshare ? S.e->pshare : S.e, x, ks, zq);
+}
+static int sys_precond_quads(const Sys<float> &S, const float *x, const KScal *ks, const float **zq) {
+    return skew_apply_quads32(S.e->pshare ? S.e->pshare : S.e, x, ks, zq);
+}
 
 static void fill_stats(fvh_eqn *e, fvh_stats *st) {
     st->niter = e->kw.h_ks->iter;
@@ -698,6 +714,7 @@ static int cg(const Sys<T> &S, const fvh_solver_cfg &cfg, T *xu, fvh_stats *st, 
         if (itmax <= 0) { k->done = 1; k->status = FVH_KRYLOV_ITMAX; }
     }));
     int enq = 0;
+    const int *qmap = sys_quad_map(S);
     FVH_TRY(read_ks(e));
     while (!w.h_ks->done && enq < itmax) {
         int nb = std::min(next_batch(enq), itmax - enq);
@@ -709,12 +726,22 @@ static int cg(const Sys<T> &S, const fvh_solver_cfg &cfg, T *xu, fvh_stats *st, 
                                    r[i] = D(r[i]) + (-al) * D(Ap[i]);
                                });
             HIP_TRY(hipGetLastError());
-            FVH_TRY(sys_precond(S, z, r, ks));
-            FVH_TRY(mapred<1>(n, ks, part, [=] __device__(int i, double(&a)[1]) {
-                const T zi = sigma * D(z[i]);
-                z[i] = zi;
-                a[0] = a[0] + D(r[i]) * D(zi);
-            }));
+            if (qmap) {  // z read straight from the sweeps' quads (the same values, no scatter pass)
+                const T *zq = nullptr;
+                FVH_TRY(sys_precond_quads(S, r, ks, &zq));
+                FVH_TRY(mapred<1>(n, ks, part, [=] __device__(int i, double(&a)[1]) {
+                    const T zi = sigma * D(zq[qmap[i]]);
+                    z[i] = zi;
+                    a[0] = a[0] + D(r[i]) * D(zi);
+                }));
+            } else {
+                FVH_TRY(sys_precond(S, z, r, ks));
+                FVH_TRY(mapred<1>(n, ks, part, [=] __device__(int i, double(&a)[1]) {
+                    const T zi = sigma * D(z[i]);
+                    z[i] = zi;
+                    a[0] = a[0] + D(r[i]) * D(zi);
+                }));
+            }
             hipLaunchKernelGGL(k_cg_beta, fused_grid(n), dim3(kBlock), 0, stream(), (int)n, part, nchunks(n), ks, itmax,
                                [=] __device__(int i, double be) { p[i] = D(z[i]) + be * D(p[i]); });
             HIP_TRY(hipGetLastError());
