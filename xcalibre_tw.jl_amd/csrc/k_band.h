@@ -37,10 +37,14 @@ int skew_apply(fvh_eqn *e, const double *x, double *y, const KScal *ks);
 int skew_to32(fvh_eqn *e);
 int skew_apply32(fvh_eqn *e, const float *x, float *y, const KScal *ks);
 // the apply without its last step: z = M^-1 x left in the sweeps' quads
-// (*zq), where natural row r's value is word skew_quad_map(m)[r]
+// (*zq), where natural row r's value is word skew_quad_map(m)[r]; x null:
+// x was written into the sweep layout by the caller (skew_rhs_target)
 int skew_apply_quads(fvh_eqn *e, const double *x, const KScal *ks, const double **zq);
 int skew_apply_quads32(fvh_eqn *e, const float *x, const KScal *ks, const float **zq);
 const int *skew_quad_map(const fvh_mesh *m);
+// where the forward sweep reads its right-hand side: natural row r's value
+// goes to word (*map)[r] * (*stride) of *dst (FP64 or FP32 layout)
+int skew_rhs_target(fvh_eqn *e, int fp32, void **dst, const int **map, int *stride);
 int skew_dinv_natural(fvh_eqn *e, double *out);
 // reports (once) a sweep whose hand-over wait ran out
 int skew_check(fvh_eqn *e);

@@ -465,6 +465,13 @@ static int sys_precond_quads(const Sys<double> &S, const double *x, const KScal 
 static int sys_precond_quads(const Sys<float> &S, const float *x, const KScal *ks, const float **zq) {
     return skew_apply_quads32(S.e->pshare ? S.e->pshare : S.e, x, ks, zq);
 }
+template <typename T>
+static int sys_rhs_target(const Sys<T> &S, T **dst, const int **map, int *stride) {
+    void *d = nullptr;
+    FVH_TRY(skew_rhs_target(S.e->pshare ? S.e->pshare : S.e, sizeof(T) == 4, &d, map, stride));
+    *dst = static_cast<T *>(d);
+    return FVH_OK;
+}
 
 static void fill_stats(fvh_eqn *e, fvh_stats *st) {
     st->niter = e->kw.h_ks->iter;
@@ -714,21 +721,35 @@ static int cg(const Sys<T> &S, const fvh_solver_cfg &cfg, T *xu, fvh_stats *st, 
         if (itmax <= 0) { k->done = 1; k->status = FVH_KRYLOV_ITMAX; }
     }));
     int enq = 0;
-    const int *qmap = sys_quad_map(S);
+    // with the banded sweeps, r goes into their layout from the alpha update
+    // and z comes out of it into the r.z product: no gather or scatter pass
+    const int *qmap = sys_quad_map(S), *bmap = nullptr;
+    T *bdst = nullptr;
+    int bstride = 0;
+    if (qmap) FVH_TRY(sys_rhs_target(S, &bdst, &bmap, &bstride));
     FVH_TRY(read_ks(e));
     while (!w.h_ks->done && enq < itmax) {
         int nb = std::min(next_batch(enq), itmax - enq);
         for (int it = 0; it < nb; ++it) {
             FVH_TRY(sys_spmv_dot(S, p, Ap, sigma, ks, p, part));
-            hipLaunchKernelGGL(k_cg_alpha, fused_grid(n), dim3(kBlock), 0, stream(), (int)n, part, nchunks(n), ks, epsT,
-                               [=] __device__(int i, double al) {
-                                   xk[i] = D(xk[i]) + al * D(p[i]);
-                                   r[i] = D(r[i]) + (-al) * D(Ap[i]);
-                               });
+            if (qmap)
+                hipLaunchKernelGGL(k_cg_alpha, fused_grid(n), dim3(kBlock), 0, stream(), (int)n, part, nchunks(n), ks,
+                                   epsT, [=] __device__(int i, double al) {
+                                       xk[i] = D(xk[i]) + al * D(p[i]);
+                                       const T ri = D(r[i]) + (-al) * D(Ap[i]);
+                                       r[i] = ri;
+                                       bdst[(size_t)bmap[i] * bstride] = ri;
+                                   });
+            else
+                hipLaunchKernelGGL(k_cg_alpha, fused_grid(n), dim3(kBlock), 0, stream(), (int)n, part, nchunks(n), ks,
+                                   epsT, [=] __device__(int i, double al) {
+                                       xk[i] = D(xk[i]) + al * D(p[i]);
+                                       r[i] = D(r[i]) + (-al) * D(Ap[i]);
+                                   });
             HIP_TRY(hipGetLastError());
-            if (qmap) {  // z read straight from the sweeps' quads (the same values, no scatter pass)
+            if (qmap) {  // r already in the sweep layout, z read straight from the sweeps' quads
                 const T *zq = nullptr;
-                FVH_TRY(sys_precond_quads(S, r, ks, &zq));
+                FVH_TRY(sys_precond_quads(S, nullptr, ks, &zq));
                 FVH_TRY(mapred<1>(n, ks, part, [=] __device__(int i, double(&a)[1]) {
                     const T zi = sigma * D(zq[qmap[i]]);
                     z[i] = zi;
