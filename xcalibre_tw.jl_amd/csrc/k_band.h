@@ -42,9 +42,19 @@ int skew_apply32(fvh_eqn *e, const float *x, float *y, const KScal *ks);
 int skew_apply_quads(fvh_eqn *e, const double *x, const KScal *ks, const double **zq);
 int skew_apply_quads32(fvh_eqn *e, const float *x, const KScal *ks, const float **zq);
 const int *skew_quad_map(const fvh_mesh *m);
-// where the forward sweep reads its right-hand side: natural row r's value
-// goes to word (*map)[r] * (*stride) of *dst (FP64 or FP32 layout)
-int skew_rhs_target(fvh_eqn *e, int fp32, void **dst, const int **map, int *stride);
+// What a caller writes in place of the gather (FP64 or FP32 sweeps): natural
+// row r's value into word map[r] * stride of dst, and the hand-over arrays
+// reset (word q: 0 where q % HS == 0, else the sentinel, whose bits are the
+// low sizeof(T) bytes of `sent`)
+struct SkewRhs {
+    void *dst;
+    const int *map;
+    int stride;
+    void *hand;
+    int nhand, HS;
+    unsigned long long sent;
+};
+int skew_rhs_target(fvh_eqn *e, int fp32, SkewRhs *t);
 int skew_dinv_natural(fvh_eqn *e, double *out);
 // reports (once) a sweep whose hand-over wait ran out
 int skew_check(fvh_eqn *e);

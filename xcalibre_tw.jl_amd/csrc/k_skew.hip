@@ -1152,9 +1152,9 @@ static int launch_sweeps_w(fvh_eqn *e, SkewSweep<T> &w, const KScal *ks) {
     return FVH_OK;
 }
 
-// gather (or, when the caller has already written x into the sweep layout,
-// only the reset of the hand-over arrays) and the two sweeps: z in quads
-// (w.zp)
+// gather with the reset of the hand-over arrays (skipped when the caller has
+// written x into the sweep layout and reset the arrays: x null) and the two
+// sweeps: z in quads (w.zp)
 template <typename T>
 static int skew_sweeps_t(fvh_eqn *e, const T *x, const KScal *ks, SkewSweep<T> &w) {
     fvh_mesh *m = e->m;
@@ -1163,11 +1163,14 @@ static int skew_sweeps_t(fvh_eqn *e, const T *x, const KScal *ks, SkewSweep<T> &
     // per-case solves: gather / scatter cover every case, the sweeps skip
     // finished cases
     const KScal *ks_all = m->cases ? nullptr : ks;
-    const int ntiles = x ? (int)L.tiles.n : 0;
-    hipLaunchKernelGGL((k_skew_gather<T>), dim3(ntiles + (nhand + 255) / 256), dim3(256), 0, stream(), ntiles,
-                       L.tiles.p, L.grids.p, x, sizeof(T) == 8 ? reinterpret_cast<Rec4T<T> *>(e->sk.F.p) : nullptr,
-                       w.bq.p, w.hand.p, nhand, L.HS, ks_all);
-    HIP_TRY(hipGetLastError());
+    if (x) {
+        const int ntiles = (int)L.tiles.n;
+        hipLaunchKernelGGL((k_skew_gather<T>), dim3(ntiles + (nhand + 255) / 256), dim3(256), 0, stream(), ntiles,
+                           L.tiles.p, L.grids.p, x,
+                           sizeof(T) == 8 ? reinterpret_cast<Rec4T<T> *>(e->sk.F.p) : nullptr, w.bq.p, w.hand.p,
+                           nhand, L.HS, ks_all);
+        HIP_TRY(hipGetLastError());
+    }
     switch (skew_waves(L, sizeof(T) == 4)) {
         case 1: FVH_TRY((launch_sweeps_w<T, 1>(e, w, ks))); break;
         case 2: FVH_TRY((launch_sweeps_w<T, 2>(e, w, ks))); break;
@@ -1264,18 +1267,24 @@ int skew_apply_quads32(fvh_eqn *e, const float *x, const KScal *ks, const float 
 
 const int *skew_quad_map(const fvh_mesh *m) { return m->skew ? m->skew->qoff.p : nullptr; }
 
-int skew_rhs_target(fvh_eqn *e, int fp32, void **dst, const int **map, int *stride) {
+int skew_rhs_target(fvh_eqn *e, int fp32, SkewRhs *t) {
     const SkewLayout &L = *e->m->skew;
+    t->nhand = 2 * L.nblk[0] * L.HS;
+    t->HS = L.HS;
     if (fp32) {
         if (!e->sk.s32.bq.p) return set_error(FVH_ERR_ARG, "FP32 sweep records not prepared");
-        *dst = e->sk.s32.bq.p;
-        *map = L.qoff.p;
-        *stride = 1;
+        t->dst = e->sk.s32.bq.p;
+        t->map = L.qoff.p;
+        t->stride = 1;
+        t->hand = e->sk.s32.hand.p;
+        t->sent = HandWord<float>::kSent;
     } else {
         if (!e->sk.F.p) return set_error(FVH_ERR_ARG, "DILU records not prepared");
-        *dst = e->sk.F.p;  // b is word 0 of each 4-word record
-        *map = L.slot_of.p;
-        *stride = 4;
+        t->dst = e->sk.F.p;  // b is word 0 of each 4-word record
+        t->map = L.slot_of.p;
+        t->stride = 4;
+        t->hand = e->sk.s64.hand.p;
+        t->sent = HandWord<double>::kSent;
     }
     return FVH_OK;
 }

@@ -466,11 +466,13 @@ static int sys_precond_quads(const Sys<float> &S, const float *x, const KScal *k
     return skew_apply_quads32(S.e->pshare ? S.e->pshare : S.e, x, ks, zq);
 }
 template <typename T>
-static int sys_rhs_target(const Sys<T> &S, T **dst, const int **map, int *stride) {
-    void *d = nullptr;
-    FVH_TRY(skew_rhs_target(S.e->pshare ? S.e->pshare : S.e, sizeof(T) == 4, &d, map, stride));
-    *dst = static_cast<T *>(d);
-    return FVH_OK;
+static int sys_rhs_target(const Sys<T> &S, SkewRhs *t) {
+    return skew_rhs_target(S.e->pshare ? S.e->pshare : S.e, sizeof(T) == 4, t);
+}
+template <typename T>
+__device__ inline T bits_as(unsigned long long u) {
+    if constexpr (sizeof(T) == 8) return __builtin_bit_cast(T, u);
+    else return __builtin_bit_cast(T, (unsigned)u);
 }
 
 static void fill_stats(fvh_eqn *e, fvh_stats *st) {
@@ -723,10 +725,13 @@ static int cg(const Sys<T> &S, const fvh_solver_cfg &cfg, T *xu, fvh_stats *st, 
     int enq = 0;
     // with the banded sweeps, r goes into their layout from the alpha update
     // and z comes out of it into the r.z product: no gather or scatter pass
-    const int *qmap = sys_quad_map(S), *bmap = nullptr;
-    T *bdst = nullptr;
-    int bstride = 0;
-    if (qmap) FVH_TRY(sys_rhs_target(S, &bdst, &bmap, &bstride));
+    const int *qmap = sys_quad_map(S);
+    SkewRhs rt{};
+    if (qmap) FVH_TRY(sys_rhs_target(S, &rt));
+    T *bdst = static_cast<T *>(rt.dst), *hand = static_cast<T *>(rt.hand);
+    const int *bmap = rt.map;
+    const int bstride = rt.stride, nhand = rt.nhand, HS = rt.HS;
+    const unsigned long long sent = rt.sent;
     FVH_TRY(read_ks(e));
     while (!w.h_ks->done && enq < itmax) {
         int nb = std::min(next_batch(enq), itmax - enq);
@@ -739,6 +744,7 @@ static int cg(const Sys<T> &S, const fvh_solver_cfg &cfg, T *xu, fvh_stats *st, 
                                        const T ri = D(r[i]) + (-al) * D(Ap[i]);
                                        r[i] = ri;
                                        bdst[(size_t)bmap[i] * bstride] = ri;
+                                       for (int q = i; q < nhand; q += n) hand[q] = q % HS == 0 ? T(0) : bits_as<T>(sent);
                                    });
             else
                 hipLaunchKernelGGL(k_cg_alpha, fused_grid(n), dim3(kBlock), 0, stream(), (int)n, part, nchunks(n), ks,
