@@ -290,7 +290,13 @@ template <typename T>
 __host__ __device__ inline T sf_sentinel() {
     return __builtin_bit_cast(T, SfWord<T>::kSent);
 }
-constexpr int kSfNap = 2;       // longest sleep between polls (s_sleep 2 units), unless FVH_SF_NAP says otherwise
+// Dynamic LDS per block of the sync-free walk, unless FVH_SF_LDS says
+// otherwise: 90,000 bytes leave room for one 256-lane block per CU (4 waves),
+// so that about 1,000 waves poll instead of up to 8,000 (measured on the
+// 1M-point Delaunay mesh: FP64 apply 14.4 ms with no cap, 12.8 ms at two
+// blocks per CU, 12.4 ms at one)
+constexpr size_t kSfLds = 90000;
+constexpr int kSfNap = 1;       // longest sleep between polls (s_sleep 2 units), unless FVH_SF_NAP says otherwise
 constexpr int kSfSpin = 1 << 19;  // polls (with back-off: about a second) before a wait is declared lost
 
 template <typename T, int MODE>
@@ -357,6 +363,26 @@ static dim3 lev_grid(int n) { return dim3((unsigned)((n + 255) / 256)); }
 static dim3 sf_grid(int nwaves) { return dim3((unsigned)((nwaves + 3) / 4)); }  // four waves per work-group
 // Longest sleep between two polls of the sync-free walk, in units of about
 // 50 ns (FVH_SF_NAP, a power of two up to 64)
+// Dynamic LDS per block of the sync-free walk (FVH_SF_LDS bytes, at most
+// 160 KiB; the kernel uses none): a way to cap how many waves are resident
+// and polling ahead of the front
+static size_t sf_lds() {
+    if (const char *env = std::getenv("FVH_SF_LDS")) {
+        const long r = std::atol(env);
+        if (r >= 0 && r <= 160 * 1024) return (size_t)r;
+    }
+    return kSfLds;
+}
+template <typename T, int MODE>
+static int sf_allow(size_t lds) {
+    static size_t allowed = 64 * 1024;
+    if (lds > allowed) {
+        HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void *>(k_lev_sf<T, MODE>),
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        allowed = lds;
+    }
+    return FVH_OK;
+}
 static int sf_napmax() {
     int v = kSfNap;
     if (const char *env = std::getenv("FVH_SF_NAP")) {
@@ -444,7 +470,8 @@ int lev_dilu_update(fvh_eqn *e, const double *val) {
                            (const double *)nullptr, (double *)nullptr, L.b.p, (double *)nullptr,
                            (const KScal *)nullptr);
         HIP_TRY(hipGetLastError());
-        hipLaunchKernelGGL((k_lev_sf<double, 0>), sf_grid(s.nswf), dim3(256), 0, stream(), s.nswf,
+        FVH_TRY((sf_allow<double, 0>(sf_lds())));
+        hipLaunchKernelGGL((k_lev_sf<double, 0>), sf_grid(s.nswf), dim3(256), sf_lds(), stream(), s.nswf,
                            reinterpret_cast<const int2 *>(s.swf.p), U, (const double *)nullptr, L.b.p, e->Dinv.p,
                            fault, sf_napmax(), (const KScal *)nullptr);
         HIP_TRY(hipGetLastError());
@@ -474,12 +501,14 @@ static int lev_apply_t(fvh_eqn *e, const LevRec<T> *Fw, const LevRec<T> *Bw, con
         FVH_TRY(lev_fault_ptr(e, &fault));
         hipLaunchKernelGGL((k_lev_sf_prep<T>), lev_grid(n), dim3(256), 0, stream(), n, s.frows.p, x, wa, wb, wc, ks);
         HIP_TRY(hipGetLastError());
-        hipLaunchKernelGGL((k_lev_sf<T, 1>), sf_grid(s.nswf), dim3(256), 0, stream(), s.nswf,
+        FVH_TRY((sf_allow<T, 1>(sf_lds())));
+        FVH_TRY((sf_allow<T, 2>(sf_lds())));
+        hipLaunchKernelGGL((k_lev_sf<T, 1>), sf_grid(s.nswf), dim3(256), sf_lds(), stream(), s.nswf,
                            reinterpret_cast<const int2 *>(s.swf.p), Fw, wa, wb, (T *)nullptr, fault, sf_napmax(), ks);
         HIP_TRY(hipGetLastError());
         hipLaunchKernelGGL((k_lev_perm<T, false>), lev_grid(n), dim3(256), 0, stream(), n, s.f2b.p, wb, wa, ks);
         HIP_TRY(hipGetLastError());
-        hipLaunchKernelGGL((k_lev_sf<T, 2>), sf_grid(s.nswb), dim3(256), 0, stream(), s.nswb,
+        hipLaunchKernelGGL((k_lev_sf<T, 2>), sf_grid(s.nswb), dim3(256), sf_lds(), stream(), s.nswb,
                            reinterpret_cast<const int2 *>(s.swb.p), Bw, wa, wc, (T *)nullptr, fault, sf_napmax(), ks);
         HIP_TRY(hipGetLastError());
         hipLaunchKernelGGL((k_lev_perm<T, true>), lev_grid(n), dim3(256), 0, stream(), n, s.brows.p, wc, y, ks);
