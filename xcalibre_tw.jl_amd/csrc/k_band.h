@@ -55,6 +55,8 @@ struct SkewRhs {
     unsigned long long sent;
 };
 int skew_rhs_target(fvh_eqn *e, int fp32, SkewRhs *t);
+// the reset of the hand-over arrays alone (the gather's other half)
+int skew_hand_reset(fvh_eqn *e, int fp32);
 int skew_dinv_natural(fvh_eqn *e, double *out);
 // reports (once) a sweep whose hand-over wait ran out
 int skew_check(fvh_eqn *e);

@@ -221,6 +221,28 @@ static int c_precond(const CSys<double> &S, double *y, const double *x, const KS
 static int c_precond(const CSys<float> &S, float *y, const float *x, const KScal *ks) {
     return precond_apply32(S.e, y, x, ks);
 }
+// With the pipelined banded DILU, CG writes r into the sweep layout in its
+// alpha update and reads z from the sweeps' result quads (no gather or
+// scatter pass; the hand-over arrays are reset by a launch of their own,
+// since the per-case maps skip finished cases): the map of rows into the
+// quads, or null
+template <typename T>
+static const int *c_quad_map(const CSys<T> &S) {
+    const fvh_eqn *P = S.e->pshare ? S.e->pshare : S.e;
+    const fvh_mesh *m = S.e->m;
+    if (P->pkind != FVH_PRECOND_DILU || !band_path(m) || !skew_active(m)) return nullptr;
+    return skew_quad_map(m);
+}
+static int c_precond_quads(const CSys<double> &S, const KScal *ks, const double **zq) {
+    fvh_eqn *P = S.e->pshare ? S.e->pshare : S.e;
+    FVH_TRY(skew_hand_reset(P, 0));
+    return skew_apply_quads(P, nullptr, ks, zq);
+}
+static int c_precond_quads(const CSys<float> &S, const KScal *ks, const float **zq) {
+    fvh_eqn *P = S.e->pshare ? S.e->pshare : S.e;
+    FVH_TRY(skew_hand_reset(P, 1));
+    return skew_apply_quads32(P, nullptr, ks, zq);
+}
 
 static int read_all(fvh_eqn *e, bool *all_done) {
     const int K = e->m->cases->K;
@@ -471,6 +493,12 @@ static int cg_c(const CSys<T> &S, const fvh_solver_cfg &cfg, T *xu, double sigma
         if (rn <= k->eps) { k->done = 1; k->status = FVH_KRYLOV_SOLVED; return; }
         if (itmax <= 0) { k->done = 1; k->status = FVH_KRYLOV_ITMAX; }
     }));
+    const int *qmap = c_quad_map(S);
+    SkewRhs rt{};
+    if (qmap) FVH_TRY(skew_rhs_target(e->pshare ? e->pshare : e, sizeof(T) == 4, &rt));
+    T *bdst = static_cast<T *>(rt.dst);
+    const int *bmap = rt.map;
+    const int bstride = rt.stride;
     auto body = [&](int nb) -> int {
         for (int it = 0; it < nb; ++it) {
             FVH_TRY(c_spmv_dot(S, p, Ap, sigma, ks, p, part));
@@ -483,17 +511,34 @@ static int cg_c(const CSys<T> &S, const fvh_solver_cfg &cfg, T *xu, double sigma
                 }
                 k->alpha = k->gamma / pAp;
             }));
-            FVH_TRY(C.map(ks, [=] __device__(int i, const KScal *K) {
-                double al = K->alpha;
-                xk[i] = D(xk[i]) + al * D(p[i]);
-                r[i] = D(r[i]) + (-al) * D(Ap[i]);
-            }));
-            FVH_TRY(c_precond(S, z, r, ks));
-            FVH_TRY(C.mapred<1>(ks, part, [=] __device__(int i, double(&a)[1], const KScal *) {
-                const T zi = sigma * D(z[i]);
-                z[i] = zi;
-                a[0] = a[0] + D(r[i]) * D(zi);
-            }));
+            if (qmap) {  // r also into the sweep layout; z from the result quads
+                FVH_TRY(C.map(ks, [=] __device__(int i, const KScal *K) {
+                    double al = K->alpha;
+                    xk[i] = D(xk[i]) + al * D(p[i]);
+                    const T ri = D(r[i]) + (-al) * D(Ap[i]);
+                    r[i] = ri;
+                    bdst[(size_t)bmap[i] * bstride] = ri;
+                }));
+                const T *zq = nullptr;
+                FVH_TRY(c_precond_quads(S, ks, &zq));
+                FVH_TRY(C.mapred<1>(ks, part, [=] __device__(int i, double(&a)[1], const KScal *) {
+                    const T zi = sigma * D(zq[qmap[i]]);
+                    z[i] = zi;
+                    a[0] = a[0] + D(r[i]) * D(zi);
+                }));
+            } else {
+                FVH_TRY(C.map(ks, [=] __device__(int i, const KScal *K) {
+                    double al = K->alpha;
+                    xk[i] = D(xk[i]) + al * D(p[i]);
+                    r[i] = D(r[i]) + (-al) * D(Ap[i]);
+                }));
+                FVH_TRY(c_precond(S, z, r, ks));
+                FVH_TRY(C.mapred<1>(ks, part, [=] __device__(int i, double(&a)[1], const KScal *) {
+                    const T zi = sigma * D(z[i]);
+                    z[i] = zi;
+                    a[0] = a[0] + D(r[i]) * D(zi);
+                }));
+            }
             FVH_TRY(C.finalize<1>(part, ks, true, [=] __device__(KScal * k, double(&g)[1]) {
                 double gn = g[0];
                 double rn = sqrt(gn);

@@ -1267,6 +1267,22 @@ int skew_apply_quads32(fvh_eqn *e, const float *x, const KScal *ks, const float 
 
 const int *skew_quad_map(const fvh_mesh *m) { return m->skew ? m->skew->qoff.p : nullptr; }
 
+int skew_hand_reset(fvh_eqn *e, int fp32) {
+    const SkewLayout &L = *e->m->skew;
+    const int nhand = 2 * L.nblk[0] * L.HS;
+    const dim3 g((unsigned)((nhand + 255) / 256));
+    if (fp32)
+        hipLaunchKernelGGL((k_skew_gather<float>), g, dim3(256), 0, stream(), 0, L.tiles.p, L.grids.p,
+                           (const float *)nullptr, (Rec4T<float> *)nullptr, e->sk.s32.bq.p, e->sk.s32.hand.p, nhand,
+                           L.HS, (const KScal *)nullptr);
+    else
+        hipLaunchKernelGGL((k_skew_gather<double>), g, dim3(256), 0, stream(), 0, L.tiles.p, L.grids.p,
+                           (const double *)nullptr, (Rec4T<double> *)nullptr, e->sk.s64.bq.p, e->sk.s64.hand.p, nhand,
+                           L.HS, (const KScal *)nullptr);
+    HIP_TRY(hipGetLastError());
+    return FVH_OK;
+}
+
 int skew_rhs_target(fvh_eqn *e, int fp32, SkewRhs *t) {
     const SkewLayout &L = *e->m->skew;
     t->nhand = 2 * L.nblk[0] * L.HS;
