@@ -123,10 +123,28 @@ __global__ void __launch_bounds__(256) k_asm_cells(DevMesh m, AsmArgs a, const d
     if (NFT >= 0) {
         diag = 0.0;
         if (NFT > 0) {
-            for (int i = m.cf_ptr[c]; i < m.cf_ptr[c + 1]; ++i) {
-                const double *p = td + (size_t)m.cf_slot[i] * (NFT > 0 ? NFT : 1);
+            // four sides at a time: their slot loads, then their terms'
+            // loads, issued together; summed in faces_range order
+            constexpr int G = 4, NT = NFT > 0 ? NFT : 1;
+            const int i1 = m.cf_ptr[c + 1];
+            for (int i = m.cf_ptr[c]; i < i1; i += G) {
+                int sl[G];
+                double v[G][NT];
 #pragma unroll
-                for (int q = 0; q < (NFT > 0 ? NFT : 0); ++q) diag += p[q];
+                for (int k = 0; k < G; ++k)
+                    if (i + k < i1) sl[k] = m.cf_slot[i + k];
+#pragma unroll
+                for (int k = 0; k < G; ++k)
+                    if (i + k < i1) {
+#pragma unroll
+                        for (int q = 0; q < NT; ++q) v[k][q] = td[(size_t)sl[k] * NT + q];
+                    }
+#pragma unroll
+                for (int k = 0; k < G; ++k)
+                    if (i + k < i1) {
+#pragma unroll
+                        for (int q = 0; q < NT; ++q) diag += v[k][q];
+                    }
             }
         }
         b = 0.0;

@@ -79,18 +79,41 @@ __global__ void k_correct_boundaries_vec(DevMesh m, BcVals bx, BcVals by, BcVals
 }
 
 // green_gauss!(dx, dy, dz, phif)
+// A cell's faces are taken kFaceGroup at a time: the group's index loads,
+// then its data loads, are issued together (not one dependent pair per
+// face), and the terms are added in faces_range order as before.
+constexpr int kFaceGroup = 4;
+
 __global__ void k_green_gauss(DevMesh m, const double *__restrict__ phif, double *__restrict__ dx,
                               double *__restrict__ dy, double *__restrict__ dz) {
     int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= m.n) return;
     double rx = 0.0, ry = 0.0, rz = 0.0;
-    for (int i = m.cf_ptr[c]; i < m.cf_ptr[c + 1]; ++i) {
-        int f = m.cf_face[i];
-        double ns = m.cf_sign[i];
-        double v = phif[f];
-        rx += v * (m.Sfx[f] * ns);
-        ry += v * (m.Sfy[f] * ns);
-        rz += v * (m.Sfz[f] * ns);
+    const int i1 = m.cf_ptr[c + 1];
+    for (int i = m.cf_ptr[c]; i < i1; i += kFaceGroup) {
+        int f[kFaceGroup];
+        double ns[kFaceGroup], v[kFaceGroup], sx[kFaceGroup], sy[kFaceGroup], sz[kFaceGroup];
+#pragma unroll
+        for (int k = 0; k < kFaceGroup; ++k)
+            if (i + k < i1) {
+                f[k] = m.cf_face[i + k];
+                ns[k] = m.cf_sign[i + k];
+            }
+#pragma unroll
+        for (int k = 0; k < kFaceGroup; ++k)
+            if (i + k < i1) {
+                v[k] = phif[f[k]];
+                sx[k] = m.Sfx[f[k]];
+                sy[k] = m.Sfy[f[k]];
+                sz[k] = m.Sfz[f[k]];
+            }
+#pragma unroll
+        for (int k = 0; k < kFaceGroup; ++k)
+            if (i + k < i1) {
+                rx += v[k] * (sx[k] * ns[k]);
+                ry += v[k] * (sy[k] * ns[k]);
+                rz += v[k] * (sz[k] * ns[k]);
+            }
     }
     double vol = m.vol[c];
     rx = rx / vol;
@@ -131,11 +154,22 @@ __global__ void k_div(DevMesh m, const double *__restrict__ fx, const double *__
     if (c >= m.n) return;
     double vol = m.vol[c];
     double acc = 0.0;
-    for (int i = m.cf_ptr[c]; i < m.cf_ptr[c + 1]; ++i) {
-        int f = m.cf_face[i];
-        double ns = m.cf_sign[i];
-        double d = (fx[f] * m.Sfx[f] + fy[f] * m.Sfy[f]) + fz[f] * m.Sfz[f];
-        acc += d * ns / vol;
+    const int i1 = m.cf_ptr[c + 1];
+    for (int i = m.cf_ptr[c]; i < i1; i += kFaceGroup) {
+        int f[kFaceGroup];
+        double ns[kFaceGroup], d[kFaceGroup];
+#pragma unroll
+        for (int k = 0; k < kFaceGroup; ++k)
+            if (i + k < i1) {
+                f[k] = m.cf_face[i + k];
+                ns[k] = m.cf_sign[i + k];
+            }
+#pragma unroll
+        for (int k = 0; k < kFaceGroup; ++k)
+            if (i + k < i1) d[k] = (fx[f[k]] * m.Sfx[f[k]] + fy[f[k]] * m.Sfy[f[k]]) + fz[f[k]] * m.Sfz[f[k]];
+#pragma unroll
+        for (int k = 0; k < kFaceGroup; ++k)
+            if (i + k < i1) acc += d[k] * ns[k] / vol;
     }
     for (int i = m.cb_ptr[c]; i < m.cb_ptr[c + 1]; ++i) {
         int f = m.cb_face[i];

@@ -511,10 +511,33 @@ static int pressure_corrector(Sim &S) {
         double *hx = S.Hx.p, *hy = S.Hy.p, *hz = S.Hz.p;
         FVH_TRY(map(n, nullptr, [=] __device__(int c) {
             double sx = 0.0, sy = 0.0;
-            for (int i = d.cf_ptr[c]; i < d.cf_ptr[c + 1]; ++i) {
-                int nb = d.cf_nbr[i], s = d.cf_slot[i];
-                sx += ax[s] * vx[nb];
-                sy += ay[s] * vy[nb];
+            // four neighbours at a time: index loads, then value loads, then
+            // the sums in faces_range order
+            constexpr int G = 4;
+            const int i1 = d.cf_ptr[c + 1];
+            for (int i = d.cf_ptr[c]; i < i1; i += G) {
+                int nb[G], sl[G];
+                double a1[G], a2[G], u1[G], u2[G];
+_Pragma("unroll")
+                for (int k = 0; k < G; ++k)
+                    if (i + k < i1) {
+                        nb[k] = d.cf_nbr[i + k];
+                        sl[k] = d.cf_slot[i + k];
+                    }
+_Pragma("unroll")
+                for (int k = 0; k < G; ++k)
+                    if (i + k < i1) {
+                        a1[k] = ax[sl[k]];
+                        u1[k] = vx[nb[k]];
+                        a2[k] = ay[sl[k]];
+                        u2[k] = vy[nb[k]];
+                    }
+_Pragma("unroll")
+                for (int k = 0; k < G; ++k)
+                    if (i + k < i1) {
+                        sx += a1[k] * u1[k];
+                        sy += a2[k] * u2[k];
+                    }
             }
             double rD = 1.0 / ax[d.diag[c]];
             hx[c] = (bx[c] - sx) * rD;
