@@ -63,17 +63,24 @@ def delaunay_full():
     return F.delaunay_unit_square(1_000_000, seed=0)
 
 
-@pytest.mark.parametrize("walk", ["syncfree", "levels", "levels-ring256"])
+@pytest.mark.parametrize("walk", ["tiles", "tiles-256", "syncfree", "levels", "levels-ring256"])
 def test_delaunay_full_size_dilu_bitwise(delaunay_full, walk, monkeypatch):
     """SpMV, the DILU update and the DILU apply (FP64) on random values,
-    bit for bit against the oracle, on the sync-free walk (the default), on
-    the one-work-group level walk, and on the level walk with its LDS ring
-    cut to 256 results (so most neighbours come through its global-memory
-    and after-barrier paths)."""
+    bit for bit against the oracle, on the tile walk (the default), on the
+    tile walk with 256-row tiles (about 8,000 tiles, most rows next to
+    another tile's), on the sync-free walk, on the one-work-group level walk,
+    and on the level walk with its LDS ring cut to 256 results (so most
+    neighbours come through its global-memory and after-barrier paths)."""
     import ctypes as C
 
     mesh = delaunay_full
-    monkeypatch.setenv("FVH_LEVEL_SYNCFREE", "1" if walk == "syncfree" else "0")
+    print(f"walk {walk}", flush=True)
+    monkeypatch.setenv("FVH_LEVEL_SYNCFREE", "0" if walk.startswith("levels") else "1")
+    monkeypatch.setenv("FVH_LEVEL_TILES", "1" if walk.startswith("tiles") else "0")
+    if walk == "tiles-256":
+        monkeypatch.setenv("FVH_TILE_ROWS", "256")
+    else:
+        monkeypatch.delenv("FVH_TILE_ROWS", raising=False)
     if walk == "levels-ring256":
         monkeypatch.setenv("FVH_LEVEL_RING", "256")
     else:

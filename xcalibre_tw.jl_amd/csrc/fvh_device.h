@@ -12,6 +12,7 @@
 #include <vector>
 
 #include "fvh_internal.h"
+#include "tile_sched.h"
 
 #define HIP_TRY(expr)                                                                               \
     do {                                                                                            \
@@ -165,6 +166,8 @@ struct CaseLayout {
 // slots for every T the sweeps touch.
 constexpr int kLevRing = 16384;    // results the level walk keeps in LDS (by schedule position)
 constexpr int kLevLate = 1 << 30;  // flag of a neighbour read after the level barrier
+constexpr int kTileRows = 2048;    // rows per tile of the tile walk, unless FVH_TILE_ROWS says otherwise
+constexpr int kTileLag = 12;       // steps a neighbour from another tile is scheduled behind its producer
 constexpr int kSkewL = 8;                    // steps per hand-over batch
 constexpr int kSkewPD = 32;                  // deepest record prefetch (steps)
 constexpr int kSkewD0 = kSkewPD + 2;         // padding diagonals below T = 0
@@ -222,6 +225,25 @@ struct Schedule {
     // count <= 64) per wave, each wave inside one level, waves in level order
     DBuf<int> swf, swb;
     int nswf = 0, nswb = 0;
+    // The tile walk, forward (tf) and backward (tb): tiles of tile_rows
+    // consecutive rows in the sweep's order, each walked by one wave in steps
+    // of independent rows; a tile's results stay in its LDS, a neighbour from
+    // another tile is staged into that LDS by a loader wave before the
+    // chunk of steps that first reads it.
+    struct TileWalk {
+        DBuf<int> rows;    // position -> row; tile t holds positions [t * tile_rows, ...)
+        DBuf<int> tiles;   // per tile: first chunk, chunk count
+        DBuf<int> chunks;  // per chunk (kTileChunkInts): first step, steps, first entry, entries, first position, rows
+        DBuf<int> steps;   // per step: first position, rows (<= 64, no dependency among them)
+        DBuf<int> ents;    // per staged neighbour: its position, the LDS word it goes to
+        int ntiles = 0, maxrem = 0;  // tiles; most neighbours one tile stages
+    };
+    TileWalk tf, tb;
+    int tile_rows = kTileRows;
+    // per entry: the LDS word where the tile walk finds the neighbour (its
+    // position in the tile, or tile_rows + its staging slot); the forward
+    // position of the row at backward position k
+    DBuf<int> tcode, tf2b;
     int ring = kLevRing;  // results the level walk's LDS ring holds (power of two <= kLevRing)
     int band_nx = 0;  // >0: A is a banded grid (deps only r-1, r-nx)
     DBuf<unsigned char> bmask;  // banded grid: bit k of row r = entry r + (-nx, -1, 0, 1, nx)[k] present

@@ -129,7 +129,7 @@ struct LevelStore {
     DBuf<LevBlob32> fwd32, bwd32;
     DBuf<double> a, b, c;
     DBuf<float> a32, b32, c32;
-    bool sf = false;         // the records hold walk positions (sync-free walk)
+    int walk = 0;            // the walk the records are for: 0 level walk, 1 sync-free, 2 tiles
     int *h_fault = nullptr;  // pinned: a wait of the sync-free walk ran out
     LevelStore() = default;
     LevelStore(const LevelStore &) = delete;

@@ -18,6 +18,7 @@
 // use coefficients premultiplied by their pivots in FP64 and rounded once,
 // with fused multiply-adds in neighbour order - the arithmetic of every FP32
 // DILU sweep of the library.
+#include <algorithm>
 #include <cstdlib>
 #include <cstring>
 
@@ -357,6 +358,169 @@ __global__ void k_lev_sf_prep(int n, const int *__restrict__ rows, const T *__re
 }
 
 // ---------------------------------------------------------------------------
+// Tile walk (the default): one work-group per tile of consecutive rows (in
+// the sweep's order), in tile order.  Its compute wave takes the tile's rows
+// in steps of at most 64 independent rows, one lane per row, and keeps every
+// result in LDS at the row's place in the tile, where the later steps of the
+// tile read it; a step's critical path is an LDS read, the row's expression
+// (lev_eval: the same bits as every other walk) and an LDS write.  Results
+// also go to global memory with device-coherent stores, into an array filled
+// with the sentinel before the sweep.  Neighbours from other (earlier) tiles
+// are staged into LDS words after the tile's results by kTileStages loader
+// waves, which take the chunks of steps in turn: a loader reads its chunk's
+// records, inputs and step table into an LDS stage buffer (its own), polls
+// the chunk's neighbours from other tiles until none is the sentinel, writes
+// them to their LDS words and raises the buffer's ready word; the compute
+// wave raises the buffer's done word when it has finished the chunk, and the
+// loader then fills it with its next chunk.  Only loaders wait for global
+// memory, so the compute wave never waits for a load or a store.  A tile
+// waits only for tiles before it, whose work-groups the in-order dispatch
+// has started, so the sweep always drains; every wait is bounded, and one
+// that runs out sets the fault word and ends the wave (a loader or compute
+// wave that ends early makes the other waves of its tile run out in turn).
+// ---------------------------------------------------------------------------
+constexpr int kTileStages = 3;  // loader waves and stage buffers per work-group
+constexpr int kTileSpin = 1 << 22;  // polls of an LDS word (about half a second) before a wait is declared lost
+
+typedef unsigned int TileWord __attribute__((ext_vector_type(4)));  // 16 bytes of a record
+
+template <typename T>
+struct alignas(16) TileStage {  // one chunk in LDS
+    LevRec<T> rec[64];  // records of the chunk's rows, by position from the chunk's first
+    T v[64];            // their inputs
+    int2 step[64];      // the chunk's steps: first position, rows
+    int first, nsteps, pad[2];
+};
+
+template <typename T>
+__host__ __device__ constexpr size_t tile_words_off() {  // byte offset of the tile's result words
+    return kTileStages * sizeof(TileStage<T>) + 16 * ((2 * kTileStages * sizeof(int) + 15) / 16);
+}
+
+template <typename T, int MODE>
+__global__ void __launch_bounds__(64 * (1 + kTileStages))
+    k_lev_tile(int tr, const int *__restrict__ tiles, const int *__restrict__ chunks, const int2 *__restrict__ steps,
+               const int2 *__restrict__ ents, const LevRec<T> *__restrict__ R, const T *__restrict__ in, T *out,
+               T *dinv, int *fault, const KScal *ks) {
+    if (ks && ks->done) return;
+    extern __shared__ __align__(16) unsigned char tile_mem[];
+    auto *stage = reinterpret_cast<TileStage<T> *>(tile_mem);
+    int *ready = reinterpret_cast<int *>(stage + kTileStages), *done = ready + kTileStages;
+    T *res = reinterpret_cast<T *>(tile_mem + tile_words_off<T>());  // [tr + staged neighbours]
+    const int t = blockIdx.x, wave = (int)(threadIdx.x >> 6), lane = threadIdx.x & 63;
+    const int c0 = tiles[2 * t], nc = tiles[2 * t + 1];
+    if (threadIdx.x < kTileStages) ready[threadIdx.x] = done[threadIdx.x] = 0;  // chunk c is c + 1
+    __syncthreads();
+    int broken = 0;
+    if (wave > 0) {  // loader of chunks wave-1, wave-1 + kTileStages, ...
+        const int b = wave - 1;
+        TileStage<T> &S = stage[b];
+        for (int c = b; c < nc; c += kTileStages) {
+            const int *C = chunks + (size_t)kTileChunkInts * (c0 + c);
+            const int fs = C[0], ns = C[1], fe = C[2], ne = C[3], fp = C[4], nr = C[5];
+            constexpr int RW = sizeof(LevRec<T>) / 16;  // the record as 16-byte words (no struct copy in memory)
+            int2 st = make_int2(0, 0), e = make_int2(0, 0);
+            TileWord r[RW];
+            T v = T(0), x = T(0);
+            if (lane < ns) st = steps[fs + lane];
+            if (lane < nr) {
+                const TileWord *src = reinterpret_cast<const TileWord *>(R + fp + lane);
+#pragma unroll
+                for (int w = 0; w < RW; ++w) r[w] = src[w];
+                if (MODE != 0) v = in[fp + lane];
+            }
+            if (lane < ne) {
+                e = ents[fe + lane];
+                x = sf_load(out + e.x);
+            }
+            for (int s = 0; __builtin_amdgcn_ballot_w64(lane < ne && sf_sent(x)); ++s) {
+                if (s >= kSfSpin) {
+                    broken = 1;
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(1);
+                if (lane < ne && sf_sent(x)) x = sf_load(out + e.x);
+            }
+            for (int s = 0; !broken && c >= kTileStages &&
+                            __hip_atomic_load(done + b, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) !=
+                                c - kTileStages + 1;
+                 ++s) {  // the compute wave is still on this buffer's previous chunk
+                if (s >= kTileSpin) broken = 1;
+                __builtin_amdgcn_s_sleep(1);
+            }
+            if (broken) break;
+            asm volatile("" ::: "memory");
+            if (lane < nr) {
+                TileWord *dst = reinterpret_cast<TileWord *>(&S.rec[lane]);
+#pragma unroll
+                for (int w = 0; w < RW; ++w) dst[w] = r[w];
+                S.v[lane] = v;
+            }
+            if (lane < ns) S.step[lane] = st;
+            if (lane == 0) {
+                S.first = fp;
+                S.nsteps = ns;
+            }
+            if (lane < ne) res[e.y] = x;
+            asm volatile("" ::: "memory");  // the stage is written (LDS keeps a wave's order) before the ready word
+            __hip_atomic_store(ready + b, c + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        }
+    } else {  // compute wave
+        const int base = t * tr;
+        for (int c = 0; c < nc; ++c) {
+            const int b = c % kTileStages;
+            const TileStage<T> &S = stage[b];
+            for (int s = 0;
+                 !broken && __hip_atomic_load(ready + b, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) != c + 1;
+                 ++s) {
+                if (s >= kTileSpin) broken = 1;
+                __builtin_amdgcn_s_sleep(1);
+            }
+            if (broken) break;  // nothing of this chunk is read
+            asm volatile("" ::: "memory");
+            const int fp = __builtin_amdgcn_readfirstlane(S.first), ns = __builtin_amdgcn_readfirstlane(S.nsteps);
+            const int2 sv = S.step[lane];
+            int sx = __builtin_amdgcn_readlane(sv.x, 0), sy = __builtin_amdgcn_readlane(sv.y, 0);
+            LevRec<T> r{};
+            T v = T(0);
+            if (lane < sy) {
+                r = S.rec[sx - fp + lane];
+                v = S.v[sx - fp + lane];
+            }
+            for (int s = 0; s < ns; ++s) {
+                T nb[kLevDeps];
+#pragma unroll
+                for (int q = 0; q < kLevDeps; ++q) nb[q] = res[q < r.m ? r.j[q] : 0];  // unused: any word in range
+                int nx = 0, ny = 0;
+                if (s + 1 < ns) {
+                    nx = __builtin_amdgcn_readlane(sv.x, s + 1);
+                    ny = __builtin_amdgcn_readlane(sv.y, s + 1);
+                }
+                LevRec<T> rn{};
+                T vn = T(0);
+                if (lane < ny) {
+                    rn = S.rec[nx - fp + lane];
+                    vn = S.v[nx - fp + lane];
+                }
+                if (lane < sy) {
+                    const T val = lev_eval<T, MODE>(r, v, nb);
+                    res[sx - base + lane] = val;
+                    sf_store(out + sx + lane, val);
+                    if (MODE == 0) dinv[r.i] = T(1) / val;
+                }
+                r = rn;
+                v = vn;
+                sx = nx;
+                sy = ny;
+            }
+            asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");  // the chunk's stage reads have returned
+            __hip_atomic_store(done + b, c + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        }
+    }
+    if (broken && lane == 0) __hip_atomic_store(fault, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
+// ---------------------------------------------------------------------------
 // Host side
 // ---------------------------------------------------------------------------
 static dim3 lev_grid(int n) { return dim3((unsigned)((n + 255) / 256)); }
@@ -410,11 +574,43 @@ static int launch_walk(int nlev, const int *lptr, const LevRec<T> *R, const T *i
     return FVH_OK;
 }
 
-// The sync-free walk unless FVH_LEVEL_SYNCFREE=0 (the one-work-group level
-// walk); both give the same bits.
-static bool lev_syncfree() {
-    const char *env = std::getenv("FVH_LEVEL_SYNCFREE");
-    return !(env && env[0] == '0' && env[1] == 0);
+// LDS bytes per work-group of the tile walk
+template <typename T>
+static size_t tile_lds(const Schedule &s, const Schedule::TileWalk &w) {
+    return tile_words_off<T>() + (size_t)(s.tile_rows + w.maxrem) * sizeof(T);
+}
+constexpr size_t kLdsMax = 160 * 1024;
+
+// The walk: the tile walk (2) unless FVH_LEVEL_TILES=0 or its tiles need more
+// LDS than a CU has (the sync-free walk, 1), or FVH_LEVEL_SYNCFREE=0 (the
+// one-work-group level walk, 0); all give the same bits.
+static bool env_off(const char *name) {
+    const char *env = std::getenv(name);
+    return env && env[0] == '0' && env[1] == 0;
+}
+static int lev_walk(const Schedule &s) {
+    if (env_off("FVH_LEVEL_SYNCFREE")) return 0;
+    if (env_off("FVH_LEVEL_TILES")) return 1;
+    const size_t lds = std::max(tile_lds<double>(s, s.tf), tile_lds<double>(s, s.tb));
+    return lds <= kLdsMax ? 2 : 1;
+}
+
+template <typename T, int MODE>
+static int launch_tiles(const Schedule &s, const Schedule::TileWalk &w, const LevRec<T> *R, const T *in, T *out,
+                        T *dinv, int *fault, const KScal *ks) {
+    const size_t lds = tile_lds<T>(s, w);
+    if (lds > kLdsMax) return set_error(FVH_ERR_UNSUPPORTED, "tile walk: %zu bytes of LDS per tile", lds);
+    static size_t allowed = 64 * 1024;
+    if (lds > allowed) {
+        HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void *>(k_lev_tile<T, MODE>),
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        allowed = lds;
+    }
+    hipLaunchKernelGGL((k_lev_tile<T, MODE>), dim3((unsigned)w.ntiles), dim3(64 * (1 + kTileStages)), lds, stream(),
+                       s.tile_rows, w.tiles.p, w.chunks.p, reinterpret_cast<const int2 *>(w.steps.p),
+                       reinterpret_cast<const int2 *>(w.ents.p), R, in, out, dinv, fault, ks);
+    HIP_TRY(hipGetLastError());
+    return FVH_OK;
 }
 
 static int lev_fault_ptr(fvh_eqn *e, int **dev) {
@@ -457,13 +653,25 @@ int lev_dilu_update(fvh_eqn *e, const double *val) {
     auto *U = reinterpret_cast<LevRec<double> *>(L.upd.p);
     auto *Fw = reinterpret_cast<LevRec<double> *>(L.fwd.p);
     auto *Bw = reinterpret_cast<LevRec<double> *>(L.bwd.p);
-    const bool sf = lev_syncfree();
-    if (sf && L.c.n != (size_t)n) FVH_TRY(L.c.alloc(n));
-    const int *codes = sf ? s.lpos.p : s.lcode.p;  // walk positions, or the level walk's ring codes
-    hipLaunchKernelGGL(k_lev_pack, lev_grid(n), dim3(256), 0, stream(), n, s.frows.p, m->d.rowptr, m->d.col, m->d.diag,
+    const int walk = lev_walk(s);
+    if (walk > 0 && L.c.n != (size_t)n) FVH_TRY(L.c.alloc(n));
+    // where each walk finds a neighbour: the tile walk's LDS words, the
+    // sync-free walk's positions, or the level walk's ring codes
+    const int *codes = walk == 2 ? s.tcode.p : walk == 1 ? s.lpos.p : s.lcode.p;
+    const int *frows = walk == 2 ? s.tf.rows.p : s.frows.p, *brows = walk == 2 ? s.tb.rows.p : s.brows.p;
+    hipLaunchKernelGGL(k_lev_pack, lev_grid(n), dim3(256), 0, stream(), n, frows, m->d.rowptr, m->d.col, m->d.diag,
                        m->d.tslot, codes, val, (const double *)nullptr, 0, U);
     HIP_TRY(hipGetLastError());
-    if (sf) {
+    if (walk == 2) {
+        int *fault = nullptr;
+        FVH_TRY(lev_fault_ptr(e, &fault));
+        hipLaunchKernelGGL((k_lev_sf_prep<double>), lev_grid(n), dim3(256), 0, stream(), n, frows,
+                           (const double *)nullptr, (double *)nullptr, L.b.p, (double *)nullptr,
+                           (const KScal *)nullptr);
+        HIP_TRY(hipGetLastError());
+        FVH_TRY((launch_tiles<double, 0>(s, s.tf, U, (const double *)nullptr, L.b.p, e->Dinv.p, fault,
+                                         (const KScal *)nullptr)));
+    } else if (walk == 1) {
         int *fault = nullptr;
         FVH_TRY(lev_fault_ptr(e, &fault));
         hipLaunchKernelGGL((k_lev_sf_prep<double>), lev_grid(n), dim3(256), 0, stream(), n, s.frows.p,
@@ -479,13 +687,13 @@ int lev_dilu_update(fvh_eqn *e, const double *val) {
         FVH_TRY((launch_walk<double, 0>(s.nlev, s.fptr.p, U, (const double *)nullptr, L.b.p, e->Dinv.p,
                                         (const KScal *)nullptr, s.ring)));
     }
-    hipLaunchKernelGGL(k_lev_pack, lev_grid(n), dim3(256), 0, stream(), n, s.frows.p, m->d.rowptr, m->d.col, m->d.diag,
+    hipLaunchKernelGGL(k_lev_pack, lev_grid(n), dim3(256), 0, stream(), n, frows, m->d.rowptr, m->d.col, m->d.diag,
                        m->d.tslot, codes, val, e->Dinv.p, 1, Fw);
     HIP_TRY(hipGetLastError());
-    hipLaunchKernelGGL(k_lev_pack, lev_grid(n), dim3(256), 0, stream(), n, s.brows.p, m->d.rowptr, m->d.col, m->d.diag,
+    hipLaunchKernelGGL(k_lev_pack, lev_grid(n), dim3(256), 0, stream(), n, brows, m->d.rowptr, m->d.col, m->d.diag,
                        m->d.tslot, codes, val, e->Dinv.p, 2, Bw);
     HIP_TRY(hipGetLastError());
-    L.sf = sf;
+    L.walk = walk;
     return FVH_OK;
 }
 
@@ -496,7 +704,20 @@ static int lev_apply_t(fvh_eqn *e, const LevRec<T> *Fw, const LevRec<T> *Bw, con
                        T *wc, const KScal *ks) {
     const Schedule &s = e->m->sched;
     const int n = (int)e->m->n;
-    if (e->lv.sf) {  // records hold walk positions: the sync-free walk
+    if (e->lv.walk == 2) {  // records hold LDS words: the tile walk
+        int *fault = nullptr;
+        FVH_TRY(lev_fault_ptr(e, &fault));
+        hipLaunchKernelGGL((k_lev_sf_prep<T>), lev_grid(n), dim3(256), 0, stream(), n, s.tf.rows.p, x, wa, wb, wc, ks);
+        HIP_TRY(hipGetLastError());
+        FVH_TRY((launch_tiles<T, 1>(s, s.tf, Fw, wa, wb, (T *)nullptr, fault, ks)));
+        hipLaunchKernelGGL((k_lev_perm<T, false>), lev_grid(n), dim3(256), 0, stream(), n, s.tf2b.p, wb, wa, ks);
+        HIP_TRY(hipGetLastError());
+        FVH_TRY((launch_tiles<T, 2>(s, s.tb, Bw, wa, wc, (T *)nullptr, fault, ks)));
+        hipLaunchKernelGGL((k_lev_perm<T, true>), lev_grid(n), dim3(256), 0, stream(), n, s.tb.rows.p, wc, y, ks);
+        HIP_TRY(hipGetLastError());
+        return FVH_OK;
+    }
+    if (e->lv.walk == 1) {  // records hold walk positions: the sync-free walk
         int *fault = nullptr;
         FVH_TRY(lev_fault_ptr(e, &fault));
         hipLaunchKernelGGL((k_lev_sf_prep<T>), lev_grid(n), dim3(256), 0, stream(), n, s.frows.p, x, wa, wb, wc, ks);

@@ -8,6 +8,7 @@
 
 #include "fvh_device.h"
 #include "k_band.h"
+#include "tile_sched.h"
 
 namespace fvh {
 
@@ -150,6 +151,22 @@ struct V3 {
 };
 inline double vnorm(V3 a) { return std::sqrt((a.x * a.x + a.y * a.y) + a.z * a.z); }
 
+// One direction of the tile walk on the device (the schedule: tile_sched.h).
+int tile_walk(int n, const std::vector<int> &rp, const std::vector<int> &col, const std::vector<int> &dg, bool upper,
+              int tr, int lag, Schedule::TileWalk &tw, std::vector<int> &pos, std::vector<int> &code) {
+    TileSchedule ts;
+    tile_schedule(n, rp, col, dg, upper, tr, lag, ts, code);
+    tw.ntiles = ts.ntiles;
+    tw.maxrem = ts.maxrem;
+    FVH_TRY(tw.rows.upload(ts.rows));
+    FVH_TRY(tw.tiles.upload(ts.tiles));
+    FVH_TRY(tw.chunks.upload(ts.chunks));
+    FVH_TRY(tw.steps.upload(ts.steps));
+    FVH_TRY(tw.ents.upload(ts.ents));
+    pos.swap(ts.pos);
+    return FVH_OK;
+}
+
 int build_schedule(fvh_mesh *m) {
     const int n = (int)m->n;
     const auto &rp = m->h_rowptr, &col = m->h_col, &dg = m->h_diag;
@@ -239,6 +256,27 @@ int build_schedule(fvh_mesh *m) {
         FVH_TRY(m->sched.swf.upload(tab));
         m->sched.nswb = waves(m->sched.h_bptr, tab);
         FVH_TRY(m->sched.swb.upload(tab));
+    }
+    {  // the tile walk (FVH_TILE_ROWS: rows per tile, a power of two 64 .. 8192; FVH_TILE_LAG: steps)
+        int tr = kTileRows, lag = kTileLag;
+        if (const char *env = std::getenv("FVH_TILE_ROWS")) {
+            const int r = std::atoi(env);
+            if (r >= 64 && r <= 8192 && (r & (r - 1)) == 0) tr = r;
+        }
+        if (const char *env = std::getenv("FVH_TILE_LAG")) {
+            const int r = std::atoi(env);
+            if (r >= 1 && r <= 1000) lag = r;
+        }
+        Schedule &s = m->sched;
+        s.tile_rows = tr;
+        std::vector<int> code(rp[n], 0), posf, posb, f2b(n);
+        FVH_TRY(tile_walk(n, rp, col, dg, false, tr, lag, s.tf, posf, code));
+        FVH_TRY(tile_walk(n, rp, col, dg, true, tr, lag, s.tb, posb, code));
+        std::vector<int> brows(n);
+        for (int i = 0; i < n; ++i) brows[posb[i]] = i;
+        for (int k = 0; k < n; ++k) f2b[k] = posf[brows[k]];
+        FVH_TRY(s.tcode.upload(code));
+        FVH_TRY(s.tf2b.upload(f2b));
     }
     int deg = 0;  // most lower / upper neighbours of a row
     for (int i = 0; i < n; ++i) deg = std::max(deg, std::max(dg[i] - rp[i], rp[i + 1] - dg[i] - 1));
