@@ -372,8 +372,11 @@ __global__ void k_lev_sf_prep(int n, const int *__restrict__ rows, const T *__re
 // the chunk's neighbours from other tiles until none is the sentinel, writes
 // them to their LDS words and raises the buffer's ready word; the compute
 // wave raises the buffer's done word when it has finished the chunk, and the
-// loader then fills it with its next chunk.  Only loaders wait for global
-// memory, so the compute wave never waits for a load or a store.  A tile
+// loader then fills it with its next chunk.  The ready word carries the
+// chunk's step count, so the compute wave takes up a chunk with one LDS
+// round trip (ready word, step table and first records together).  Only
+// loaders wait for global memory, so the compute wave never waits for a load
+// or a store.  A tile
 // waits only for tiles before it, whose work-groups the in-order dispatch
 // has started, so the sweep always drains; every wait is bounded, and one
 // that runs out sets the fault word and ends the wave (a loader or compute
@@ -381,23 +384,64 @@ __global__ void k_lev_sf_prep(int n, const int *__restrict__ rows, const T *__re
 // ---------------------------------------------------------------------------
 constexpr int kTileStages = 3;  // loader waves and stage buffers per work-group
 constexpr int kTileSpin = 1 << 22;  // polls of an LDS word (about half a second) before a wait is declared lost
+// ready word of a stage buffer: (chunk + 1) | (steps << kTileStepShift)
+constexpr int kTileStepShift = 20, kTileChunkMask = (1 << kTileStepShift) - 1;
 
 typedef unsigned int TileWord __attribute__((ext_vector_type(4)));  // 16 bytes of a record
 
+// lev_eval for rows of at most D neighbours, with the test for an absent
+// neighbour on the term rather than on the running sum (off the chain of
+// dependent operations): an absent term is +0, and x - (+0) = x and
+// fma(-(+0), +0, x) = x + (-0) = x exactly for every x (signed zeros, NaN
+// and infinities included), so the results are lev_eval's bits.
+template <typename T, int MODE, int D>
+__device__ inline T tile_eval(const LevRec<T> &r, T in, const T (&nb)[kLevDeps]) {
+    if (MODE == 0) {
+        T d = r.d[0];
+#pragma unroll
+        for (int q = 0; q < D; ++q) d = d - (q < r.m ? r.a[q] / nb[q] : T(0));
+        return d;
+    } else if (MODE == 1) {
+        T s = in;
+        if constexpr (sizeof(T) == 4) {
+#pragma unroll
+            for (int q = D - 1; q >= 0; --q)
+                s = __builtin_fmaf(-(q < r.m ? r.a[q] : 0.0f), q < r.m ? nb[q] : 0.0f, s);
+        } else {
+#pragma unroll
+            for (int q = 0; q < D; ++q) s = s - (q < r.m ? r.a[q] * nb[q] * r.d[q] : T(0));
+        }
+        return s;
+    } else {
+        const T di = r.d[0];
+        T s = in * di;
+#pragma unroll
+        for (int q = 0; q < D; ++q) {
+            if constexpr (sizeof(T) == 4) s = __builtin_fmaf(-(q < r.m ? r.a[q] : 0.0f), q < r.m ? nb[q] : 0.0f, s);
+            else s = s - (q < r.m ? r.a[q] * nb[q] * di : T(0));
+        }
+        return s;
+    }
+}
+
 template <typename T>
 struct alignas(16) TileStage {  // one chunk in LDS
-    LevRec<T> rec[64];  // records of the chunk's rows, by position from the chunk's first
-    T v[64];            // their inputs
+    LevRec<T> rec[64];  // records of the chunk's rows, by position from the chunk's first, the
+                        // row's input in pad; every slot holds a record (m = 0 past the chunk)
     int2 step[64];      // the chunk's steps: first position, rows
-    int first, nsteps, pad[2];
 };
+template <typename T>
+__device__ inline T tile_input(const LevRec<T> &r) {
+    if constexpr (sizeof(T) == 8) return __builtin_bit_cast(T, make_int2(r.pad[0], r.pad[1]));
+    else return __builtin_bit_cast(T, r.pad[0]);
+}
 
 template <typename T>
 __host__ __device__ constexpr size_t tile_words_off() {  // byte offset of the tile's result words
     return kTileStages * sizeof(TileStage<T>) + 16 * ((2 * kTileStages * sizeof(int) + 15) / 16);
 }
 
-template <typename T, int MODE>
+template <typename T, int MODE, int D>
 __global__ void __launch_bounds__(64 * (1 + kTileStages))
     k_lev_tile(int tr, const int *__restrict__ tiles, const int *__restrict__ chunks, const int2 *__restrict__ steps,
                const int2 *__restrict__ ents, const LevRec<T> *__restrict__ R, const T *__restrict__ in, T *out,
@@ -415,19 +459,41 @@ __global__ void __launch_bounds__(64 * (1 + kTileStages))
     if (wave > 0) {  // loader of chunks wave-1, wave-1 + kTileStages, ...
         const int b = wave - 1;
         TileStage<T> &S = stage[b];
-        for (int c = b; c < nc; c += kTileStages) {
-            const int *C = chunks + (size_t)kTileChunkInts * (c0 + c);
-            const int fs = C[0], ns = C[1], fe = C[2], ne = C[3], fp = C[4], nr = C[5];
+        int4 ca = make_int4(0, 0, 0, 0);  // lane k: the loader's k-th chunk record (of the current 64)
+        int2 cb = make_int2(0, 0);
+        for (int c = b, k = 0; c < nc; c += kTileStages, k = (k + 1) & 63) {
+            if (k == 0) {  // the records of this loader's next 64 chunks, one per lane
+                const int cl = c + lane * kTileStages;
+                if (cl < nc) {
+                    const int *C = chunks + (size_t)kTileChunkInts * (c0 + cl);
+                    ca = *reinterpret_cast<const int4 *>(C);
+                    cb = *reinterpret_cast<const int2 *>(C + 4);
+                }
+            }
+            const int fs = __builtin_amdgcn_readlane(ca.x, k), ns = __builtin_amdgcn_readlane(ca.y, k),
+                      fe = __builtin_amdgcn_readlane(ca.z, k), ne = __builtin_amdgcn_readlane(ca.w, k),
+                      fp = __builtin_amdgcn_readlane(cb.x, k), nr = __builtin_amdgcn_readlane(cb.y, k);
             constexpr int RW = sizeof(LevRec<T>) / 16;  // the record as 16-byte words (no struct copy in memory)
             int2 st = make_int2(0, 0), e = make_int2(0, 0);
             TileWord r[RW];
-            T v = T(0), x = T(0);
+            T x = T(0);
+#pragma unroll
+            for (int w = 0; w < RW; ++w) r[w] = TileWord{0u, 0u, 0u, 0u};  // m = 0, words 0
             if (lane < ns) st = steps[fs + lane];
             if (lane < nr) {
                 const TileWord *src = reinterpret_cast<const TileWord *>(R + fp + lane);
 #pragma unroll
                 for (int w = 0; w < RW; ++w) r[w] = src[w];
-                if (MODE != 0) v = in[fp + lane];
+                if (MODE != 0) {  // the input into the record's pad words (bytes 24 .. 31)
+                    const T v = in[fp + lane];
+                    if constexpr (sizeof(T) == 8) {
+                        const int2 b = __builtin_bit_cast(int2, v);
+                        r[1].z = (unsigned)b.x;
+                        r[1].w = (unsigned)b.y;
+                    } else {
+                        r[1].z = __builtin_bit_cast(unsigned, v);
+                    }
+                }
             }
             if (lane < ne) {
                 e = ents[fe + lane];
@@ -450,70 +516,87 @@ __global__ void __launch_bounds__(64 * (1 + kTileStages))
             }
             if (broken) break;
             asm volatile("" ::: "memory");
-            if (lane < nr) {
+            {
                 TileWord *dst = reinterpret_cast<TileWord *>(&S.rec[lane]);
 #pragma unroll
                 for (int w = 0; w < RW; ++w) dst[w] = r[w];
-                S.v[lane] = v;
             }
             if (lane < ns) S.step[lane] = st;
-            if (lane == 0) {
-                S.first = fp;
-                S.nsteps = ns;
-            }
             if (lane < ne) res[e.y] = x;
             asm volatile("" ::: "memory");  // the stage is written (LDS keeps a wave's order) before the ready word
-            __hip_atomic_store(ready + b, c + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            __hip_atomic_store(ready + b, (c + 1) | (ns << kTileStepShift), __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_WORKGROUP);
         }
     } else {  // compute wave
         const int base = t * tr;
         for (int c = 0; c < nc; ++c) {
             const int b = c % kTileStages;
             const TileStage<T> &S = stage[b];
-            for (int s = 0;
-                 !broken && __hip_atomic_load(ready + b, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) != c + 1;
-                 ++s) {
-                if (s >= kTileSpin) broken = 1;
+            // the ready word, the step table and the first step's records (its
+            // rows are slots 0 .. rows-1) in one LDS round trip: LDS serves the
+            // wave's reads in order, so once the ready word shows the chunk, the
+            // reads after it see what the loader wrote before it
+            constexpr int RW = sizeof(LevRec<T>) / 16;
+            int word = 0;
+            int2 sv;
+            TileWord rw[RW];
+            for (int s = 0;; ++s) {
+                word = __hip_atomic_load(ready + b, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                asm volatile("" ::: "memory");
+                sv = S.step[lane];
+                const TileWord *src = reinterpret_cast<const TileWord *>(&S.rec[lane]);
+#pragma unroll
+                for (int w = 0; w < RW; ++w) rw[w] = src[w];
+                // the values are taken here, so the three reads go out together
+                asm volatile("" : "+v"(word), "+v"(sv.x), "+v"(sv.y));
+#pragma unroll
+                for (int w = 0; w < RW; ++w) asm volatile("" : "+v"(rw[w]));
+                word = __builtin_amdgcn_readfirstlane(word);
+                if ((word & kTileChunkMask) == c + 1) break;
+                if (s >= kTileSpin) {
+                    broken = 1;
+                    break;
+                }
                 __builtin_amdgcn_s_sleep(1);
             }
-            if (broken) break;  // nothing of this chunk is read
-            asm volatile("" ::: "memory");
-            const int fp = __builtin_amdgcn_readfirstlane(S.first), ns = __builtin_amdgcn_readfirstlane(S.nsteps);
-            const int2 sv = S.step[lane];
-            int sx = __builtin_amdgcn_readlane(sv.x, 0), sy = __builtin_amdgcn_readlane(sv.y, 0);
-            LevRec<T> r{};
-            T v = T(0);
-            if (lane < sy) {
-                r = S.rec[sx - fp + lane];
-                v = S.v[sx - fp + lane];
+            if (broken) break;  // nothing of this chunk is used
+            LevRec<T> r;
+            {
+                TileWord *dst = reinterpret_cast<TileWord *>(&r);
+#pragma unroll
+                for (int w = 0; w < RW; ++w) dst[w] = rw[w];
             }
-            for (int s = 0; s < ns; ++s) {
+            const int ns = word >> kTileStepShift, fp = __builtin_amdgcn_readlane(sv.x, 0);
+            int sx = fp, sy = __builtin_amdgcn_readlane(sv.y, 0);
+            // one step: its rows' neighbours from LDS, the next step's records
+            // read ahead (lanes past a step's rows read some record of the chunk:
+            // every slot holds one, its words in range; they store nothing)
+            auto step = [&](int s, const LevRec<T> &cur, LevRec<T> &nxt) {
                 T nb[kLevDeps];
 #pragma unroll
-                for (int q = 0; q < kLevDeps; ++q) nb[q] = res[q < r.m ? r.j[q] : 0];  // unused: any word in range
+                for (int q = 0; q < D; ++q) nb[q] = res[q < cur.m ? cur.j[q] : 0];  // absent: any word in range
                 int nx = 0, ny = 0;
                 if (s + 1 < ns) {
                     nx = __builtin_amdgcn_readlane(sv.x, s + 1);
                     ny = __builtin_amdgcn_readlane(sv.y, s + 1);
                 }
-                LevRec<T> rn{};
-                T vn = T(0);
-                if (lane < ny) {
-                    rn = S.rec[nx - fp + lane];
-                    vn = S.v[nx - fp + lane];
-                }
+                nxt = S.rec[(nx - fp + lane) & 63];
                 if (lane < sy) {
-                    const T val = lev_eval<T, MODE>(r, v, nb);
+                    const T val = tile_eval<T, MODE, D>(cur, MODE != 0 ? tile_input(cur) : T(0), nb);
                     res[sx - base + lane] = val;
                     sf_store(out + sx + lane, val);
-                    if (MODE == 0) dinv[r.i] = T(1) / val;
+                    if (MODE == 0) dinv[cur.i] = T(1) / val;
                 }
-                r = rn;
-                v = vn;
                 sx = nx;
                 sy = ny;
+            };
+            LevRec<T> r2;
+            for (int s = 0; s < ns; s += 2) {  // two register sets take turns
+                step(s, r, r2);
+                if (s + 1 >= ns) break;
+                step(s + 1, r2, r);
             }
-            asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");  // the chunk's stage reads have returned
+            asm volatile("" ::: "memory");  // LDS serves the stage reads before this write
             __hip_atomic_store(done + b, c + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
         }
     }
@@ -595,22 +678,29 @@ static int lev_walk(const Schedule &s) {
     return lds <= kLdsMax ? 2 : 1;
 }
 
-template <typename T, int MODE>
-static int launch_tiles(const Schedule &s, const Schedule::TileWalk &w, const LevRec<T> *R, const T *in, T *out,
-                        T *dinv, int *fault, const KScal *ks) {
+template <typename T, int MODE, int D>
+static int launch_tiles_d(const Schedule &s, const Schedule::TileWalk &w, const LevRec<T> *R, const T *in, T *out,
+                          T *dinv, int *fault, const KScal *ks) {
     const size_t lds = tile_lds<T>(s, w);
     if (lds > kLdsMax) return set_error(FVH_ERR_UNSUPPORTED, "tile walk: %zu bytes of LDS per tile", lds);
     static size_t allowed = 64 * 1024;
     if (lds > allowed) {
-        HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void *>(k_lev_tile<T, MODE>),
+        HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void *>(k_lev_tile<T, MODE, D>),
                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
         allowed = lds;
     }
-    hipLaunchKernelGGL((k_lev_tile<T, MODE>), dim3((unsigned)w.ntiles), dim3(64 * (1 + kTileStages)), lds, stream(),
-                       s.tile_rows, w.tiles.p, w.chunks.p, reinterpret_cast<const int2 *>(w.steps.p),
+    hipLaunchKernelGGL((k_lev_tile<T, MODE, D>), dim3((unsigned)w.ntiles), dim3(64 * (1 + kTileStages)), lds,
+                       stream(), s.tile_rows, w.tiles.p, w.chunks.p, reinterpret_cast<const int2 *>(w.steps.p),
                        reinterpret_cast<const int2 *>(w.ents.p), R, in, out, dinv, fault, ks);
     HIP_TRY(hipGetLastError());
     return FVH_OK;
+}
+// rows of at most three neighbours (triangles) or kLevDeps
+template <typename T, int MODE>
+static int launch_tiles(const Schedule &s, const Schedule::TileWalk &w, const LevRec<T> *R, const T *in, T *out,
+                        T *dinv, int *fault, const KScal *ks) {
+    if (s.max_deps <= 3) return launch_tiles_d<T, MODE, 3>(s, w, R, in, out, dinv, fault, ks);
+    return launch_tiles_d<T, MODE, kLevDeps>(s, w, R, in, out, dinv, fault, ks);
 }
 
 static int lev_fault_ptr(fvh_eqn *e, int **dev) {
