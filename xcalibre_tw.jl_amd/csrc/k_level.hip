@@ -426,10 +426,22 @@ __device__ inline T tile_eval(const LevRec<T> &r, T in, const T (&nb)[kLevDeps])
 
 template <typename T>
 struct alignas(16) TileStage {  // one chunk in LDS
-    LevRec<T> rec[64];  // records of the chunk's rows, by position from the chunk's first, the
-                        // row's input in pad; every slot holds a record (m = 0 past the chunk)
-    int2 step[64];      // the chunk's steps: first position, rows
+    // records of the chunk's rows, by position from the chunk's first, the
+    // row's input in pad; every slot holds a record (m = 0 past the chunk).
+    // Word-major (16-byte word w of slot k at rec[w][k]), so that a wave's
+    // read of one word of 64 consecutive slots is free of bank conflicts.
+    TileWord rec[sizeof(LevRec<T>) / 16][64];
+    int2 step[64];  // the chunk's steps: first position, rows
 };
+// slot k of a stage into a record in registers
+template <typename T>
+__device__ inline LevRec<T> stage_rec(const TileStage<T> &S, int k) {
+    constexpr int RW = sizeof(LevRec<T>) / 16;
+    TileWord w[RW];
+#pragma unroll
+    for (int q = 0; q < RW; ++q) w[q] = S.rec[q][k];
+    return __builtin_bit_cast(LevRec<T>, w);
+}
 template <typename T>
 __device__ inline T tile_input(const LevRec<T> &r) {
     if constexpr (sizeof(T) == 8) return __builtin_bit_cast(T, make_int2(r.pad[0], r.pad[1]));
@@ -484,6 +496,13 @@ __global__ void __launch_bounds__(64 * (1 + kTileStages))
                 const TileWord *src = reinterpret_cast<const TileWord *>(R + fp + lane);
 #pragma unroll
                 for (int w = 0; w < RW; ++w) r[w] = src[w];
+                // neighbour words as byte offsets into the LDS words (absent: 0),
+                // so the compute wave reads them with no address arithmetic
+                const int m = (int)r[0].y;
+                r[0].z = m > 0 ? r[0].z * (unsigned)sizeof(T) : 0u;
+                r[0].w = m > 1 ? r[0].w * (unsigned)sizeof(T) : 0u;
+                r[1].x = m > 2 ? r[1].x * (unsigned)sizeof(T) : 0u;
+                r[1].y = m > 3 ? r[1].y * (unsigned)sizeof(T) : 0u;
                 if (MODE != 0) {  // the input into the record's pad words (bytes 24 .. 31)
                     const T v = in[fp + lane];
                     if constexpr (sizeof(T) == 8) {
@@ -516,11 +535,8 @@ __global__ void __launch_bounds__(64 * (1 + kTileStages))
             }
             if (broken) break;
             asm volatile("" ::: "memory");
-            {
-                TileWord *dst = reinterpret_cast<TileWord *>(&S.rec[lane]);
 #pragma unroll
-                for (int w = 0; w < RW; ++w) dst[w] = r[w];
-            }
+            for (int w = 0; w < RW; ++w) S.rec[w][lane] = r[w];
             if (lane < ns) S.step[lane] = st;
             if (lane < ne) res[e.y] = x;
             asm volatile("" ::: "memory");  // the stage is written (LDS keeps a wave's order) before the ready word
@@ -544,9 +560,8 @@ __global__ void __launch_bounds__(64 * (1 + kTileStages))
                 word = __hip_atomic_load(ready + b, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
                 asm volatile("" ::: "memory");
                 sv = S.step[lane];
-                const TileWord *src = reinterpret_cast<const TileWord *>(&S.rec[lane]);
 #pragma unroll
-                for (int w = 0; w < RW; ++w) rw[w] = src[w];
+                for (int w = 0; w < RW; ++w) rw[w] = S.rec[w][lane];
                 // the values are taken here, so the three reads go out together
                 asm volatile("" : "+v"(word), "+v"(sv.x), "+v"(sv.y));
 #pragma unroll
@@ -560,12 +575,7 @@ __global__ void __launch_bounds__(64 * (1 + kTileStages))
                 __builtin_amdgcn_s_sleep(1);
             }
             if (broken) break;  // nothing of this chunk is used
-            LevRec<T> r;
-            {
-                TileWord *dst = reinterpret_cast<TileWord *>(&r);
-#pragma unroll
-                for (int w = 0; w < RW; ++w) dst[w] = rw[w];
-            }
+            LevRec<T> r = __builtin_bit_cast(LevRec<T>, rw);
             const int ns = word >> kTileStepShift, fp = __builtin_amdgcn_readlane(sv.x, 0);
             int sx = fp, sy = __builtin_amdgcn_readlane(sv.y, 0);
             // one step: its rows' neighbours from LDS, the next step's records
@@ -574,13 +584,14 @@ __global__ void __launch_bounds__(64 * (1 + kTileStages))
             auto step = [&](int s, const LevRec<T> &cur, LevRec<T> &nxt) {
                 T nb[kLevDeps];
 #pragma unroll
-                for (int q = 0; q < D; ++q) nb[q] = res[q < cur.m ? cur.j[q] : 0];  // absent: any word in range
+                for (int q = 0; q < D; ++q)  // byte offsets (absent: word 0)
+                    nb[q] = *reinterpret_cast<const T *>(reinterpret_cast<const char *>(res) + cur.j[q]);
                 int nx = 0, ny = 0;
                 if (s + 1 < ns) {
                     nx = __builtin_amdgcn_readlane(sv.x, s + 1);
                     ny = __builtin_amdgcn_readlane(sv.y, s + 1);
                 }
-                nxt = S.rec[(nx - fp + lane) & 63];
+                nxt = stage_rec(S, (nx - fp + lane) & 63);
                 if (lane < sy) {
                     const T val = tile_eval<T, MODE, D>(cur, MODE != 0 ? tile_input(cur) : T(0), nb);
                     res[sx - base + lane] = val;
