@@ -12,7 +12,8 @@
 // rows of waves dispatched before it, so the solve always drains; each poll
 // is bounded, and a wait that runs out sets a fault word (pinned host
 // memory) that the caller reports.  Per row: lane l sums its entries l,
-// l + 64, ... in order from 0, then the lanes are added by a fixed tree.
+// l + 64, ... in order from 0, then the lanes are added by a fixed tree; a
+// lane issues the loads of up to kLdlBatch inputs before it waits for any.
 #include <algorithm>
 #include <vector>
 
@@ -26,6 +27,7 @@ namespace {
 constexpr unsigned long long kLdlSent = 0x7FF4DEAD5EA11A57ull;
 constexpr int kLdlSpin = 1 << 20;  // polls (a few 100 ms) before a wait is declared lost
 constexpr int kLdlWaves = 4;       // waves (rows) per work-group
+constexpr int kLdlBatch = 16;      // inputs per lane loaded before any is waited for
 
 __device__ inline double ldl_load(const double *p) {
     return __builtin_bit_cast(double, __hip_atomic_load(reinterpret_cast<const unsigned long long *>(p),
@@ -82,8 +84,40 @@ __global__ void __launch_bounds__(64 * kLdlWaves)
     if (w >= n) return;
     const int lane = threadIdx.x & 63;
     const int r = BWD ? n - 1 - w : w;
+    const int p1 = ptr[r + 1];
     double acc = 0.0;
-    for (int p = ptr[r] + lane; p < ptr[r + 1]; p += 64) acc = acc + val[p] * ldl_wait(src, idx[p], fault);
+    // blocks of 64 * kLdlBatch entries: a lane loads all its inputs of a block
+    // at once and then polls only those not yet published, so a row waits
+    // about one load latency per block rather than one per entry
+    for (int pb = ptr[r] + lane; pb < p1; pb += 64 * kLdlBatch) {
+        double a[kLdlBatch], v[kLdlBatch];
+        int c[kLdlBatch];
+#pragma unroll
+        for (int k = 0; k < kLdlBatch; ++k) {
+            const int p = pb + 64 * k;
+            c[k] = p < p1 ? idx[p] : 0;
+            a[k] = p < p1 ? val[p] : 0.0;
+        }
+#pragma unroll
+        for (int k = 0; k < kLdlBatch; ++k) v[k] = pb + 64 * k < p1 ? ldl_load(src + c[k]) : 0.0;
+        for (int s = 0;; ++s) {
+            bool missing = false;
+#pragma unroll
+            for (int k = 0; k < kLdlBatch; ++k) missing |= ldl_sent(v[k]);
+            if (!missing) break;
+            if (s >= kLdlSpin) {
+                __hip_atomic_store(fault, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+                break;
+            }
+            __builtin_amdgcn_s_sleep(1);
+#pragma unroll
+            for (int k = 0; k < kLdlBatch; ++k)
+                if (ldl_sent(v[k])) v[k] = ldl_load(src + c[k]);
+        }
+#pragma unroll
+        for (int k = 0; k < kLdlBatch; ++k)
+            if (pb + 64 * k < p1) acc = acc + a[k] * v[k];  // the lane's entries in order
+    }
     const double s = wave_sum(acc);
     if (lane == 0) {
         const double base = BWD ? ldl_wait(rhs, r, fault) / D[r] : rhs[r];
