@@ -43,8 +43,13 @@ def _pressure_matrix(mesh, gamma, dm):
     return eq, g, vals
 
 
+@pytest.mark.parametrize("solves", ["ranges", "rows"])
 @pytest.mark.parametrize("name", REFERENCE_MESHES)
-def test_ldl_apply_against_the_oracle(ref_mesh, name):
+def test_ldl_apply_against_the_oracle(ref_mesh, name, solves, monkeypatch):
+    """Both triangular-solve schedules: the range walk over the
+    nested-dissection leaves and separators (the default) and one wave per
+    row (FVH_LDL_RANGES=0)."""
+    monkeypatch.setenv("FVH_LDL_RANGES", "1" if solves == "ranges" else "0")
     mesh = ref_mesh(name)
     dm = mesh.device()
     n = mesh.ncells
@@ -62,7 +67,7 @@ def test_ldl_apply_against_the_oracle(ref_mesh, name):
             want = ref.precond_apply_csr(mesh, vals, L.PRECOND_LDL, x)
             err = np.linalg.norm(y - want) / np.linalg.norm(want)
             res = np.linalg.norm(ref.spmv_csr(mesh, vals, y) - x) / np.linalg.norm(x)
-            print(f"{name}: apply {t}: rel diff to the oracle {err:.2e}, residual {res:.2e}", flush=True)
+            print(f"{name} ({solves}): apply {t}: rel diff to the oracle {err:.2e}, residual {res:.2e}", flush=True)
             assert err <= 1e-10 and res <= 1e-12
         # the factor is kept: a changed matrix and another update leave M alone
         L.check(LIB().fvh_precond_update(eq), "fvh_precond_update")

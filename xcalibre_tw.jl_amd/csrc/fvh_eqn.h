@@ -148,6 +148,8 @@ struct LdlStore {
     bool ready = false;
     int nnzL = 0;
     DBuf<int> perm, fptr, fidx, bptr, bidx;
+    DBuf<int> ranges;  // bounds of the nested-dissection leaves and separators (at most kLdlRangeMax rows each)
+    int nranges = 0;
     DBuf<double> fval, bval, D, rhs, y, z;
     int *h_fault = nullptr;
     LdlStore() = default;

@@ -3,18 +3,22 @@
 // update as the reference freezes opLDL (preconditioners_0_types.jl:36-42,
 // preconditioners_2_functions.jl:52-57).
 //
-// The two triangular solves are sync-free: one wave per row of L (forward,
-// rows in factor order) or of L^T (backward, rows in reverse order), the
-// lanes splitting the row's entries.  A row's inputs are earlier rows'
-// results, which their waves publish with device-coherent stores into an
-// array filled with a signalling-NaN sentinel before the solve; a lane polls
-// a value until it is no longer the sentinel.  Every wave waits only for
-// rows of waves dispatched before it, so the solve always drains; each poll
-// is bounded, and a wait that runs out sets a fault word (pinned host
-// memory) that the caller reports.  Per row: lane l sums its entries l,
-// l + 64, ... in order from 0, then the lanes are added by a fixed tree; a
-// lane issues the loads of up to kLdlBatch inputs before it waits for any.
+// The two triangular solves walk the ranges of the nested-dissection order
+// (k_ldl_range, below).  FVH_LDL_RANGES=0 keeps the earlier sync-free
+// solves: one wave per row of L (forward, rows in factor order) or of L^T
+// (backward, rows in reverse order), the lanes splitting the row's entries.
+// In both, a row's inputs from elsewhere are results that other work-groups
+// publish with device-coherent stores into an array filled with a
+// signalling-NaN sentinel before the solve; a lane polls a value until it is
+// no longer the sentinel.  Every work-group waits only for rows of
+// work-groups dispatched before it, so the solve always drains; each poll is
+// bounded, and a wait that runs out sets a fault word (pinned host memory)
+// that the caller reports.  Per row (sync-free solves): lane l sums its
+// entries l, l + 64, ... in order from 0, then the lanes are added by a
+// fixed tree; a lane issues the loads of up to kLdlBatch inputs before it
+// waits for any.
 #include <algorithm>
+#include <cstdlib>
 #include <vector>
 
 #include "fvh_eqn.h"
@@ -125,6 +129,86 @@ __global__ void __launch_bounds__(64 * kLdlWaves)
     }
 }
 
+// Range walk (the default): one work-group per range of the factor order
+// (a nested-dissection leaf, or a separator, cut at kLdlRangeMax rows),
+// ranges in solve order.  The work-group takes its rows one after another,
+// its threads splitting each row's entries: an input from the range itself
+// comes from the range's results in LDS, one from an earlier range from the
+// global result array (device-coherent loads polled until published, with
+// up to kLdlRangeBatch loads in flight per thread).  Per row: thread t sums
+// its entries t, t + 256, ... in order from 0, the lanes of each wave are
+// added by a fixed tree and the four waves in order; every thread then
+// forms the result and writes it to the row's LDS word (each wave reads only
+// words it wrote itself), thread 0 also publishes it.  One barrier per row
+// (the wave sums in two alternating slots).  A separator's dependent rows
+// thus follow each other at LDS and barrier latency instead of a
+// publish-to-poll hop between CUs.
+constexpr int kLdlRangeMax = 4096;   // rows per range (its results in LDS)
+constexpr int kLdlRangeThreads = 256;
+constexpr int kLdlRangeBatch = 8;
+
+template <bool BWD>
+__global__ void __launch_bounds__(kLdlRangeThreads)
+    k_ldl_range(int nranges, const int *__restrict__ bounds, const int *__restrict__ ptr,
+                const int *__restrict__ idx, const double *__restrict__ val, const double *__restrict__ rhs,
+                const double *__restrict__ D, double *out, int *fault, const KScal *ks) {
+    if (ks && ks->done) return;
+    __shared__ double loc[kLdlRangeMax];
+    __shared__ double red[2][kLdlRangeThreads / 64];
+    const int g = BWD ? nranges - 1 - (int)blockIdx.x : (int)blockIdx.x;
+    const int r0 = bounds[g], r1 = bounds[g + 1];
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    int lost = 0;
+    for (int q = 0; q < r1 - r0; ++q) {
+        const int r = BWD ? r1 - 1 - q : r0 + q;
+        const int p1 = ptr[r + 1];
+        double acc = 0.0;
+        for (int pb = ptr[r] + t; pb < p1; pb += kLdlRangeThreads * kLdlRangeBatch) {
+            double a[kLdlRangeBatch], v[kLdlRangeBatch];
+            int c[kLdlRangeBatch];
+            bool far[kLdlRangeBatch];  // an input from another range (global memory)
+#pragma unroll
+            for (int k = 0; k < kLdlRangeBatch; ++k) {
+                const int p = pb + kLdlRangeThreads * k;
+                c[k] = p < p1 ? idx[p] : r0;
+                a[k] = p < p1 ? val[p] : 0.0;
+            }
+#pragma unroll
+            for (int k = 0; k < kLdlRangeBatch; ++k) {
+                far[k] = c[k] < r0 || c[k] >= r1;
+                v[k] = far[k] ? ldl_load(out + c[k]) : loc[c[k] - r0];
+            }
+            for (int s = 0;; ++s) {
+                bool missing = false;
+#pragma unroll
+                for (int k = 0; k < kLdlRangeBatch; ++k) missing |= far[k] && ldl_sent(v[k]);
+                if (!missing) break;
+                if (s >= kLdlSpin) {
+                    lost = 1;
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(1);
+#pragma unroll
+                for (int k = 0; k < kLdlRangeBatch; ++k)
+                    if (far[k] && ldl_sent(v[k])) v[k] = ldl_load(out + c[k]);
+            }
+#pragma unroll
+            for (int k = 0; k < kLdlRangeBatch; ++k)
+                if (pb + kLdlRangeThreads * k < p1) acc = acc + a[k] * v[k];  // the thread's entries in order
+        }
+        acc = wave_sum(acc);
+        if (lane == 0) red[q & 1][w] = acc;
+        __syncthreads();
+        double sum = red[q & 1][0];
+#pragma unroll
+        for (int k = 1; k < kLdlRangeThreads / 64; ++k) sum = sum + red[q & 1][k];
+        const double y = (BWD ? rhs[r] / D[r] : rhs[r]) - sum;
+        loc[r - r0] = y;
+        if (t == 0) ldl_store(out + r, y);
+    }
+    if (lost && t == 0) __hip_atomic_store(fault, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
 template <typename T>
 __global__ void k_ldl_out(int n, const int *__restrict__ perm, const double *__restrict__ z, T *__restrict__ out,
                           const KScal *ks) {
@@ -149,8 +233,16 @@ int ldl_setup(fvh_eqn *e, const double *val_dev) {
     HIP_TRY(hipMemcpy(val.data(), val_dev, sizeof(double) * m->nnz, hipMemcpyDeviceToHost));
     HIP_TRY(hipMemcpy(cx.data(), m->ccx.p, sizeof(double) * n, hipMemcpyDeviceToHost));
     HIP_TRY(hipMemcpy(cy.data(), m->ccy.p, sizeof(double) * n, hipMemcpyDeviceToHost));
-    std::vector<int> perm;
-    ldl_order(n, m->h_rowptr.data(), m->h_col.data(), cx.data(), cy.data(), perm);
+    std::vector<int> perm, bounds;
+    ldl_order(n, m->h_rowptr.data(), m->h_col.data(), cx.data(), cy.data(), perm, &bounds);
+    {  // ranges of at most kLdlRangeMax rows
+        std::vector<int> cut{0};
+        for (size_t g = 0; g + 1 < bounds.size(); ++g)
+            for (int a = bounds[g]; a < bounds[g + 1]; a += kLdlRangeMax) cut.push_back(std::min(a + kLdlRangeMax, bounds[g + 1]));
+        if (cut.back() != n) return set_error(FVH_ERR_ARG, "LDL: ordering ranges end at %d of %d rows", cut.back(), n);
+        FVH_TRY(S.ranges.upload(cut));
+        S.nranges = (int)cut.size() - 1;
+    }
     LdlHost F;
     FVH_TRY(ldl_factor(n, m->h_rowptr.data(), m->h_col.data(), val.data(), perm, F));
     // rows of L (strictly lower, columns ascending) from its columns
@@ -188,6 +280,12 @@ int ldl_setup(fvh_eqn *e, const double *val_dev) {
     return FVH_OK;
 }
 
+// the range walk unless FVH_LDL_RANGES=0 (one wave per row, sync-free)
+static bool ldl_ranges() {
+    const char *env = std::getenv("FVH_LDL_RANGES");
+    return !(env && env[0] == '0' && env[1] == 0);
+}
+
 template <typename T>
 int ldl_apply(fvh_eqn *e, const T *x, T *out, const KScal *ks) {
     LdlStore &S = e->ldl;
@@ -197,6 +295,18 @@ int ldl_apply(fvh_eqn *e, const T *x, T *out, const KScal *ks) {
     HIP_TRY(hipHostGetDevicePointer((void **)&fault, S.h_fault, 0));
     hipLaunchKernelGGL((k_ldl_prep<T>), grid1d(n), dim3(256), 0, stream(), n, S.perm.p, x, S.rhs.p, S.y.p, S.z.p, ks);
     HIP_TRY(hipGetLastError());
+    if (ldl_ranges() && S.nranges > 0) {
+        const dim3 gr((unsigned)S.nranges);
+        hipLaunchKernelGGL((k_ldl_range<false>), gr, dim3(kLdlRangeThreads), 0, stream(), S.nranges, S.ranges.p,
+                           S.fptr.p, S.fidx.p, S.fval.p, S.rhs.p, S.D.p, S.y.p, fault, ks);
+        HIP_TRY(hipGetLastError());
+        hipLaunchKernelGGL((k_ldl_range<true>), gr, dim3(kLdlRangeThreads), 0, stream(), S.nranges, S.ranges.p,
+                           S.bptr.p, S.bidx.p, S.bval.p, S.y.p, S.D.p, S.z.p, fault, ks);
+        HIP_TRY(hipGetLastError());
+        hipLaunchKernelGGL((k_ldl_out<T>), grid1d(n), dim3(256), 0, stream(), n, S.perm.p, S.z.p, out, ks);
+        HIP_TRY(hipGetLastError());
+        return FVH_OK;
+    }
     const dim3 g((unsigned)((n + kLdlWaves - 1) / kLdlWaves));
     hipLaunchKernelGGL((k_ldl_sweep<false>), g, dim3(64 * kLdlWaves), 0, stream(), n, S.fptr.p, S.fidx.p, S.fval.p,
                        S.rhs.p, S.D.p, S.y.p, S.y.p, fault, ks);

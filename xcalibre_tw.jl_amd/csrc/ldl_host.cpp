@@ -33,6 +33,7 @@ struct NdState {
     std::vector<int> mark;  // per cell: side stamp of the current split
     int stamp = 0;
     std::vector<int> *out;
+    std::vector<int> *bounds;  // end of every leaf and separator appended (or null)
 };
 
 constexpr int kNdLeaf = 48;
@@ -50,12 +51,14 @@ void nd_split(NdState &S, std::vector<int> cells) {
         Frame &F = st.back();
         if (F.stage == 1) {
             S.out->insert(S.out->end(), F.sep.begin(), F.sep.end());
+            if (S.bounds && !F.sep.empty()) S.bounds->push_back((int)S.out->size());
             st.pop_back();
             continue;
         }
         if ((int)F.cells.size() <= kNdLeaf) {
             std::sort(F.cells.begin(), F.cells.end());
             S.out->insert(S.out->end(), F.cells.begin(), F.cells.end());
+            if (S.bounds && !F.cells.empty()) S.bounds->push_back((int)S.out->size());
             st.pop_back();
             continue;
         }
@@ -95,10 +98,12 @@ void nd_split(NdState &S, std::vector<int> cells) {
 
 }  // namespace
 
-void ldl_order(int n, const int *rowptr, const int *col, const double *x, const double *y, std::vector<int> &perm) {
-    NdState S{rowptr, col, x, y, std::vector<int>(n, 0), 0, &perm};
+void ldl_order(int n, const int *rowptr, const int *col, const double *x, const double *y, std::vector<int> &perm,
+               std::vector<int> *bounds) {
+    NdState S{rowptr, col, x, y, std::vector<int>(n, 0), 0, &perm, bounds};
     perm.clear();
     perm.reserve(n);
+    if (bounds) bounds->assign(1, 0);
     std::vector<int> all(n);
     std::iota(all.begin(), all.end(), 0);
     nd_split(S, std::move(all));

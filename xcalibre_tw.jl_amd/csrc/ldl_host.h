@@ -14,8 +14,11 @@ struct LdlHost {
     std::vector<double> Lx, D;
 };
 
-// Nested-dissection ordering of a symmetric pattern from the cells' centres.
-void ldl_order(int n, const int *rowptr, const int *col, const double *x, const double *y, std::vector<int> &perm);
+// Nested-dissection ordering of a symmetric pattern from the cells' centres;
+// with `bounds`, also the ranges of positions that each leaf and each
+// separator occupy (range g is [bounds[g], bounds[g + 1]), in order).
+void ldl_order(int n, const int *rowptr, const int *col, const double *x, const double *y, std::vector<int> &perm,
+               std::vector<int> *bounds = nullptr);
 
 // Numeric factorization; A symmetric (values read from its upper triangle).
 int ldl_factor(int n, const int *rowptr, const int *col, const double *val, const std::vector<int> &perm, LdlHost &F);
