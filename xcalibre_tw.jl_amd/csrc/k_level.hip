@@ -545,67 +545,87 @@ __global__ void __launch_bounds__(64 * (1 + kTileStages))
         }
     } else {  // compute wave
         const int base = t * tr;
+        constexpr int RW = sizeof(LevRec<T>) / 16;
+        // a chunk's ready word, step table and first step's records (its rows
+        // are slots 0 .. rows-1), read in that order: LDS serves the wave's
+        // reads in order, so once the ready word shows the chunk, the reads
+        // after it see what the loader wrote before it.  They are issued
+        // during the previous chunk's last step; a chunk whose loader was late
+        // is read again until its ready word shows it.
+        int word = 0;
+        int2 sv;
+        TileWord rw[RW];
+        auto take = [&](int c) {
+            const TileStage<T> &S = stage[c % kTileStages];
+            word = __hip_atomic_load(ready + c % kTileStages, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            asm volatile("" ::: "memory");
+            sv = S.step[lane];
+#pragma unroll
+            for (int w = 0; w < RW; ++w) rw[w] = S.rec[w][lane];
+        };
+        take(0);
         for (int c = 0; c < nc; ++c) {
             const int b = c % kTileStages;
             const TileStage<T> &S = stage[b];
-            // the ready word, the step table and the first step's records (its
-            // rows are slots 0 .. rows-1) in one LDS round trip: LDS serves the
-            // wave's reads in order, so once the ready word shows the chunk, the
-            // reads after it see what the loader wrote before it
-            constexpr int RW = sizeof(LevRec<T>) / 16;
-            int word = 0;
-            int2 sv;
-            TileWord rw[RW];
-            for (int s = 0;; ++s) {
-                word = __hip_atomic_load(ready + b, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-                asm volatile("" ::: "memory");
-                sv = S.step[lane];
-#pragma unroll
-                for (int w = 0; w < RW; ++w) rw[w] = S.rec[w][lane];
-                // the values are taken here, so the three reads go out together
-                asm volatile("" : "+v"(word), "+v"(sv.x), "+v"(sv.y));
-#pragma unroll
-                for (int w = 0; w < RW; ++w) asm volatile("" : "+v"(rw[w]));
-                word = __builtin_amdgcn_readfirstlane(word);
-                if ((word & kTileChunkMask) == c + 1) break;
+            word = __builtin_amdgcn_readfirstlane(word);
+            for (int s = 0; (word & kTileChunkMask) != c + 1; ++s) {
                 if (s >= kTileSpin) {
                     broken = 1;
                     break;
                 }
                 __builtin_amdgcn_s_sleep(1);
+                take(c);
+                // the values are taken here, so the reads go out together
+                asm volatile("" : "+v"(word), "+v"(sv.x), "+v"(sv.y));
+#pragma unroll
+                for (int w = 0; w < RW; ++w) asm volatile("" : "+v"(rw[w]));
+                word = __builtin_amdgcn_readfirstlane(word);
             }
             if (broken) break;  // nothing of this chunk is used
             LevRec<T> r = __builtin_bit_cast(LevRec<T>, rw);
             const int ns = word >> kTileStepShift, fp = __builtin_amdgcn_readlane(sv.x, 0);
+            const int2 svc = sv;
             int sx = fp, sy = __builtin_amdgcn_readlane(sv.y, 0);
-            // one step: its rows' neighbours from LDS, the next step's records
-            // read ahead (lanes past a step's rows read some record of the chunk:
+            // one step's rows: neighbours from LDS, the row's expression, the
+            // result to LDS and global memory; `ahead` issues the reads for the
+            // next step (lanes past a step's rows read some record of the chunk:
             // every slot holds one, its words in range; they store nothing)
-            auto step = [&](int s, const LevRec<T> &cur, LevRec<T> &nxt) {
+            auto step = [&](const LevRec<T> &cur, auto ahead) {
                 T nb[kLevDeps];
 #pragma unroll
                 for (int q = 0; q < D; ++q)  // byte offsets (absent: word 0)
                     nb[q] = *reinterpret_cast<const T *>(reinterpret_cast<const char *>(res) + cur.j[q]);
-                int nx = 0, ny = 0;
-                if (s + 1 < ns) {
-                    nx = __builtin_amdgcn_readlane(sv.x, s + 1);
-                    ny = __builtin_amdgcn_readlane(sv.y, s + 1);
-                }
-                nxt = stage_rec(S, (nx - fp + lane) & 63);
+                asm volatile("" ::: "memory");  // the neighbour reads go out first (they are on the chain)
+                ahead();
                 if (lane < sy) {
                     const T val = tile_eval<T, MODE, D>(cur, MODE != 0 ? tile_input(cur) : T(0), nb);
                     res[sx - base + lane] = val;
                     sf_store(out + sx + lane, val);
                     if (MODE == 0) dinv[cur.i] = T(1) / val;
                 }
+            };
+            auto within = [&](int s, const LevRec<T> &cur, LevRec<T> &nxt) {  // step s, not the last
+                const int nx = __builtin_amdgcn_readlane(svc.x, s + 1), ny = __builtin_amdgcn_readlane(svc.y, s + 1);
+                step(cur, [&]() { nxt = stage_rec(S, (nx - fp + lane) & 63); });
                 sx = nx;
                 sy = ny;
             };
+            auto last = [&](const LevRec<T> &cur) {  // the chunk's last step reads the next chunk's stage
+                step(cur, [&]() {
+                    if (c + 1 < nc) take(c + 1);
+                });
+            };
             LevRec<T> r2;
-            for (int s = 0; s < ns; s += 2) {  // two register sets take turns
-                step(s, r, r2);
-                if (s + 1 >= ns) break;
-                step(s + 1, r2, r);
+            int s = 0;
+            for (; s + 2 < ns; s += 2) {  // two register sets take turns
+                within(s, r, r2);
+                within(s + 1, r2, r);
+            }
+            if (s + 1 < ns) {
+                within(s, r, r2);
+                last(r2);
+            } else {
+                last(r);
             }
             asm volatile("" ::: "memory");  // LDS serves the stage reads before this write
             __hip_atomic_store(done + b, c + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
