@@ -167,7 +167,7 @@ struct CaseLayout {
 constexpr int kLevRing = 16384;    // results the level walk keeps in LDS (by schedule position)
 constexpr int kLevLate = 1 << 30;  // flag of a neighbour read after the level barrier
 constexpr int kTileRows = 1024;    // rows per tile of the tile walk, unless FVH_TILE_ROWS says otherwise
-constexpr int kTileLag = 12;       // steps a neighbour from another tile is scheduled behind its producer
+constexpr int kTileLag = 8;        // steps a neighbour from another tile is scheduled behind its producer
 constexpr int kSkewL = 8;                    // steps per hand-over batch
 constexpr int kSkewPD = 32;                  // deepest record prefetch (steps)
 constexpr int kSkewD0 = kSkewPD + 2;         // padding diagonals below T = 0
