@@ -117,3 +117,55 @@ def test_delaunay_full_size_dilu_bitwise(delaunay_full, walk, monkeypatch):
     finally:
         L.lib().fvh_eqn_destroy(h)
         mesh._dev = None
+
+
+def test_delaunay_full_size_fp32_walks_agree(delaunay_full, monkeypatch):
+    """The FP32 DILU apply (mixed precision's inner solves) at C3's real size:
+    the tile walk (1,024- and 256-row tiles), the sync-free walk and the
+    one-work-group level walk evaluate the same FP32 expression per row, so
+    their results agree bit for bit; and they are the FP64 apply to FP32
+    accuracy."""
+    import ctypes as C
+
+    mesh = delaunay_full
+    rng = np.random.default_rng(23)
+    x = rng.standard_normal(mesh.ncells)
+    got = {}
+    for walk in ("tiles", "tiles-256", "syncfree", "levels"):
+        print(f"walk {walk}", flush=True)
+        monkeypatch.setenv("FVH_LEVEL_SYNCFREE", "0" if walk == "levels" else "1")
+        monkeypatch.setenv("FVH_LEVEL_TILES", "1" if walk.startswith("tiles") else "0")
+        if walk == "tiles-256":
+            monkeypatch.setenv("FVH_TILE_ROWS", "256")
+        else:
+            monkeypatch.delenv("FVH_TILE_ROWS", raising=False)
+        mesh._dev = None
+        dm = mesh.device()
+        h = C.c_void_p()
+        L.check(L.lib().fvh_eqn_create(dm.h, C.byref(h)))
+        try:
+            nz = C.c_void_p()
+            L.check(L.lib().fvh_eqn_arrays(h, C.byref(nz), None))
+            rp, ci = dm.pattern()
+            r2 = np.random.default_rng(29)
+            rows = np.repeat(np.arange(mesh.ncells), np.diff(rp))
+            vals = np.where(rows == ci, 4.5, 0.0) - r2.uniform(0.1, 1.0, ci.size) * (rows != ci)
+            L.check(L.lib().fvh_memcpy(nz, vals.ctypes.data_as(C.c_void_p), vals.nbytes))
+            L.check(L.lib().fvh_precond_set(h, L.PRECOND_DILU, None))
+            L.check(L.lib().fvh_precond_update(h), "fvh_precond_update")
+            xf = L.DeviceArray.from_host(x.astype(np.float32))
+            yf = L.DeviceArray(mesh.ncells, np.float32)
+            L.check(L.lib().fvh_precond_apply_f32(h, yf.vp, xf.vp), "fvh_precond_apply_f32")
+            got[walk] = yf.host()
+            if walk == "tiles":
+                xd, yd = L.DeviceArray.from_host(x), L.DeviceArray(mesh.ncells)
+                L.check(L.lib().fvh_precond_apply(h, yd.vp, xd.vp), "fvh_precond_apply")
+                y64 = yd.host()
+        finally:
+            L.lib().fvh_eqn_destroy(h)
+            mesh._dev = None
+    for walk, y in got.items():
+        assert np.array_equal(y.view(np.int32), got["tiles"].view(np.int32)), walk
+    err = np.linalg.norm(got["tiles"] - y64) / np.linalg.norm(y64)
+    print(f"FP32 apply against FP64: rel diff {err:.2e}", flush=True)
+    assert err < 1e-5
