@@ -389,19 +389,67 @@ constexpr int kTileStepShift = 20, kTileChunkMask = (1 << kTileStepShift) - 1;
 
 typedef unsigned int TileWord __attribute__((ext_vector_type(4)));  // 16 bytes of a record
 
+// The compute wave's record of a row, built by the loaders from the level
+// record: the neighbours' LDS words as byte offsets (absent: 0), the row's
+// input v, and the coefficients and pivots the mode reads (update: a[q] =
+// A_ij A_ji, d[0] = A_ii, the row i; forward: a[q] = A_ij and, in FP64, d[q] =
+// Dinv_j; backward: a[q] = A_ij, d[0] = Dinv_i), packed so that a step reads
+// as few 16-byte words as the mode needs.
+template <int N>
+struct TileWordArray {
+    TileWord w[N];
+};
+
+template <typename T, int MODE, int D>
+struct alignas(16) TileRec {
+    static constexpr bool has_i = true, has_v = true;
+    static constexpr int nd = D;
+    int i, m, j[D];
+    T v;
+    T a[D], d[D];
+};
+template <>
+struct alignas(16) TileRec<double, 0, 3> {  // 64 bytes
+    static constexpr bool has_i = true, has_v = false;
+    static constexpr int nd = 1;
+    int i, m, j[3];
+    double a[3], d[1];
+};
+template <>
+struct alignas(16) TileRec<double, 2, 3> {  // 64 bytes
+    static constexpr bool has_i = false, has_v = true;
+    static constexpr int nd = 1;
+    int m, j[3];
+    double v, a[3], d[1];
+};
+template <>
+struct alignas(16) TileRec<float, 1, 3> {  // 32 bytes (the FP32 coefficients carry their pivots)
+    static constexpr bool has_i = false, has_v = true;
+    static constexpr int nd = 0;
+    int m, j[3];
+    float v, a[3];
+};
+template <>
+struct alignas(16) TileRec<float, 2, 3> {  // 48 bytes
+    static constexpr bool has_i = false, has_v = true;
+    static constexpr int nd = 1;
+    int m, j[3];
+    float v, a[3], d[1];
+};
+
 // lev_eval for rows of at most D neighbours, with the test for an absent
 // neighbour on the term rather than on the running sum (off the chain of
 // dependent operations): an absent term is +0, and x - (+0) = x and
 // fma(-(+0), +0, x) = x + (-0) = x exactly for every x (signed zeros, NaN
 // and infinities included), so the results are lev_eval's bits.
-template <typename T, int MODE, int D>
-__device__ inline T tile_eval(const LevRec<T> &r, T in, const T (&nb)[kLevDeps]) {
-    if (MODE == 0) {
+template <typename T, int MODE, int D, typename R>
+__device__ inline T tile_eval(const R &r, T in, const T (&nb)[kLevDeps]) {
+    if constexpr (MODE == 0) {
         T d = r.d[0];
 #pragma unroll
         for (int q = 0; q < D; ++q) d = d - (q < r.m ? r.a[q] / nb[q] : T(0));
         return d;
-    } else if (MODE == 1) {
+    } else if constexpr (MODE == 1) {
         T s = in;
         if constexpr (sizeof(T) == 4) {
 #pragma unroll
@@ -424,33 +472,33 @@ __device__ inline T tile_eval(const LevRec<T> &r, T in, const T (&nb)[kLevDeps])
     }
 }
 
-template <typename T>
+static_assert(sizeof(TileRec<double, 1, 3>) == 80 && sizeof(TileRec<double, 2, 3>) == 64 &&
+                  sizeof(TileRec<double, 0, 3>) == 64 && sizeof(TileRec<float, 1, 3>) == 32 &&
+                  sizeof(TileRec<float, 2, 3>) == 48 && sizeof(TileRec<double, 1, 4>) == 96,
+              "tile records: whole 16-byte words");
+
+template <typename R>
 struct alignas(16) TileStage {  // one chunk in LDS
-    // records of the chunk's rows, by position from the chunk's first, the
-    // row's input in pad; every slot holds a record (m = 0 past the chunk).
-    // Word-major (16-byte word w of slot k at rec[w][k]), so that a wave's
-    // read of one word of 64 consecutive slots is free of bank conflicts.
-    TileWord rec[sizeof(LevRec<T>) / 16][64];
+    // records of the chunk's rows, by position from the chunk's first;
+    // every slot holds a record (m = 0 past the chunk).  Word-major (16-byte
+    // word w of slot k at rec[w][k]), so that a wave's read of one word of 64
+    // consecutive slots is free of bank conflicts.
+    TileWord rec[sizeof(R) / 16][64];
     int2 step[64];  // the chunk's steps: first position, rows
 };
 // slot k of a stage into a record in registers
-template <typename T>
-__device__ inline LevRec<T> stage_rec(const TileStage<T> &S, int k) {
-    constexpr int RW = sizeof(LevRec<T>) / 16;
+template <typename R>
+__device__ inline R stage_rec(const TileStage<R> &S, int k) {
+    constexpr int RW = sizeof(R) / 16;
     TileWord w[RW];
 #pragma unroll
     for (int q = 0; q < RW; ++q) w[q] = S.rec[q][k];
-    return __builtin_bit_cast(LevRec<T>, w);
-}
-template <typename T>
-__device__ inline T tile_input(const LevRec<T> &r) {
-    if constexpr (sizeof(T) == 8) return __builtin_bit_cast(T, make_int2(r.pad[0], r.pad[1]));
-    else return __builtin_bit_cast(T, r.pad[0]);
+    return __builtin_bit_cast(R, w);
 }
 
-template <typename T>
+template <typename R>
 __host__ __device__ constexpr size_t tile_words_off() {  // byte offset of the tile's result words
-    return kTileStages * sizeof(TileStage<T>) + 16 * ((2 * kTileStages * sizeof(int) + 15) / 16);
+    return kTileStages * sizeof(TileStage<R>) + 16 * ((2 * kTileStages * sizeof(int) + 15) / 16);
 }
 
 template <typename T, int MODE, int D>
@@ -459,10 +507,11 @@ __global__ void __launch_bounds__(64 * (1 + kTileStages))
                const int2 *__restrict__ ents, const LevRec<T> *__restrict__ R, const T *__restrict__ in, T *out,
                T *dinv, int *fault, const KScal *ks) {
     if (ks && ks->done) return;
+    using Rec = TileRec<T, MODE, D>;
     extern __shared__ __align__(16) unsigned char tile_mem[];
-    auto *stage = reinterpret_cast<TileStage<T> *>(tile_mem);
+    auto *stage = reinterpret_cast<TileStage<Rec> *>(tile_mem);
     int *ready = reinterpret_cast<int *>(stage + kTileStages), *done = ready + kTileStages;
-    T *res = reinterpret_cast<T *>(tile_mem + tile_words_off<T>());  // [tr + staged neighbours]
+    T *res = reinterpret_cast<T *>(tile_mem + tile_words_off<Rec>());  // [tr + staged neighbours]
     const int t = blockIdx.x, wave = (int)(threadIdx.x >> 6), lane = threadIdx.x & 63;
     const int c0 = tiles[2 * t], nc = tiles[2 * t + 1];
     if (threadIdx.x < kTileStages) ready[threadIdx.x] = done[threadIdx.x] = 0;  // chunk c is c + 1
@@ -470,7 +519,7 @@ __global__ void __launch_bounds__(64 * (1 + kTileStages))
     int broken = 0;
     if (wave > 0) {  // loader of chunks wave-1, wave-1 + kTileStages, ...
         const int b = wave - 1;
-        TileStage<T> &S = stage[b];
+        TileStage<Rec> &S = stage[b];
         int4 ca = make_int4(0, 0, 0, 0);  // lane k: the loader's k-th chunk record (of the current 64)
         int2 cb = make_int2(0, 0);
         for (int c = b, k = 0; c < nc; c += kTileStages, k = (k + 1) & 63) {
@@ -485,34 +534,41 @@ __global__ void __launch_bounds__(64 * (1 + kTileStages))
             const int fs = __builtin_amdgcn_readlane(ca.x, k), ns = __builtin_amdgcn_readlane(ca.y, k),
                       fe = __builtin_amdgcn_readlane(ca.z, k), ne = __builtin_amdgcn_readlane(ca.w, k),
                       fp = __builtin_amdgcn_readlane(cb.x, k), nr = __builtin_amdgcn_readlane(cb.y, k);
-            constexpr int RW = sizeof(LevRec<T>) / 16;  // the record as 16-byte words (no struct copy in memory)
+            constexpr int LW = sizeof(LevRec<T>) / 16, RW = sizeof(Rec) / 16;  // as 16-byte words (no struct copy in memory)
             int2 st = make_int2(0, 0), e = make_int2(0, 0);
-            TileWord r[RW];
-            T x = T(0);
+            TileWord lw[LW];
+            T x = T(0), v = T(0);
 #pragma unroll
-            for (int w = 0; w < RW; ++w) r[w] = TileWord{0u, 0u, 0u, 0u};  // m = 0, words 0
+            for (int w = 0; w < LW; ++w) lw[w] = TileWord{0u, 0u, 0u, 0u};  // m = 0, words 0
             if (lane < ns) st = steps[fs + lane];
             if (lane < nr) {
                 const TileWord *src = reinterpret_cast<const TileWord *>(R + fp + lane);
 #pragma unroll
-                for (int w = 0; w < RW; ++w) r[w] = src[w];
-                // neighbour words as byte offsets into the LDS words (absent: 0),
-                // so the compute wave reads them with no address arithmetic
-                const int m = (int)r[0].y;
-                r[0].z = m > 0 ? r[0].z * (unsigned)sizeof(T) : 0u;
-                r[0].w = m > 1 ? r[0].w * (unsigned)sizeof(T) : 0u;
-                r[1].x = m > 2 ? r[1].x * (unsigned)sizeof(T) : 0u;
-                r[1].y = m > 3 ? r[1].y * (unsigned)sizeof(T) : 0u;
-                if (MODE != 0) {  // the input into the record's pad words (bytes 24 .. 31)
-                    const T v = in[fp + lane];
-                    if constexpr (sizeof(T) == 8) {
-                        const int2 b = __builtin_bit_cast(int2, v);
-                        r[1].z = (unsigned)b.x;
-                        r[1].w = (unsigned)b.y;
-                    } else {
-                        r[1].z = __builtin_bit_cast(unsigned, v);
-                    }
-                }
+                for (int w = 0; w < LW; ++w) lw[w] = src[w];
+                if (MODE != 0) v = in[fp + lane];
+            }
+            // the compute wave's record (neighbour words as byte offsets into
+            // the LDS words, absent 0, so the compute wave reads them with no
+            // address arithmetic)
+            const LevRec<T> lr = __builtin_bit_cast(LevRec<T>, lw);
+            Rec rc{};
+            rc.m = lr.m;
+#pragma unroll
+            for (int q = 0; q < D; ++q) {
+                rc.j[q] = q < lr.m ? lr.j[q] * (int)sizeof(T) : 0;
+                rc.a[q] = lr.a[q];
+            }
+            if constexpr (Rec::nd > 0) {
+#pragma unroll
+                for (int q = 0; q < Rec::nd; ++q) rc.d[q] = lr.d[q];
+            }
+            if constexpr (Rec::has_i) rc.i = lr.i;
+            if constexpr (Rec::has_v) rc.v = v;
+            TileWord r[RW];
+            {
+                const auto rb = __builtin_bit_cast(TileWordArray<RW>, rc);
+#pragma unroll
+                for (int w = 0; w < RW; ++w) r[w] = rb.w[w];
             }
             if (lane < ne) {
                 e = ents[fe + lane];
@@ -545,7 +601,7 @@ __global__ void __launch_bounds__(64 * (1 + kTileStages))
         }
     } else {  // compute wave
         const int base = t * tr;
-        constexpr int RW = sizeof(LevRec<T>) / 16;
+        constexpr int RW = sizeof(Rec) / 16;
         // a chunk's ready word, step table and first step's records (its rows
         // are slots 0 .. rows-1), read in that order: LDS serves the wave's
         // reads in order, so once the ready word shows the chunk, the reads
@@ -556,7 +612,7 @@ __global__ void __launch_bounds__(64 * (1 + kTileStages))
         int2 sv;
         TileWord rw[RW];
         auto take = [&](int c) {
-            const TileStage<T> &S = stage[c % kTileStages];
+            const TileStage<Rec> &S = stage[c % kTileStages];
             word = __hip_atomic_load(ready + c % kTileStages, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
             asm volatile("" ::: "memory");
             sv = S.step[lane];
@@ -566,7 +622,7 @@ __global__ void __launch_bounds__(64 * (1 + kTileStages))
         take(0);
         for (int c = 0; c < nc; ++c) {
             const int b = c % kTileStages;
-            const TileStage<T> &S = stage[b];
+            const TileStage<Rec> &S = stage[b];
             word = __builtin_amdgcn_readfirstlane(word);
             for (int s = 0; (word & kTileChunkMask) != c + 1; ++s) {
                 if (s >= kTileSpin) {
@@ -582,7 +638,7 @@ __global__ void __launch_bounds__(64 * (1 + kTileStages))
                 word = __builtin_amdgcn_readfirstlane(word);
             }
             if (broken) break;  // nothing of this chunk is used
-            LevRec<T> r = __builtin_bit_cast(LevRec<T>, rw);
+            Rec r = __builtin_bit_cast(Rec, rw);
             const int ns = word >> kTileStepShift, fp = __builtin_amdgcn_readlane(sv.x, 0);
             const int2 svc = sv;
             int sx = fp, sy = __builtin_amdgcn_readlane(sv.y, 0);
@@ -590,7 +646,7 @@ __global__ void __launch_bounds__(64 * (1 + kTileStages))
             // result to LDS and global memory; `ahead` issues the reads for the
             // next step (lanes past a step's rows read some record of the chunk:
             // every slot holds one, its words in range; they store nothing)
-            auto step = [&](const LevRec<T> &cur, auto ahead) {
+            auto step = [&](const Rec &cur, auto ahead) {
                 T nb[kLevDeps];
 #pragma unroll
                 for (int q = 0; q < D; ++q)  // byte offsets (absent: word 0)
@@ -598,24 +654,26 @@ __global__ void __launch_bounds__(64 * (1 + kTileStages))
                 asm volatile("" ::: "memory");  // the neighbour reads go out first (they are on the chain)
                 ahead();
                 if (lane < sy) {
-                    const T val = tile_eval<T, MODE, D>(cur, MODE != 0 ? tile_input(cur) : T(0), nb);
+                    T in_ = T(0);
+                    if constexpr (Rec::has_v) in_ = cur.v;
+                    const T val = tile_eval<T, MODE, D>(cur, in_, nb);
                     res[sx - base + lane] = val;
                     sf_store(out + sx + lane, val);
-                    if (MODE == 0) dinv[cur.i] = T(1) / val;
+                    if constexpr (MODE == 0) dinv[cur.i] = T(1) / val;
                 }
             };
-            auto within = [&](int s, const LevRec<T> &cur, LevRec<T> &nxt) {  // step s, not the last
+            auto within = [&](int s, const Rec &cur, Rec &nxt) {  // step s, not the last
                 const int nx = __builtin_amdgcn_readlane(svc.x, s + 1), ny = __builtin_amdgcn_readlane(svc.y, s + 1);
                 step(cur, [&]() { nxt = stage_rec(S, (nx - fp + lane) & 63); });
                 sx = nx;
                 sy = ny;
             };
-            auto last = [&](const LevRec<T> &cur) {  // the chunk's last step reads the next chunk's stage
+            auto last = [&](const Rec &cur) {  // the chunk's last step reads the next chunk's stage
                 step(cur, [&]() {
                     if (c + 1 < nc) take(c + 1);
                 });
             };
-            LevRec<T> r2;
+            Rec r2;
             int s = 0;
             for (; s + 2 < ns; s += 2) {  // two register sets take turns
                 within(s, r, r2);
@@ -689,9 +747,9 @@ static int launch_walk(int nlev, const int *lptr, const LevRec<T> *R, const T *i
 }
 
 // LDS bytes per work-group of the tile walk
-template <typename T>
+template <typename T, int MODE, int D>
 static size_t tile_lds(const Schedule &s, const Schedule::TileWalk &w) {
-    return tile_words_off<T>() + (size_t)(s.tile_rows + w.maxrem) * sizeof(T);
+    return tile_words_off<TileRec<T, MODE, D>>() + (size_t)(s.tile_rows + w.maxrem) * sizeof(T);
 }
 constexpr size_t kLdsMax = 160 * 1024;
 
@@ -705,14 +763,14 @@ static bool env_off(const char *name) {
 static int lev_walk(const Schedule &s) {
     if (env_off("FVH_LEVEL_SYNCFREE")) return 0;
     if (env_off("FVH_LEVEL_TILES")) return 1;
-    const size_t lds = std::max(tile_lds<double>(s, s.tf), tile_lds<double>(s, s.tb));
+    const size_t lds = std::max(tile_lds<double, 1, kLevDeps>(s, s.tf), tile_lds<double, 2, kLevDeps>(s, s.tb));
     return lds <= kLdsMax ? 2 : 1;
 }
 
 template <typename T, int MODE, int D>
 static int launch_tiles_d(const Schedule &s, const Schedule::TileWalk &w, const LevRec<T> *R, const T *in, T *out,
                           T *dinv, int *fault, const KScal *ks) {
-    const size_t lds = tile_lds<T>(s, w);
+    const size_t lds = tile_lds<T, MODE, D>(s, w);
     if (lds > kLdsMax) return set_error(FVH_ERR_UNSUPPORTED, "tile walk: %zu bytes of LDS per tile", lds);
     static size_t allowed = 64 * 1024;
     if (lds > allowed) {
