@@ -131,7 +131,8 @@ __global__ void __launch_bounds__(64 * kLdlWaves)
 
 // Range walk (the default): one work-group per range of the factor order
 // (a nested-dissection leaf, or a separator, cut at kLdlRangeMax rows),
-// ranges in solve order.  The work-group takes its rows one after another,
+// ranges in solve order; the range's row pointers are staged in LDS and the
+// next row's first batch of entries is loaded while a row is summed.  The work-group takes its rows one after another,
 // its threads splitting each row's entries: an input from the range itself
 // comes from the range's results in LDS, one from an earlier range from the
 // global result array (device-coherent loads polled until published, with
@@ -143,58 +144,102 @@ __global__ void __launch_bounds__(64 * kLdlWaves)
 // (the wave sums in two alternating slots).  A separator's dependent rows
 // thus follow each other at LDS and barrier latency instead of a
 // publish-to-poll hop between CUs.
-constexpr int kLdlRangeMax = 4096;   // rows per range (its results in LDS)
+constexpr int kLdlRangeMax = 2048;   // rows per range (its results and row pointers in LDS)
 constexpr int kLdlRangeThreads = 256;
 constexpr int kLdlRangeBatch = 8;
 
 template <bool BWD>
 __global__ void __launch_bounds__(kLdlRangeThreads)
     k_ldl_range(int nranges, const int *__restrict__ bounds, const int *__restrict__ ptr,
-                const int *__restrict__ idx, const double *__restrict__ val, const double *__restrict__ rhs,
-                const double *__restrict__ D, double *out, int *fault, const KScal *ks) {
+                const int *__restrict__ idx, const double *__restrict__ val, int nnz,
+                const double *__restrict__ rhs, const double *__restrict__ D, double *out, int *fault,
+                const KScal *ks) {
     if (ks && ks->done) return;
+    constexpr int B = kLdlRangeBatch, SPAN = kLdlRangeThreads * kLdlRangeBatch;
     __shared__ double loc[kLdlRangeMax];
+    __shared__ int rp[kLdlRangeMax + 1];  // ptr of the range's rows (and of the row after)
     __shared__ double red[2][kLdlRangeThreads / 64];
     const int g = BWD ? nranges - 1 - (int)blockIdx.x : (int)blockIdx.x;
-    const int r0 = bounds[g], r1 = bounds[g + 1];
+    const int r0 = bounds[g], r1 = bounds[g + 1], nrow = r1 - r0;
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    for (int i = t; i <= nrow; i += kLdlRangeThreads) rp[i] = ptr[r0 + i];
+    __syncthreads();
+    auto row_of = [&](int q) { return BWD ? r1 - 1 - q : r0 + q; };
+    // the first batch of row q's entries and its right-hand side, loaded
+    // whatever the row's length (clamped indices, unused beyond the row), so
+    // that the next row's loads are in flight while this row waits
+    struct Head {
+        int c[B];
+        double a[B];
+        int p0, p1;
+        double base, piv;
+    };
+    auto fetch = [&](int q, Head &h) {
+        const int r = row_of(q);
+        h.p0 = rp[r - r0];
+        h.p1 = rp[r - r0 + 1];
+#pragma unroll
+        for (int k = 0; k < B; ++k) {
+            const int p = min(h.p0 + t + kLdlRangeThreads * k, nnz - 1);
+            h.c[k] = idx[p];
+            h.a[k] = val[p];
+        }
+        h.base = rhs[r];
+        h.piv = BWD ? D[r] : 1.0;
+    };
+    // inputs of entries pb + 256 k < p1 with columns c: from the range's LDS
+    // words or, polled until published, from the global result array
     int lost = 0;
-    for (int q = 0; q < r1 - r0; ++q) {
-        const int r = BWD ? r1 - 1 - q : r0 + q;
-        const int p1 = ptr[r + 1];
+    auto gather = [&](const int (&c)[B], int pb, int p1, double (&v)[B]) {
+        bool far[B];
+#pragma unroll
+        for (int k = 0; k < B; ++k) {
+            const bool in = pb + kLdlRangeThreads * k < p1;
+            far[k] = in && (c[k] < r0 || c[k] >= r1);
+            v[k] = far[k] ? ldl_load(out + c[k]) : loc[in ? c[k] - r0 : 0];
+        }
+        for (int s = 0;; ++s) {
+            bool missing = false;
+#pragma unroll
+            for (int k = 0; k < B; ++k) missing |= far[k] && ldl_sent(v[k]);
+            if (!missing) break;
+            if (s >= kLdlSpin) {
+                lost = 1;
+                break;
+            }
+            __builtin_amdgcn_s_sleep(1);
+#pragma unroll
+            for (int k = 0; k < B; ++k)
+                if (far[k] && ldl_sent(v[k])) v[k] = ldl_load(out + c[k]);
+        }
+    };
+    Head h;
+    if (nrow > 0) fetch(0, h);
+    for (int q = 0; q < nrow; ++q) {
+        const int r = row_of(q);
+        Head hn;
+        fetch(min(q + 1, nrow - 1), hn);  // the next row's (the last row's again at the end)
         double acc = 0.0;
-        for (int pb = ptr[r] + t; pb < p1; pb += kLdlRangeThreads * kLdlRangeBatch) {
-            double a[kLdlRangeBatch], v[kLdlRangeBatch];
-            int c[kLdlRangeBatch];
-            bool far[kLdlRangeBatch];  // an input from another range (global memory)
+        {
+            double v[B];
+            gather(h.c, h.p0 + t, h.p1, v);
 #pragma unroll
-            for (int k = 0; k < kLdlRangeBatch; ++k) {
+            for (int k = 0; k < B; ++k)
+                if (h.p0 + t + kLdlRangeThreads * k < h.p1) acc = acc + h.a[k] * v[k];  // the thread's entries in order
+        }
+        for (int pb = h.p0 + t + SPAN; pb < h.p1; pb += SPAN) {  // rows longer than one batch
+            int c[B];
+            double a[B], v[B];
+#pragma unroll
+            for (int k = 0; k < B; ++k) {
                 const int p = pb + kLdlRangeThreads * k;
-                c[k] = p < p1 ? idx[p] : r0;
-                a[k] = p < p1 ? val[p] : 0.0;
+                c[k] = p < h.p1 ? idx[p] : r0;
+                a[k] = p < h.p1 ? val[p] : 0.0;
             }
+            gather(c, pb, h.p1, v);
 #pragma unroll
-            for (int k = 0; k < kLdlRangeBatch; ++k) {
-                far[k] = c[k] < r0 || c[k] >= r1;
-                v[k] = far[k] ? ldl_load(out + c[k]) : loc[c[k] - r0];
-            }
-            for (int s = 0;; ++s) {
-                bool missing = false;
-#pragma unroll
-                for (int k = 0; k < kLdlRangeBatch; ++k) missing |= far[k] && ldl_sent(v[k]);
-                if (!missing) break;
-                if (s >= kLdlSpin) {
-                    lost = 1;
-                    break;
-                }
-                __builtin_amdgcn_s_sleep(1);
-#pragma unroll
-                for (int k = 0; k < kLdlRangeBatch; ++k)
-                    if (far[k] && ldl_sent(v[k])) v[k] = ldl_load(out + c[k]);
-            }
-#pragma unroll
-            for (int k = 0; k < kLdlRangeBatch; ++k)
-                if (pb + kLdlRangeThreads * k < p1) acc = acc + a[k] * v[k];  // the thread's entries in order
+            for (int k = 0; k < B; ++k)
+                if (pb + kLdlRangeThreads * k < h.p1) acc = acc + a[k] * v[k];
         }
         acc = wave_sum(acc);
         if (lane == 0) red[q & 1][w] = acc;
@@ -202,9 +247,10 @@ __global__ void __launch_bounds__(kLdlRangeThreads)
         double sum = red[q & 1][0];
 #pragma unroll
         for (int k = 1; k < kLdlRangeThreads / 64; ++k) sum = sum + red[q & 1][k];
-        const double y = (BWD ? rhs[r] / D[r] : rhs[r]) - sum;
+        const double y = (BWD ? h.base / h.piv : h.base) - sum;
         loc[r - r0] = y;
         if (t == 0) ldl_store(out + r, y);
+        h = hn;
     }
     if (lost && t == 0) __hip_atomic_store(fault, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
 }
@@ -297,11 +343,12 @@ int ldl_apply(fvh_eqn *e, const T *x, T *out, const KScal *ks) {
     HIP_TRY(hipGetLastError());
     if (ldl_ranges() && S.nranges > 0) {
         const dim3 gr((unsigned)S.nranges);
+        const int nnz = std::max(S.nnzL, 1);  // the entry arrays hold at least one word
         hipLaunchKernelGGL((k_ldl_range<false>), gr, dim3(kLdlRangeThreads), 0, stream(), S.nranges, S.ranges.p,
-                           S.fptr.p, S.fidx.p, S.fval.p, S.rhs.p, S.D.p, S.y.p, fault, ks);
+                           S.fptr.p, S.fidx.p, S.fval.p, nnz, S.rhs.p, S.D.p, S.y.p, fault, ks);
         HIP_TRY(hipGetLastError());
         hipLaunchKernelGGL((k_ldl_range<true>), gr, dim3(kLdlRangeThreads), 0, stream(), S.nranges, S.ranges.p,
-                           S.bptr.p, S.bidx.p, S.bval.p, S.y.p, S.D.p, S.z.p, fault, ks);
+                           S.bptr.p, S.bidx.p, S.bval.p, nnz, S.y.p, S.D.p, S.z.p, fault, ks);
         HIP_TRY(hipGetLastError());
         hipLaunchKernelGGL((k_ldl_out<T>), grid1d(n), dim3(256), 0, stream(), n, S.perm.p, S.z.p, out, ks);
         HIP_TRY(hipGetLastError());
