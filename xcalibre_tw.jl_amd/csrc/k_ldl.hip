@@ -62,6 +62,31 @@ __device__ inline double wave_sum(double a) {
     return a;
 }
 
+// DPP move of an FP64 value (both halves); lanes that read outside their
+// row, and rows the row mask leaves out, get 0
+template <int CTRL, int ROWS, bool BOUND>
+__device__ inline double dpp_f64(double v) {
+    const int2 b = __builtin_bit_cast(int2, v);
+    int2 r;
+    r.x = __builtin_amdgcn_update_dpp(0, b.x, CTRL, ROWS, 0xF, BOUND);
+    r.y = __builtin_amdgcn_update_dpp(0, b.y, CTRL, ROWS, 0xF, BOUND);
+    return __builtin_bit_cast(double, r);
+}
+// Sum of a wave's 64 lanes in a fixed order, without LDS: shifts by 1, 2,
+// 4, 8 within each row of 16 lanes (lane 15 of a row then holds the row's
+// sum), the sum of rows 0 and 2 broadcast into rows 1 and 3, then lane 31's
+// into rows 2 and 3; lane 63 holds (row 2 + row 3) + (row 0 + row 1).
+__device__ inline double wave_total(double a) {
+    a = a + dpp_f64<0x111, 0xF, true>(a);   // row_shr:1
+    a = a + dpp_f64<0x112, 0xF, true>(a);   // row_shr:2
+    a = a + dpp_f64<0x114, 0xF, true>(a);   // row_shr:4
+    a = a + dpp_f64<0x118, 0xF, true>(a);   // row_shr:8
+    a = a + dpp_f64<0x142, 0xA, false>(a);  // row_bcast:15 into rows 1 and 3
+    a = a + dpp_f64<0x143, 0xC, false>(a);  // row_bcast:31 into rows 2 and 3
+    const int2 b = __builtin_bit_cast(int2, a);
+    return __builtin_bit_cast(double, make_int2(__builtin_amdgcn_readlane(b.x, 63), __builtin_amdgcn_readlane(b.y, 63)));
+}
+
 }  // namespace
 
 // rhs[k] = x[perm[k]] (in FP64); both result arrays to the sentinel
@@ -138,7 +163,7 @@ __global__ void __launch_bounds__(64 * kLdlWaves)
 // global result array (device-coherent loads polled until published, with
 // up to kLdlRangeBatch loads in flight per thread).  Per row: thread t sums
 // its entries t, t + 256, ... in order from 0, the lanes of each wave are
-// added by a fixed tree and the four waves in order; every thread then
+// added by a fixed DPP tree (wave_total) and the four waves in order; every thread then
 // forms the result and writes it to the row's LDS word (each wave reads only
 // words it wrote itself), thread 0 also publishes it.  One barrier per row
 // (the wave sums in two alternating slots).  A separator's dependent rows
@@ -241,7 +266,7 @@ __global__ void __launch_bounds__(kLdlRangeThreads)
             for (int k = 0; k < B; ++k)
                 if (pb + kLdlRangeThreads * k < h.p1) acc = acc + a[k] * v[k];
         }
-        acc = wave_sum(acc);
+        acc = wave_total(acc);
         if (lane == 0) red[q & 1][w] = acc;
         __syncthreads();
         double sum = red[q & 1][0];
