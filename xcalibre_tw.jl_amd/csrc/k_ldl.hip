@@ -267,11 +267,14 @@ __global__ void __launch_bounds__(kLdlRangeThreads)
                 if (pb + kLdlRangeThreads * k < h.p1) acc = acc + a[k] * v[k];
         }
         acc = wave_total(acc);
-        if (lane == 0) red[q & 1][w] = acc;
-        __syncthreads();
-        double sum = red[q & 1][0];
+        double sum = acc;
+        if constexpr (kLdlRangeThreads > 64) {  // the waves' sums in order
+            if (lane == 0) red[q & 1][w] = acc;
+            __syncthreads();
+            sum = red[q & 1][0];
 #pragma unroll
-        for (int k = 1; k < kLdlRangeThreads / 64; ++k) sum = sum + red[q & 1][k];
+            for (int k = 1; k < kLdlRangeThreads / 64; ++k) sum = sum + red[q & 1][k];
+        }
         const double y = (BWD ? h.base / h.piv : h.base) - sum;
         loc[r - r0] = y;
         if (t == 0) ldl_store(out + r, y);
