@@ -363,24 +363,24 @@ __global__ void k_lev_sf_prep(int n, const int *__restrict__ rows, const T *__re
 // in steps of at most 64 independent rows, one lane per row, and keeps every
 // result in LDS at the row's place in the tile, where the later steps of the
 // tile read it; a step's critical path is an LDS read, the row's expression
-// (lev_eval: the same bits as every other walk) and an LDS write.  Results
+// (tile_eval: the same bits as every other walk) and an LDS write.  Results
 // also go to global memory with device-coherent stores, into an array filled
 // with the sentinel before the sweep.  Neighbours from other (earlier) tiles
 // are staged into LDS words after the tile's results by kTileStages loader
-// waves, which take the chunks of steps in turn: a loader reads its chunk's
-// records, inputs and step table into an LDS stage buffer (its own), polls
-// the chunk's neighbours from other tiles until none is the sentinel, writes
-// them to their LDS words and raises the buffer's ready word; the compute
-// wave raises the buffer's done word when it has finished the chunk, and the
-// loader then fills it with its next chunk.  The ready word carries the
-// chunk's step count, so the compute wave takes up a chunk with one LDS
-// round trip (ready word, step table and first records together).  Only
-// loaders wait for global memory, so the compute wave never waits for a load
-// or a store.  A tile
-// waits only for tiles before it, whose work-groups the in-order dispatch
-// has started, so the sweep always drains; every wait is bounded, and one
-// that runs out sets the fault word and ends the wave (a loader or compute
-// wave that ends early makes the other waves of its tile run out in turn).
+// waves, which take the chunks of steps in turn: a loader packs its chunk's
+// records (with the rows' inputs) and step table into an LDS stage buffer
+// (its own), polls the chunk's neighbours from other tiles until none is the
+// sentinel, writes them to their LDS words and raises the buffer's ready
+// word; the compute wave raises the buffer's done word when it has finished
+// the chunk, and the loader then fills it with its next chunk.  The ready
+// word carries the chunk's step count; the compute wave reads it with the
+// step table and the first records during the previous chunk's last step.
+// Only loaders wait for global memory, so the compute wave never waits for a
+// load or a store.  A tile waits only for tiles before it, whose work-groups
+// the in-order dispatch has started, so the sweep always drains; every wait
+// is bounded, and one that runs out sets the fault word and ends the wave (a
+// loader or compute wave that ends early makes the other waves of its tile
+// run out in turn).
 // ---------------------------------------------------------------------------
 constexpr int kTileStages = 3;  // loader waves and stage buffers per work-group
 constexpr int kTileSpin = 1 << 22;  // polls of an LDS word (about half a second) before a wait is declared lost
