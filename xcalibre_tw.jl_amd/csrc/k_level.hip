@@ -382,7 +382,7 @@ __global__ void k_lev_sf_prep(int n, const int *__restrict__ rows, const T *__re
 // loader or compute wave that ends early makes the other waves of its tile
 // run out in turn).
 // ---------------------------------------------------------------------------
-constexpr int kTileStages = 3;  // loader waves and stage buffers per work-group
+constexpr int kTileStages = 4;  // loader waves and stage buffers per work-group
 constexpr int kTileSpin = 1 << 22;  // polls of an LDS word (about half a second) before a wait is declared lost
 // ready word of a stage buffer: (chunk + 1) | (steps << kTileStepShift)
 constexpr int kTileStepShift = 20, kTileChunkMask = (1 << kTileStepShift) - 1;
