@@ -14,7 +14,7 @@ import pytest
 import fvmhip as F
 from fvmhip import _lib as L
 from oracle import ref
-from tests.cases import flatplate_model, bfs_model, cavity_model, config, rel_l2
+from tests.cases import flatplate_model, bfs_model, cavity_model, config, rel_l2, same_bits
 
 pytestmark = pytest.mark.gpu
 
@@ -210,6 +210,21 @@ def test_cavity_c1_mixed_fields(quad40):
     assert n < 1000  # converged to the break
     for f, err in errs.items():
         assert err <= 1e-6, f"{f}: {err:.3e}"
+
+
+@pytest.mark.parametrize("mesh_name", ["quad40", "trig40"])
+def test_fp32_cg_fused_matches_unfused(ref_mesh, mesh_name, monkeypatch):
+    """The FP32 inner CG with each beta update fused into the next SpMV
+    (k_cg_beta_spmv) gives the same bits as with the two kernels apart."""
+    model = cavity_model(ref_mesh(mesh_name))
+    out = {}
+    for fused in ("1", "0"):
+        monkeypatch.setenv("FVH_CG_FUSED", fused)
+        sim = F.Simulation(model, config(20, precision=L.MIXED), 0.0)
+        sim.run(20)
+        out[fused] = [sim.get(f) for f in ("Ux", "Uy", "p")] + [np.asarray(x) for x in sim.residuals()]
+    for a, b in zip(out["1"], out["0"]):
+        assert same_bits(a, b)
 
 
 def test_bfs_mixed_converges_to_oracle():

@@ -75,6 +75,24 @@ def test_cavity_c1_full_run(quad40):
         assert same_bits(a, b), name
 
 
+@pytest.mark.parametrize("fused", ["1", "0"])
+@pytest.mark.parametrize("mesh_name", ["quad40", "trig40"])
+def test_cg_beta_fused_with_spmv_bitwise(ref_mesh, mesh_name, fused, monkeypatch):
+    """The pressure CG with each beta update fused into the next iteration's
+    SpMV (k_cg_beta_spmv, the default) and with the two apart
+    (FVH_CG_FUSED=0), on a banded mesh (banded SpMV, DILU sweeps that read r
+    in their own layout) and a triangle mesh (CSR SpMV): both bitwise."""
+    monkeypatch.setenv("FVH_CG_FUSED", fused)
+    model = cavity_model(ref_mesh(mesh_name))
+    g, r = _sims(model, config(25), pref=0.0)
+    assert g.run(25) == r.run(25)
+    for a, b in zip(g.residuals(), r.residuals()):
+        assert same_bits(a, b)
+    for name in ("Ux", "Uy", "p", "mdotf"):
+        assert same_bits(g.get(name), r.get(name)), name
+    assert g.stats("p")["niter"] == r.stats("p")["niter"]
+
+
 def test_upwind_midpoint_euler_piso(quad40):
     """Upwind divergence, Midpoint gradients, Euler time term, PISO loop."""
     model = cavity_model(quad40)

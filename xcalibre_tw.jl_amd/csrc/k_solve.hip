@@ -13,6 +13,7 @@
 // iteration returns at once when ks->done is set, so the host enqueues
 // iterations in batches and reads the flag only between batches.
 #include <cmath>
+#include <cstdlib>
 #include <limits>
 
 #include "fvh_eqn.h"
@@ -669,6 +670,97 @@ __global__ void __launch_bounds__(kBlock) k_cg_beta(int n, const double *part, i
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) op(i, beta);
 }
 
+// k_cg_beta and the next iteration's k_spmv_dot in one launch, one block of
+// kWide threads per chunk: beta as k_cg_beta forms it, then p' = z + beta p
+// for the chunk's rows, stored into a second array (the old p of rows in
+// other blocks must stay readable), and y = sigma A p' with the partial sums
+// of p'.y.  A row's neighbours' p' are formed again from z and p by the same
+// expression, so the product reads the bits that the owners store.  At itmax
+// the blocks still run the product: nothing reads it.
+template <typename V, class Ax>
+__global__ void __launch_bounds__(kWide)
+    k_cg_beta_spmv(int n, const double *rz, int nch, KScal *ks, int itmax, const V *__restrict__ z,
+                   const V *__restrict__ p, V *__restrict__ pn, V *__restrict__ y, V sigma, double *part, Ax ax) {
+    if (block_done(ks)) return;
+    double g[1];
+    final_reduce<1>(rz, nch, g);
+    const double gn = g[0], rn = sqrt(gn);
+    const bool solved = (rn <= ks->eps) || (rn + 1.0 <= 1.0);
+    const double beta = gn / ks->gprev;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        ks->rnorm = rn;
+        if (!solved) {
+            ks->pnorm2 = gn + beta * beta * ks->pnorm2;
+            ks->gamma = gn;
+            ks->beta = beta;
+        }
+        ks->iter += 1;
+        const bool tired = ks->iter >= itmax;
+        if (solved || tired) {
+            ks->done = 1;
+            ks->status = solved ? FVH_KRYLOV_SOLVED : FVH_KRYLOV_ITMAX;
+        }
+    }
+    if (solved) return;
+    const auto pnew = [=](int j) -> V { return (V)((double)z[j] + beta * (double)p[j]); };
+    __shared__ double sp[1][kChunk];
+    const int c0 = blockIdx.x * kChunk;
+#pragma unroll
+    for (int h = 0; h < kChunk / kWide; ++h) {
+        const int j = threadIdx.x + h * kWide, i = c0 + j;
+        double pr = 0.0;
+        if (i < n) {
+            const V pi = pnew(i);
+            pn[i] = pi;
+            const V s = sigma * ax(pnew, i);
+            y[i] = s;
+            pr = (double)pi * (double)s;
+        }
+        sp[0][j] = pr;
+    }
+    chunk_reduce_staged<1>(sp, part, nch);
+}
+
+template <typename V>
+struct BandRows {
+    int n, nx;
+    const unsigned char *mask;
+    const V *C;
+    template <class X>
+    __device__ V operator()(X xat, int i) const {
+        return band_ax_at(n, nx, mask, C, xat, i);
+    }
+};
+template <typename V>
+struct CsrRows {
+    const int *rowptr, *col;
+    const V *val;
+    template <class X>
+    __device__ V operator()(X xat, int i) const {
+        return row_ax_at(rowptr, col, val, xat, i);
+    }
+};
+
+// The fused beta update and SpMV with the operator the system's own SpMV
+// uses (banded or CSR values, FP64 or the FP32 copy).
+template <typename T>
+static int launch_cg_beta_spmv(fvh_eqn *e, const double *rz, KScal *ks, int itmax, const T *z, const T *p, T *pn,
+                               T *y, double sigma, double *part) {
+    const fvh_mesh *m = e->m;
+    const int n = (int)m->n, nch = nchunks(m->n);
+    const T sg = (T)sigma;
+    const T *band = sizeof(T) == 8 ? (const T *)e->kw.bval.p : (const T *)e->kw.bval32.p;
+    const T *csr = sizeof(T) == 8 ? (const T *)e->nz.p : (const T *)e->kw.valf.p;
+    if (e->kw.band_live && band)
+        hipLaunchKernelGGL((k_cg_beta_spmv<T, BandRows<T>>), dim3(nch), dim3(kWide), 0, stream(), n, rz, nch, ks,
+                           itmax, z, p, pn, y, sg, part, BandRows<T>{n, m->sched.band_nx, m->sched.bmask.p, band});
+    else
+        hipLaunchKernelGGL((k_cg_beta_spmv<T, CsrRows<T>>), dim3(nch), dim3(kWide), 0, stream(), n, rz, nch, ks,
+                           itmax, z, p, pn, y, sg, part, CsrRows<T>{m->d.rowptr, m->d.col, csr});
+    HIP_TRY(hipGetLastError());
+    return FVH_OK;
+}
+
 // ---------------------------------------------------------------------------
 // CG (Krylov.jl cg!) on sigma*A x = sigma*b
 // ---------------------------------------------------------------------------
@@ -677,9 +769,11 @@ static int cg(const Sys<T> &S, const fvh_solver_cfg &cfg, T *xu, fvh_stats *st, 
     fvh_eqn *e = S.e;
     const int n = S.n;
     KrylovWork &w = e->kw;
-    T *xk = S.v[0], *dx = S.v[1], *r = S.v[2], *z = S.v[3], *p = S.v[4], *Ap = S.v[5];
+    T *xk = S.v[0], *dx = S.v[1], *r = S.v[2], *z = S.v[3], *p = S.v[4], *Ap = S.v[5], *p2 = S.v[6];
     const T *b = S.b;
-    double *part = w.part.p;
+    // r.z partials in part, p.Ap partials after them (the fused beta + SpMV
+    // launch reads the one and writes the other)
+    double *part = w.part.p, *ppap = w.part.p + nchunks(n);
     KScal *ks = w.ks.p;
     const double atol = cfg.atol, rtol = cfg.rtol;
     const int itmax = cfg.itmax;
@@ -733,23 +827,39 @@ static int cg(const Sys<T> &S, const fvh_solver_cfg &cfg, T *xu, fvh_stats *st, 
     const int bstride = rt.stride, nhand = rt.nhand, HS = rt.HS;
     const unsigned long long sent = rt.sent;
     FVH_TRY(read_ks(e));
+    // Each iteration's beta update runs fused with the next iteration's SpMV
+    // (k_cg_beta_spmv, p alternating between two arrays); the last one of a
+    // batch runs alone, in place, before the host reads the scalars.
+    const char *fuse_env = std::getenv("FVH_CG_FUSED");
+    const bool fuse = !(fuse_env && fuse_env[0] == '0');
+    T *pc = p, *pa = p2;
     while (!w.h_ks->done && enq < itmax) {
         int nb = std::min(next_batch(enq), itmax - enq);
         for (int it = 0; it < nb; ++it) {
-            FVH_TRY(sys_spmv_dot(S, p, Ap, sigma, ks, p, part));
+            if (it > 0 && fuse) {
+                FVH_TRY(launch_cg_beta_spmv<T>(e, part, ks, itmax, z, pc, pa, Ap, sigma, ppap));
+                std::swap(pc, pa);
+            } else {
+                if (it > 0) {
+                    hipLaunchKernelGGL(k_cg_beta, fused_grid(n), dim3(kBlock), 0, stream(), (int)n, part, nchunks(n),
+                                       ks, itmax, [=] __device__(int i, double be) { pc[i] = D(z[i]) + be * D(pc[i]); });
+                    HIP_TRY(hipGetLastError());
+                }
+                FVH_TRY(sys_spmv_dot(S, pc, Ap, sigma, ks, pc, ppap));
+            }
             if (qmap)
-                hipLaunchKernelGGL(k_cg_alpha, fused_grid(n), dim3(kBlock), 0, stream(), (int)n, part, nchunks(n), ks,
+                hipLaunchKernelGGL(k_cg_alpha, fused_grid(n), dim3(kBlock), 0, stream(), (int)n, ppap, nchunks(n), ks,
                                    epsT, [=] __device__(int i, double al) {
-                                       xk[i] = D(xk[i]) + al * D(p[i]);
+                                       xk[i] = D(xk[i]) + al * D(pc[i]);
                                        const T ri = D(r[i]) + (-al) * D(Ap[i]);
                                        r[i] = ri;
                                        bdst[(size_t)bmap[i] * bstride] = ri;
                                        for (int q = i; q < nhand; q += n) hand[q] = q % HS == 0 ? T(0) : bits_as<T>(sent);
                                    });
             else
-                hipLaunchKernelGGL(k_cg_alpha, fused_grid(n), dim3(kBlock), 0, stream(), (int)n, part, nchunks(n), ks,
+                hipLaunchKernelGGL(k_cg_alpha, fused_grid(n), dim3(kBlock), 0, stream(), (int)n, ppap, nchunks(n), ks,
                                    epsT, [=] __device__(int i, double al) {
-                                       xk[i] = D(xk[i]) + al * D(p[i]);
+                                       xk[i] = D(xk[i]) + al * D(pc[i]);
                                        r[i] = D(r[i]) + (-al) * D(Ap[i]);
                                    });
             HIP_TRY(hipGetLastError());
@@ -769,10 +879,10 @@ static int cg(const Sys<T> &S, const fvh_solver_cfg &cfg, T *xu, fvh_stats *st, 
                     a[0] = a[0] + D(r[i]) * D(zi);
                 }));
             }
-            hipLaunchKernelGGL(k_cg_beta, fused_grid(n), dim3(kBlock), 0, stream(), (int)n, part, nchunks(n), ks, itmax,
-                               [=] __device__(int i, double be) { p[i] = D(z[i]) + be * D(p[i]); });
-            HIP_TRY(hipGetLastError());
         }
+        hipLaunchKernelGGL(k_cg_beta, fused_grid(n), dim3(kBlock), 0, stream(), (int)n, part, nchunks(n), ks, itmax,
+                           [=] __device__(int i, double be) { pc[i] = D(z[i]) + be * D(pc[i]); });
+        HIP_TRY(hipGetLastError());
         enq += nb;
         FVH_TRY(read_ks(e));
     }

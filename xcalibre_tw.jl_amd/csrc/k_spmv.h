@@ -17,9 +17,11 @@ namespace fvh {
 // instead of two per entry.  Slots past the row's end re-read its last entry
 // (in bounds, same cache line) and are dropped by a select.
 constexpr int kRowBatch = 5;
-template <typename V>
-__device__ inline V row_ax(const int *__restrict__ rowptr, const int *__restrict__ col, const V *__restrict__ val,
-                           const V *__restrict__ x, int i) {
+// The row products read x through `xat(j)`: a plain array, or values formed
+// on the fly from other arrays (CG's p = z + beta p).
+template <typename V, class X>
+__device__ inline V row_ax_at(const int *__restrict__ rowptr, const int *__restrict__ col, const V *__restrict__ val,
+                              X xat, int i) {
     V s = 0;
     const int p1 = rowptr[i + 1];
     for (int p0 = rowptr[i]; p0 < p1; p0 += kRowBatch) {
@@ -32,7 +34,7 @@ __device__ inline V row_ax(const int *__restrict__ rowptr, const int *__restrict
             a[k] = val[p];
         }
 #pragma unroll
-        for (int k = 0; k < kRowBatch; ++k) xv[k] = x[c[k]];
+        for (int k = 0; k < kRowBatch; ++k) xv[k] = xat(c[k]);
 #pragma unroll
         for (int k = 0; k < kRowBatch; ++k) {
             const V t = s + a[k] * xv[k];
@@ -41,20 +43,25 @@ __device__ inline V row_ax(const int *__restrict__ rowptr, const int *__restrict
     }
     return s;
 }
+template <typename V>
+__device__ inline V row_ax(const int *__restrict__ rowptr, const int *__restrict__ col, const V *__restrict__ val,
+                           const V *__restrict__ x, int i) {
+    return row_ax_at(rowptr, col, val, [=](int j) { return x[j]; }, i);
+}
 
 // Row i of a banded grid's A x: entries among i-nx, i-1, i, i+1, i+nx (bit k
 // of mask[i]), values C[k n + i]; the present ones summed from 0 in
 // ascending column order, as row_ax does, so the two agree bit for bit.
-template <typename V>
-__device__ inline V band_ax(int n, int nx, const unsigned char *__restrict__ mask, const V *__restrict__ C,
-                            const V *__restrict__ x, int i) {
+template <typename V, class X>
+__device__ inline V band_ax_at(int n, int nx, const unsigned char *__restrict__ mask, const V *__restrict__ C, X xat,
+                               int i) {
     const int mk = mask[i];
     const int off[5] = {-nx, -1, 0, 1, nx};
     V a[5], xv[5];
 #pragma unroll
     for (int k = 0; k < 5; ++k) a[k] = C[(size_t)k * n + i];
 #pragma unroll
-    for (int k = 0; k < 5; ++k) xv[k] = x[((mk >> k) & 1) ? i + off[k] : i];
+    for (int k = 0; k < 5; ++k) xv[k] = xat(((mk >> k) & 1) ? i + off[k] : i);
     V s = 0;
 #pragma unroll
     for (int k = 0; k < 5; ++k) {
@@ -62,6 +69,11 @@ __device__ inline V band_ax(int n, int nx, const unsigned char *__restrict__ mas
         s = ((mk >> k) & 1) ? t : s;
     }
     return s;
+}
+template <typename V>
+__device__ inline V band_ax(int n, int nx, const unsigned char *__restrict__ mask, const V *__restrict__ C,
+                            const V *__restrict__ x, int i) {
+    return band_ax_at(n, nx, mask, C, [=](int j) { return x[j]; }, i);
 }
 
 template <typename V>

@@ -37,18 +37,21 @@ __device__ inline void chunk_reduce(double (&acc)[NV], double *__restrict__ part
     }
 }
 
-// One block of kBlock threads reduces NV partial arrays of `cnt` entries
-// (array v at part + v*stride + base); all threads get r.
+// The first kBlock threads of a block reduce NV partial arrays of `cnt`
+// entries (array v at part + v*stride + base); every thread of the block
+// (kBlock or more) calls it and gets r.
 template <int NV>
 __device__ inline void final_reduce_at(const double *__restrict__ part, int stride, int base, int cnt,
                                        double (&r)[NV]) {
     __shared__ double sh[NV][kBlock];
     const int t = threadIdx.x;
+    if (t < kBlock) {
 #pragma unroll
-    for (int v = 0; v < NV; ++v) {
-        double a = 0.0;
-        for (int j = t; j < cnt; j += kBlock) a = a + part[v * stride + base + j];
-        sh[v][t] = a;
+        for (int v = 0; v < NV; ++v) {
+            double a = 0.0;
+            for (int j = t; j < cnt; j += kBlock) a = a + part[v * stride + base + j];
+            sh[v][t] = a;
+        }
     }
     __syncthreads();
     if (t < 128) {
