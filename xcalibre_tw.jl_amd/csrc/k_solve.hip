@@ -409,6 +409,13 @@ __device__ inline double dsign(double x) { return x > 0.0 ? 1.0 : (x < 0.0 ? -1.
 // then as many as are done, up to 16 (each check idles the device for a
 // host round trip; iterations enqueued past the end skip at once).
 static int next_batch(int done_iters) { return done_iters < 4 ? 2 : std::min(16, done_iters); }
+// The first batch is enqueued without reading the scalars set up before it:
+// a solve that is already done (zero or small residual, itmax <= 0) finds
+// ks->done set on the device, and every kernel of the batch returns at once.
+// The read after each batch waits for the batch's kernels (and so for every
+// sweep the solve ran, before the caller's fault check); a solve that ran no
+// batch reads the scalars once at its end.
+static void first_batch(fvh_eqn *e) { e->kw.h_ks->done = 0; }
 
 // A linear system as the Krylov recurrences see it.  T is the storage type
 // of the vectors: double for the reference path (start value in x, A and M
@@ -540,7 +547,7 @@ static int bicgstab(const Sys<T> &S, const fvh_solver_cfg &cfg, T *xu, fvh_stats
         if (itmax <= 0) { k->done = 1; k->status = FVH_KRYLOV_ITMAX; }
     }));
     int enq = 0;
-    FVH_TRY(read_ks(e));
+    first_batch(e);
     while (!w.h_ks->done && enq < itmax) {
         int nb = std::min(next_batch(enq), itmax - enq);
         for (int it = 0; it < nb; ++it) {
@@ -598,7 +605,7 @@ static int bicgstab(const Sys<T> &S, const fvh_solver_cfg &cfg, T *xu, fvh_stats
     }
     if (S.warm) FVH_TRY(map(n, nullptr, [=] __device__(int i) { xu[i] = xk[i] + dx[i]; }));
     else FVH_TRY(map(n, nullptr, [=] __device__(int i) { xu[i] = xk[i]; }));
-    FVH_TRY(read_ks(e));
+    if (enq == 0) FVH_TRY(read_ks(e));
     fill_stats(e, st);
     return FVH_OK;
 }
@@ -826,7 +833,7 @@ static int cg(const Sys<T> &S, const fvh_solver_cfg &cfg, T *xu, fvh_stats *st, 
     const int *bmap = rt.map;
     const int bstride = rt.stride, nhand = rt.nhand, HS = rt.HS;
     const unsigned long long sent = rt.sent;
-    FVH_TRY(read_ks(e));
+    first_batch(e);
     // Each iteration's beta update runs fused with the next iteration's SpMV
     // (k_cg_beta_spmv, p alternating between two arrays); the last one of a
     // batch runs alone, in place, before the host reads the scalars.
@@ -888,7 +895,7 @@ static int cg(const Sys<T> &S, const fvh_solver_cfg &cfg, T *xu, fvh_stats *st, 
     }
     if (S.warm) FVH_TRY(map(n, nullptr, [=] __device__(int i) { xu[i] = xk[i] + dx[i]; }));
     else FVH_TRY(map(n, nullptr, [=] __device__(int i) { xu[i] = xk[i]; }));
-    FVH_TRY(read_ks(e));
+    if (enq == 0) FVH_TRY(read_ks(e));
     fill_stats(e, st);
     return FVH_OK;
 }

@@ -490,14 +490,15 @@ static int turbulence(Sim &S) {
 // ---------------------------------------------------------------------------
 // Loop body
 // ---------------------------------------------------------------------------
-static int momentum(Sim &S, fvh_eqn *e, double *U, const double *gp, int comp, bool relax, const char *name) {
+static int momentum(Sim &S, fvh_eqn *e, double *U, const double *gp, int comp, bool relax, const char *name,
+                    bool update_pre) {
     const int n = (int)S.m->n;
     FVH_TRY(copy_dev(S.prev.p, U, n));
     std::vector<KTerm> t = {term(FVH_OP_TIME, S.cs.sU.time, 1, nullptr, U),
                             term(FVH_OP_DIVERGENCE, S.cs.sU.divergence, 1, S.mdotf.p, U),
                             term(FVH_OP_LAPLACIAN, S.cs.sU.laplacian, -1, S.nueff.p, U)};
     FVH_TRY(assemble(S, e, t, {{-1.0, gp}}, S.bU, comp, S.planU, relax, S.cs.vU.relax, false));
-    FVH_TRY(precond_update(e));
+    if (update_pre) FVH_TRY(precond_update(e));
     FVH_TRY(run_solve(S, e, S.cs.vU, U, name));
     return residual_slot(S, e, U, comp);
 }
@@ -597,8 +598,12 @@ static int iterate_once(Sim &S) {
     const int n = (int)m->n;
     const int iter = S.it;
     const bool piso = S.cs.algorithm == FVH_PISO;
-    FVH_TRY(momentum(S, S.ux.get(), S.Ux.p, S.gpx.p, 0, !piso, "Ux"));
-    FVH_TRY(momentum(S, S.uy.get(), S.Uy.p, S.gpy.p, 1, !piso, "Uy"));
+    FVH_TRY(momentum(S, S.ux.get(), S.Ux.p, S.gpx.p, 0, !piso, "Ux", true));
+    // Uy's preconditioner is Ux's (Solvers_1_SIMPLE.jl:45), so the update
+    // before the Uy solve (:141) recomputes it from Ux's matrix, which has not
+    // changed since the update before the Ux solve (:132): the recomputation
+    // would store the same values, and is skipped.
+    FVH_TRY(momentum(S, S.uy.get(), S.Uy.p, S.gpy.p, 1, !piso, "Uy", S.uy->pshare != S.ux.get()));
     {
         const double *vol = m->d.vol, *ax = S.ux->nz.p, *gx = S.gpx.p, *gy = S.gpy.p;
         const int *diag = m->d.diag;
