@@ -227,6 +227,21 @@ def test_fp32_cg_fused_matches_unfused(ref_mesh, mesh_name, monkeypatch):
         assert same_bits(a, b)
 
 
+def test_mixed_momentum_pair_matches_sequential(quad40, monkeypatch):
+    """Mixed precision with the two momentum solves at once (Uy on a second
+    stream, its own FP32 copies of Ux's records) gives the same bits as with
+    the solves one after the other (FVH_U_PAIR=0)."""
+    model = cavity_model(quad40)
+    out = {}
+    for pair in ("1", "0"):
+        monkeypatch.setenv("FVH_U_PAIR", pair)
+        sim = F.Simulation(model, config(20, precision=L.MIXED), 0.0)
+        sim.run(20)
+        out[pair] = [sim.get(f) for f in ("Ux", "Uy", "p")] + [np.asarray(x) for x in sim.residuals()]
+    for a, b in zip(out["1"], out["0"]):
+        assert same_bits(a, b)
+
+
 def test_bfs_mixed_converges_to_oracle():
     """A small backward-facing step run to 1e-6 residuals in both forms."""
     mesh = F.backward_facing_step(nx=120, ny=40)

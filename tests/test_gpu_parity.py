@@ -93,6 +93,25 @@ def test_cg_beta_fused_with_spmv_bitwise(ref_mesh, mesh_name, fused, monkeypatch
     assert g.stats("p")["niter"] == r.stats("p")["niter"]
 
 
+@pytest.mark.parametrize("pair", ["1", "0"])
+def test_momentum_pair_bitwise(quad40, pair, monkeypatch):
+    """The two momentum solves at once (the default on banded meshes: Uy on a
+    second stream and thread, applying a copy of Ux's DILU records) and one
+    after the other (FVH_U_PAIR=0: Uy borrows Ux's preconditioner): both
+    bitwise, with BiCGSTAB and GMRES for U."""
+    monkeypatch.setenv("FVH_U_PAIR", pair)
+    model = cavity_model(quad40)
+    for usolver in (F.BicgstabSolver, F.GmresSolver):
+        g, r = _sims(model, config(15, U_solver=usolver), pref=0.0)
+        assert g.run(15) == r.run(15)
+        for a, b in zip(g.residuals(), r.residuals()):
+            assert same_bits(a, b)
+        for name in ("Ux", "Uy", "p", "mdotf"):
+            assert same_bits(g.get(name), r.get(name)), name
+        for fld in ("Ux", "Uy"):
+            assert g.stats(fld)["niter"] == r.stats(fld)["niter"], fld
+
+
 def test_upwind_midpoint_euler_piso(quad40):
     """Upwind divergence, Midpoint gradients, Euler time term, PISO loop."""
     model = cavity_model(quad40)

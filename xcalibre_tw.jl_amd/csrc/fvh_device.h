@@ -41,6 +41,7 @@ constexpr int kChunk = kBlock * kPer;
 inline int nchunks(int64_t n) { return (int)((n + kChunk - 1) / kChunk); }
 
 hipStream_t stream();
+void set_thread_stream(hipStream_t s);  // this thread's launches (null: the library's stream)
 int ensure_device();
 
 template <class T>

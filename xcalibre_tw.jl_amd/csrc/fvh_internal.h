@@ -26,6 +26,8 @@ struct HMesh {
 
 // Records a formatted message for fvh_last_error and returns `code`.
 int set_error(int code, const char *fmt, ...) __attribute__((format(printf, 2, 3)));
+// The calling thread's last message (each thread records its own).
+std::string last_error_text();
 
 }  // namespace fvh
 

@@ -33,6 +33,8 @@ struct SkewRect {
 int skew_setup(fvh_mesh *m, const std::vector<SkewRect> &rects);
 bool skew_active(const fvh_mesh *m);
 int skew_update(fvh_eqn *e, const double *val);
+int skew_copy(fvh_eqn *e, const fvh_eqn *src);
+bool skew_tuned32(const fvh_mesh *m);  // the FP32 sweep width is chosen (tuned or forced)
 int skew_apply(fvh_eqn *e, const double *x, double *y, const KScal *ks);
 int skew_to32(fvh_eqn *e);
 int skew_apply32(fvh_eqn *e, const float *x, float *y, const KScal *ks);

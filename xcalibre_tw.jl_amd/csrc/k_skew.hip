@@ -1135,6 +1135,24 @@ int skew_update(fvh_eqn *e, const double *val) {
     return FVH_OK;
 }
 
+// e's records become a copy of src's (both on the same mesh's pipelined
+// sweeps): F, Bk and the FP64 backward quads, which are what an update
+// writes and the applies read.  e then applies src's preconditioner with
+// buffers of its own (right-hand sides, results, hand-over arrays, fault
+// flag), so that the two equations' solves can run at the same time.
+int skew_copy(fvh_eqn *e, const fvh_eqn *src) {
+    const SkewLayout &L = *e->m->skew;
+    if (src->m != e->m || src->sk.F.n != L.NS || src->sk.s64.kb.n != (size_t)kBwdN * L.NS)
+        return set_error(FVH_ERR_ARG, "DILU records not prepared");
+    FVH_TRY(skew_alloc(e));
+    FVH_TRY(sweep_alloc(e->sk.s64, L));
+    HIP_TRY(hipMemcpyAsync(e->sk.F.p, src->sk.F.p, sizeof(Rec4) * L.NS, hipMemcpyDeviceToDevice, stream()));
+    HIP_TRY(hipMemcpyAsync(e->sk.Bk.p, src->sk.Bk.p, sizeof(Rec4) * L.NS, hipMemcpyDeviceToDevice, stream()));
+    HIP_TRY(hipMemcpyAsync(e->sk.s64.kb.p, src->sk.s64.kb.p, sizeof(double) * src->sk.s64.kb.n,
+                           hipMemcpyDeviceToDevice, stream()));
+    return FVH_OK;
+}
+
 template <typename T, int W>
 static int launch_sweeps_w(fvh_eqn *e, SkewSweep<T> &w, const KScal *ks) {
     const SkewLayout &L = *e->m->skew;
@@ -1235,6 +1253,8 @@ int skew_apply(fvh_eqn *e, const double *x, double *y, const KScal *ks) {
     if (!s.s64.kf.p) return set_error(FVH_ERR_ARG, "DILU records not prepared");
     return skew_apply_t<double>(e, x, y, ks, s.s64);
 }
+
+bool skew_tuned32(const fvh_mesh *m) { return forced_waves() || m->skew->W32 != 0; }
 
 int skew_to32(fvh_eqn *e) {
     SkewStore &s = e->sk;

@@ -16,6 +16,8 @@ static thread_local std::string g_last_error;
 static hipStream_t g_stream = nullptr;
 static int g_device_ok = -1;
 
+std::string last_error_text() { return g_last_error; }
+
 int set_error(int code, const char *fmt, ...) {
     char buf[1024];
     va_list ap;
@@ -44,7 +46,12 @@ int ensure_device() {
     return FVH_OK;
 }
 
-hipStream_t stream() { return g_stream; }
+// A thread may route its launches to a stream of its own (the two momentum
+// solves of a SIMPLE iteration run at once, one of them on a second thread);
+// every other thread uses the library's stream.
+static thread_local hipStream_t t_stream = nullptr;
+hipStream_t stream() { return t_stream ? t_stream : g_stream; }
+void set_thread_stream(hipStream_t s) { t_stream = s; }
 
 // Per-cell boundary-condition lists for one BC tuple, cached on the mesh.
 int bc_plan(fvh_mesh *m, const fvh_bc *bcs, int nbc, BcPlan **out) {

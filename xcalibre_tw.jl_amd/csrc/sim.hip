@@ -5,9 +5,13 @@
 // iteration for the convergence break and nothing else.
 #include <chrono>
 #include <cmath>
+#include <cstdlib>
 #include <limits>
+#include <string>
+#include <thread>
 
 #include "fvh_eqn.h"
+#include "k_band.h"
 #include "k_util.h"
 
 namespace fvh {
@@ -91,6 +95,16 @@ struct Sim {
     std::vector<int> case_break;    // iteration of each case's break, -1 = running
     DBuf<double> snapUx, snapUy, snapP;
     std::map<std::string, std::vector<fvh_stats>> cstats;
+    // the second stream of the concurrent momentum solves, and the event it
+    // waits on (created at first use)
+    struct Side {
+        hipStream_t s = nullptr;
+        hipEvent_t ev = nullptr;
+        ~Side() {
+            if (ev) (void)hipEventDestroy(ev);
+            if (s) (void)hipStreamDestroy(s);
+        }
+    } side;
 };
 
 // ---------------------------------------------------------------------------
@@ -490,14 +504,18 @@ static int turbulence(Sim &S) {
 // ---------------------------------------------------------------------------
 // Loop body
 // ---------------------------------------------------------------------------
-static int momentum(Sim &S, fvh_eqn *e, double *U, const double *gp, int comp, bool relax, const char *name,
-                    bool update_pre) {
+static int momentum_assemble(Sim &S, fvh_eqn *e, double *U, const double *gp, int comp, bool relax) {
     const int n = (int)S.m->n;
     FVH_TRY(copy_dev(S.prev.p, U, n));
     std::vector<KTerm> t = {term(FVH_OP_TIME, S.cs.sU.time, 1, nullptr, U),
                             term(FVH_OP_DIVERGENCE, S.cs.sU.divergence, 1, S.mdotf.p, U),
                             term(FVH_OP_LAPLACIAN, S.cs.sU.laplacian, -1, S.nueff.p, U)};
-    FVH_TRY(assemble(S, e, t, {{-1.0, gp}}, S.bU, comp, S.planU, relax, S.cs.vU.relax, false));
+    return assemble(S, e, t, {{-1.0, gp}}, S.bU, comp, S.planU, relax, S.cs.vU.relax, false);
+}
+
+static int momentum(Sim &S, fvh_eqn *e, double *U, const double *gp, int comp, bool relax, const char *name,
+                    bool update_pre) {
+    FVH_TRY(momentum_assemble(S, e, U, gp, comp, relax));
     if (update_pre) FVH_TRY(precond_update(e));
     FVH_TRY(run_solve(S, e, S.cs.vU, U, name));
     return residual_slot(S, e, U, comp);
@@ -593,17 +611,79 @@ _Pragma("unroll")
     return FVH_OK;
 }
 
+// The two momentum solves run at once when the U preconditioner is DILU on
+// the pipelined sweeps (one case; FVH_U_PAIR=0 turns this off).
+static bool u_pair(const Sim &S) {
+    const char *env = std::getenv("FVH_U_PAIR");
+    return !(env && env[0] == '0') && !S.cases && S.cs.vU.precond == FVH_PRECOND_DILU && band_path(S.m) &&
+           skew_active(S.m);
+}
+
+// Both momentum equations are assembled (neither reads the other's
+// solution), Ux's preconditioner is updated and Uy's records are copied from
+// it: Uy's preconditioner is Ux's (Solvers_1_SIMPLE.jl:45), and the update
+// before the Uy solve (:141) recomputes it from Ux's matrix, unchanged since
+// the update before the Ux solve (:132).  The Uy solve and residual then run
+// on a second stream from a second thread while this thread runs Ux's: the
+// solves share no buffer that either writes, so their results are those of
+// the reference's order.  The first mixed-precision solve on a mesh times
+// its FP32 sweep widths and runs alone.
+static int momentum_pair(Sim &S, bool piso) {
+    fvh_eqn *ex = S.ux.get(), *ey = S.uy.get();
+    FVH_TRY(momentum_assemble(S, ex, S.Ux.p, S.gpx.p, 0, !piso));
+    FVH_TRY(precond_update(ex));
+    FVH_TRY(momentum_assemble(S, ey, S.Uy.p, S.gpy.p, 1, !piso));
+    FVH_TRY(skew_copy(ey, ex));
+    if (S.cs.vU.precision != FVH_FP64 && !skew_tuned32(S.m)) {
+        FVH_TRY(run_solve(S, ex, S.cs.vU, S.Ux.p, "Ux"));
+        FVH_TRY(residual_slot(S, ex, S.Ux.p, 0));
+        FVH_TRY(run_solve(S, ey, S.cs.vU, S.Uy.p, "Uy"));
+        return residual_slot(S, ey, S.Uy.p, 1);
+    }
+    if (!S.side.s) HIP_TRY(hipStreamCreateWithFlags(&S.side.s, hipStreamNonBlocking));
+    if (!S.side.ev) HIP_TRY(hipEventCreateWithFlags(&S.side.ev, hipEventDisableTiming));
+    HIP_TRY(hipEventRecord(S.side.ev, stream()));
+    fvh_stats sx{}, sy{};
+    int rc_y = FVH_OK;
+    std::string err_y;
+    std::thread th([&] {
+        set_thread_stream(S.side.s);
+        if (hipStreamWaitEvent(S.side.s, S.side.ev, 0) != hipSuccess)
+            rc_y = set_error(FVH_ERR_HIP, "hipStreamWaitEvent failed");
+        if (rc_y == FVH_OK) rc_y = solve(ey, S.cs.vU, S.Uy.p, &sy);
+        if (rc_y == FVH_OK) rc_y = residual_slot(S, ey, S.Uy.p, 1);
+        const hipError_t se = hipStreamSynchronize(S.side.s);
+        if (rc_y == FVH_OK && se != hipSuccess) rc_y = set_error(FVH_ERR_HIP, "Uy stream: %s", hipGetErrorString(se));
+        if (rc_y != FVH_OK) err_y = last_error_text();
+        set_thread_stream(nullptr);
+    });
+    int rc_x = solve(ex, S.cs.vU, S.Ux.p, &sx);
+    if (rc_x == FVH_OK) rc_x = residual_slot(S, ex, S.Ux.p, 0);
+    th.join();
+    S.stats["Ux"] = sx;
+    S.stats["Uy"] = sy;
+    if (rc_x != FVH_OK) return rc_x;
+    if (rc_y != FVH_OK) return set_error(rc_y, "%s", err_y.c_str());
+    return FVH_OK;
+}
+
 static int iterate_once(Sim &S) {
     fvh_mesh *m = S.m;
     const int n = (int)m->n;
     const int iter = S.it;
     const bool piso = S.cs.algorithm == FVH_PISO;
-    FVH_TRY(momentum(S, S.ux.get(), S.Ux.p, S.gpx.p, 0, !piso, "Ux", true));
-    // Uy's preconditioner is Ux's (Solvers_1_SIMPLE.jl:45), so the update
-    // before the Uy solve (:141) recomputes it from Ux's matrix, which has not
-    // changed since the update before the Ux solve (:132): the recomputation
-    // would store the same values, and is skipped.
-    FVH_TRY(momentum(S, S.uy.get(), S.Uy.p, S.gpy.p, 1, !piso, "Uy", S.uy->pshare != S.ux.get()));
+    if (u_pair(S)) {
+        S.uy->pshare = nullptr;  // Uy applies its copy of Ux's records
+        FVH_TRY(momentum_pair(S, piso));
+    } else {
+        S.uy->pshare = S.ux.get();
+        FVH_TRY(momentum(S, S.ux.get(), S.Ux.p, S.gpx.p, 0, !piso, "Ux", true));
+        // Uy's preconditioner is Ux's (Solvers_1_SIMPLE.jl:45), so the update
+        // before the Uy solve (:141) recomputes it from Ux's matrix, which has
+        // not changed since the update before the Ux solve (:132): the
+        // recomputation would store the same values, and is skipped.
+        FVH_TRY(momentum(S, S.uy.get(), S.Uy.p, S.gpy.p, 1, !piso, "Uy", false));
+    }
     {
         const double *vol = m->d.vol, *ax = S.ux->nz.p, *gx = S.gpx.p, *gy = S.gpy.p;
         const int *diag = m->d.diag;
@@ -700,6 +780,9 @@ static int sim_init(Sim &S, const double *Ux, const double *Uy, const double *Uz
     FVH_TRY(fvh_eqn_create(m, &raw));
     S.pe.reset(raw);
     FVH_TRY(fvh_precond_set(S.ux.get(), S.cs.vU.precond, nullptr));
+    // Uy borrows Ux's preconditioner, or, when the two solves run at once,
+    // applies a copy of its records with buffers of its own
+    if (S.cs.vU.precond == FVH_PRECOND_DILU) FVH_TRY(fvh_precond_set(S.uy.get(), FVH_PRECOND_DILU, nullptr));
     S.uy->pshare = S.ux.get();
     FVH_TRY(fvh_precond_set(S.pe.get(), S.cs.vp.precond, nullptr));
     if (S.cs.model == FVH_KOMEGA) {
