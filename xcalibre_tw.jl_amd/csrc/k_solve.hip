@@ -628,10 +628,25 @@ __device__ inline bool block_done(const KScal *ks) {
 }
 inline dim3 fused_grid(int64_t n) { return dim3((unsigned)std::max<int64_t>(1, std::min<int64_t>((n + kBlock - 1) / kBlock, 1024))); }
 
-// alpha = gamma / pAp (or stop on zero curvature), then x += alpha p, r -= alpha Ap
-template <class Op>
-__global__ void __launch_bounds__(kBlock) k_cg_alpha(int n, const double *part, int nch, KScal *ks, double epsT, Op op) {
+// alpha = gamma / pAp (or stop on zero curvature), then x += alpha p,
+// r -= alpha Ap: `load(i)` reads an element's inputs, `op(i, v, alpha)`
+// writes its results.  Each thread has at most kAlphaPer elements (the grid
+// is sized for that), loaded before the reduction so that the two
+// latencies overlap.
+constexpr int kAlphaPer = 4;
+inline dim3 alpha_grid(int64_t n) {
+    return dim3((unsigned)std::max<int64_t>(1, (n + kBlock * kAlphaPer - 1) / (kBlock * kAlphaPer)));
+}
+template <class Ld, class Op>
+__global__ void __launch_bounds__(kBlock)
+    k_cg_alpha(int n, const double *part, int nch, KScal *ks, double epsT, Ld load, Op op) {
     if (block_done(ks)) return;
+    using St = decltype(load(0));
+    St v[kAlphaPer];
+    const int i0 = blockIdx.x * blockDim.x + threadIdx.x, stride = gridDim.x * blockDim.x;
+#pragma unroll
+    for (int k = 0; k < kAlphaPer; ++k)
+        if (i0 + k * stride < n) v[k] = load(i0 + k * stride);
     double pa[1];
     final_reduce<1>(part, nch, pa);
     const double pAp = pa[0], gamma = ks->gamma, pn2 = ks->pnorm2;
@@ -647,7 +662,9 @@ __global__ void __launch_bounds__(kBlock) k_cg_alpha(int n, const double *part, 
         }
     }
     if (zc) return;
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) op(i, al);
+#pragma unroll
+    for (int k = 0; k < kAlphaPer; ++k)
+        if (i0 + k * stride < n) op(i0 + k * stride, v[k], al);
 }
 
 // beta = gamma_new / gamma (or stop: solved, itmax), then p = z + beta p
@@ -768,6 +785,14 @@ static int launch_cg_beta_spmv(fvh_eqn *e, const double *rz, KScal *ks, int itma
     return FVH_OK;
 }
 
+// An element's inputs to CG's alpha update (and its slot in the sweeps'
+// records when r goes there).
+template <typename T>
+struct AlphaIn {
+    T x, p, r, ap;
+    int slot;
+};
+
 // ---------------------------------------------------------------------------
 // CG (Krylov.jl cg!) on sigma*A x = sigma*b
 // ---------------------------------------------------------------------------
@@ -854,20 +879,22 @@ static int cg(const Sys<T> &S, const fvh_solver_cfg &cfg, T *xu, fvh_stats *st, 
                 }
                 FVH_TRY(sys_spmv_dot(S, pc, Ap, sigma, ks, pc, ppap));
             }
+            const T *pk = pc;
+            const auto ld = [=] __device__(int i) { return AlphaIn<T>{xk[i], pk[i], r[i], Ap[i], qmap ? bmap[i] : 0}; };
             if (qmap)
-                hipLaunchKernelGGL(k_cg_alpha, fused_grid(n), dim3(kBlock), 0, stream(), (int)n, ppap, nchunks(n), ks,
-                                   epsT, [=] __device__(int i, double al) {
-                                       xk[i] = D(xk[i]) + al * D(pc[i]);
-                                       const T ri = D(r[i]) + (-al) * D(Ap[i]);
+                hipLaunchKernelGGL(k_cg_alpha, alpha_grid(n), dim3(kBlock), 0, stream(), (int)n, ppap, nchunks(n), ks,
+                                   epsT, ld, [=] __device__(int i, const AlphaIn<T> &v, double al) {
+                                       xk[i] = D(v.x) + al * D(v.p);
+                                       const T ri = D(v.r) + (-al) * D(v.ap);
                                        r[i] = ri;
-                                       bdst[(size_t)bmap[i] * bstride] = ri;
+                                       bdst[(size_t)v.slot * bstride] = ri;
                                        for (int q = i; q < nhand; q += n) hand[q] = q % HS == 0 ? T(0) : bits_as<T>(sent);
                                    });
             else
-                hipLaunchKernelGGL(k_cg_alpha, fused_grid(n), dim3(kBlock), 0, stream(), (int)n, ppap, nchunks(n), ks,
-                                   epsT, [=] __device__(int i, double al) {
-                                       xk[i] = D(xk[i]) + al * D(pc[i]);
-                                       r[i] = D(r[i]) + (-al) * D(Ap[i]);
+                hipLaunchKernelGGL(k_cg_alpha, alpha_grid(n), dim3(kBlock), 0, stream(), (int)n, ppap, nchunks(n), ks,
+                                   epsT, ld, [=] __device__(int i, const AlphaIn<T> &v, double al) {
+                                       xk[i] = D(v.x) + al * D(v.p);
+                                       r[i] = D(v.r) + (-al) * D(v.ap);
                                    });
             HIP_TRY(hipGetLastError());
             if (qmap) {  // r already in the sweep layout, z read straight from the sweeps' quads
