@@ -628,6 +628,16 @@ static bool u_pair(const Sim &S) {
 // solves share no buffer that either writes, so their results are those of
 // the reference's order.  The first mixed-precision solve on a mesh times
 // its FP32 sweep widths and runs alone.
+// The Uy half of the pair, on the calling thread's stream (exceptions become
+// error codes here: none may leave the thread).
+static int uy_side(Sim &S, fvh_eqn *ey, fvh_stats *sy) {
+    FVH_GUARD_BEGIN
+    HIP_TRY(hipStreamWaitEvent(stream(), S.side.ev, 0));
+    FVH_TRY(solve(ey, S.cs.vU, S.Uy.p, sy));
+    return residual_slot(S, ey, S.Uy.p, 1);
+    FVH_GUARD_END
+}
+
 static int momentum_pair(Sim &S, bool piso) {
     fvh_eqn *ex = S.ux.get(), *ey = S.uy.get();
     FVH_TRY(momentum_assemble(S, ex, S.Ux.p, S.gpx.p, 0, !piso));
@@ -648,17 +658,20 @@ static int momentum_pair(Sim &S, bool piso) {
     std::string err_y;
     std::thread th([&] {
         set_thread_stream(S.side.s);
-        if (hipStreamWaitEvent(S.side.s, S.side.ev, 0) != hipSuccess)
-            rc_y = set_error(FVH_ERR_HIP, "hipStreamWaitEvent failed");
-        if (rc_y == FVH_OK) rc_y = solve(ey, S.cs.vU, S.Uy.p, &sy);
-        if (rc_y == FVH_OK) rc_y = residual_slot(S, ey, S.Uy.p, 1);
-        const hipError_t se = hipStreamSynchronize(S.side.s);
+        rc_y = uy_side(S, ey, &sy);
+        const hipError_t se = hipStreamSynchronize(S.side.s);  // also after a failure: nothing left in flight
         if (rc_y == FVH_OK && se != hipSuccess) rc_y = set_error(FVH_ERR_HIP, "Uy stream: %s", hipGetErrorString(se));
         if (rc_y != FVH_OK) err_y = last_error_text();
         set_thread_stream(nullptr);
     });
-    int rc_x = solve(ex, S.cs.vU, S.Ux.p, &sx);
-    if (rc_x == FVH_OK) rc_x = residual_slot(S, ex, S.Ux.p, 0);
+    int rc_x = FVH_OK;
+    try {  // the thread is joined on every path
+        rc_x = solve(ex, S.cs.vU, S.Ux.p, &sx);
+        if (rc_x == FVH_OK) rc_x = residual_slot(S, ex, S.Ux.p, 0);
+    } catch (...) {
+        th.join();
+        throw;
+    }
     th.join();
     S.stats["Ux"] = sx;
     S.stats["Uy"] = sy;
