@@ -653,12 +653,15 @@ static int momentum_pair(Sim &S, bool piso) {
     if (!S.side.s) HIP_TRY(hipStreamCreateWithFlags(&S.side.s, hipStreamNonBlocking));
     if (!S.side.ev) HIP_TRY(hipEventCreateWithFlags(&S.side.ev, hipEventDisableTiming));
     HIP_TRY(hipEventRecord(S.side.ev, stream()));
+    int dev = 0;
+    HIP_TRY(hipGetDevice(&dev));
     fvh_stats sx{}, sy{};
     int rc_y = FVH_OK;
     std::string err_y;
     std::thread th([&] {
         set_thread_stream(S.side.s);
-        rc_y = uy_side(S, ey, &sy);
+        // the new thread's allocations go to this thread's device
+        rc_y = hipSetDevice(dev) == hipSuccess ? uy_side(S, ey, &sy) : set_error(FVH_ERR_HIP, "hipSetDevice failed");
         const hipError_t se = hipStreamSynchronize(S.side.s);  // also after a failure: nothing left in flight
         if (rc_y == FVH_OK && se != hipSuccess) rc_y = set_error(FVH_ERR_HIP, "Uy stream: %s", hipGetErrorString(se));
         if (rc_y != FVH_OK) err_y = last_error_text();
